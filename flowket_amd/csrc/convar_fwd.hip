@@ -1,0 +1,776 @@
+// Fused ConvNetAutoregressive2D forward (machines/conv_net_autoregressive_2D.py
+// :33-74 + machines/abstract_machine.py:31-71) for gfx950.
+//
+// One workgroup evaluates a tile of T configurations through the whole network
+// with every activation resident in LDS (buffers V0, H0, V1, H1, CAT of
+// T*H*W pixels x C channels).  Each convolution is an implicit GEMM on MFMA:
+// A = weights (16 output channels x k input slots), B = activations (k slots
+// x 16 pixels) read from LDS at the tap offset, D = 16 channels x 16 pixels.
+// The 1x1 convolutions (x, y, head) take their B operand straight from the
+// registers of the preceding convolution's D fragment: input slots are
+// numbered so that a D fragment *is* a B fragment (see convar_layout.h).
+// Zero padding, DownShift and RightShift are index arithmetic (tap validity
+// masks and shifted write targets), never materialised.
+// HBM traffic: H*W int8 spins in, 8 bytes out per configuration.
+#include "fk_common.h"
+#include "convar_layout.h"
+
+namespace fk {
+
+template <int PREC>
+struct Prec;
+
+template <>
+struct Prec<FK_PREC_BF16> {
+  using frag = bf16x8;
+  static constexpr int ESZ = 2;
+  static constexpr int KW = 32;
+  __device__ static __forceinline__ f32x4 mfma(frag a, frag b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+  __device__ static __forceinline__ frag zero() {
+    frag z;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) z[e] = (__bf16)0.0f;
+    return z;
+  }
+};
+
+template <>
+struct Prec<FK_PREC_FP32> {
+  using frag = float;
+  static constexpr int ESZ = 4;
+  static constexpr int KW = 4;
+  __device__ static __forceinline__ f32x4 mfma(frag a, frag b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static __forceinline__ frag zero() { return 0.0f; }
+};
+
+template <int ACT>
+__device__ __forceinline__ float act1(float x) {
+  if constexpr (ACT == FK_ACT_RELU) return fmaxf(x, 0.0f);
+  else return x > 0.0f ? x : expm1f(x);
+}
+template <int ACT>
+__device__ __forceinline__ f32x4 act4(f32x4 v) {
+  f32x4 r;
+  r.x = act1<ACT>(v.x);
+  r.y = act1<ACT>(v.y);
+  r.z = act1<ACT>(v.z);
+  r.w = act1<ACT>(v.w);
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// LDS addressing.  A pixel holds C channels as 16-byte chunks; chunk indices
+// are XOR-swizzled with the pixel index so that 16 lanes reading the same
+// chunk of 16 consecutive pixels hit distinct banks.
+template <int PREC, int C>
+struct Lds {
+  static constexpr int ESZ = Prec<PREC>::ESZ;
+  static constexpr int PIXB = C * ESZ;
+  static constexpr int CPP = PIXB / 16;  // chunks per pixel
+  static constexpr int ROWB = 256;       // bytes per LDS bank row
+  static constexpr int PPR = PIXB >= ROWB ? 1 : ROWB / PIXB;
+  static constexpr bool SWZ = (CPP & (CPP - 1)) == 0;
+  __device__ static __forceinline__ int chunk(int p, int c) {
+    const int sw = SWZ ? ((p / PPR) & (CPP - 1)) : 0;
+    return p * PIXB + ((c ^ sw) << 4);
+  }
+  // byte offset of the 4-channel quad q (channels 4q..4q+3) of pixel p
+  __device__ static __forceinline__ int quad(int p, int q) {
+    if constexpr (PREC == FK_PREC_BF16) {
+      const int ks = q >> 3, w = q & 7;
+      return chunk(p, 4 * ks + (w & 3)) + ((w >> 2) << 3);
+    } else {
+      return chunk(p, (q & 3) * (C / 16) + (q >> 2));
+    }
+  }
+  __device__ static __forceinline__ void store_quad(uint8_t* buf, int p, int q, f32x4 v) {
+    if constexpr (PREC == FK_PREC_BF16) {
+      bf16x4 h;
+      h[0] = (__bf16)v.x;
+      h[1] = (__bf16)v.y;
+      h[2] = (__bf16)v.z;
+      h[3] = (__bf16)v.w;
+      *reinterpret_cast<bf16x4*>(buf + quad(p, q)) = h;
+    } else {
+      *reinterpret_cast<f32x4*>(buf + quad(p, q)) = v;
+    }
+  }
+  __device__ static __forceinline__ f32x4 load_quad(const uint8_t* buf, int p, int q) {
+    if constexpr (PREC == FK_PREC_BF16) {
+      bf16x4 h = *reinterpret_cast<const bf16x4*>(buf + quad(p, q));
+      f32x4 v;
+      v.x = (float)h[0];
+      v.y = (float)h[1];
+      v.z = (float)h[2];
+      v.w = (float)h[3];
+      return v;
+    } else {
+      return *reinterpret_cast<const f32x4*>(buf + quad(p, q));
+    }
+  }
+  // All B fragments (one per k-step) of pixel p for lane group g.
+  template <int KS>
+  __device__ static __forceinline__ void load_bfrags(const uint8_t* buf, int p, int g,
+                                                     typename Prec<PREC>::frag (&b)[KS]) {
+    if constexpr (PREC == FK_PREC_BF16) {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) b[ks] = *reinterpret_cast<const bf16x8*>(buf + chunk(p, 4 * ks + g));
+    } else {
+#pragma unroll
+      for (int j = 0; j < KS / 4; ++j) {
+        f32x4 v = *reinterpret_cast<const f32x4*>(buf + chunk(p, g * (C / 16) + j));
+        b[4 * j + 0] = v.x;
+        b[4 * j + 1] = v.y;
+        b[4 * j + 2] = v.z;
+        b[4 * j + 3] = v.w;
+      }
+    }
+  }
+};
+
+// D fragments (MT tiles of 4 channels per lane) -> B fragments of the next
+// 1x1 contraction, in registers.
+template <int PREC, int MT, int KS>
+__device__ __forceinline__ void d_to_b(const f32x4 (&d)[MT], typename Prec<PREC>::frag (&b)[KS]) {
+  if constexpr (PREC == FK_PREC_BF16) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      bf16x8 f;
+      f[0] = (__bf16)d[2 * ks].x;
+      f[1] = (__bf16)d[2 * ks].y;
+      f[2] = (__bf16)d[2 * ks].z;
+      f[3] = (__bf16)d[2 * ks].w;
+      f[4] = (__bf16)d[2 * ks + 1].x;
+      f[5] = (__bf16)d[2 * ks + 1].y;
+      f[6] = (__bf16)d[2 * ks + 1].z;
+      f[7] = (__bf16)d[2 * ks + 1].w;
+      b[ks] = f;
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < KS; ++t) b[t] = d[t >> 2][t & 3];
+  }
+}
+
+// Tap tables: KIND 0 = vertical kxk (rows -2..0, cols -1..1), 1 = horizontal
+// 1xk (cols -2..0), 2 = h3conv kxk on the concatenation (rows -2..0, cols
+// -2..0).  Tap index = di*kw + dj in the Keras kernel.
+template <int KIND>
+struct Taps {
+  static constexpr int N = (KIND == 1) ? 3 : 9;
+  __device__ static __forceinline__ int dr(int t) { return KIND == 1 ? 0 : (t / 3) - 2; }
+  __device__ static __forceinline__ int dc(int t) {
+    return KIND == 0 ? (t % 3) - 1 : (KIND == 1 ? t - 2 : (t % 3) - 2);
+  }
+  // bit of the (dr, dc) validity mask
+  __device__ static __forceinline__ int bit(int t) { return (dr(t) + 2) * 4 + (dc(t) + 2); }
+};
+
+struct FwdArgs {
+  const void* w;
+  const float* aux;
+  int H, W, HW, nblocks, T, npix;
+  const int8_t* spins;
+  const int32_t* pairs;
+  const int64_t* m_dev;
+  int64_t m_count;
+  int conn_kind;
+  const int32_t* conn_a;
+  const int32_t* conn_b;
+  float* logpsi;
+  float* cond;
+};
+
+// acc[u][mt] += sum over taps of W[tap][mt] x B(pixel + tap offset)
+template <int PREC, int C, int MAXT, int KIND, int MTO>
+__device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t* buf,
+                                          const typename Prec<PREC>::frag* __restrict__ wg, const int (&pix)[MAXT],
+                                          const uint32_t (&msk)[MAXT], int nt, int W, int lane, int g) {
+  using P = Prec<PREC>;
+  using frag = typename P::frag;
+  using L = Lds<PREC, C>;
+  constexpr int KS = C / P::KW;
+  constexpr int NTAP = Taps<KIND>::N;
+  // weights of the current tap; the next tap's are fetched one iteration
+  // ahead so the L2 latency overlaps the current tap's MFMAs
+  frag w[MTO][KS], wn[MTO][KS];
+#pragma unroll
+  for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) w[mt][ks] = wg[(mt * KS + ks) * 64 + lane];
+#pragma unroll 1
+  for (int tap = 0; tap < NTAP; ++tap) {
+    const int tn = (tap + 1 < NTAP) ? tap + 1 : tap;
+#pragma unroll
+    for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) wn[mt][ks] = wg[((tn * MTO + mt) * KS + ks) * 64 + lane];
+    const int off = Taps<KIND>::dr(tap) * W + Taps<KIND>::dc(tap);
+    const int bit = Taps<KIND>::bit(tap);
+#pragma unroll
+    for (int u = 0; u < MAXT; ++u) {
+      if (u < nt) {
+        const bool valid = (msk[u] >> bit) & 1u;
+        frag b[KS];
+        L::template load_bfrags<KS>(buf, valid ? pix[u] + off : 0, g, b);
+        if (!valid) {
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) b[ks] = P::zero();
+        }
+#pragma unroll
+        for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) acc[u][mt] = P::mfma(w[mt][ks], b[ks], acc[u][mt]);
+      }
+    }
+#pragma unroll
+    for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) w[mt][ks] = wn[mt][ks];
+  }
+}
+
+__device__ __forceinline__ f32x4 load_bias4(const float* b, int ch, int limit) {
+  f32x4 v;
+  v.x = (ch + 0 < limit) ? b[ch + 0] : 0.0f;
+  v.y = (ch + 1 < limit) ? b[ch + 1] : 0.0f;
+  v.z = (ch + 2 < limit) ? b[ch + 2] : 0.0f;
+  v.w = (ch + 3 < limit) ? b[ch + 3] : 0.0f;
+  return v;
+}
+
+enum BlockKind { BK_FIRST = 0, BK_A = 1, BK_B = 2, BK_FINAL = 3 };
+
+template <int PREC, int C, int MAXT, int ACT>
+__global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
+  using P = Prec<PREC>;
+  using frag = typename P::frag;
+  using L = Lds<PREC, C>;
+  constexpr int MT = C / 16;
+  constexpr int KS = C / P::KW;
+  constexpr int MTH = (C / 2 + 15) / 16;
+  constexpr int PIXB = L::PIXB;
+  const ConvarLayout lay(C, PREC);
+
+  extern __shared__ __align__(16) uint8_t lds[];
+  const int npix = a.npix;
+  uint8_t* V0 = lds;
+  uint8_t* H0 = V0 + npix * PIXB;
+  uint8_t* V1 = H0 + npix * PIXB;
+  uint8_t* H1 = V1 + npix * PIXB;
+  uint8_t* CAT = H1 + npix * PIXB;
+  float* sp = reinterpret_cast<float*>(CAT + npix * PIXB);
+  float2* part = reinterpret_cast<float2*>(sp + npix);
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, NW = blockDim.x >> 6;
+  const int n16 = lane & 15, g = lane >> 4;
+  const int H = a.H, W = a.W, HW = a.HW, T = a.T;
+  const int ntiles = npix >> 4;
+  const int nt = (ntiles > wave) ? (ntiles - wave + NW - 1) / NW : 0;
+
+  int pix[MAXT], pi[MAXT], pj[MAXT];
+  uint32_t msk[MAXT];
+  bool inp[MAXT];
+#pragma unroll
+  for (int u = 0; u < MAXT; ++u) {
+    const int p = (wave + u * NW) * 16 + n16;
+    const bool in = (u < nt) && (p < T * HW);
+    const int t = in ? p / HW : 0;
+    const int rem = in ? p - t * HW : 0;
+    const int i = rem / W, j = rem - (rem / W) * W;
+    uint32_t m = 0;
+    if (in) {
+#pragma unroll
+      for (int dr = -2; dr <= 0; ++dr)
+#pragma unroll
+        for (int dc = -2; dc <= 1; ++dc)
+          if (i + dr >= 0 && j + dc >= 0 && j + dc < W) m |= 1u << ((dr + 2) * 4 + (dc + 2));
+    }
+    pix[u] = p;
+    pi[u] = i;
+    pj[u] = j;
+    msk[u] = m;
+    inp[u] = in;
+  }
+
+  int64_t M = a.m_dev ? *a.m_dev : a.m_count;
+  if (M > a.m_count) M = a.m_count;
+  const int64_t ngroups = (M + T - 1) / T;
+  const int nb = a.nblocks;
+  const frag* wall = reinterpret_cast<const frag*>(a.w);
+
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    // ---- configurations -> LDS (float spins; connection applied on the fly)
+    for (int p = threadIdx.x; p < npix; p += blockDim.x) {
+      float s = 0.0f;
+      const int t = p / HW;
+      const int64_t m = grp * T + t;
+      if (p < T * HW && m < M) {
+        const int q = p - t * HW;
+        int64_t b = m;
+        int c = 0;
+        if (a.pairs) {
+          b = a.pairs[2 * m];
+          c = a.pairs[2 * m + 1];
+        }
+        const int8_t* row = a.spins + b * HW;
+        int8_t v = row[q];
+        if (c > 0) {
+          const int ca = a.conn_a[c];
+          if (a.conn_kind == FK_CONN_FLIP) {
+            if (q == ca) v = (int8_t)-v;
+          } else if (ca >= 0) {
+            const int cb = a.conn_b[c];
+            if (q == ca) v = row[cb];
+            else if (q == cb) v = row[ca];
+          }
+        }
+        s = (float)v;
+      }
+      sp[p] = s;
+    }
+    __syncthreads();
+
+    for (int blk = 0; blk < nb; ++blk) {
+      const int kind = (blk == 0) ? BK_FIRST : (blk == nb - 1 ? BK_FINAL : ((blk & 1) ? BK_A : BK_B));
+      const uint8_t* Vin = (kind == BK_B) ? V1 : V0;
+      const uint8_t* Hin = (kind == BK_B) ? H1 : H0;
+      const float* bias = a.aux + lay.aux_bias(blk);
+
+      // ================= phase VH: vertical stack + horizontal 1xk =========
+      {
+        // -- vertical kxk (pre-activation v')
+        f32x4 acc[MAXT][MT];
+#pragma unroll
+        for (int u = 0; u < MAXT; ++u)
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = load_bias4(bias, 16 * mt + 4 * g, C);
+        if (kind == BK_FIRST) {
+          const float* kv = a.aux + lay.aux_kv0(nb);
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u) {
+            if (u < nt) {
+#pragma unroll
+              for (int tap = 0; tap < 9; ++tap) {
+                const bool valid = (msk[u] >> Taps<0>::bit(tap)) & 1u;
+                const float s = valid ? sp[pix[u] + Taps<0>::dr(tap) * W + Taps<0>::dc(tap)] : 0.0f;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                  const f32x4 k = *reinterpret_cast<const f32x4*>(kv + tap * C + 16 * mt + 4 * g);
+                  acc[u][mt] += k * s;
+                }
+              }
+            }
+          }
+        } else {
+          conv_pass<PREC, C, MAXT, 0, MT>(acc, Vin, wall + lay.off_v(blk) * 64,
+                                          pix, msk, nt, W, lane, g);
+        }
+        // epilogue: vertical output + y = act(1x1(DownShift(act(v'))))
+        frag wy[MTH][KS];
+#pragma unroll
+        for (int mh = 0; mh < MTH; ++mh)
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) wy[mh][ks] = wall[(lay.off_y(blk) + mh * KS + ks) * 64 + lane];
+        const float* by = bias + 2 * C + C / 2;
+#pragma unroll
+        for (int u = 0; u < MAXT; ++u) {
+          if (u < nt) {
+            f32x4 av[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+              av[mt] = act4<ACT>(acc[u][mt]);
+              const int q = 4 * mt + g;
+              if (inp[u]) {
+                if (kind == BK_FIRST) L::store_quad(V0, pix[u], q, av[mt]);
+                else if (kind == BK_A) L::store_quad(V1, pix[u], q, av[mt]);
+                else if (kind == BK_B)
+                  L::store_quad(V0, pix[u], q, act4<ACT>(L::load_quad(V0, pix[u], q) + acc[u][mt]));
+              }
+            }
+            frag bf[KS];
+            d_to_b<PREC, MT, KS>(av, bf);
+            const bool down = pi[u] + 1 < H;
+            const int tgt = down ? pix[u] + W : pix[u] - (H - 1) * W;
+#pragma unroll
+            for (int mh = 0; mh < MTH; ++mh) {
+              const f32x4 b0 = load_bias4(by, 16 * mh + 4 * g, C / 2);
+              f32x4 y = b0;
+#pragma unroll
+              for (int ks = 0; ks < KS; ++ks) y = P::mfma(wy[mh][ks], bf[ks], y);
+              y = act4<ACT>(down ? y : b0);
+              if (inp[u] && 16 * mh + 4 * g < C / 2) L::store_quad(CAT, tgt, C / 8 + 4 * mh + g, y);
+            }
+          }
+        }
+      }
+      {
+        // -- horizontal 1xk, then x = act(1x1(act(.))) into CAT (RightShift
+        //    of the final block folded into the write target)
+        f32x4 acc[MAXT][MT];
+#pragma unroll
+        for (int u = 0; u < MAXT; ++u)
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = load_bias4(bias + C, 16 * mt + 4 * g, C);
+        if (kind == BK_FIRST) {
+          const float* kh = a.aux + lay.aux_kh0(nb);
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u) {
+            if (u < nt) {
+#pragma unroll
+              for (int tap = 0; tap < 3; ++tap) {
+                const bool valid = (msk[u] >> Taps<1>::bit(tap)) & 1u;
+                const float s = valid ? sp[pix[u] + Taps<1>::dc(tap)] : 0.0f;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                  const f32x4 k = *reinterpret_cast<const f32x4*>(kh + tap * C + 16 * mt + 4 * g);
+                  acc[u][mt] += k * s;
+                }
+              }
+            }
+          }
+        } else {
+          conv_pass<PREC, C, MAXT, 1, MT>(acc, Hin, wall + lay.off_h(blk) * 64, pix, msk, nt, W, lane, g);
+        }
+        frag wx[MTH][KS];
+#pragma unroll
+        for (int mh = 0; mh < MTH; ++mh)
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) wx[mh][ks] = wall[(lay.off_x(blk) + mh * KS + ks) * 64 + lane];
+        const float* bx = bias + 2 * C;
+#pragma unroll
+        for (int u = 0; u < MAXT; ++u) {
+          if (u < nt) {
+            f32x4 ax[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) ax[mt] = act4<ACT>(acc[u][mt]);
+            frag bf[KS];
+            d_to_b<PREC, MT, KS>(ax, bf);
+            bool keep = true;
+            int tgt = pix[u];
+            if (kind == BK_FINAL) {
+              keep = pj[u] + 1 < W;
+              tgt = keep ? pix[u] + 1 : pix[u] - (W - 1);
+            }
+#pragma unroll
+            for (int mh = 0; mh < MTH; ++mh) {
+              const f32x4 b0 = load_bias4(bx, 16 * mh + 4 * g, C / 2);
+              f32x4 x = b0;
+#pragma unroll
+              for (int ks = 0; ks < KS; ++ks) x = P::mfma(wx[mh][ks], bf[ks], x);
+              x = act4<ACT>(keep ? x : b0);
+              if (inp[u] && 16 * mh + 4 * g < C / 2) L::store_quad(CAT, tgt, 4 * mh + g, x);
+            }
+          }
+        }
+      }
+      __syncthreads();
+
+      // ================= phase H3: kxk on the concatenation ================
+      {
+        f32x4 acc[MAXT][MT];
+#pragma unroll
+        for (int u = 0; u < MAXT; ++u)
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = load_bias4(bias + 3 * C, 16 * mt + 4 * g, C);
+        conv_pass<PREC, C, MAXT, 2, MT>(acc, CAT, wall + lay.off_c(blk) * 64, pix, msk, nt, W, lane, g);
+        if (kind != BK_FINAL) {
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u) {
+            if (u < nt && inp[u]) {
+#pragma unroll
+              for (int mt = 0; mt < MT; ++mt) {
+                const int q = 4 * mt + g;
+                if (kind == BK_FIRST) L::store_quad(H0, pix[u], q, act4<ACT>(acc[u][mt]));
+                else if (kind == BK_A) L::store_quad(H1, pix[u], q, act4<ACT>(acc[u][mt]));
+                else L::store_quad(H0, pix[u], q, act4<ACT>(L::load_quad(H0, pix[u], q) + acc[u][mt]));
+              }
+            }
+          }
+        } else {
+          // head: 1x1 C->4 on act(h'), complex logits, 2-way log-normalise,
+          // gather at the actual spin (abstract_machine.py:34-37,54-57)
+          frag wh[KS];
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) wh[ks] = wall[(lay.head_base(nb) + ks) * 64 + lane];
+          const f32x4 hb = load_bias4(a.aux + lay.aux_head_bias(nb), 4 * g, 4);
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u) {
+            if (u < nt) {
+              f32x4 ah[MT];
+#pragma unroll
+              for (int mt = 0; mt < MT; ++mt) ah[mt] = act4<ACT>(acc[u][mt]);
+              frag bf[KS];
+              d_to_b<PREC, MT, KS>(ah, bf);
+              f32x4 z = hb;
+#pragma unroll
+              for (int ks = 0; ks < KS; ++ks) z = P::mfma(wh[ks], bf[ks], z);
+              if (g == 0 && inp[u]) {
+                // z = (Re z0, Re z1, Im z0, Im z1) for spin values (+1, -1)
+                const float mx = fmaxf(z.x, z.y);
+                const float lse = mx + 0.5f * log1pf(expf(-2.0f * fabsf(z.x - z.y)));
+                const float s = sp[pix[u]];
+                const bool up = s > 0.0f;
+                part[pix[u]] = make_float2((up ? z.x : z.y) - lse, up ? z.z : z.w);
+                if (a.cond) {
+                  const int t = pix[u] / HW;
+                  const int64_t m = grp * T + t;
+                  if (m < M) {
+                    float* o = a.cond + (m * HW + (pix[u] - t * HW)) * 2;
+                    o[0] = 2.0f * (z.x - lse);
+                    o[1] = 2.0f * (z.y - lse);
+                  }
+                }
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- per-configuration site sums (deterministic tree per wave)
+    for (int t = wave; t < T; t += NW) {
+      float sr = 0.0f, si = 0.0f;
+      for (int q = lane; q < HW; q += 64) {
+        const float2 v = part[t * HW + q];
+        sr += v.x;
+        si += v.y;
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off, 64);
+        si += __shfl_xor(si, off, 64);
+      }
+      const int64_t m = grp * T + t;
+      if (lane == 0 && m < M) {
+        a.logpsi[2 * m] = sr;
+        a.logpsi[2 * m + 1] = si;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void pack_gather_kernel(const float* __restrict__ src, const int32_t* __restrict__ map, int64_t n,
+                                   int prec, void* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t k = map[i];
+  const float v = k >= 0 ? src[k] : 0.0f;
+  if (prec == FK_PREC_BF16) reinterpret_cast<__bf16*>(dst)[i] = (__bf16)v;
+  else reinterpret_cast<float*>(dst)[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+struct FwdPlan {
+  int T, npix, maxt;
+  size_t lds;
+};
+
+static size_t fwd_lds_bytes(int npix, int pixb) { return (size_t)npix * pixb * 5 + (size_t)npix * (4 + 8); }
+
+static int plan_forward(int HW, int C, int prec, int64_t m, FwdPlan* plan) {
+  const int pixb = C * (prec == FK_PREC_BF16 ? 2 : 4);
+  const size_t cap = 160 * 1024;
+  const int NW = 8;
+  int best = 0;
+  for (int T = 1; T <= 256; ++T) {
+    const int npix = ((T * HW + 15) / 16) * 16;
+    if (fwd_lds_bytes(npix, pixb) > cap) break;
+    const int per_wave = (npix / 16 + NW - 1) / NW;
+    if (per_wave > 4) break;
+    best = T;
+    if ((int64_t)T >= m) break;
+  }
+  if (best == 0) return FK_ERR_UNSUPPORTED;
+  plan->T = best;
+  plan->npix = ((best * HW + 15) / 16) * 16;
+  plan->lds = fwd_lds_bytes(plan->npix, pixb);
+  const int per_wave = (plan->npix / 16 + NW - 1) / NW;
+  plan->maxt = per_wave <= 2 ? 2 : 4;
+  return FK_OK;
+}
+
+template <int PREC, int C, int MAXT, int ACT>
+static int launch_fwd(const FwdArgs& a, const FwdPlan& pl, hipStream_t s) {
+  auto kern = convar_fwd_kernel<PREC, C, MAXT, ACT>;
+  FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)pl.lds));
+  const int64_t ngroups = (a.m_count + pl.T - 1) / pl.T;
+  const int per_cu = (int)((160 * 1024) / pl.lds) > 0 ? (int)((160 * 1024) / pl.lds) : 1;
+  int64_t grid = (int64_t)device_cu_count() * per_cu;
+  if (grid > ngroups) grid = ngroups;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(512), pl.lds, s, a);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+template <int PREC, int C>
+static int dispatch_fwd_c(const FwdArgs& a, const FwdPlan& pl, int act, hipStream_t s) {
+  if (pl.maxt == 2) {
+    return act == FK_ACT_ELU ? launch_fwd<PREC, C, 2, FK_ACT_ELU>(a, pl, s)
+                             : launch_fwd<PREC, C, 2, FK_ACT_RELU>(a, pl, s);
+  }
+  return act == FK_ACT_ELU ? launch_fwd<PREC, C, 4, FK_ACT_ELU>(a, pl, s)
+                           : launch_fwd<PREC, C, 4, FK_ACT_RELU>(a, pl, s);
+}
+
+bool convar_supported(int channels, int precision) {
+  if (precision == FK_PREC_BF16) return channels == 32 || channels == 64;
+  if (precision == FK_PREC_FP32) return channels == 16 || channels == 32 || channels == 64;
+  return false;
+}
+
+}  // namespace fk
+
+using namespace fk;
+
+extern "C" {
+
+int fk_convar_packed_sizes(int32_t depth, int32_t channels, int32_t precision, int64_t* n_weights, int64_t* n_aux) {
+  FK_REQUIRE(n_weights && n_aux, "fk_convar_packed_sizes: null pointer");
+  FK_REQUIRE(depth >= 1, "fk_convar_packed_sizes: depth must be >= 1");
+  if (!convar_supported(channels, precision)) {
+    set_error("ConvNetAutoregressive2D kernels: channels=%d precision=%d not implemented", channels, precision);
+    return FK_ERR_UNSUPPORTED;
+  }
+  const ConvarLayout lay(channels, precision);
+  const int nb = num_causal_blocks(depth);
+  *n_weights = lay.total_frags(nb) * 64 * lay.E;
+  *n_aux = lay.aux_total(nb);
+  return FK_OK;
+}
+
+int fk_convar_pack_maps(int32_t depth, int32_t channels, int32_t precision, int32_t* wmap, int32_t* amap) {
+  int64_t nw = 0, na = 0;
+  int rc = fk_convar_packed_sizes(depth, channels, precision, &nw, &na);
+  if (rc != FK_OK) return rc;
+  FK_REQUIRE(wmap && amap, "fk_convar_pack_maps: null pointer");
+  const int C = channels;
+  const ConvarLayout lay(C, precision);
+  const int nb = num_causal_blocks(depth);
+  for (int64_t i = 0; i < nw; ++i) wmap[i] = -1;
+  // canonical offsets
+  int64_t src = 0;
+  struct Conv {
+    int64_t koff, boff;
+    int kh, kw, cin, cout;
+  };
+  auto take = [&](int kh, int kw, int cin, int cout) {
+    Conv c{src, src + (int64_t)kh * kw * cin * cout, kh, kw, cin, cout};
+    src = c.boff + cout;
+    return c;
+  };
+  auto fill = [&](int64_t frag_base, const Conv& cv, int mto) {
+    const int ntap = cv.kh * cv.kw;
+    for (int tap = 0; tap < ntap; ++tap)
+      for (int mt = 0; mt < mto; ++mt)
+        for (int ks = 0; ks < lay.KS; ++ks) {
+          const int64_t frag = frag_base + ((int64_t)tap * mto + mt) * lay.KS + ks;
+          for (int lane = 0; lane < 64; ++lane) {
+            const int m = lane % 16, g = lane / 16;
+            for (int e = 0; e < lay.E; ++e) {
+              const int cin = precision == FK_PREC_BF16 ? slot_channel_bf16(32 * ks + 8 * g + e)
+                                                        : slot_channel_f32(ks, g);
+              const int cout = 16 * mt + m;
+              const int64_t idx = (frag * 64 + lane) * lay.E + e;
+              if (cout < cv.cout && cin < cv.cin)
+                wmap[idx] = (int32_t)(cv.koff + ((int64_t)tap * cv.cin + cin) * cv.cout + cout);
+            }
+          }
+        }
+  };
+  auto bias_copy = [&](int64_t dst, const Conv& cv) {
+    for (int c = 0; c < cv.cout; ++c) amap[dst + c] = (int32_t)(cv.boff + c);
+  };
+  for (int b = 0; b < nb; ++b) {
+    const int cin = b == 0 ? 1 : C;
+    Conv v = take(3, 3, cin, C), h = take(1, 3, cin, C), x = take(1, 1, C, C / 2), y = take(1, 1, C, C / 2),
+         cc = take(3, 3, C, C);
+    const int64_t ab = lay.aux_bias(b);
+    bias_copy(ab, v);
+    bias_copy(ab + C, h);
+    bias_copy(ab + 2 * C, x);
+    bias_copy(ab + 2 * C + C / 2, y);
+    bias_copy(ab + 3 * C, cc);
+    if (b == 0) {
+      for (int i = 0; i < 9 * C; ++i) amap[lay.aux_kv0(nb) + i] = (int32_t)(v.koff + i);
+      for (int i = 0; i < 3 * C; ++i) amap[lay.aux_kh0(nb) + i] = (int32_t)(h.koff + i);
+    } else {
+      fill(lay.off_v(b), v, lay.MT);
+      fill(lay.off_h(b), h, lay.MT);
+    }
+    fill(lay.off_x(b), x, lay.MTH);
+    fill(lay.off_y(b), y, lay.MTH);
+    fill(lay.off_c(b), cc, lay.MT);
+  }
+  Conv head = take(1, 1, C, 4);
+  fill(lay.head_base(nb), head, 1);
+  for (int c = 0; c < 4; ++c) amap[lay.aux_head_bias(nb) + c] = (int32_t)(head.boff + c);
+  return FK_OK;
+}
+
+int fk_pack_gather(const float* params, const int32_t* map, int64_t n, int32_t precision, void* dst, void* stream) {
+  FK_REQUIRE(params && map && dst, "fk_pack_gather: null pointer");
+  FK_REQUIRE(precision == FK_PREC_FP32 || precision == FK_PREC_BF16, "fk_pack_gather: bad precision");
+  if (n <= 0) return FK_OK;
+  hipLaunchKernelGGL(pack_gather_kernel, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, params, map, n,
+                     precision, dst);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_convar_forward(const void* wpacked, const float* aux, int32_t precision, int32_t H, int32_t W, int32_t depth,
+                      int32_t channels, int32_t activation, const int8_t* spins, const int32_t* pairs,
+                      const int64_t* m_dev, int64_t m_count, int32_t conn_kind, const int32_t* conn_a,
+                      const int32_t* conn_b, float* logpsi, float* cond_logp, void* stream) {
+  FK_REQUIRE(wpacked && aux && spins && logpsi, "fk_convar_forward: null pointer");
+  FK_REQUIRE(H >= 1 && W >= 1 && depth >= 1, "fk_convar_forward: bad shape");
+  FK_REQUIRE(activation == FK_ACT_RELU || activation == FK_ACT_ELU, "fk_convar_forward: bad activation");
+  FK_REQUIRE(!pairs || (conn_a && conn_b), "fk_convar_forward: pairs need a connection table");
+  if (!convar_supported(channels, precision)) {
+    set_error("ConvNetAutoregressive2D kernels: channels=%d precision=%d not implemented", channels, precision);
+    return FK_ERR_UNSUPPORTED;
+  }
+  if (m_count <= 0) return FK_OK;
+  FwdPlan pl;
+  if (plan_forward(H * W, channels, precision, m_count, &pl) != FK_OK) {
+    set_error("ConvNetAutoregressive2D kernels: lattice %dx%d with %d channels does not fit in LDS", H, W, channels);
+    return FK_ERR_UNSUPPORTED;
+  }
+  FwdArgs a;
+  a.w = wpacked;
+  a.aux = aux;
+  a.H = H;
+  a.W = W;
+  a.HW = H * W;
+  a.nblocks = num_causal_blocks(depth);
+  a.T = pl.T;
+  a.npix = pl.npix;
+  a.spins = spins;
+  a.pairs = pairs;
+  a.m_dev = m_dev;
+  a.m_count = m_count;
+  a.conn_kind = conn_kind;
+  a.conn_a = conn_a;
+  a.conn_b = conn_b;
+  a.logpsi = logpsi;
+  a.cond = cond_logp;
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == FK_PREC_BF16) {
+    if (channels == 32) return dispatch_fwd_c<FK_PREC_BF16, 32>(a, pl, activation, s);
+    return dispatch_fwd_c<FK_PREC_BF16, 64>(a, pl, activation, s);
+  }
+  if (channels == 16) return dispatch_fwd_c<FK_PREC_FP32, 16>(a, pl, activation, s);
+  if (channels == 32) return dispatch_fwd_c<FK_PREC_FP32, 32>(a, pl, activation, s);
+  return dispatch_fwd_c<FK_PREC_FP32, 64>(a, pl, activation, s);
+}
+
+}  // extern "C"

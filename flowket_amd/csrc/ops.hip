@@ -1,0 +1,369 @@
+// Operators (find_conn), the ragged index map and local-energy reductions.
+// Integer/byte and small-FP64 work: HBM/latency bound, no MFMA.
+#include "fk_common.h"
+
+#include <cstdarg>
+#include <string>
+
+namespace fk {
+
+static thread_local char g_err[1024];
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int device_cu_count() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    return 256;
+  return cus;
+}
+
+// One workgroup per sample.  Connection slot c of sample b follows
+// HeisenbergFindConn.calc_conn_and_mel (operators/heisenberg.py:84-121),
+// Ising.find_conn (operators/ising.py:17-46) or the J1J2 bond operator
+// (operators/j1j2.py:37-55 through operators/netket_operator.py:46-66).
+__global__ __launch_bounds__(256) void find_conn_kernel(
+    const int8_t* __restrict__ spins, int64_t batch, int n_sites, int kind,
+    const int32_t* __restrict__ conn_a, const int32_t* __restrict__ conn_b,
+    const double* __restrict__ conn_mel, int n_conn,
+    const int32_t* __restrict__ diag_a, const int32_t* __restrict__ diag_b,
+    const double* __restrict__ diag_j, int n_diag,
+    int8_t* __restrict__ conn_out, double* __restrict__ mel_out,
+    uint8_t* __restrict__ use_out, int32_t* __restrict__ n_used_out) {
+  extern __shared__ int8_t s_spin[];  // n_sites spins + per-connection use flags
+  int8_t* s_use = s_spin + ((n_sites + 15) & ~15);
+  __shared__ int s_count[4];
+  const int64_t b = blockIdx.x;
+  const int tid = threadIdx.x;
+  for (int q = tid; q < n_sites; q += blockDim.x) s_spin[q] = spins[b * n_sites + q];
+  __syncthreads();
+
+  // Diagonal element: sequential double accumulation in table order on one
+  // thread (exact for the integer/dyadic couplings of the reference models).
+  if (tid == 0) {
+    double d = 0.0;
+    for (int k = 0; k < n_diag; ++k)
+      d += diag_j[k] * (double)s_spin[diag_a[k]] * (double)s_spin[diag_b[k]];
+    mel_out[b] = d;
+    use_out[b] = 1;
+    s_use[0] = 1;
+  }
+  int local = (tid == 0) ? 1 : 0;
+  for (int c = 1 + tid; c < n_conn; c += blockDim.x) {
+    const int a = conn_a[c];
+    int u = 0;
+    double m = 0.0;
+    if (kind == FK_CONN_FLIP) {
+      u = (a >= 0);
+      m = u ? conn_mel[c] : 0.0;
+    } else if (a >= 0) {
+      const int bb = conn_b[c];
+      const int anti = s_spin[a] != s_spin[bb];
+      m = anti ? conn_mel[c] : 0.0;
+      u = anti && (m != 0.0);
+    }
+    mel_out[(int64_t)c * batch + b] = m;
+    use_out[(int64_t)c * batch + b] = (uint8_t)u;
+    s_use[c] = (int8_t)u;
+    local += u;
+  }
+  // used count (integers: any order is exact)
+  for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
+  if ((tid & 63) == 0) s_count[tid >> 6] = local;
+  __syncthreads();
+  if (tid == 0 && n_used_out) {
+    int t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_count[w];
+    n_used_out[b] = t;
+  }
+  if (!conn_out) return;
+  // Dense connected configurations, slot-major (n_conn, batch, n_sites).
+  const int64_t total = (int64_t)n_conn * n_sites;
+  for (int64_t idx = tid; idx < total; idx += blockDim.x) {
+    const int c = (int)(idx / n_sites);
+    const int q = (int)(idx - (int64_t)c * n_sites);
+    int8_t v = s_spin[q];
+    if (c > 0) {
+      const int a = conn_a[c];
+      if (kind == FK_CONN_FLIP) {
+        if (q == a) v = (int8_t)-v;
+      } else if (a >= 0) {
+        const int bb = conn_b[c];
+        if (kind == FK_CONN_SWAP_ZERO_PAD && !s_use[c]) {
+          v = 0;
+        } else {
+          if (q == a) v = s_spin[bb];
+          else if (q == bb) v = s_spin[a];
+        }
+      } else if (kind == FK_CONN_SWAP_ZERO_PAD) {
+        v = 0;
+      }
+    }
+    conn_out[((int64_t)c * batch + b) * n_sites + q] = v;
+  }
+}
+
+// counts[b] -> offsets[b+1]; one thread per sample reads use[c][b] (coalesced
+// over b for each c).
+__global__ void count_used_kernel(const uint8_t* __restrict__ use, int64_t batch, int n_conn,
+                                  int64_t* __restrict__ offsets) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  int64_t t = 0;
+  for (int c = 0; c < n_conn; ++c) t += use[(int64_t)c * batch + b] ? 1 : 0;
+  offsets[b + 1] = t;
+  if (b == 0) offsets[0] = 0;
+}
+
+// In-place inclusive scan of offsets[1..batch] by a single workgroup.
+__global__ __launch_bounds__(1024) void scan_offsets_kernel(int64_t* __restrict__ offsets, int64_t batch) {
+  __shared__ int64_t s_wave[16];
+  __shared__ int64_t s_carry;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < batch; base += blockDim.x) {
+    const int64_t i = base + tid;
+    int64_t v = (i < batch) ? offsets[i + 1] : 0;
+    for (int off = 1; off < 64; off <<= 1) {
+      int64_t o = __shfl_up(v, off, 64);
+      if (lane >= off) v += o;
+    }
+    if (lane == 63) s_wave[wave] = v;
+    __syncthreads();
+    if (tid == 0) {
+      int64_t run = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        int64_t t = s_wave[w];
+        s_wave[w] = run;
+        run += t;
+      }
+    }
+    __syncthreads();
+    v += s_wave[wave] + s_carry;
+    if (i < batch) offsets[i + 1] = v;
+    __syncthreads();
+    if (tid == (int)blockDim.x - 1) s_carry = v;
+    __syncthreads();
+  }
+}
+
+__global__ void write_pairs_kernel(const uint8_t* __restrict__ use, int64_t batch, int n_conn,
+                                   const int64_t* __restrict__ offsets, int32_t* __restrict__ pairs) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  int64_t k = offsets[b];
+  for (int c = 0; c < n_conn; ++c) {
+    if (use[(int64_t)c * batch + b]) {
+      pairs[2 * k] = (int32_t)b;
+      pairs[2 * k + 1] = c;
+      ++k;
+    }
+  }
+}
+
+__device__ __forceinline__ void cexp_diff(float dr, float di, double& er, double& ei) {
+  // exp of a complex64 difference, evaluated in float like the reference's
+  // numpy complex64 arithmetic (observables/monte_carlo/operator.py:31).
+  float m = expf(dr);
+  float s, c;
+  sincosf(di, &s, &c);
+  er = (double)(m * c);
+  ei = (double)(m * s);
+}
+
+__global__ void local_energy_kernel(const int64_t* __restrict__ offsets, const int32_t* __restrict__ pairs,
+                                    const float* __restrict__ logpsi, const double* __restrict__ mel,
+                                    int64_t batch, double* __restrict__ eloc) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  const int64_t k0 = offsets[b], k1 = offsets[b + 1];
+  double accr = 0.0, acci = 0.0;
+  if (k1 > k0) {
+    const float r0 = logpsi[2 * k0], i0 = logpsi[2 * k0 + 1];
+    for (int64_t k = k0; k < k1; ++k) {
+      const int c = pairs[2 * k + 1];
+      double er, ei;
+      cexp_diff(logpsi[2 * k] - r0, logpsi[2 * k + 1] - i0, er, ei);
+      const double m = mel[(int64_t)c * batch + b];
+      accr += m * er;
+      acci += m * ei;
+    }
+  }
+  eloc[2 * b] = accr;
+  eloc[2 * b + 1] = acci;
+}
+
+__global__ void local_energy_dense_kernel(const float* __restrict__ logpsi, const double* __restrict__ mel,
+                                          int n_conn, int64_t batch, double* __restrict__ eloc) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  const float r0 = logpsi[2 * b], i0 = logpsi[2 * b + 1];
+  double accr = 0.0, acci = 0.0;
+  for (int c = 0; c < n_conn; ++c) {
+    const int64_t k = (int64_t)c * batch + b;
+    double er, ei;
+    cexp_diff(logpsi[2 * k] - r0, logpsi[2 * k + 1] - i0, er, ei);
+    accr += mel[k] * er;
+    acci += mel[k] * ei;
+  }
+  eloc[2 * b] = accr;
+  eloc[2 * b + 1] = acci;
+}
+
+// Two-pass FP64 mean / population variance of Re (numpy.mean / numpy.var in
+// observables/monte_carlo/observable.py:12-13), fixed reduction order.
+__global__ __launch_bounds__(1024) void energy_moments_kernel(const double* __restrict__ eloc, int64_t batch,
+                                                              double* __restrict__ out) {
+  __shared__ double s_r[1024], s_i[1024];
+  const int tid = threadIdx.x;
+  double r = 0.0, im = 0.0;
+  for (int64_t b = tid; b < batch; b += blockDim.x) {
+    r += eloc[2 * b];
+    im += eloc[2 * b + 1];
+  }
+  s_r[tid] = r;
+  s_i[tid] = im;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      s_r[tid] += s_r[tid + s];
+      s_i[tid] += s_i[tid + s];
+    }
+    __syncthreads();
+  }
+  const double mr = s_r[0] / (double)batch, mi = s_i[0] / (double)batch;
+  __syncthreads();
+  double v = 0.0;
+  for (int64_t b = tid; b < batch; b += blockDim.x) {
+    const double d = eloc[2 * b] - mr;
+    v += d * d;
+  }
+  s_r[tid] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) s_r[tid] += s_r[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[0] = mr;
+    out[1] = mi;
+    out[2] = s_r[0] / (double)batch;
+    out[3] = (double)batch;
+  }
+}
+
+__global__ void ar_draw_site_kernel(const float* __restrict__ cond_logp, const double* __restrict__ u,
+                                    int64_t batch, int n_sites, int site, int8_t* __restrict__ spins) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  const float lp = cond_logp[(b * n_sites + site) * 2];
+  // exp of the float32 log-probability compared with the float64 uniform,
+  // strict '>' (deepar/samplers/autoregressive.py:42-44).
+  const double p = (double)expf(lp);
+  spins[b * n_sites + site] = (p > u[b * n_sites + site]) ? (int8_t)1 : (int8_t)-1;
+}
+
+}  // namespace fk
+
+using namespace fk;
+
+extern "C" {
+
+int fk_version(void) { return 1; }
+
+const char* fk_last_error(void) { return g_err; }
+
+int fk_find_conn(const int8_t* spins, int64_t batch, int32_t n_sites, int32_t kind,
+                 const int32_t* conn_a, const int32_t* conn_b, const double* conn_mel, int32_t n_conn,
+                 const int32_t* diag_a, const int32_t* diag_b, const double* diag_j, int32_t n_diag,
+                 int8_t* conn_out, double* mel_out, uint8_t* use_out, int32_t* n_used_out, void* stream) {
+  FK_REQUIRE(spins && conn_a && conn_b && conn_mel && mel_out && use_out, "fk_find_conn: null pointer");
+  FK_REQUIRE(batch >= 0 && n_sites > 0 && n_sites <= 65536 && n_conn >= 1, "fk_find_conn: bad sizes");
+  FK_REQUIRE(kind == FK_CONN_SWAP || kind == FK_CONN_SWAP_ZERO_PAD || kind == FK_CONN_FLIP,
+             "fk_find_conn: bad kind %d", kind);
+  FK_REQUIRE(n_diag == 0 || (diag_a && diag_b && diag_j), "fk_find_conn: null diagonal table");
+  if (batch == 0) return FK_OK;
+  FK_REQUIRE(batch <= 0x7fffffff, "fk_find_conn: batch too large");
+  const size_t lds = ((n_sites + 15) & ~15) + ((n_conn + 15) & ~15);
+  FK_REQUIRE(lds <= 65536, "fk_find_conn: table too large for LDS");
+  hipLaunchKernelGGL(find_conn_kernel, dim3((unsigned)batch), dim3(256), lds, (hipStream_t)stream, spins, batch,
+                     n_sites, kind, conn_a, conn_b, conn_mel, n_conn, diag_a, diag_b, diag_j, n_diag, conn_out,
+                     mel_out, use_out, n_used_out);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int64_t fk_compact_scratch_bytes(int64_t batch) {
+  (void)batch;
+  return 0;
+}
+
+int fk_compact_connections(const uint8_t* use, int64_t batch, int32_t n_conn, int64_t* offsets, int32_t* pairs,
+                           void* scratch, void* stream) {
+  (void)scratch;
+  FK_REQUIRE(use && offsets && pairs, "fk_compact_connections: null pointer");
+  FK_REQUIRE(batch >= 0 && n_conn >= 1, "fk_compact_connections: bad sizes");
+  hipStream_t s = (hipStream_t)stream;
+  if (batch == 0) {
+    FK_CHECK_HIP(hipMemsetAsync(offsets, 0, sizeof(int64_t), s));
+    return FK_OK;
+  }
+  const int tb = 256;
+  hipLaunchKernelGGL(count_used_kernel, dim3(div_up(batch, tb)), dim3(tb), 0, s, use, batch, n_conn, offsets);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(scan_offsets_kernel, dim3(1), dim3(1024), 0, s, offsets, batch);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(write_pairs_kernel, dim3(div_up(batch, tb)), dim3(tb), 0, s, use, batch, n_conn, offsets,
+                     pairs);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_local_energy(const int64_t* offsets, const int32_t* pairs, const float* logpsi, const double* mel,
+                    int64_t batch, double* eloc_out, void* stream) {
+  FK_REQUIRE(offsets && pairs && logpsi && mel && eloc_out, "fk_local_energy: null pointer");
+  if (batch <= 0) return FK_OK;
+  hipLaunchKernelGGL(local_energy_kernel, dim3(div_up(batch, 256)), dim3(256), 0, (hipStream_t)stream, offsets,
+                     pairs, logpsi, mel, batch, eloc_out);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_local_energy_dense(const float* logpsi, const double* mel, int32_t n_conn, int64_t batch, double* eloc_out,
+                          void* stream) {
+  FK_REQUIRE(logpsi && mel && eloc_out, "fk_local_energy_dense: null pointer");
+  if (batch <= 0) return FK_OK;
+  hipLaunchKernelGGL(local_energy_dense_kernel, dim3(div_up(batch, 256)), dim3(256), 0, (hipStream_t)stream,
+                     logpsi, mel, n_conn, batch, eloc_out);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_energy_moments(const double* eloc, int64_t batch, double* out, void* stream) {
+  FK_REQUIRE(eloc && out, "fk_energy_moments: null pointer");
+  FK_REQUIRE(batch > 0, "fk_energy_moments: empty batch");
+  hipLaunchKernelGGL(energy_moments_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, eloc, batch, out);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_ar_draw_site(const float* cond_logp, const double* uniforms, int64_t batch, int32_t n_sites, int32_t site,
+                    int8_t* spins, void* stream) {
+  FK_REQUIRE(cond_logp && uniforms && spins, "fk_ar_draw_site: null pointer");
+  FK_REQUIRE(site >= 0 && site < n_sites, "fk_ar_draw_site: bad site");
+  if (batch <= 0) return FK_OK;
+  hipLaunchKernelGGL(ar_draw_site_kernel, dim3(div_up(batch, 256)), dim3(256), 0, (hipStream_t)stream, cond_logp,
+                     uniforms, batch, n_sites, site, spins);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,248 @@
+"""Wave functions evaluated by the HIP kernels.
+
+``ConvNetAutoregressive2D`` mirrors machines/conv_net_autoregressive_2D.py
+(with AutoNormalizedAutoregressiveMachine, machines/abstract_machine.py:49-71):
+same constructor arguments, same architecture, same outputs -- log psi (B, 1)
+complex64 and conditional log-probabilities (B, H, W, 2).  Parameters are one
+flat float32 device tensor in the canonical (Keras) order; weight
+normalisation (deepar/layers/wrappers.py:123-135) is a per-step transform of
+that vector.  Every evaluation runs in the fused forward kernel
+(fk_convar_forward).
+"""
+import numpy as np
+import torch
+
+from . import _lib
+
+_ACT = {'relu': _lib.FK_ACT_RELU, 'elu': _lib.FK_ACT_ELU}
+_PREC = {'fp32': _lib.FK_PREC_FP32, 'float32': _lib.FK_PREC_FP32, 'bf16': _lib.FK_PREC_BF16,
+         'bfloat16': _lib.FK_PREC_BF16}
+
+
+def num_causal_blocks(depth):
+    """Block 0, (depth-2) residual blocks of two causal blocks, and the masked
+    final block (conv_net_autoregressive_2D.py:68-72)."""
+    return 2 + 2 * max(depth - 2, 0)
+
+
+def layer_shapes(depth, channels, kernel_size=3):
+    """(kernel shape, bias shape) of every convolution in network order:
+    per causal block vconv, hconv, x1x1, y1x1, h3conv
+    (conv_net_autoregressive_2D.py:38-50), then the 1x1 C->4 head (:73)."""
+    k, C = kernel_size, channels
+    shapes = []
+    for b in range(num_causal_blocks(depth)):
+        cin = 1 if b == 0 else C
+        shapes += [((k, k, cin, C), (C,)), ((1, k, cin, C), (C,)), ((1, 1, C, C // 2), (C // 2,)),
+                   ((1, 1, C, C // 2), (C // 2,)), ((k, k, C, C), (C,))]
+    shapes.append(((1, 1, C, 4), (4,)))
+    return shapes
+
+
+class _ConditionalLogProbsModel:
+    """Stand-in for ``Model(inputs, machine.conditional_log_probs)``."""
+
+    def __init__(self, machine):
+        self.machine = machine
+        self.input_shape = (None,) + machine.input_size
+
+    def predict(self, x, batch_size=None):
+        return self.machine.conditional_log_probs(x)
+
+
+class ConvNetAutoregressive2D:
+    """machines/conv_net_autoregressive_2D.py:8-78.
+
+    ``keras_input_layer`` is the lattice shape (H, W) (or any object with a
+    ``shape`` attribute ending in (H, W)).  ``precision`` selects the kernel
+    arithmetic: 'fp32' (FP32 MFMA) or 'bf16' (BF16 MFMA, FP32 accumulate)."""
+
+    def __init__(self, keras_input_layer, depth, num_of_channels, kernel_size=3, strides=1, activation='relu',
+                 weights_normalization=True, exponential_norm=True, precision='fp32', seed=0, device=None,
+                 params=None):
+        shape = getattr(keras_input_layer, 'shape', keras_input_layer)
+        shape = tuple(int(s) for s in shape if s is not None)[-2:]
+        if len(shape) != 2:
+            raise ValueError('ConvNetAutoregressive2D needs a 2-D input shape')
+        if kernel_size != 3 or strides != 1:
+            raise NotImplementedError('the HIP kernels implement kernel_size=3, strides=1')
+        if activation not in _ACT:
+            raise NotImplementedError(f'activation {activation!r} (supported: relu, elu)')
+        self.input_size = shape
+        self.H, self.W = shape
+        self.depth = int(depth)
+        self.num_of_channels = int(num_of_channels)
+        self.kernel_size = kernel_size
+        self.activation = activation
+        self.weights_normalization = bool(weights_normalization)
+        self.exponential_norm = bool(exponential_norm)
+        self.precision = precision
+        self.input_shape = (None,) + shape
+        if device is None:
+            device = torch.device('cuda', torch.cuda.current_device()) if torch.cuda.is_available() else 'cpu'
+        self.device = torch.device(device)
+        self.shapes = layer_shapes(self.depth, self.num_of_channels, kernel_size)
+        self.n_params = sum(int(np.prod(k)) + int(np.prod(b)) for k, b in self.shapes)
+        if params is None:
+            params = self.glorot_init(seed)
+        self.set_params(params)
+        self._packs = {}
+        self._maps = {}
+        self.conditional_log_probs_model = _ConditionalLogProbsModel(self)
+
+    # ------------------------------------------------------------------ params
+    def glorot_init(self, seed):
+        """Glorot-uniform kernels and zero biases (Keras Conv2D defaults) drawn
+        from a seeded torch generator; with weight normalisation the g of each
+        layer starts at log|v| per output channel (wrappers.py:26-39,104-111)."""
+        gen = torch.Generator().manual_seed(int(seed))
+        out = []
+        for kshape, bshape in self.shapes:
+            fan_in = kshape[0] * kshape[1] * kshape[2]
+            fan_out = kshape[0] * kshape[1] * kshape[3]
+            lim = float(np.sqrt(6.0 / (fan_in + fan_out)))
+            out.append((torch.rand(kshape, generator=gen, dtype=torch.float64) * 2 - 1) * lim)
+            out.append(torch.zeros(bshape, dtype=torch.float64))
+        flat = torch.cat([t.reshape(-1) for t in out]).to(torch.float32)
+        if self.weights_normalization:
+            self.wn_g = self._wn_initial_g(flat)
+        return flat
+
+    def _wn_layer_views(self, flat):
+        views, off = [], 0
+        for idx, (kshape, bshape) in enumerate(self.shapes):
+            nk = int(np.prod(kshape))
+            views.append((idx, off, kshape))
+            off += nk + int(np.prod(bshape))
+        return views
+
+    def _wn_initial_g(self, flat):
+        gs = []
+        for idx, off, kshape in self._wn_layer_views(flat):
+            if idx == len(self.shapes) - 1:
+                continue  # the head is never weight-normalised (:73)
+            v = flat[off:off + int(np.prod(kshape))].reshape(kshape).to(torch.float64)
+            norm = torch.sqrt((v * v).sum(dim=(0, 1, 2)))
+            gs.append(torch.log(norm + 1e-10) if self.exponential_norm else norm)
+        return torch.cat(gs).to(torch.float32)
+
+    def set_params(self, params):
+        p = torch.as_tensor(params, dtype=torch.float32).reshape(-1)
+        if p.numel() != self.n_params:
+            raise ValueError(f'expected {self.n_params} parameters, got {p.numel()}')
+        self.params = p.to(self.device).contiguous()
+        if self.weights_normalization and not hasattr(self, 'wn_g'):
+            self.wn_g = self._wn_initial_g(self.params.cpu())
+        if self.weights_normalization:
+            self.wn_g = torch.as_tensor(self.wn_g, dtype=torch.float32).to(self.device)
+        self._version = getattr(self, '_version', 0) + 1
+
+    def mark_params_updated(self):
+        self._version += 1
+
+    def effective_params(self):
+        """The flat kernel/bias vector the network actually uses: the raw
+        parameters, or with weight normalisation kernel = l2_normalize(v)
+        * exp(g) per output channel (wrappers.py:123-135)."""
+        if not self.weights_normalization:
+            return self.params
+        parts, gi = [], 0
+        p = self.params
+        for idx, off, kshape in self._wn_layer_views(p):
+            nk = int(np.prod(kshape))
+            nb = self.shapes[idx][1][0]
+            v = p[off:off + nk].reshape(kshape)
+            if idx < len(self.shapes) - 1:
+                cout = kshape[3]
+                g = self.wn_g[gi:gi + cout]
+                gi += cout
+                scale = torch.exp(g) if self.exponential_norm else g
+                norm = torch.sqrt((v * v).sum(dim=(0, 1, 2)))
+                v = v / torch.clamp(norm, min=1e-12) * scale
+            parts += [v.reshape(-1), p[off + nk:off + nk + nb]]
+        return torch.cat(parts)
+
+    def _pack(self, precision):
+        prec = _PREC[precision]
+        key = (prec, self._version)
+        if key in self._packs:
+            return self._packs[key]
+        _lib.require_gpu()
+        if prec not in self._maps:
+            nw, na = _lib.ctypes.c_int64(), _lib.ctypes.c_int64()
+            _lib.call('fk_convar_packed_sizes', self.depth, self.num_of_channels, prec, _lib.ctypes.byref(nw),
+                      _lib.ctypes.byref(na))
+            wmap = np.empty(nw.value, np.int32)
+            amap = np.empty(na.value, np.int32)
+            _lib.call('fk_convar_pack_maps', self.depth, self.num_of_channels, prec,
+                      wmap.ctypes.data_as(_lib.ctypes.c_void_p), amap.ctypes.data_as(_lib.ctypes.c_void_p))
+            self._maps[prec] = (torch.from_numpy(wmap).to(self.device), torch.from_numpy(amap).to(self.device))
+        wmap, amap = self._maps[prec]
+        eff = self.effective_params().contiguous()
+        wdtype = torch.bfloat16 if prec == _lib.FK_PREC_BF16 else torch.float32
+        wbuf = torch.empty(wmap.numel(), dtype=wdtype, device=self.device)
+        abuf = torch.empty(amap.numel(), dtype=torch.float32, device=self.device)
+        s = _lib.stream_ptr()
+        _lib.call('fk_pack_gather', _lib.ptr(eff), _lib.ptr(wmap), wmap.numel(), prec, _lib.ptr(wbuf), s)
+        _lib.call('fk_pack_gather', _lib.ptr(eff), _lib.ptr(amap), amap.numel(), _lib.FK_PREC_FP32,
+                  _lib.ptr(abuf), s)
+        self._packs = {key: (wbuf, abuf)}
+        return wbuf, abuf
+
+    # ------------------------------------------------------------------ eval
+    def _spins(self, x):
+        if not isinstance(x, torch.Tensor):
+            x = torch.as_tensor(np.asarray(x))
+        x = x.to(self.device)
+        if x.dtype != torch.int8:
+            x = x.to(torch.int8)
+        return x.reshape(x.shape[0], self.H * self.W).contiguous()
+
+    def forward(self, spins, pairs=None, m=None, m_dev=None, operator=None, want_cond=False, precision=None,
+                stream=None):
+        """Raw fused launch.  spins (B, H*W) int8; with ``pairs`` (m, 2)
+        evaluates connected configurations of ``operator``.  Returns
+        (logpsi (m, 2) float32, cond (m, H, W, 2) float32 or None)."""
+        precision = precision or self.precision
+        prec = _PREC[precision]
+        wbuf, abuf = self._pack(precision)
+        if pairs is None:
+            m = spins.shape[0]
+        elif m is None:
+            m = pairs.shape[0]
+        out = torch.empty((m, 2), dtype=torch.float32, device=self.device)
+        cond = torch.empty((m, self.H, self.W, 2), dtype=torch.float32, device=self.device) if want_cond else None
+        kind, ca, cb = 0, None, None
+        if pairs is not None:
+            tb = operator.tables(self.device)
+            kind, ca, cb = operator.kind, tb['conn_a'], tb['conn_b']
+        _lib.call('fk_convar_forward', _lib.ptr(wbuf), _lib.ptr(abuf), prec, self.H, self.W, self.depth,
+                  self.num_of_channels, _ACT[self.activation], _lib.ptr(spins), _lib.ptr(pairs), _lib.ptr(m_dev),
+                  int(m), kind, _lib.ptr(ca), _lib.ptr(cb), _lib.ptr(out), _lib.ptr(cond), _lib.stream_ptr(stream))
+        return out, cond
+
+    def log_psi(self, x, precision=None):
+        """(B,) complex64 log psi."""
+        out, _ = self.forward(self._spins(x), precision=precision)
+        return torch.complex(out[:, 0], out[:, 1])
+
+    def predict(self, x, batch_size=None):
+        """Keras ``Model(inputs, machine.predictions).predict``: (B, 1) complex64."""
+        return self.log_psi(x)[:, None]
+
+    __call__ = predict
+
+    def conditional_log_probs(self, x, precision=None):
+        """(B, H, W, 2) float32; index 0 is spin +1 (deepar/layers/one_hot.py:7-9)."""
+        _, cond = self.forward(self._spins(x), want_cond=True, precision=precision)
+        return cond
+
+    def log_psi_connected(self, spins, pairs, m, operator, m_dev=None, precision=None, stream=None):
+        """log psi of the connected configurations (sample, connection) in
+        ``pairs`` (first m rows, or *m_dev rows when given)."""
+        out, _ = self.forward(spins, pairs=pairs, m=m, m_dev=m_dev, operator=operator, precision=precision,
+                              stream=stream)
+        return out
+
+    def count_params(self):
+        return self.n_params

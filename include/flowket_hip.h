@@ -1,0 +1,154 @@
+/*
+ * flowket_hip.h -- C ABI of the MI355X (gfx950) VMC hot-path library
+ * (libflowket_hip.so).
+ *
+ * Every entry point takes plain device pointers, sizes and a HIP stream passed
+ * as `void*` (a hipStream_t; NULL = the null stream).  Work is enqueued on
+ * that stream; no entry point synchronises unless stated.  Return value:
+ * FK_OK (0) or an error code, with a message from fk_last_error().
+ * Buffers are owned by the caller.  Calls that share a stream are ordered;
+ * the library keeps no per-call state.
+ *
+ * The reference (FlowKet, Python/TensorFlow) has no C ABI: each entry point
+ * below replaces the reference function cited next to it (paths relative to
+ * FlowKet's src/flowket/), and the Python package flowket_amd binds them with
+ * ctypes behind FlowKet's own class names.
+ */
+#ifndef FLOWKET_HIP_H
+#define FLOWKET_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FK_OK 0
+#define FK_ERR_ARG 1
+#define FK_ERR_HIP 2
+#define FK_ERR_UNSUPPORTED 3
+
+/* Connection kinds of fk_find_conn. */
+#define FK_CONN_SWAP 0          /* exchange spins a,b (Heisenberg)              */
+#define FK_CONN_SWAP_ZERO_PAD 1 /* exchange a,b; unused slots all-zero (J1J2)   */
+#define FK_CONN_FLIP 2          /* flip spin a (transverse-field Ising)         */
+
+/* Activations of ConvNetAutoregressive2D. */
+#define FK_ACT_RELU 0
+#define FK_ACT_ELU 1
+
+/* Arithmetic of the fused network kernels. */
+#define FK_PREC_FP32 0
+#define FK_PREC_BF16 1
+
+int fk_version(void);
+const char* fk_last_error(void);
+
+/* ---------------------------------------------------------------------------
+ * Operators: connected configurations and matrix elements.
+ * Replaces HeisenbergFindConn.find_conn (operators/heisenberg.py:72-121),
+ * Ising.find_conn (operators/ising.py:17-46) and the J1J2 netket wrapper
+ * (operators/j1j2.py:7-69, operators/netket_operator.py:46-66).
+ *
+ * The operator is described by a connection table built once on the host:
+ *   conn_a[c], conn_b[c]  (c = 0..n_conn-1; entry 0 is the diagonal, ignored)
+ *       sites of connection c, -1 for "no bond" (use=0, mel=0)
+ *   conn_mel[c]           off-diagonal element of connection c when it applies
+ *   diag_a/diag_b/diag_j  the diagonal is sum_k diag_j[k]*s[diag_a[k]]*s[diag_b[k]]
+ *                         accumulated in double in k order
+ * spins: (batch, n_sites) int8 +-1.
+ * Outputs (any may be NULL except mel/use):
+ *   conn_out (n_conn, batch, n_sites) int8, mel_out (n_conn, batch) double,
+ *   use_out (n_conn, batch) uint8, n_used_out (batch) int32.
+ * ------------------------------------------------------------------------- */
+int fk_find_conn(const int8_t* spins, int64_t batch, int32_t n_sites,
+                 int32_t kind, const int32_t* conn_a, const int32_t* conn_b,
+                 const double* conn_mel, int32_t n_conn,
+                 const int32_t* diag_a, const int32_t* diag_b,
+                 const double* diag_j, int32_t n_diag,
+                 int8_t* conn_out, double* mel_out, uint8_t* use_out,
+                 int32_t* n_used_out, void* stream);
+
+/* Ragged index map of the unbalanced local-energy path
+ * (observables/monte_carlo/operator.py:6-10,26-30): from use (n_conn, batch)
+ * writes offsets (batch+1, exclusive prefix sum of used counts, offsets[batch]
+ * = total) and pairs (total, 2) = (sample, connection) in sample-major,
+ * connection-minor order.  `pairs` must hold n_conn*batch entries.
+ * `scratch` must hold fk_compact_scratch_bytes(batch) bytes. */
+int64_t fk_compact_scratch_bytes(int64_t batch);
+int fk_compact_connections(const uint8_t* use, int64_t batch, int32_t n_conn,
+                           int64_t* offsets, int32_t* pairs, void* scratch,
+                           void* stream);
+
+/* E_loc (observables/monte_carlo/operator.py:13-33): for sample b,
+ *   sum_k mel[c_k, b] * exp(logpsi[k] - logpsi[offsets[b]])
+ * over k in [offsets[b], offsets[b+1]), difference and exp in complex64 as the
+ * reference, product and sum in complex128.  logpsi (total, 2) float (re, im);
+ * eloc_out (batch, 2) double. */
+int fk_local_energy(const int64_t* offsets, const int32_t* pairs,
+                    const float* logpsi, const double* mel, int64_t batch,
+                    double* eloc_out, void* stream);
+
+/* Balanced path (operator.py:35-42): logpsi (n_conn, batch, 2) for every
+ * slot, eloc_out (batch, 2). */
+int fk_local_energy_dense(const float* logpsi, const double* mel,
+                          int32_t n_conn, int64_t batch, double* eloc_out,
+                          void* stream);
+
+/* BaseObservable.estimate (observables/monte_carlo/observable.py:10-14):
+ * out[0..3] = (Re mean, Im mean, population variance of Re, batch) in double,
+ * reduced on the device in two passes. */
+int fk_energy_moments(const double* eloc, int64_t batch, double* out,
+                      void* stream);
+
+/* ---------------------------------------------------------------------------
+ * ConvNetAutoregressive2D (machines/conv_net_autoregressive_2D.py:8-78 with
+ * AutoNormalizedAutoregressiveMachine, machines/abstract_machine.py:49-71).
+ *
+ * Parameters live in a flat float32 vector in network order (per causal
+ * block: vconv k,b; hconv k,b; x1x1 k,b; y1x1 k,b; h3conv k,b; then head k,b;
+ * Keras kernel layout (kh, kw, cin, cout)).  The kernels read them from two
+ * packed device buffers (MFMA fragment order + an fp32 side buffer) produced
+ * by fk_pack_gather with the maps from fk_convar_pack_maps.
+ * ------------------------------------------------------------------------- */
+
+/* Host-only.  Sizes (elements) of the packed buffers; returns FK_OK or
+ * FK_ERR_UNSUPPORTED for shapes the kernels do not implement. */
+int fk_convar_packed_sizes(int32_t depth, int32_t channels, int32_t precision,
+                           int64_t* n_weights, int64_t* n_aux);
+/* Host-only.  Fills wmap (n_weights) and amap (n_aux) with source indices
+ * into the flat parameter vector (-1 = zero). */
+int fk_convar_pack_maps(int32_t depth, int32_t channels, int32_t precision,
+                        int32_t* wmap, int32_t* amap);
+/* Gathers params[map[i]] (0 where map[i] < 0) into dst as float32
+ * (precision FP32) or bfloat16 round-to-nearest-even (BF16). */
+int fk_pack_gather(const float* params, const int32_t* map, int64_t n,
+                   int32_t precision, void* dst, void* stream);
+
+/* Fused forward of the whole network.  Configurations are either the rows of
+ * `spins` (pairs == NULL: m_count rows) or connected configurations
+ * (pairs (m, 2) = (sample, connection) applied to spins with the connection
+ * table conn_a/conn_b of `conn_kind`).  If m_dev != NULL the count is read
+ * from device memory (*m_dev, int64), m_count is then an upper bound used to
+ * size the launch.  Outputs: logpsi (m, 2) float (re, im); cond_logp
+ * (m, H, W, 2) float = 2*Re of the normalised conditionals, or NULL. */
+int fk_convar_forward(const void* wpacked, const float* aux, int32_t precision,
+                      int32_t H, int32_t W, int32_t depth, int32_t channels,
+                      int32_t activation, const int8_t* spins,
+                      const int32_t* pairs, const int64_t* m_dev,
+                      int64_t m_count, int32_t conn_kind,
+                      const int32_t* conn_a, const int32_t* conn_b,
+                      float* logpsi, float* cond_logp, void* stream);
+
+/* Plain autoregressive draw at one site (deepar/samplers/autoregressive.py:
+ * 42-44): spins[b, site] = +1 if exp(cond_logp[b, site, 0]) > u[b, site]
+ * else -1, for all b; cond_logp (batch, n_sites, 2) float, u (batch, n_sites)
+ * double, spins (batch, n_sites) int8. */
+int fk_ar_draw_site(const float* cond_logp, const double* uniforms,
+                    int64_t batch, int32_t n_sites, int32_t site,
+                    int8_t* spins, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLOWKET_HIP_H */

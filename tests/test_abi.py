@@ -1,0 +1,79 @@
+"""The C-ABI library builds, loads, and exports every declared symbol; host-only
+entry points (packing maps) behave.  No GPU compute here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'flowket_hip.h')
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(fk_[a-z0-9_]+)\s*\(', text)))
+
+
+@pytest.fixture(scope='module')
+def lib():
+    from flowket_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        from flowket_amd import build
+        build.build()
+    return _lib.load()
+
+
+def test_header_symbols_exported(lib):
+    syms = declared_symbols()
+    assert len(syms) >= 10
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_binding_covers_header():
+    from flowket_amd import _lib
+    assert set(declared_symbols()) == set(_lib.SIGNATURES)
+
+
+def test_version(lib):
+    from flowket_amd import _lib
+    assert lib.fk_version() >= 1
+    assert isinstance(lib.fk_last_error(), bytes)
+    _ = _lib
+
+
+def _maps(depth, C, prec):
+    from flowket_amd import _lib
+    nw, na = ctypes.c_int64(), ctypes.c_int64()
+    rc = _lib.load().fk_convar_packed_sizes(depth, C, prec, ctypes.byref(nw), ctypes.byref(na))
+    assert rc == 0
+    wmap = np.empty(nw.value, np.int32)
+    amap = np.empty(na.value, np.int32)
+    rc = _lib.load().fk_convar_pack_maps(depth, C, prec, wmap.ctypes.data_as(ctypes.c_void_p),
+                                         amap.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0
+    return wmap, amap
+
+
+@pytest.mark.parametrize('depth,C,prec', [(5, 16, 0), (4, 32, 0), (20, 32, 1), (3, 64, 1), (2, 32, 1), (1, 16, 0)])
+def test_pack_maps_cover_every_parameter_once(lib, depth, C, prec):
+    """Every network parameter appears in the packed buffers exactly once
+    (weights of MFMA convolutions once in the fragment buffer, biases and the
+    first block's 1-channel kernels once in the aux buffer)."""
+    from flowket_amd.machines import layer_shapes
+    wmap, amap = _maps(depth, C, prec)
+    n = sum(int(np.prod(k)) + int(np.prod(b)) for k, b in layer_shapes(depth, C))
+    used = np.concatenate([wmap[wmap >= 0], amap[amap >= 0]])
+    assert used.min() >= 0 and used.max() < n
+    counts = np.bincount(used, minlength=n)
+    assert np.all(counts == 1)
+
+
+def test_unsupported_shapes_are_errors(lib):
+    from flowket_amd import _lib
+    nw, na = ctypes.c_int64(), ctypes.c_int64()
+    assert _lib.load().fk_convar_packed_sizes(3, 24, 1, ctypes.byref(nw), ctypes.byref(na)) == 3
+    assert b'not implemented' in _lib.load().fk_last_error()

@@ -1,0 +1,132 @@
+"""Fused ConvNetAutoregressive2D forward on the GPU vs the float64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import convnet as ocn
+from oracle import exact as oex
+from tests.conftest import gpu_banner
+
+pytestmark = pytest.mark.gpu
+
+# FP32 kernels: log psi within 1e-5 relative; BF16: 2e-2 relative (tolerances
+# of the FP32 and BF16 paths).
+FP32_RTOL = 1e-5
+BF16_RTOL = 2e-2
+
+
+def _oracle_params(machine):
+    flat = machine.effective_params().detach().double().cpu().numpy()
+    out, off = [], 0
+    for kshape, bshape in machine.shapes:
+        nk, nb = int(np.prod(kshape)), int(np.prod(bshape))
+        out.append((flat[off:off + nk].reshape(kshape), flat[off + nk:off + nk + nb]))
+        off += nk + nb
+    return out
+
+
+def _machine(shape, depth, C, act, prec, seed=0, bias_scale=0.3, wn=False):
+    import flowket_amd as fk
+    m = fk.ConvNetAutoregressive2D(shape, depth=depth, num_of_channels=C, activation=act,
+                                   weights_normalization=wn, precision=prec, seed=seed)
+    if bias_scale:
+        # non-zero biases so every bias path is exercised
+        g = torch.Generator().manual_seed(seed + 1)
+        p = m.params.cpu().clone()
+        off = 0
+        for kshape, bshape in m.shapes:
+            nk, nb = int(np.prod(kshape)), int(np.prod(bshape))
+            p[off + nk:off + nk + nb] = bias_scale * torch.randn(nb, generator=g)
+            off += nk + nb
+        m.set_params(p)
+    return m
+
+
+def _check_logpsi(got, ref, rtol):
+    got = got.detach().cpu().numpy().astype(np.complex128)
+    scale = np.abs(ref.real).max() + 1.0
+    err_re = np.abs(got.real - ref.real).max() / scale
+    # phases: compare directly (absolute, same scale)
+    err_im = np.abs(got.imag - ref.imag).max() / scale
+    assert err_re < rtol and err_im < rtol, (err_re, err_im)
+
+
+CASES = [
+    ((4, 4), 5, 16, 'elu', 'fp32'),
+    ((4, 4), 2, 16, 'relu', 'fp32'),
+    ((4, 3), 2, 16, 'relu', 'fp32'),
+    ((6, 5), 3, 32, 'relu', 'fp32'),
+    ((10, 10), 4, 32, 'elu', 'fp32'),
+    ((10, 10), 4, 32, 'elu', 'bf16'),
+    ((7, 9), 3, 32, 'relu', 'bf16'),
+    ((10, 10), 3, 64, 'elu', 'fp32'),
+    ((10, 10), 3, 64, 'elu', 'bf16'),
+    ((4, 4), 1, 32, 'relu', 'bf16'),
+]
+
+
+@pytest.mark.parametrize('shape,depth,C,act,prec', CASES)
+def test_forward_matches_oracle(rng, shape, depth, C, act, prec):
+    gpu_banner(f'convnet forward shape={shape} depth={depth} C={C} act={act} prec={prec}')
+    m = _machine(shape, depth, C, act, prec)
+    B = 67
+    s = rng.choice([-1, 1], size=(B,) + shape).astype(np.int8)
+    got = m.log_psi(s)
+    ref = ocn.log_psi(_oracle_params(m), s, depth, activation=act)
+    _check_logpsi(got, ref, FP32_RTOL if prec == 'fp32' else BF16_RTOL)
+    cond = m.conditional_log_probs(s).cpu().numpy()
+    rcond = ocn.conditional_log_probs(_oracle_params(m), s, depth, activation=act)
+    tol = 1e-4 if prec == 'fp32' else 5e-2
+    assert np.abs(cond - rcond).max() < tol
+
+
+def test_forward_weight_normalisation(rng):
+    gpu_banner('convnet forward with weight normalisation')
+    m = _machine((6, 6), 3, 32, 'relu', 'fp32', wn=True)
+    s = rng.choice([-1, 1], size=(40, 6, 6)).astype(np.int8)
+    got = m.log_psi(s)
+    ref = ocn.log_psi(_oracle_params(m), s, 3, activation='relu')
+    _check_logpsi(got, ref, FP32_RTOL)
+
+
+def test_normalised_distribution_4x4():
+    """tests/test_autoregressive.py:18-30: sum_s |psi(s)|^2 == 1 (1e-5)."""
+    gpu_banner('convnet normalisation 4x4 depth 4 C 16')
+    m = _machine((4, 4), 4, 16, 'relu', 'fp32')
+    states = oex.all_states((4, 4)).astype(np.int8)
+    lp = m.log_psi(states).cpu().numpy().astype(np.complex128)
+    from scipy.special import logsumexp
+    assert np.exp(logsumexp(2 * lp.real)) == pytest.approx(1.0, abs=1e-5)
+
+
+def test_causality(rng):
+    """Conditional at site k is unchanged by any perturbation at sites >= k."""
+    gpu_banner('convnet causality 5x4')
+    m = _machine((5, 4), 3, 16, 'elu', 'fp32')
+    s = rng.choice([-1, 1], size=(1, 5, 4)).astype(np.int8)
+    base = m.conditional_log_probs(s).cpu().numpy()[0]
+    for k in range(20):
+        t = s.copy().reshape(1, -1)
+        t[0, k:] = rng.choice([-1, 1], size=20 - k)
+        c = m.conditional_log_probs(t.reshape(1, 5, 4)).cpu().numpy()[0].reshape(20, 2)
+        assert np.abs(c[:k + 1] - base.reshape(20, 2)[:k + 1]).max() < 1e-5
+
+
+def test_connected_equals_dense(rng):
+    """Ragged (sample, connection) evaluation == evaluation of the materialised
+    connected configurations."""
+    import flowket_amd as fk
+    gpu_banner('connected vs dense 8x8 heisenberg bf16+fp32')
+    for prec in ('fp32', 'bf16'):
+        m = _machine((8, 8), 3, 32, 'elu', prec)
+        op = fk.Heisenberg(hilbert_state_shape=(8, 8), pbc=True)
+        B = 50
+        s = torch.as_tensor(rng.choice([-1, 1], size=(B, 64)).astype(np.int8)).cuda()
+        r = op.compact(s)
+        total = int(r['offsets'][-1])
+        lp = m.log_psi_connected(s, r['pairs'], total, op)
+        conn = op.find_conn_device(s)['conn']
+        pairs = r['pairs'][:total].long()
+        dense = conn[pairs[:, 1], pairs[:, 0]]
+        lp2, _ = m.forward(dense.contiguous())
+        assert torch.equal(lp, lp2)
