@@ -1,0 +1,198 @@
+"""Benchmark of the VMC hot path on one MI355X.
+
+Workload (the headline configuration): 2-D antiferromagnetic Heisenberg on a
+10x10 periodic lattice (Marshall-rotated, J = 1), ConvNetAutoregressive2D with
+depth 20, 32 channels, 3x3 kernels, ReLU (the reference default), no weight
+normalisation, random Glorot weights from a seed, batch 8192 configurations.
+
+One step = one pass of the hot path over the batch:
+    find_conn -> ragged index map -> log psi of every used connected
+    configuration (fused BF16 MFMA forward) -> E_loc reduction -> FP64 energy
+    mean/variance.
+The headline ``value`` is local-energy evaluations per second of that step
+(BF16 MFMA network, ragged batched evaluation).  Reported beside it: the same
+local energies through the plain FP32 per-sample-padded path (every one of the
+201 connection slots of every sample evaluated in FP32), the samples/s of the
+samplers, and the CPU oracle on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+H = W = 10
+DEPTH = 20
+CHANNELS = 32
+BATCH = 8192
+ACT = 'relu'
+PEAK_BF16 = 2.5e15      # MI355X dense BF16 MFMA FLOP/s
+PEAK_FP32 = 157.3e12    # MI355X FP32 FLOP/s
+PEAK_HBM = 8.0e12       # MI355X HBM3E bytes/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--batch', type=int, default=BATCH)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline')
+    ap.add_argument('--no-fp32', action='store_true', help='skip the FP32 plain-path comparison')
+    return ap.parse_args()
+
+
+def flops_per_forward():
+    C = CHANNELS
+    nb = 2 * DEPTH - 2
+    return 2 * H * W * ((10 * C * C + 12 * C) + (nb - 1) * 22 * C * C + 4 * C)
+
+
+def cpu_baseline(machine, spins_np, n_samples=8):
+    """The CPU oracle (numpy float64 restatement) on a bounded sample: local
+    energies of the first n_samples configurations."""
+    from threadpoolctl import threadpool_limits
+    from oracle import convnet as ocn
+    from oracle import observables as oob
+    from oracle import operators as oop
+    flat = machine.effective_params().double().cpu().numpy()
+    params, off = [], 0
+    for ks, bs in machine.shapes:
+        nk, nb = int(np.prod(ks)), int(np.prod(bs))
+        params.append((flat[off:off + nk].reshape(ks), flat[off + nk:off + nk + nb]))
+        off += nk + nb
+    cores = len(os.sched_getaffinity(0))
+    s = spins_np[:n_samples].astype(np.float64)
+    with threadpool_limits(limits=cores):
+        t0 = time.perf_counter()
+        conn, mel, use = oop.heisenberg_find_conn(s, pbc=True)
+        oob.local_values(lambda x: ocn.log_psi(params, x, DEPTH, activation=ACT), conn, mel, use)
+        dt = time.perf_counter() - t0
+    return {'value': n_samples / dt, 'unit': 'E_loc evals/s', 'cores': cores, 'kind': 'port',
+            'sample': f'{n_samples} configurations of the benchmark batch, numpy float64 oracle '
+                      f'(find_conn + unbalanced E_loc with {int(use.sum())} network forwards), {dt:.1f} s'}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU; multi-GPU sharding is a design note only '
+                                   '(see DESIGN.md), --gpus must be 1'}))
+        return 2
+    import torch
+    import flowket_amd as fk
+    from flowket_amd import _lib
+    _lib.require_gpu()
+    dev = torch.device('cuda', 0)
+    torch.cuda.set_device(dev)
+    B = args.batch
+
+    m16 = fk.ConvNetAutoregressive2D((H, W), depth=DEPTH, num_of_channels=CHANNELS, activation=ACT,
+                                     weights_normalization=False, precision='bf16', seed=1234)
+    m32 = fk.ConvNetAutoregressive2D((H, W), depth=DEPTH, num_of_channels=CHANNELS, activation=ACT,
+                                     weights_normalization=False, precision='fp32', params=m16.params)
+    op = fk.Heisenberg(hilbert_state_shape=(H, W), pbc=True)
+    obs = fk.Observable(op)
+
+    # ---- synthetic batch: exact autoregressive samples of the random network
+    t0 = time.perf_counter()
+    ar = fk.AutoregressiveSampler(m16, B, seed=7)
+    spins = next(ar).reshape(B, H * W).contiguous()
+    torch.cuda.synchronize()
+    ar_time = time.perf_counter() - t0
+    samples_per_s_plain_ar = B / ar_time
+
+    # ---- the step
+    fwd_times = []
+
+    def step(record):
+        r = op.compact(spins)
+        offsets, pairs = r['offsets'], r['pairs']
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        logpsi = m16.log_psi_connected(spins, pairs, pairs.shape[0], op, m_dev=offsets[B:])
+        e1.record()
+        eloc = torch.empty((B, 2), dtype=torch.float64, device=dev)
+        _lib.call('fk_local_energy', _lib.ptr(offsets), _lib.ptr(pairs), _lib.ptr(logpsi), _lib.ptr(r['mel']),
+                  B, _lib.ptr(eloc), _lib.stream_ptr())
+        mom = torch.empty(4, dtype=torch.float64, device=dev)
+        _lib.call('fk_energy_moments', _lib.ptr(eloc), B, _lib.ptr(mom), _lib.stream_ptr())
+        if record:
+            fwd_times.append((e0, e1))
+        return offsets, mom
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        offsets, mom = step(True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = B * args.steps / elapsed
+    n_conn = int(offsets[-1].item())
+    fwd_ms = float(np.mean([a.elapsed_time(b) for a, b in fwd_times]))
+    flop_launch = n_conn * flops_per_forward()
+    achieved = flop_launch / (fwd_ms * 1e-3)
+    mom = mom.cpu().numpy()
+
+    out = {
+        'metric': 'samples/s & E_loc evals/s, 10x10 Heisenberg: bf16 batched vs fp32 plain',
+        'value': round(value, 2),
+        'unit': 'E_loc evals/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(ms_per_step, 3),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'bf16',
+        'data': 'synthetic: exact autoregressive samples of a seeded random-weight network',
+        'config': {'workload': '10x10 periodic Heisenberg, ConvNetAutoregressive2D depth 20 C 32 relu, '
+                               'batch 8192, bf16 MFMA ragged local energies', 'batch': B,
+                   'connected_evaluations_per_step': n_conn, 'params': m16.n_params},
+        'energy_per_site': float(mom[0]) / (H * W),
+        'roofline': {'bound': 'mfma', 'achieved': round(achieved / 1e12, 2), 'peak': PEAK_BF16 / 1e12,
+                     'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_BF16, 4), 'traffic': None,
+                     'kernel': 'convar_fwd_kernel<bf16,32>', 'kernel_ms': round(fwd_ms, 3),
+                     'flop_per_launch': flop_launch},
+        'samples_per_s': {'plain_autoregressive_bf16': round(samples_per_s_plain_ar, 1)},
+        'connected_logpsi_per_s': round(n_conn * args.steps / elapsed, 1),
+    }
+
+    if not args.no_fp32:
+        # plain FP32 path: every slot of every sample (per-sample padding), FP32 MFMA
+        nc = op.num_connections
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = op.find_conn_device(spins, want_conn=True)
+        flat = r['conn'].reshape(nc * B, H * W)
+        lp, _ = m32.forward(flat)
+        eloc32 = torch.empty((B, 2), dtype=torch.float64, device=dev)
+        _lib.call('fk_local_energy_dense', _lib.ptr(lp), _lib.ptr(r['mel']), nc, B, _lib.ptr(eloc32),
+                  _lib.stream_ptr())
+        torch.cuda.synchronize()
+        dt32 = time.perf_counter() - t0
+        e32 = torch.complex(eloc32[:, 0], eloc32[:, 1]).mean().item()
+        out['fp32_plain'] = {'E_loc_evals_per_s': round(B / dt32, 2), 'ms_per_step': round(dt32 * 1e3, 1),
+                             'evaluations': nc * B, 'energy_per_site': e32.real / (H * W)}
+        out['speedup_vs_fp32_plain'] = round(value / (B / dt32), 2)
+
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_baseline(m32, spins.cpu().numpy())
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

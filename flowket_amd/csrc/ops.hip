@@ -110,6 +110,8 @@ __global__ __launch_bounds__(256) void find_conn_kernel(
   }
 }
 
+__global__ void zero_i64_kernel(int64_t* p) { *p = 0; }
+
 // counts[b] -> offsets[b+1]; one thread per sample reads use[c][b] (coalesced
 // over b for each c).
 __global__ void count_used_kernel(const uint8_t* __restrict__ use, int64_t batch, int n_conn,
@@ -285,8 +287,9 @@ int fk_find_conn(const int8_t* spins, int64_t batch, int32_t n_sites, int32_t ki
                  const int32_t* conn_a, const int32_t* conn_b, const double* conn_mel, int32_t n_conn,
                  const int32_t* diag_a, const int32_t* diag_b, const double* diag_j, int32_t n_diag,
                  int8_t* conn_out, double* mel_out, uint8_t* use_out, int32_t* n_used_out, void* stream) {
-  FK_REQUIRE(spins && conn_a && conn_b && conn_mel && mel_out && use_out, "fk_find_conn: null pointer");
   FK_REQUIRE(batch >= 0 && n_sites > 0 && n_sites <= 65536 && n_conn >= 1, "fk_find_conn: bad sizes");
+  if (batch == 0) return FK_OK;
+  FK_REQUIRE(spins && conn_a && conn_b && conn_mel && mel_out && use_out, "fk_find_conn: null pointer");
   FK_REQUIRE(kind == FK_CONN_SWAP || kind == FK_CONN_SWAP_ZERO_PAD || kind == FK_CONN_FLIP,
              "fk_find_conn: bad kind %d", kind);
   FK_REQUIRE(n_diag == 0 || (diag_a && diag_b && diag_j), "fk_find_conn: null diagonal table");
@@ -309,13 +312,15 @@ int64_t fk_compact_scratch_bytes(int64_t batch) {
 int fk_compact_connections(const uint8_t* use, int64_t batch, int32_t n_conn, int64_t* offsets, int32_t* pairs,
                            void* scratch, void* stream) {
   (void)scratch;
-  FK_REQUIRE(use && offsets && pairs, "fk_compact_connections: null pointer");
   FK_REQUIRE(batch >= 0 && n_conn >= 1, "fk_compact_connections: bad sizes");
+  FK_REQUIRE(offsets, "fk_compact_connections: null offsets");
   hipStream_t s = (hipStream_t)stream;
   if (batch == 0) {
-    FK_CHECK_HIP(hipMemsetAsync(offsets, 0, sizeof(int64_t), s));
+    hipLaunchKernelGGL(zero_i64_kernel, dim3(1), dim3(1), 0, s, offsets);
+    FK_CHECK_LAUNCH();
     return FK_OK;
   }
+  FK_REQUIRE(use && pairs, "fk_compact_connections: null pointer");
   const int tb = 256;
   hipLaunchKernelGGL(count_used_kernel, dim3(div_up(batch, tb)), dim3(tb), 0, s, use, batch, n_conn, offsets);
   FK_CHECK_LAUNCH();

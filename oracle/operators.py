@@ -174,7 +174,9 @@ def j1j2_find_conn(sample, j2=0.5, pbc=False):
         mel[c] = np.where(anti, 2.0 * J[col], 0.0)
         swapped = flat.copy()
         swapped[:, a], swapped[:, b] = sb, sa
-        conn[c] = np.where(anti[:, None], swapped, 0.0)
+        # a connection exists only where its element is non-zero; every other
+        # slot is padding and holds the all-zero configuration (:56-60)
+        conn[c] = np.where((mel[c] != 0.0)[:, None], swapped, 0.0)
     use = mel != 0.0
     use[0] = True
     return conn.reshape((nc, B, L1, L2)), mel, use
