@@ -69,7 +69,7 @@ def cpu_baseline(machine, spins_np, n_samples=8):
         params.append((flat[off:off + nk].reshape(ks), flat[off + nk:off + nk + nb]))
         off += nk + nb
     cores = len(os.sched_getaffinity(0))
-    s = spins_np[:n_samples].astype(np.float64)
+    s = spins_np[:n_samples].reshape(n_samples, H, W).astype(np.float64)
     with threadpool_limits(limits=cores):
         t0 = time.perf_counter()
         conn, mel, use = oop.heisenberg_find_conn(s, pbc=True)
@@ -101,13 +101,31 @@ def main():
     op = fk.Heisenberg(hilbert_state_shape=(H, W), pbc=True)
     obs = fk.Observable(op)
 
-    # ---- synthetic batch: exact autoregressive samples of the random network
-    t0 = time.perf_counter()
-    ar = fk.AutoregressiveSampler(m16, B, seed=7)
-    spins = next(ar).reshape(B, H * W).contiguous()
-    torch.cuda.synchronize()
-    ar_time = time.perf_counter() - t0
-    samples_per_s_plain_ar = B / ar_time
+    # ---- samplers: fast (each (layer, site) once) and plain (H*W forwards)
+    def time_sampler(sampler, reps):
+        next(sampler)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            x = next(sampler)
+        torch.cuda.synchronize()
+        return x, B * reps / (time.perf_counter() - t0)
+
+    fast16 = fk.FastAutoregressiveSampler(m16, B, seed=7)
+    spins, sps_fast16 = time_sampler(fast16, 3)
+    spins = spins.reshape(B, H * W).contiguous()
+    _, sps_fast32 = time_sampler(fk.FastAutoregressiveSampler(m32, B, seed=8), 1)
+    sps = {'fast_autoregressive_bf16': round(sps_fast16, 1), 'fast_autoregressive_fp32': round(sps_fast32, 1)}
+    if not args.no_fp32:
+        t0 = time.perf_counter()
+        next(fk.AutoregressiveSampler(m16, B, seed=9))
+        torch.cuda.synchronize()
+        sps['plain_autoregressive_bf16'] = round(B / (time.perf_counter() - t0), 1)
+        t0 = time.perf_counter()
+        next(fk.AutoregressiveSampler(m32, B, seed=9))
+        torch.cuda.synchronize()
+        sps['plain_autoregressive_fp32'] = round(B / (time.perf_counter() - t0), 1)
+        sps['fast_bf16_vs_plain_fp32'] = round(sps_fast16 / sps['plain_autoregressive_fp32'], 1)
 
     # ---- the step
     fwd_times = []
@@ -157,7 +175,7 @@ def main():
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'bf16',
-        'data': 'synthetic: exact autoregressive samples of a seeded random-weight network',
+        'data': 'synthetic: exact (fast) autoregressive samples of a seeded random-weight network',
         'config': {'workload': '10x10 periodic Heisenberg, ConvNetAutoregressive2D depth 20 C 32 relu, '
                                'batch 8192, bf16 MFMA ragged local energies', 'batch': B,
                    'connected_evaluations_per_step': n_conn, 'params': m16.n_params},
@@ -166,7 +184,7 @@ def main():
                      'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_BF16, 4), 'traffic': None,
                      'kernel': 'convar_fwd_kernel<bf16,32>', 'kernel_ms': round(fwd_ms, 3),
                      'flop_per_launch': flop_launch},
-        'samples_per_s': {'plain_autoregressive_bf16': round(samples_per_s_plain_ar, 1)},
+        'samples_per_s': sps,
         'connected_logpsi_per_s': round(n_conn * args.steps / elapsed, 1),
     }
 

@@ -7,7 +7,7 @@ mirrors FlowKet's operator / machine / sampler / VMC classes.
 from .operators import Operator, OperatorOnGrid, cube_shape, Heisenberg, Ising, J1J2, j1j2_edges  # noqa: F401
 from .machines import ConvNetAutoregressive2D  # noqa: F401
 from .observables import BaseObservable, Observable, energy_moments  # noqa: F401
-from .samplers import Sampler, AutoregressiveSampler  # noqa: F401
+from .samplers import Sampler, AutoregressiveSampler, FastAutoregressiveSampler  # noqa: F401
 from .optimization import VariationalMonteCarlo, MiniBatchGenerator, loss_for_energy_minimization  # noqa: F401
 
 __version__ = '0.1.0'

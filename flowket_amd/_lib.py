@@ -41,6 +41,8 @@ SIGNATURES = {
     'fk_convar_forward': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _I32,
                                          _P, _P, _P, _P, _P]),
     'fk_ar_draw_site': (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P]),
+    'fk_fast_sampler_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I32, _I64]),
+    'fk_fast_sample': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I64, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -14,53 +14,9 @@
 // HBM traffic: H*W int8 spins in, 8 bytes out per configuration.
 #include "fk_common.h"
 #include "convar_layout.h"
+#include "convar_dev.h"
 
 namespace fk {
-
-template <int PREC>
-struct Prec;
-
-template <>
-struct Prec<FK_PREC_BF16> {
-  using frag = bf16x8;
-  static constexpr int ESZ = 2;
-  static constexpr int KW = 32;
-  __device__ static __forceinline__ f32x4 mfma(frag a, frag b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-  }
-  __device__ static __forceinline__ frag zero() {
-    frag z;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) z[e] = (__bf16)0.0f;
-    return z;
-  }
-};
-
-template <>
-struct Prec<FK_PREC_FP32> {
-  using frag = float;
-  static constexpr int ESZ = 4;
-  static constexpr int KW = 4;
-  __device__ static __forceinline__ f32x4 mfma(frag a, frag b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  __device__ static __forceinline__ frag zero() { return 0.0f; }
-};
-
-template <int ACT>
-__device__ __forceinline__ float act1(float x) {
-  if constexpr (ACT == FK_ACT_RELU) return fmaxf(x, 0.0f);
-  else return x > 0.0f ? x : expm1f(x);
-}
-template <int ACT>
-__device__ __forceinline__ f32x4 act4(f32x4 v) {
-  f32x4 r;
-  r.x = act1<ACT>(v.x);
-  r.y = act1<ACT>(v.y);
-  r.z = act1<ACT>(v.z);
-  r.w = act1<ACT>(v.w);
-  return r;
-}
 
 // ---------------------------------------------------------------------------
 // LDS addressing.  A pixel holds C channels as 16-byte chunks; chunk indices
@@ -132,44 +88,6 @@ struct Lds {
   }
 };
 
-// D fragments (MT tiles of 4 channels per lane) -> B fragments of the next
-// 1x1 contraction, in registers.
-template <int PREC, int MT, int KS>
-__device__ __forceinline__ void d_to_b(const f32x4 (&d)[MT], typename Prec<PREC>::frag (&b)[KS]) {
-  if constexpr (PREC == FK_PREC_BF16) {
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      bf16x8 f;
-      f[0] = (__bf16)d[2 * ks].x;
-      f[1] = (__bf16)d[2 * ks].y;
-      f[2] = (__bf16)d[2 * ks].z;
-      f[3] = (__bf16)d[2 * ks].w;
-      f[4] = (__bf16)d[2 * ks + 1].x;
-      f[5] = (__bf16)d[2 * ks + 1].y;
-      f[6] = (__bf16)d[2 * ks + 1].z;
-      f[7] = (__bf16)d[2 * ks + 1].w;
-      b[ks] = f;
-    }
-  } else {
-#pragma unroll
-    for (int t = 0; t < KS; ++t) b[t] = d[t >> 2][t & 3];
-  }
-}
-
-// Tap tables: KIND 0 = vertical kxk (rows -2..0, cols -1..1), 1 = horizontal
-// 1xk (cols -2..0), 2 = h3conv kxk on the concatenation (rows -2..0, cols
-// -2..0).  Tap index = di*kw + dj in the Keras kernel.
-template <int KIND>
-struct Taps {
-  static constexpr int N = (KIND == 1) ? 3 : 9;
-  __device__ static __forceinline__ int dr(int t) { return KIND == 1 ? 0 : (t / 3) - 2; }
-  __device__ static __forceinline__ int dc(int t) {
-    return KIND == 0 ? (t % 3) - 1 : (KIND == 1 ? t - 2 : (t % 3) - 2);
-  }
-  // bit of the (dr, dc) validity mask
-  __device__ static __forceinline__ int bit(int t) { return (dr(t) + 2) * 4 + (dc(t) + 2); }
-};
-
 struct FwdArgs {
   const void* w;
   const float* aux;
@@ -233,17 +151,6 @@ __device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t
       for (int ks = 0; ks < KS; ++ks) w[mt][ks] = wn[mt][ks];
   }
 }
-
-__device__ __forceinline__ f32x4 load_bias4(const float* b, int ch, int limit) {
-  f32x4 v;
-  v.x = (ch + 0 < limit) ? b[ch + 0] : 0.0f;
-  v.y = (ch + 1 < limit) ? b[ch + 1] : 0.0f;
-  v.z = (ch + 2 < limit) ? b[ch + 2] : 0.0f;
-  v.w = (ch + 3 < limit) ? b[ch + 3] : 0.0f;
-  return v;
-}
-
-enum BlockKind { BK_FIRST = 0, BK_A = 1, BK_B = 2, BK_FINAL = 3 };
 
 template <int PREC, int C, int MAXT, int ACT>
 __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
@@ -336,7 +243,7 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
     __syncthreads();
 
     for (int blk = 0; blk < nb; ++blk) {
-      const int kind = (blk == 0) ? BK_FIRST : (blk == nb - 1 ? BK_FINAL : ((blk & 1) ? BK_A : BK_B));
+      const int kind = block_kind(blk, nb);
       const uint8_t* Vin = (kind == BK_B) ? V1 : V0;
       const uint8_t* Hin = (kind == BK_B) ? H1 : H0;
       const float* bias = a.aux + lay.aux_bias(blk);

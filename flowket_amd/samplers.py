@@ -82,3 +82,54 @@ class AutoregressiveSampler(Sampler):
             _, cond = m.forward(batch, want_cond=True)
             _lib.call('fk_ar_draw_site', _lib.ptr(cond), _lib.ptr(u), self.batch_size, n, site, _lib.ptr(batch), s)
         return batch.reshape((self.batch_size,) + self.input_size)
+
+
+class FastAutoregressiveSampler(AutoregressiveSampler):
+    """deepar/samplers/fast_autoregressive.py:13-76: same distribution and,
+    given the same uniforms, the same draws as AutoregressiveSampler, with each
+    (layer, site) activation computed once (fk_fast_sample: one wavefront per
+    16 samples walking the lattice, activation cache in device memory).
+    ``mini_batch_size`` bounds the cache: batches larger than it are drawn in
+    chunks (:30-35)."""
+
+    def __init__(self, conditional_log_probs_machine, batch_size, mini_batch_size=None, seed=None,
+                 uniforms=None, precision=None, **kwargs):
+        super().__init__(conditional_log_probs_machine, batch_size, mini_batch_size, seed=seed, uniforms=uniforms)
+        self.precision = precision
+        self._ws = None
+        self.last_cond_logp = None
+        self.keep_cond_logp = False
+
+    def _workspace(self, rows, prec_code):
+        m = self.machine
+        nbytes = _lib.load().fk_fast_sampler_workspace_bytes(m.H, m.W, m.depth, m.num_of_channels, prec_code, rows)
+        if nbytes < 0:
+            raise _lib.FlowketHipError(
+                f'fast sampler does not implement channels={m.num_of_channels} (supported: 32, 64)')
+        if self._ws is None or self._ws.numel() < nbytes:
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=m.device)
+        return self._ws
+
+    def __next__(self):
+        _lib.require_gpu()
+        from .machines import _ACT, _PREC
+        m = self.machine
+        precision = self.precision or m.precision
+        prec = _PREC[precision]
+        n = m.H * m.W
+        u = self._draw_uniforms(m.device)
+        wbuf, abuf = m._pack(precision)
+        out = torch.empty((self.batch_size, n), dtype=torch.int8, device=m.device)
+        cond = torch.empty((self.batch_size, n, 2), dtype=torch.float32, device=m.device) \
+            if self.keep_cond_logp else None
+        chunk = self.mini_batch_size
+        s = _lib.stream_ptr()
+        for start in range(0, self.batch_size, chunk):
+            rows = min(chunk, self.batch_size - start)
+            ws = self._workspace(rows, prec)
+            _lib.call('fk_fast_sample', _lib.ptr(wbuf), _lib.ptr(abuf), prec, m.H, m.W, m.depth,
+                      m.num_of_channels, _ACT[m.activation], rows, _lib.ptr(u[start:start + rows]),
+                      _lib.ptr(ws), _lib.ptr(out[start:start + rows]),
+                      _lib.ptr(cond[start:start + rows]) if cond is not None else None, s)
+        self.last_cond_logp = cond
+        return out.reshape((self.batch_size,) + self.input_size)

@@ -148,6 +148,24 @@ int fk_ar_draw_site(const float* cond_logp, const double* uniforms,
                     int64_t batch, int32_t n_sites, int32_t site,
                     int8_t* spins, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Fast autoregressive sampling (deepar/samplers/fast_autoregressive.py:13-76):
+ * each (layer, site) activation computed once, spin drawn at every site with
+ * the AutoregressiveSampler rule s = +1 iff exp(log p(+1)) > u
+ * (deepar/samplers/autoregressive.py:42-44) from injected uniforms
+ * (batch, H*W) double.  Output spins (batch, H*W) int8; cond_logp (batch,
+ * H*W, 2) float = log-probabilities of each site's conditional, or NULL.
+ * `workspace` holds fk_fast_sampler_workspace_bytes(...) bytes (the
+ * activation cache; contents need no initialisation).  Channels 32 or 64.
+ * ------------------------------------------------------------------------- */
+int64_t fk_fast_sampler_workspace_bytes(int32_t H, int32_t W, int32_t depth,
+                                        int32_t channels, int32_t precision,
+                                        int64_t batch);
+int fk_fast_sample(const void* wpacked, const float* aux, int32_t precision,
+                   int32_t H, int32_t W, int32_t depth, int32_t channels,
+                   int32_t activation, int64_t batch, const double* uniforms,
+                   void* workspace, int8_t* out, float* cond_logp, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
