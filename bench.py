@@ -46,7 +46,28 @@ def parse():
     ap.add_argument('--batch', type=int, default=BATCH)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline')
     ap.add_argument('--no-fp32', action='store_true', help='skip the FP32 plain-path comparison')
+    ap.add_argument('--profile-forward', type=int, default=0, metavar='N',
+                    help='only run two fused-forward launches over the connected configurations of N random '
+                         'samples (a short program for rocprofv3 counter passes) and exit')
+    ap.add_argument('--precision', default='bf16', choices=['bf16', 'fp32'])
     return ap.parse_args()
+
+
+def profile_forward(n, precision):
+    import torch
+    import flowket_amd as fk
+    m = fk.ConvNetAutoregressive2D((H, W), depth=DEPTH, num_of_channels=CHANNELS, activation=ACT,
+                                   weights_normalization=False, precision=precision, seed=1234)
+    op = fk.Heisenberg(hilbert_state_shape=(H, W), pbc=True)
+    gen = torch.Generator().manual_seed(0)
+    spins = (torch.randint(0, 2, (n, H * W), generator=gen) * 2 - 1).to(torch.int8).cuda()
+    r = op.compact(spins)
+    total = int(r['offsets'][-1])
+    for _ in range(2):
+        m.log_psi_connected(spins, r['pairs'], total, op)
+    torch.cuda.synchronize()
+    print(json.dumps({'profile_forward': {'samples': n, 'configurations': total, 'precision': precision}}))
+    return 0
 
 
 def flops_per_forward():
@@ -82,6 +103,8 @@ def cpu_baseline(machine, spins_np, n_samples=8):
 
 def main():
     args = parse()
+    if args.profile_forward:
+        return profile_forward(args.profile_forward, args.precision)
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; multi-GPU sharding is a design note only '
                                    '(see DESIGN.md), --gpus must be 1'}))

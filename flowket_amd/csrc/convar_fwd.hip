@@ -27,7 +27,7 @@ struct Lds {
   static constexpr int ESZ = Prec<PREC>::ESZ;
   static constexpr int PIXB = C * ESZ;
   static constexpr int CPP = PIXB / 16;  // chunks per pixel
-  static constexpr int ROWB = 256;       // bytes per LDS bank row
+  static constexpr int ROWB = 128;       // bytes per LDS bank row (32 banks x 4 B)
   static constexpr int PPR = PIXB >= ROWB ? 1 : ROWB / PIXB;
   static constexpr bool SWZ = (CPP & (CPP - 1)) == 0;
   __device__ static __forceinline__ int chunk(int p, int c) {
@@ -68,6 +68,59 @@ struct Lds {
       return *reinterpret_cast<const f32x4*>(buf + quad(p, q));
     }
   }
+  // Chunk of k-step 0 for lane group g (the address that precomputed tap
+  // offsets point at); further k-steps / chunks are reached by XOR.
+  __device__ static __forceinline__ int base_chunk(int g) {
+    return PREC == FK_PREC_BF16 ? g : g * (C / 16);
+  }
+  template <int KS>
+  __device__ static __forceinline__ void load_bfrags_at(const uint8_t* buf, uint32_t off,
+                                                        typename Prec<PREC>::frag (&b)[KS]) {
+    if constexpr (PREC == FK_PREC_BF16) {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) b[ks] = *reinterpret_cast<const bf16x8*>(buf + (off ^ (64u * ks)));
+    } else {
+#pragma unroll
+      for (int j = 0; j < KS / 4; ++j) {
+        f32x4 v = *reinterpret_cast<const f32x4*>(buf + (off ^ (16u * j)));
+        b[4 * j + 0] = v.x;
+        b[4 * j + 1] = v.y;
+        b[4 * j + 2] = v.z;
+        b[4 * j + 3] = v.w;
+      }
+    }
+  }
+  // byte offset of M-tile mt's quad of this lane from the lane's own
+  // k-step-0 chunk offset
+  __device__ static __forceinline__ uint32_t self_quad(uint32_t off, int mt) {
+    if constexpr (PREC == FK_PREC_BF16) return (off ^ (64u * (mt >> 1))) + 8u * (mt & 1);
+    else return off ^ (16u * mt);
+  }
+  __device__ static __forceinline__ void store_quad_at(uint8_t* buf, uint32_t off, f32x4 v) {
+    if constexpr (PREC == FK_PREC_BF16) {
+      bf16x4 h;
+      h[0] = (__bf16)v.x;
+      h[1] = (__bf16)v.y;
+      h[2] = (__bf16)v.z;
+      h[3] = (__bf16)v.w;
+      *reinterpret_cast<bf16x4*>(buf + off) = h;
+    } else {
+      *reinterpret_cast<f32x4*>(buf + off) = v;
+    }
+  }
+  __device__ static __forceinline__ f32x4 load_quad_at(const uint8_t* buf, uint32_t off) {
+    if constexpr (PREC == FK_PREC_BF16) {
+      bf16x4 h = *reinterpret_cast<const bf16x4*>(buf + off);
+      f32x4 v;
+      v.x = (float)h[0];
+      v.y = (float)h[1];
+      v.z = (float)h[2];
+      v.w = (float)h[3];
+      return v;
+    } else {
+      return *reinterpret_cast<const f32x4*>(buf + off);
+    }
+  }
   // All B fragments (one per k-step) of pixel p for lane group g.
   template <int KS>
   __device__ static __forceinline__ void load_bfrags(const uint8_t* buf, int p, int g,
@@ -103,54 +156,40 @@ struct FwdArgs {
   float* cond;
 };
 
-// acc[u][mt] += sum over taps of W[tap][mt] x B(pixel + tap offset)
+// acc[u][mt] += sum over taps of W[tap][mt] x B(pixel + tap offset).
+// toff[u][k] is this lane's byte offset of k-step 0 at tap k = (dr+2)*4+(dc+2)
+// of tile u, pointing at the all-zero pixel for padding taps.
 template <int PREC, int C, int MAXT, int KIND, int MTO>
 __device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t* buf,
-                                          const typename Prec<PREC>::frag* __restrict__ wg, const int (&pix)[MAXT],
-                                          const uint32_t (&msk)[MAXT], int nt, int W, int lane, int g) {
+                                          const typename Prec<PREC>::frag* __restrict__ wg,
+                                          const uint32_t (&toff)[MAXT][12], int nt, int lane) {
   using P = Prec<PREC>;
   using frag = typename P::frag;
   using L = Lds<PREC, C>;
   constexpr int KS = C / P::KW;
   constexpr int NTAP = Taps<KIND>::N;
-  // weights of the current tap; the next tap's are fetched one iteration
-  // ahead so the L2 latency overlaps the current tap's MFMAs
-  frag w[MTO][KS], wn[MTO][KS];
 #pragma unroll
-  for (int mt = 0; mt < MTO; ++mt)
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) w[mt][ks] = wg[(mt * KS + ks) * 64 + lane];
-#pragma unroll 1
   for (int tap = 0; tap < NTAP; ++tap) {
-    const int tn = (tap + 1 < NTAP) ? tap + 1 : tap;
+    frag w[MTO][KS];
 #pragma unroll
     for (int mt = 0; mt < MTO; ++mt)
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) wn[mt][ks] = wg[((tn * MTO + mt) * KS + ks) * 64 + lane];
-    const int off = Taps<KIND>::dr(tap) * W + Taps<KIND>::dc(tap);
-    const int bit = Taps<KIND>::bit(tap);
+      for (int ks = 0; ks < KS; ++ks) w[mt][ks] = wg[((tap * MTO + mt) * KS + ks) * 64 + lane];
 #pragma unroll
     for (int u = 0; u < MAXT; ++u) {
       if (u < nt) {
-        const bool valid = (msk[u] >> bit) & 1u;
         frag b[KS];
-        L::template load_bfrags<KS>(buf, valid ? pix[u] + off : 0, g, b);
-        if (!valid) {
-#pragma unroll
-          for (int ks = 0; ks < KS; ++ks) b[ks] = P::zero();
-        }
+        L::template load_bfrags_at<KS>(buf, toff[u][Taps<KIND>::bit(tap)], b);
 #pragma unroll
         for (int mt = 0; mt < MTO; ++mt)
 #pragma unroll
           for (int ks = 0; ks < KS; ++ks) acc[u][mt] = P::mfma(w[mt][ks], b[ks], acc[u][mt]);
       }
     }
-#pragma unroll
-    for (int mt = 0; mt < MTO; ++mt)
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks) w[mt][ks] = wn[mt][ks];
   }
 }
+
+__device__ __forceinline__ f32x4 bias4(const float* b) { return *reinterpret_cast<const f32x4*>(b); }
 
 template <int PREC, int C, int MAXT, int ACT>
 __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
@@ -165,12 +204,15 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
 
   extern __shared__ __align__(16) uint8_t lds[];
   const int npix = a.npix;
+  // each buffer has npix pixels plus one all-zero pixel (index npix) that
+  // padding taps read
+  const int BUFB = (npix + 1) * PIXB;
   uint8_t* V0 = lds;
-  uint8_t* H0 = V0 + npix * PIXB;
-  uint8_t* V1 = H0 + npix * PIXB;
-  uint8_t* H1 = V1 + npix * PIXB;
-  uint8_t* CAT = H1 + npix * PIXB;
-  float* sp = reinterpret_cast<float*>(CAT + npix * PIXB);
+  uint8_t* H0 = V0 + BUFB;
+  uint8_t* V1 = H0 + BUFB;
+  uint8_t* H1 = V1 + BUFB;
+  uint8_t* CAT = H1 + BUFB;
+  float* sp = reinterpret_cast<float*>(CAT + BUFB);
   float2* part = reinterpret_cast<float2*>(sp + npix);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, NW = blockDim.x >> 6;
@@ -181,6 +223,7 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
 
   int pix[MAXT], pi[MAXT], pj[MAXT];
   uint32_t msk[MAXT];
+  uint32_t toff[MAXT][12];
   bool inp[MAXT];
 #pragma unroll
   for (int u = 0; u < MAXT; ++u) {
@@ -202,6 +245,16 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
     pj[u] = j;
     msk[u] = m;
     inp[u] = in;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      const int dr = k / 4 - 2, dc = k % 4 - 2;
+      const int pp = ((m >> k) & 1u) ? p + dr * W + dc : npix;
+      toff[u][k] = (uint32_t)L::chunk(pp, L::base_chunk(g));
+    }
+  }
+  for (int z = threadIdx.x; z < 5 * (PIXB / 4); z += blockDim.x) {
+    const int buf = z / (PIXB / 4), w4 = z - buf * (PIXB / 4);
+    reinterpret_cast<uint32_t*>(lds + buf * BUFB + npix * PIXB)[w4] = 0u;
   }
 
   int64_t M = a.m_dev ? *a.m_dev : a.m_count;
@@ -255,7 +308,7 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
 #pragma unroll
         for (int u = 0; u < MAXT; ++u)
 #pragma unroll
-          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = load_bias4(bias, 16 * mt + 4 * g, C);
+          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + 16 * mt + 4 * g);
         if (kind == BK_FIRST) {
           const float* kv = a.aux + lay.aux_kv0(nb);
 #pragma unroll
@@ -274,8 +327,7 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
             }
           }
         } else {
-          conv_pass<PREC, C, MAXT, 0, MT>(acc, Vin, wall + lay.off_v(blk) * 64,
-                                          pix, msk, nt, W, lane, g);
+          conv_pass<PREC, C, MAXT, 0, MT>(acc, Vin, wall + lay.off_v(blk) * 64, toff, nt, lane);
         }
         // epilogue: vertical output + y = act(1x1(DownShift(act(v'))))
         frag wy[MTH][KS];
@@ -293,10 +345,10 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
               av[mt] = act4<ACT>(acc[u][mt]);
               const int q = 4 * mt + g;
               if (inp[u]) {
-                if (kind == BK_FIRST) L::store_quad(V0, pix[u], q, av[mt]);
-                else if (kind == BK_A) L::store_quad(V1, pix[u], q, av[mt]);
-                else if (kind == BK_B)
-                  L::store_quad(V0, pix[u], q, act4<ACT>(L::load_quad(V0, pix[u], q) + acc[u][mt]));
+                const uint32_t so = L::self_quad(toff[u][10], mt);
+                if (kind == BK_FIRST) L::store_quad_at(V0, so, av[mt]);
+                else if (kind == BK_A) L::store_quad_at(V1, so, av[mt]);
+                else if (kind == BK_B) L::store_quad_at(V0, so, act4<ACT>(L::load_quad_at(V0, so) + acc[u][mt]));
               }
             }
             frag bf[KS];
@@ -322,7 +374,7 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
 #pragma unroll
         for (int u = 0; u < MAXT; ++u)
 #pragma unroll
-          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = load_bias4(bias + C, 16 * mt + 4 * g, C);
+          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + C + 16 * mt + 4 * g);
         if (kind == BK_FIRST) {
           const float* kh = a.aux + lay.aux_kh0(nb);
 #pragma unroll
@@ -341,7 +393,7 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
             }
           }
         } else {
-          conv_pass<PREC, C, MAXT, 1, MT>(acc, Hin, wall + lay.off_h(blk) * 64, pix, msk, nt, W, lane, g);
+          conv_pass<PREC, C, MAXT, 1, MT>(acc, Hin, wall + lay.off_h(blk) * 64, toff, nt, lane);
         }
         frag wx[MTH][KS];
 #pragma unroll
@@ -383,8 +435,8 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
 #pragma unroll
         for (int u = 0; u < MAXT; ++u)
 #pragma unroll
-          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = load_bias4(bias + 3 * C, 16 * mt + 4 * g, C);
-        conv_pass<PREC, C, MAXT, 2, MT>(acc, CAT, wall + lay.off_c(blk) * 64, pix, msk, nt, W, lane, g);
+          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + 3 * C + 16 * mt + 4 * g);
+        conv_pass<PREC, C, MAXT, 2, MT>(acc, CAT, wall + lay.off_c(blk) * 64, toff, nt, lane);
         if (kind != BK_FINAL) {
 #pragma unroll
           for (int u = 0; u < MAXT; ++u) {
@@ -392,9 +444,10 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
 #pragma unroll
               for (int mt = 0; mt < MT; ++mt) {
                 const int q = 4 * mt + g;
-                if (kind == BK_FIRST) L::store_quad(H0, pix[u], q, act4<ACT>(acc[u][mt]));
-                else if (kind == BK_A) L::store_quad(H1, pix[u], q, act4<ACT>(acc[u][mt]));
-                else L::store_quad(H0, pix[u], q, act4<ACT>(L::load_quad(H0, pix[u], q) + acc[u][mt]));
+                const uint32_t so = L::self_quad(toff[u][10], mt);
+                if (kind == BK_FIRST) L::store_quad_at(H0, so, act4<ACT>(acc[u][mt]));
+                else if (kind == BK_A) L::store_quad_at(H1, so, act4<ACT>(acc[u][mt]));
+                else L::store_quad_at(H0, so, act4<ACT>(L::load_quad_at(H0, so) + acc[u][mt]));
               }
             }
           }
@@ -480,7 +533,7 @@ struct FwdPlan {
   size_t lds;
 };
 
-static size_t fwd_lds_bytes(int npix, int pixb) { return (size_t)npix * pixb * 5 + (size_t)npix * (4 + 8); }
+static size_t fwd_lds_bytes(int npix, int pixb) { return (size_t)(npix + 1) * pixb * 5 + (size_t)npix * (4 + 8); }
 
 static int plan_forward(int HW, int C, int prec, int64_t m, FwdPlan* plan) {
   const int pixb = C * (prec == FK_PREC_BF16 ? 2 : 4);
