@@ -191,8 +191,45 @@ __device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t
 
 __device__ __forceinline__ f32x4 bias4(const float* b) { return *reinterpret_cast<const f32x4*>(b); }
 
-template <int PREC, int C, int MAXT, int ACT>
-__global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
+// Register-resident weights of one convolution (all taps) for the tile-outer
+// schedule.
+template <int PREC, int C, int KIND, int MTO>
+struct ConvW {
+  static constexpr int KS = C / Prec<PREC>::KW;
+  static constexpr int N = Taps<KIND>::N;
+  typename Prec<PREC>::frag w[N][MTO][KS];
+  __device__ __forceinline__ void load(const typename Prec<PREC>::frag* __restrict__ wg, int lane) {
+#pragma unroll
+    for (int t = 0; t < N; ++t)
+#pragma unroll
+      for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) w[t][mt][ks] = wg[((t * MTO + mt) * KS + ks) * 64 + lane];
+  }
+};
+
+// One 16-pixel tile of a convolution with register-resident weights: every
+// tap's B fragments are read from LDS first, then the MFMAs run back to back.
+template <int PREC, int C, int KIND, int MTO>
+__device__ __forceinline__ void conv_tile(f32x4 (&acc)[MTO], const uint8_t* buf, const ConvW<PREC, C, KIND, MTO>& cw,
+                                          const uint32_t (&to)[12]) {
+  using P = Prec<PREC>;
+  using L = Lds<PREC, C>;
+  constexpr int KS = C / P::KW;
+  constexpr int N = Taps<KIND>::N;
+  typename P::frag b[N][KS];
+#pragma unroll
+  for (int t = 0; t < N; ++t) L::template load_bfrags_at<KS>(buf, to[Taps<KIND>::bit(t)], b[t]);
+#pragma unroll
+  for (int t = 0; t < N; ++t)
+#pragma unroll
+    for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) acc[mt] = P::mfma(cw.w[t][mt][ks], b[t][ks], acc[mt]);
+}
+
+template <int PREC, int C, int MAXT, int ACT, int NWAVE>
+__global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   using P = Prec<PREC>;
   using frag = typename P::frag;
   using L = Lds<PREC, C>;
@@ -200,6 +237,10 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
   constexpr int KS = C / P::KW;
   constexpr int MTH = (C / 2 + 15) / 16;
   constexpr int PIXB = L::PIXB;
+  // bf16 C=32: a 3x3 convolution's weights fit in 72 VGPRs, so each wave
+  // loads them once per phase and walks its tiles; otherwise taps are the
+  // outer loop and every tap's weights serve all tiles of the wave.
+  constexpr bool TILE_OUTER = (PREC == FK_PREC_BF16 && C == 32);
   const ConvarLayout lay(C, PREC);
 
   extern __shared__ __align__(16) uint8_t lds[];
@@ -215,7 +256,11 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
   float* sp = reinterpret_cast<float*>(CAT + BUFB);
   float2* part = reinterpret_cast<float2*>(sp + npix);
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, NW = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // wave index and tile count are wave-uniform: keep them in SGPRs so tile
+  // predicates are scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int NW = blockDim.x >> 6;
   const int n16 = lane & 15, g = lane >> 4;
   const int H = a.H, W = a.W, HW = a.HW, T = a.T;
   const int ntiles = npix >> 4;
@@ -300,195 +345,240 @@ __global__ __launch_bounds__(512) void convar_fwd_kernel(FwdArgs a) {
       const uint8_t* Vin = (kind == BK_B) ? V1 : V0;
       const uint8_t* Hin = (kind == BK_B) ? H1 : H0;
       const float* bias = a.aux + lay.aux_bias(blk);
+      const float* by = bias + 2 * C + C / 2;
+      const float* bx = bias + 2 * C;
+
+      frag wy[MTH][KS], wx[MTH][KS];
+#pragma unroll
+      for (int mh = 0; mh < MTH; ++mh)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          wy[mh][ks] = wall[(lay.off_y(blk) + mh * KS + ks) * 64 + lane];
+          wx[mh][ks] = wall[(lay.off_x(blk) + mh * KS + ks) * 64 + lane];
+        }
+
+      // -- block-0 convolutions on the 1-channel input (VALU)
+      auto valu_v = [&](int u, f32x4 (&acc)[MT]) {
+        const float* kv = a.aux + lay.aux_kv0(nb);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+          const bool valid = (msk[u] >> Taps<0>::bit(tap)) & 1u;
+          const float s = valid ? sp[pix[u] + Taps<0>::dr(tap) * W + Taps<0>::dc(tap)] : 0.0f;
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            const f32x4 k = *reinterpret_cast<const f32x4*>(kv + tap * C + 16 * mt + 4 * g);
+            acc[mt] += k * s;
+          }
+        }
+      };
+      auto valu_h = [&](int u, f32x4 (&acc)[MT]) {
+        const float* kh = a.aux + lay.aux_kh0(nb);
+#pragma unroll
+        for (int tap = 0; tap < 3; ++tap) {
+          const bool valid = (msk[u] >> Taps<1>::bit(tap)) & 1u;
+          const float s = valid ? sp[pix[u] + Taps<1>::dc(tap)] : 0.0f;
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            const f32x4 k = *reinterpret_cast<const f32x4*>(kh + tap * C + 16 * mt + 4 * g);
+            acc[mt] += k * s;
+          }
+        }
+      };
+      // -- epilogue of the vertical kxk: vertical output + y = act(1x1(
+      //    DownShift(act(v')))) written one row down
+      auto epi_v = [&](int u, const f32x4 (&acc)[MT]) {
+        f32x4 av[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          av[mt] = act4<ACT>(acc[mt]);
+          if (inp[u]) {
+            const uint32_t so = L::self_quad(toff[u][10], mt);
+            if (kind == BK_FIRST) L::store_quad_at(V0, so, av[mt]);
+            else if (kind == BK_A) L::store_quad_at(V1, so, av[mt]);
+            else if (kind == BK_B) L::store_quad_at(V0, so, act4<ACT>(L::load_quad_at(V0, so) + acc[mt]));
+          }
+        }
+        frag bf[KS];
+        d_to_b<PREC, MT, KS>(av, bf);
+        const bool down = pi[u] + 1 < H;
+        const int tgt = down ? pix[u] + W : pix[u] - (H - 1) * W;
+#pragma unroll
+        for (int mh = 0; mh < MTH; ++mh) {
+          const f32x4 b0 = bias4(by + 16 * mh + 4 * g);
+          f32x4 y = b0;
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) y = P::mfma(wy[mh][ks], bf[ks], y);
+          y = act4<ACT>(down ? y : b0);
+          if (inp[u] && 16 * mh + 4 * g < C / 2) L::store_quad(CAT, tgt, C / 8 + 4 * mh + g, y);
+        }
+      };
+      // -- epilogue of the horizontal 1xk: x = act(1x1(act(.))) into CAT (the
+      //    final block's RightShift folded into the write target)
+      auto epi_h = [&](int u, const f32x4 (&acc)[MT]) {
+        f32x4 ax[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) ax[mt] = act4<ACT>(acc[mt]);
+        frag bf[KS];
+        d_to_b<PREC, MT, KS>(ax, bf);
+        bool keep = true;
+        int tgt = pix[u];
+        if (kind == BK_FINAL) {
+          keep = pj[u] + 1 < W;
+          tgt = keep ? pix[u] + 1 : pix[u] - (W - 1);
+        }
+#pragma unroll
+        for (int mh = 0; mh < MTH; ++mh) {
+          const f32x4 b0 = bias4(bx + 16 * mh + 4 * g);
+          f32x4 x = b0;
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) x = P::mfma(wx[mh][ks], bf[ks], x);
+          x = act4<ACT>(keep ? x : b0);
+          if (inp[u] && 16 * mh + 4 * g < C / 2) L::store_quad(CAT, tgt, 4 * mh + g, x);
+        }
+      };
 
       // ================= phase VH: vertical stack + horizontal 1xk =========
-      {
-        // -- vertical kxk (pre-activation v')
-        f32x4 acc[MAXT][MT];
-#pragma unroll
-        for (int u = 0; u < MAXT; ++u)
-#pragma unroll
-          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + 16 * mt + 4 * g);
-        if (kind == BK_FIRST) {
-          const float* kv = a.aux + lay.aux_kv0(nb);
+      if constexpr (TILE_OUTER) {
+        {
+          ConvW<PREC, C, 0, MT> wv;
+          if (kind != BK_FIRST) wv.load(wall + lay.off_v(blk) * 64, lane);
 #pragma unroll
           for (int u = 0; u < MAXT; ++u) {
             if (u < nt) {
+              f32x4 acc[MT];
 #pragma unroll
-              for (int tap = 0; tap < 9; ++tap) {
-                const bool valid = (msk[u] >> Taps<0>::bit(tap)) & 1u;
-                const float s = valid ? sp[pix[u] + Taps<0>::dr(tap) * W + Taps<0>::dc(tap)] : 0.0f;
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt) {
-                  const f32x4 k = *reinterpret_cast<const f32x4*>(kv + tap * C + 16 * mt + 4 * g);
-                  acc[u][mt] += k * s;
-                }
-              }
-            }
-          }
-        } else {
-          conv_pass<PREC, C, MAXT, 0, MT>(acc, Vin, wall + lay.off_v(blk) * 64, toff, nt, lane);
-        }
-        // epilogue: vertical output + y = act(1x1(DownShift(act(v'))))
-        frag wy[MTH][KS];
-#pragma unroll
-        for (int mh = 0; mh < MTH; ++mh)
-#pragma unroll
-          for (int ks = 0; ks < KS; ++ks) wy[mh][ks] = wall[(lay.off_y(blk) + mh * KS + ks) * 64 + lane];
-        const float* by = bias + 2 * C + C / 2;
-#pragma unroll
-        for (int u = 0; u < MAXT; ++u) {
-          if (u < nt) {
-            f32x4 av[MT];
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-              av[mt] = act4<ACT>(acc[u][mt]);
-              const int q = 4 * mt + g;
-              if (inp[u]) {
-                const uint32_t so = L::self_quad(toff[u][10], mt);
-                if (kind == BK_FIRST) L::store_quad_at(V0, so, av[mt]);
-                else if (kind == BK_A) L::store_quad_at(V1, so, av[mt]);
-                else if (kind == BK_B) L::store_quad_at(V0, so, act4<ACT>(L::load_quad_at(V0, so) + acc[u][mt]));
-              }
-            }
-            frag bf[KS];
-            d_to_b<PREC, MT, KS>(av, bf);
-            const bool down = pi[u] + 1 < H;
-            const int tgt = down ? pix[u] + W : pix[u] - (H - 1) * W;
-#pragma unroll
-            for (int mh = 0; mh < MTH; ++mh) {
-              const f32x4 b0 = load_bias4(by, 16 * mh + 4 * g, C / 2);
-              f32x4 y = b0;
-#pragma unroll
-              for (int ks = 0; ks < KS; ++ks) y = P::mfma(wy[mh][ks], bf[ks], y);
-              y = act4<ACT>(down ? y : b0);
-              if (inp[u] && 16 * mh + 4 * g < C / 2) L::store_quad(CAT, tgt, C / 8 + 4 * mh + g, y);
+              for (int mt = 0; mt < MT; ++mt) acc[mt] = bias4(bias + 16 * mt + 4 * g);
+              if (kind == BK_FIRST) valu_v(u, acc);
+              else conv_tile<PREC, C, 0, MT>(acc, Vin, wv, toff[u]);
+              epi_v(u, acc);
             }
           }
         }
-      }
-      {
-        // -- horizontal 1xk, then x = act(1x1(act(.))) into CAT (RightShift
-        //    of the final block folded into the write target)
-        f32x4 acc[MAXT][MT];
-#pragma unroll
-        for (int u = 0; u < MAXT; ++u)
-#pragma unroll
-          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + C + 16 * mt + 4 * g);
-        if (kind == BK_FIRST) {
-          const float* kh = a.aux + lay.aux_kh0(nb);
+        {
+          ConvW<PREC, C, 1, MT> wh;
+          if (kind != BK_FIRST) wh.load(wall + lay.off_h(blk) * 64, lane);
 #pragma unroll
           for (int u = 0; u < MAXT; ++u) {
             if (u < nt) {
+              f32x4 acch[MT];
 #pragma unroll
-              for (int tap = 0; tap < 3; ++tap) {
-                const bool valid = (msk[u] >> Taps<1>::bit(tap)) & 1u;
-                const float s = valid ? sp[pix[u] + Taps<1>::dc(tap)] : 0.0f;
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt) {
-                  const f32x4 k = *reinterpret_cast<const f32x4*>(kh + tap * C + 16 * mt + 4 * g);
-                  acc[u][mt] += k * s;
-                }
-              }
+              for (int mt = 0; mt < MT; ++mt) acch[mt] = bias4(bias + C + 16 * mt + 4 * g);
+              if (kind == BK_FIRST) valu_h(u, acch);
+              else conv_tile<PREC, C, 1, MT>(acch, Hin, wh, toff[u]);
+              epi_h(u, acch);
             }
           }
-        } else {
-          conv_pass<PREC, C, MAXT, 1, MT>(acc, Hin, wall + lay.off_h(blk) * 64, toff, nt, lane);
         }
-        frag wx[MTH][KS];
+      } else {
+        {
+          f32x4 acc[MAXT][MT];
 #pragma unroll
-        for (int mh = 0; mh < MTH; ++mh)
+          for (int u = 0; u < MAXT; ++u)
 #pragma unroll
-          for (int ks = 0; ks < KS; ++ks) wx[mh][ks] = wall[(lay.off_x(blk) + mh * KS + ks) * 64 + lane];
-        const float* bx = bias + 2 * C;
+            for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + 16 * mt + 4 * g);
+          if (kind == BK_FIRST) {
 #pragma unroll
-        for (int u = 0; u < MAXT; ++u) {
-          if (u < nt) {
-            f32x4 ax[MT];
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) ax[mt] = act4<ACT>(acc[u][mt]);
-            frag bf[KS];
-            d_to_b<PREC, MT, KS>(ax, bf);
-            bool keep = true;
-            int tgt = pix[u];
-            if (kind == BK_FINAL) {
-              keep = pj[u] + 1 < W;
-              tgt = keep ? pix[u] + 1 : pix[u] - (W - 1);
-            }
-#pragma unroll
-            for (int mh = 0; mh < MTH; ++mh) {
-              const f32x4 b0 = load_bias4(bx, 16 * mh + 4 * g, C / 2);
-              f32x4 x = b0;
-#pragma unroll
-              for (int ks = 0; ks < KS; ++ks) x = P::mfma(wx[mh][ks], bf[ks], x);
-              x = act4<ACT>(keep ? x : b0);
-              if (inp[u] && 16 * mh + 4 * g < C / 2) L::store_quad(CAT, tgt, 4 * mh + g, x);
-            }
+            for (int u = 0; u < MAXT; ++u)
+              if (u < nt) valu_v(u, acc[u]);
+          } else {
+            conv_pass<PREC, C, MAXT, 0, MT>(acc, Vin, wall + lay.off_v(blk) * 64, toff, nt, lane);
           }
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u)
+            if (u < nt) epi_v(u, acc[u]);
+        }
+        {
+          f32x4 acc[MAXT][MT];
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + C + 16 * mt + 4 * g);
+          if (kind == BK_FIRST) {
+#pragma unroll
+            for (int u = 0; u < MAXT; ++u)
+              if (u < nt) valu_h(u, acc[u]);
+          } else {
+            conv_pass<PREC, C, MAXT, 1, MT>(acc, Hin, wall + lay.off_h(blk) * 64, toff, nt, lane);
+          }
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u)
+            if (u < nt) epi_h(u, acc[u]);
         }
       }
       __syncthreads();
 
       // ================= phase H3: kxk on the concatenation ================
-      {
+      frag whd[KS];
+      f32x4 hb = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (kind == BK_FINAL) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) whd[ks] = wall[(lay.head_base(nb) + ks) * 64 + lane];
+        hb = load_bias4(a.aux + lay.aux_head_bias(nb), 4 * g, 4);
+      }
+      auto epi_c = [&](int u, const f32x4 (&acc)[MT]) {
+        if (kind != BK_FINAL) {
+          if (inp[u]) {
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+              const uint32_t so = L::self_quad(toff[u][10], mt);
+              if (kind == BK_FIRST) L::store_quad_at(H0, so, act4<ACT>(acc[mt]));
+              else if (kind == BK_A) L::store_quad_at(H1, so, act4<ACT>(acc[mt]));
+              else L::store_quad_at(H0, so, act4<ACT>(L::load_quad_at(H0, so) + acc[mt]));
+            }
+          }
+          return;
+        }
+        // head: 1x1 C->4 on act(h'), complex logits, 2-way log-normalise,
+        // gather at the actual spin (abstract_machine.py:34-37,54-57)
+        f32x4 ah[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) ah[mt] = act4<ACT>(acc[mt]);
+        frag bf[KS];
+        d_to_b<PREC, MT, KS>(ah, bf);
+        f32x4 z = hb;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) z = P::mfma(whd[ks], bf[ks], z);
+        if (g == 0 && inp[u]) {
+          // z = (Re z0, Re z1, Im z0, Im z1) for spin values (+1, -1)
+          const float mx = fmaxf(z.x, z.y);
+          const float lse = mx + 0.5f * log1pf(expf(-2.0f * fabsf(z.x - z.y)));
+          const float s = sp[pix[u]];
+          const bool up = s > 0.0f;
+          part[pix[u]] = make_float2((up ? z.x : z.y) - lse, up ? z.z : z.w);
+          if (a.cond) {
+            const int t = pix[u] / HW;
+            const int64_t m = grp * T + t;
+            if (m < M) {
+              float* o = a.cond + (m * HW + (pix[u] - t * HW)) * 2;
+              o[0] = 2.0f * (z.x - lse);
+              o[1] = 2.0f * (z.y - lse);
+            }
+          }
+        }
+      };
+      if constexpr (TILE_OUTER) {
+        ConvW<PREC, C, 2, MT> wc;
+        wc.load(wall + lay.off_c(blk) * 64, lane);
+#pragma unroll
+        for (int u = 0; u < MAXT; ++u) {
+          if (u < nt) {
+            f32x4 acc[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) acc[mt] = bias4(bias + 3 * C + 16 * mt + 4 * g);
+            conv_tile<PREC, C, 2, MT>(acc, CAT, wc, toff[u]);
+            epi_c(u, acc);
+          }
+        }
+      } else {
         f32x4 acc[MAXT][MT];
 #pragma unroll
         for (int u = 0; u < MAXT; ++u)
 #pragma unroll
           for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + 3 * C + 16 * mt + 4 * g);
         conv_pass<PREC, C, MAXT, 2, MT>(acc, CAT, wall + lay.off_c(blk) * 64, toff, nt, lane);
-        if (kind != BK_FINAL) {
 #pragma unroll
-          for (int u = 0; u < MAXT; ++u) {
-            if (u < nt && inp[u]) {
-#pragma unroll
-              for (int mt = 0; mt < MT; ++mt) {
-                const int q = 4 * mt + g;
-                const uint32_t so = L::self_quad(toff[u][10], mt);
-                if (kind == BK_FIRST) L::store_quad_at(H0, so, act4<ACT>(acc[u][mt]));
-                else if (kind == BK_A) L::store_quad_at(H1, so, act4<ACT>(acc[u][mt]));
-                else L::store_quad_at(H0, so, act4<ACT>(L::load_quad_at(H0, so) + acc[u][mt]));
-              }
-            }
-          }
-        } else {
-          // head: 1x1 C->4 on act(h'), complex logits, 2-way log-normalise,
-          // gather at the actual spin (abstract_machine.py:34-37,54-57)
-          frag wh[KS];
-#pragma unroll
-          for (int ks = 0; ks < KS; ++ks) wh[ks] = wall[(lay.head_base(nb) + ks) * 64 + lane];
-          const f32x4 hb = load_bias4(a.aux + lay.aux_head_bias(nb), 4 * g, 4);
-#pragma unroll
-          for (int u = 0; u < MAXT; ++u) {
-            if (u < nt) {
-              f32x4 ah[MT];
-#pragma unroll
-              for (int mt = 0; mt < MT; ++mt) ah[mt] = act4<ACT>(acc[u][mt]);
-              frag bf[KS];
-              d_to_b<PREC, MT, KS>(ah, bf);
-              f32x4 z = hb;
-#pragma unroll
-              for (int ks = 0; ks < KS; ++ks) z = P::mfma(wh[ks], bf[ks], z);
-              if (g == 0 && inp[u]) {
-                // z = (Re z0, Re z1, Im z0, Im z1) for spin values (+1, -1)
-                const float mx = fmaxf(z.x, z.y);
-                const float lse = mx + 0.5f * log1pf(expf(-2.0f * fabsf(z.x - z.y)));
-                const float s = sp[pix[u]];
-                const bool up = s > 0.0f;
-                part[pix[u]] = make_float2((up ? z.x : z.y) - lse, up ? z.z : z.w);
-                if (a.cond) {
-                  const int t = pix[u] / HW;
-                  const int64_t m = grp * T + t;
-                  if (m < M) {
-                    float* o = a.cond + (m * HW + (pix[u] - t * HW)) * 2;
-                    o[0] = 2.0f * (z.x - lse);
-                    o[1] = 2.0f * (z.y - lse);
-                  }
-                }
-              }
-            }
-          }
-        }
+        for (int u = 0; u < MAXT; ++u)
+          if (u < nt) epi_c(u, acc[u]);
       }
       __syncthreads();
     }
@@ -529,22 +619,26 @@ __global__ void pack_gather_kernel(const float* __restrict__ src, const int32_t*
 
 // ---------------------------------------------------------------------------
 struct FwdPlan {
-  int T, npix, maxt;
+  int T, npix, maxt, nwave;
   size_t lds;
 };
 
 static size_t fwd_lds_bytes(int npix, int pixb) { return (size_t)(npix + 1) * pixb * 5 + (size_t)npix * (4 + 8); }
 
+// Largest tile of configurations whose five activation lattices fit in the
+// 160 KB LDS; bf16 C=32 runs one wave per SIMD (the whole register file per
+// wave: register-resident convolution weights), the other shapes two.
 static int plan_forward(int HW, int C, int prec, int64_t m, FwdPlan* plan) {
   const int pixb = C * (prec == FK_PREC_BF16 ? 2 : 4);
   const size_t cap = 160 * 1024;
-  const int NW = 8;
+  const int NW = (prec == FK_PREC_BF16 && C == 32) ? 4 : 8;
+  const int maxt_cap = (NW == 4) ? 8 : 4;
   int best = 0;
   for (int T = 1; T <= 256; ++T) {
     const int npix = ((T * HW + 15) / 16) * 16;
     if (fwd_lds_bytes(npix, pixb) > cap) break;
     const int per_wave = (npix / 16 + NW - 1) / NW;
-    if (per_wave > 4) break;
+    if (per_wave > maxt_cap) break;
     best = T;
     if ((int64_t)T >= m) break;
   }
@@ -552,14 +646,15 @@ static int plan_forward(int HW, int C, int prec, int64_t m, FwdPlan* plan) {
   plan->T = best;
   plan->npix = ((best * HW + 15) / 16) * 16;
   plan->lds = fwd_lds_bytes(plan->npix, pixb);
+  plan->nwave = NW;
   const int per_wave = (plan->npix / 16 + NW - 1) / NW;
-  plan->maxt = per_wave <= 2 ? 2 : 4;
+  plan->maxt = per_wave <= 2 ? 2 : (per_wave <= 4 ? 4 : 8);
   return FK_OK;
 }
 
-template <int PREC, int C, int MAXT, int ACT>
+template <int PREC, int C, int MAXT, int ACT, int NWAVE>
 static int launch_fwd(const FwdArgs& a, const FwdPlan& pl, hipStream_t s) {
-  auto kern = convar_fwd_kernel<PREC, C, MAXT, ACT>;
+  auto kern = convar_fwd_kernel<PREC, C, MAXT, ACT, NWAVE>;
   FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)pl.lds));
   const int64_t ngroups = (a.m_count + pl.T - 1) / pl.T;
@@ -567,19 +662,26 @@ static int launch_fwd(const FwdArgs& a, const FwdPlan& pl, hipStream_t s) {
   int64_t grid = (int64_t)device_cu_count() * per_cu;
   if (grid > ngroups) grid = ngroups;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(512), pl.lds, s, a);
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * NWAVE), pl.lds, s, a);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }
 
 template <int PREC, int C>
 static int dispatch_fwd_c(const FwdArgs& a, const FwdPlan& pl, int act, hipStream_t s) {
-  if (pl.maxt == 2) {
-    return act == FK_ACT_ELU ? launch_fwd<PREC, C, 2, FK_ACT_ELU>(a, pl, s)
-                             : launch_fwd<PREC, C, 2, FK_ACT_RELU>(a, pl, s);
+  if constexpr (PREC == FK_PREC_BF16 && C == 32) {
+    if (pl.maxt <= 4)
+      return act == FK_ACT_ELU ? launch_fwd<PREC, C, 4, FK_ACT_ELU, 4>(a, pl, s)
+                               : launch_fwd<PREC, C, 4, FK_ACT_RELU, 4>(a, pl, s);
+    return act == FK_ACT_ELU ? launch_fwd<PREC, C, 8, FK_ACT_ELU, 4>(a, pl, s)
+                             : launch_fwd<PREC, C, 8, FK_ACT_RELU, 4>(a, pl, s);
+  } else {
+    if (pl.maxt == 2)
+      return act == FK_ACT_ELU ? launch_fwd<PREC, C, 2, FK_ACT_ELU, 8>(a, pl, s)
+                               : launch_fwd<PREC, C, 2, FK_ACT_RELU, 8>(a, pl, s);
+    return act == FK_ACT_ELU ? launch_fwd<PREC, C, 4, FK_ACT_ELU, 8>(a, pl, s)
+                             : launch_fwd<PREC, C, 4, FK_ACT_RELU, 8>(a, pl, s);
   }
-  return act == FK_ACT_ELU ? launch_fwd<PREC, C, 4, FK_ACT_ELU>(a, pl, s)
-                           : launch_fwd<PREC, C, 4, FK_ACT_RELU>(a, pl, s);
 }
 
 bool convar_supported(int channels, int precision) {
