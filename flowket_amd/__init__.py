@@ -9,5 +9,6 @@ from .machines import ConvNetAutoregressive2D  # noqa: F401
 from .observables import BaseObservable, Observable, energy_moments  # noqa: F401
 from .samplers import Sampler, AutoregressiveSampler, FastAutoregressiveSampler  # noqa: F401
 from .optimization import VariationalMonteCarlo, MiniBatchGenerator, loss_for_energy_minimization  # noqa: F401
+from .optimizers import ComplexValuesStochasticReconfiguration, Adam, energy_gradient, conjugate_gradient  # noqa: F401
 
 __version__ = '0.1.0'

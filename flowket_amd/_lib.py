@@ -43,6 +43,16 @@ SIGNATURES = {
     'fk_ar_draw_site': (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P]),
     'fk_fast_sampler_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I32, _I64]),
     'fk_fast_sample': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I64, _P, _P, _P, _P, _P]),
+    'fk_convar_grad_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I64]),
+    'fk_convar_grad': (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I64, _P, _P, _P, _P, _P]),
+    'fk_o_mean': (ctypes.c_int, [_P, _P, _I64, _I64, _P, _P, _P]),
+    'fk_o_times_v': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I64, _P, _P, _P]),
+    'fk_oh_times_u': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I64, ctypes.c_double, ctypes.c_double, _P, _P,
+                                     _P, _P]),
+    'fk_sr_scratch_doubles': (_I64, []),
+    'fk_dot_f64': (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    'fk_axpy': (ctypes.c_int, [ctypes.c_double, _P, _P, _I64, _P]),
+    'fk_xpby': (ctypes.c_int, [_P, ctypes.c_double, _P, _I64, _P]),
 }
 
 _lib = None

@@ -1,0 +1,465 @@
+// Per-sample log-psi gradients O_k = d log psi(s) / d theta of
+// ConvNetAutoregressive2D (the quantity of Machine.predictions_jacobian,
+// machines/abstract_machine.py:24-28, re-paired into complex O_k as in
+// optimizers/complex_values_optimizer.py:8-9 and computed per example like
+// utils/jacobian.py:114-123).
+//
+// One workgroup per sample, FP32 throughout.  A forward pass stores every
+// post-activation tensor of the sample to a per-sample workspace; the backward
+// walks the causal blocks in reverse twice -- once with the cotangent of
+// Re log psi and once with that of Im log psi -- and writes d/d theta of both
+// into O_re and O_im (batch, P) in the canonical flat parameter order.
+// Derivatives of the activations are taken from the stored outputs
+// (relu' = [y > 0], elu' = y > 0 ? 1 : y + 1).
+#include "fk_common.h"
+#include "convar_layout.h"
+
+namespace fk {
+
+template <int ACT>
+__device__ __forceinline__ float bact(float x) {
+  if constexpr (ACT == FK_ACT_RELU) return fmaxf(x, 0.0f);
+  else return x > 0.0f ? x : expm1f(x);
+}
+template <int ACT>
+__device__ __forceinline__ float bdact(float y) {  // derivative from the output
+  if constexpr (ACT == FK_ACT_RELU) return y > 0.0f ? 1.0f : 0.0f;
+  else return y > 0.0f ? 1.0f : y + 1.0f;
+}
+
+struct BwdArgs {
+  const float* params;  // flat parameters (effective kernels), P floats
+  int H, W, HW, C, nb;
+  int64_t batch;
+  const int8_t* spins;  // (batch, HW)
+  float* work;          // per-workgroup activation store
+  int64_t work_floats;  // floats per workgroup
+  float* o_re;          // (batch, P)
+  float* o_im;
+  float* logpsi;        // (batch, 2) or null
+  int64_t P;
+};
+
+// Offsets of layer parameters in the flat vector.
+struct ParamOffsets {
+  int64_t kv, bv, kh, bh, kx, bx, ky, by, kc, bc;
+};
+__device__ __forceinline__ ParamOffsets param_offsets(int blk, int C) {
+  // block 0 has cin = 1; blocks >= 1 have cin = C
+  const int64_t blk0 = 9LL * C + C + 3LL * C + C + (int64_t)C * (C / 2) * 2 + C + 9LL * C * C + C;
+  const int64_t blkn = 9LL * C * C + C + 3LL * C * C + C + (int64_t)C * (C / 2) * 2 + C + 9LL * C * C + C;
+  const int64_t base = blk == 0 ? 0 : blk0 + (int64_t)(blk - 1) * blkn;
+  const int cin = blk == 0 ? 1 : C;
+  ParamOffsets o;
+  o.kv = base;
+  o.bv = o.kv + 9LL * cin * C;
+  o.kh = o.bv + C;
+  o.bh = o.kh + 3LL * cin * C;
+  o.kx = o.bh + C;
+  o.bx = o.kx + (int64_t)C * (C / 2);
+  o.ky = o.bx + C / 2;
+  o.by = o.ky + (int64_t)C * (C / 2);
+  o.kc = o.by + C / 2;
+  o.bc = o.kc + 9LL * C * C;
+  return o;
+}
+__device__ __forceinline__ int64_t head_offset(int nb, int C) {
+  const ParamOffsets o = param_offsets(nb - 1, C);
+  return o.bc + C;
+}
+
+// Tap offsets: vertical kxk rows -2..0 cols -1..1; horizontal 1xk cols -2..0;
+// h3 kxk rows -2..0 cols -2..0.
+__device__ __forceinline__ void tap_rc(int kind, int t, int& dr, int& dc) {
+  if (kind == 0) {
+    dr = t / 3 - 2;
+    dc = t % 3 - 1;
+  } else if (kind == 1) {
+    dr = 0;
+    dc = t - 2;
+  } else {
+    dr = t / 3 - 2;
+    dc = t % 3 - 2;
+  }
+}
+
+// out[p][co] = b[co] + sum_{tap,ci} K[tap][ci][co] * in[p + off][ci]
+// (cin channels; in/out are [HW][cs] with channel strides cs_in/cs_out).
+__device__ void conv_fwd(float* out, int cs_out, const float* in, int cs_in, int cin, int cout, const float* K,
+                         const float* b, int kind, int ntap, int H, int W) {
+  const int HW = H * W;
+  for (int idx = threadIdx.x; idx < HW * cout; idx += blockDim.x) {
+    const int p = idx / cout, co = idx - (idx / cout) * cout;
+    const int i = p / W, j = p - (p / W) * W;
+    float acc = b ? b[co] : 0.0f;
+    for (int t = 0; t < ntap; ++t) {
+      int dr, dc;
+      tap_rc(kind, t, dr, dc);
+      const int r = i + dr, c = j + dc;
+      if (r < 0 || r >= H || c < 0 || c >= W) continue;
+      const float* src = in + (r * W + c) * cs_in;
+      const float* k = K + (int64_t)t * cin * cout + co;
+      for (int ci = 0; ci < cin; ++ci) acc += k[(int64_t)ci * cout] * src[ci];
+    }
+    out[p * cs_out + co] = acc;
+  }
+}
+
+// din[q][ci] (+)= sum_{tap,co} K[tap][ci][co] * dout[q - off][co]
+__device__ void conv_bwd_data(float* din, int cs_in, bool accumulate, const float* dout, int cs_out, int cin,
+                              int cout, const float* K, int kind, int ntap, int H, int W) {
+  const int HW = H * W;
+  for (int idx = threadIdx.x; idx < HW * cin; idx += blockDim.x) {
+    const int q = idx / cin, ci = idx - (idx / cin) * cin;
+    const int i = q / W, j = q - (q / W) * W;
+    float acc = 0.0f;
+    for (int t = 0; t < ntap; ++t) {
+      int dr, dc;
+      tap_rc(kind, t, dr, dc);
+      const int r = i - dr, c = j - dc;
+      if (r < 0 || r >= H || c < 0 || c >= W) continue;
+      const float* g = dout + (r * W + c) * cs_out;
+      const float* k = K + ((int64_t)t * cin + ci) * cout;
+      for (int co = 0; co < cout; ++co) acc += k[co] * g[co];
+    }
+    if (accumulate) din[q * cs_in + ci] += acc;
+    else din[q * cs_in + ci] = acc;
+  }
+}
+
+// dK[tap][ci][co] = sum_p in[p + off][ci] * dout[p][co]; db[co] = sum_p dout[p][co]
+__device__ void conv_bwd_weight(float* dK, float* db, const float* in, int cs_in, const float* dout, int cs_out,
+                                int cin, int cout, int kind, int ntap, int H, int W) {
+  const int HW = H * W;
+  const int nk = ntap * cin * cout;
+  for (int idx = threadIdx.x; idx < nk + cout; idx += blockDim.x) {
+    if (idx >= nk) {
+      const int co = idx - nk;
+      float acc = 0.0f;
+      for (int p = 0; p < HW; ++p) acc += dout[p * cs_out + co];
+      db[co] = acc;
+      continue;
+    }
+    const int t = idx / (cin * cout);
+    const int rem = idx - t * cin * cout;
+    const int ci = rem / cout, co = rem - (rem / cout) * cout;
+    int dr, dc;
+    tap_rc(kind, t, dr, dc);
+    float acc = 0.0f;
+    for (int i = 0; i < H; ++i) {
+      const int r = i + dr;
+      if (r < 0 || r >= H) continue;
+      for (int j = 0; j < W; ++j) {
+        const int c = j + dc;
+        if (c < 0 || c >= W) continue;
+        acc += in[(r * W + c) * cs_in + ci] * dout[(i * W + j) * cs_out + co];
+      }
+    }
+    dK[idx] = acc;
+  }
+}
+
+// Per-sample workspace layout (floats): spins HW, then per block
+// VA (act(v')), XA (act(hx)), CAT (x2|y2), VO (vertical output), HO
+// (horizontal output), each HW*C.
+__device__ __forceinline__ float* ws_tensor(float* ws, int HW, int C, int blk, int which) {
+  return ws + HW + ((int64_t)blk * 5 + which) * HW * C;
+}
+enum { T_VA = 0, T_XA = 1, T_CAT = 2, T_VO = 3, T_HO = 4 };
+
+template <int ACT>
+__global__ __launch_bounds__(256) void convar_grad_kernel(BwdArgs a) {
+  extern __shared__ float sm[];
+  const int H = a.H, W = a.W, HW = a.HW, C = a.C, nb = a.nb;
+  const int F = nb - 1;
+  const int64_t b = blockIdx.x;
+  if (b >= a.batch) return;
+  float* ws = a.work + (int64_t)blockIdx.x * a.work_floats;
+  // LDS: gradient buffers
+  float* dV = sm;                // HW*C  gradient w.r.t. the vertical input of the current block
+  float* dH = dV + HW * C;       // HW*C
+  float* dVr = dH + HW * C;      // HW*C  pending residual gradient (vertical)
+  float* dHr = dVr + HW * C;     // HW*C
+  float* t0 = dHr + HW * C;      // HW*C  scratch
+  float* t1 = t0 + HW * C;       // HW*C  scratch
+  float* dz = t1 + HW * C;       // HW*4
+  const float* prm = a.params;
+
+  // ------------------------------------------------------------ forward
+  float* sp = ws;
+  for (int q = threadIdx.x; q < HW; q += blockDim.x) sp[q] = (float)a.spins[b * HW + q];
+  __syncthreads();
+  for (int blk = 0; blk < nb; ++blk) {
+    const ParamOffsets o = param_offsets(blk, C);
+    const bool first = blk == 0, fin = blk == F;
+    const bool kb = !first && !fin && ((blk & 1) == 0);  // second block of a residual pair
+    const int cin = first ? 1 : C;
+    const float* Vin = first ? sp : ws_tensor(ws, HW, C, blk - 1, T_VO);
+    const float* Hin = first ? sp : ws_tensor(ws, HW, C, blk - 1, T_HO);
+    const int cs = first ? 1 : C;
+    float* VA = ws_tensor(ws, HW, C, blk, T_VA);
+    float* XA = ws_tensor(ws, HW, C, blk, T_XA);
+    float* CAT = ws_tensor(ws, HW, C, blk, T_CAT);
+    float* VO = ws_tensor(ws, HW, C, blk, T_VO);
+    float* HO = ws_tensor(ws, HW, C, blk, T_HO);
+    // v' -> t0, hx -> t1
+    conv_fwd(t0, C, Vin, cs, cin, C, prm + o.kv, prm + o.bv, 0, 9, H, W);
+    conv_fwd(t1, C, Hin, cs, cin, C, prm + o.kh, prm + o.bh, 1, 3, H, W);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+      const float v = t0[idx];
+      VA[idx] = bact<ACT>(v);
+      XA[idx] = bact<ACT>(t1[idx]);
+      if (!fin) {
+        float vo;
+        if (kb) vo = bact<ACT>(ws_tensor(ws, HW, C, blk - 2, T_VO)[idx] + v);
+        else vo = bact<ACT>(v);
+        VO[idx] = vo;
+      }
+    }
+    __syncthreads();
+    // x2 = act(1x1(shift?(xa))), y2 = act(1x1(downshift(va)))
+    for (int idx = threadIdx.x; idx < HW * (C / 2); idx += blockDim.x) {
+      const int p = idx / (C / 2), co = idx - (idx / (C / 2)) * (C / 2);
+      const int i = p / W, j = p - (p / W) * W;
+      float x = prm[o.bx + co];
+      if (!fin || j > 0) {
+        const float* src = XA + (fin ? p - 1 : p) * C;
+        for (int ci = 0; ci < C; ++ci) x += prm[o.kx + (int64_t)ci * (C / 2) + co] * src[ci];
+      }
+      float y = prm[o.by + co];
+      if (i > 0) {
+        const float* src = VA + (p - W) * C;
+        for (int ci = 0; ci < C; ++ci) y += prm[o.ky + (int64_t)ci * (C / 2) + co] * src[ci];
+      }
+      CAT[p * C + co] = bact<ACT>(x);
+      CAT[p * C + C / 2 + co] = bact<ACT>(y);
+    }
+    __syncthreads();
+    conv_fwd(t0, C, CAT, C, C, C, prm + o.kc, prm + o.bc, 2, 9, H, W);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+      const float h = t0[idx];
+      float ho;
+      if (kb) ho = bact<ACT>(ws_tensor(ws, HW, C, blk - 2, T_HO)[idx] + h);
+      else ho = bact<ACT>(h);
+      HO[idx] = ho;
+    }
+    __syncthreads();
+  }
+
+  // head: z = Khead act(h_F) + bhead; cotangents of Re / Im log psi
+  const int64_t ho = head_offset(nb, C);
+  const float* HF = ws_tensor(ws, HW, C, F, T_HO);
+  __shared__ float s_lp[2];
+  if (threadIdx.x == 0) s_lp[0] = s_lp[1] = 0.0f;
+  __syncthreads();
+  for (int part = 0; part < 2; ++part) {
+    // per-pixel cotangent of the 4 logits
+    for (int p = threadIdx.x; p < HW; p += blockDim.x) {
+      float z[4];
+      for (int k = 0; k < 4; ++k) {
+        float acc = prm[ho + 4 * C + k];
+        for (int ci = 0; ci < C; ++ci) acc += prm[ho + ci * 4 + k] * HF[p * C + ci];
+        z[k] = acc;
+      }
+      const float mx = fmaxf(z[0], z[1]);
+      const float e0 = expf(2.0f * (z[0] - mx)), e1 = expf(2.0f * (z[1] - mx));
+      const float p0 = e0 / (e0 + e1), p1 = e1 / (e0 + e1);
+      const bool up = sp[p] > 0.0f;
+      if (part == 0) {
+        dz[p * 4 + 0] = (up ? 1.0f : 0.0f) - p0;
+        dz[p * 4 + 1] = (up ? 0.0f : 1.0f) - p1;
+        dz[p * 4 + 2] = 0.0f;
+        dz[p * 4 + 3] = 0.0f;
+        const float lse = mx + 0.5f * logf(e0 + e1);
+        atomicAdd(&s_lp[0], (up ? z[0] : z[1]) - lse);
+        atomicAdd(&s_lp[1], up ? z[2] : z[3]);
+      } else {
+        dz[p * 4 + 0] = 0.0f;
+        dz[p * 4 + 1] = 0.0f;
+        dz[p * 4 + 2] = up ? 1.0f : 0.0f;
+        dz[p * 4 + 3] = up ? 0.0f : 1.0f;
+      }
+    }
+    __syncthreads();
+    float* O = (part == 0 ? a.o_re : a.o_im) + b * a.P;
+    // head weight / bias gradients and d act(h_F)
+    for (int idx = threadIdx.x; idx < 4 * C + 4; idx += blockDim.x) {
+      float acc = 0.0f;
+      if (idx < 4 * C) {
+        const int ci = idx / 4, k = idx - (idx / 4) * 4;
+        for (int p = 0; p < HW; ++p) acc += HF[p * C + ci] * dz[p * 4 + k];
+      } else {
+        for (int p = 0; p < HW; ++p) acc += dz[p * 4 + idx - 4 * C];
+      }
+      O[ho + idx] = acc;
+    }
+    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+      const int p = idx / C, ci = idx - (idx / C) * C;
+      float g = 0.0f;
+      for (int k = 0; k < 4; ++k) g += prm[ho + ci * 4 + k] * dz[p * 4 + k];
+      dH[idx] = g;   // gradient w.r.t. HO of the final block
+      dV[idx] = 0.0f;  // the final block's vertical output is discarded
+      dVr[idx] = 0.0f;
+      dHr[idx] = 0.0f;
+    }
+    __syncthreads();
+
+    // ---------------------------------------------------------- backward
+    for (int blk = F; blk >= 0; --blk) {
+      const ParamOffsets o = param_offsets(blk, C);
+      const bool first = blk == 0, fin = blk == F;
+      const bool kb = !first && !fin && ((blk & 1) == 0);
+      const bool ka = !first && !fin && ((blk & 1) == 1);
+      const int cin = first ? 1 : C;
+      const float* Vin = first ? sp : ws_tensor(ws, HW, C, blk - 1, T_VO);
+      const float* Hin = first ? sp : ws_tensor(ws, HW, C, blk - 1, T_HO);
+      const int cs = first ? 1 : C;
+      const float* VA = ws_tensor(ws, HW, C, blk, T_VA);
+      const float* XA = ws_tensor(ws, HW, C, blk, T_XA);
+      const float* CAT = ws_tensor(ws, HW, C, blk, T_CAT);
+      const float* VO = ws_tensor(ws, HW, C, blk, T_VO);
+      const float* HO = ws_tensor(ws, HW, C, blk, T_HO);
+      // dh' -> t1 ; dv' -> t0 (from the vertical output; 0 for the final block)
+      for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+        const float gh = dH[idx] * bdact<ACT>(HO[idx]);
+        t1[idx] = gh;
+        float gv = 0.0f;
+        if (!fin) gv = dV[idx] * bdact<ACT>(VO[idx]);
+        t0[idx] = gv;
+        if (kb) {  // the residual inputs receive the pre-activation gradient
+          dVr[idx] = gv;
+          dHr[idx] = gh;
+        }
+      }
+      __syncthreads();
+      // h3conv: weights, bias, and d cat -> dV (reused as scratch)
+      conv_bwd_weight(O + o.kc, O + o.bc, CAT, C, t1, C, C, C, 2, 9, H, W);
+      conv_bwd_data(dV, C, false, t1, C, C, C, prm + o.kc, 2, 9, H, W);
+      __syncthreads();
+      // d x2p (in dV[:, :C/2]) and d y2p (dV[:, C/2:]) through act
+      for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) dV[idx] *= bdact<ACT>(CAT[idx]);
+      __syncthreads();
+      // x1x1 / y1x1 weight and bias gradients
+      for (int idx = threadIdx.x; idx < 2 * (C * (C / 2) + C / 2); idx += blockDim.x) {
+        const int which = idx / (C * (C / 2) + C / 2);  // 0: x, 1: y
+        const int r = idx - which * (C * (C / 2) + C / 2);
+        float acc = 0.0f;
+        if (r < C * (C / 2)) {
+          const int ci = r / (C / 2), co = r - (r / (C / 2)) * (C / 2);
+          for (int p = 0; p < HW; ++p) {
+            const int i = p / W, j = p - (p / W) * W;
+            float in;
+            if (which == 0) in = (fin ? (j > 0 ? XA[(p - 1) * C + ci] : 0.0f) : XA[p * C + ci]);
+            else in = (i > 0 ? VA[(p - W) * C + ci] : 0.0f);
+            acc += in * dV[p * C + which * (C / 2) + co];
+          }
+          O[(which == 0 ? o.kx : o.ky) + r] = acc;
+        } else {
+          const int co = r - C * (C / 2);
+          for (int p = 0; p < HW; ++p) acc += dV[p * C + which * (C / 2) + co];
+          O[(which == 0 ? o.bx : o.by) + co] = acc;
+        }
+      }
+      // d xa (with the RightShift adjoint) -> dH, d va (DownShift adjoint)
+      // added into dv' (t0)
+      for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+        const int p = idx / C, ci = idx - (idx / C) * C;
+        const int i = p / W, j = p - (p / W) * W;
+        float gx = 0.0f;
+        const int px = fin ? p + 1 : p;
+        if (!fin || j + 1 < W)
+          for (int co = 0; co < C / 2; ++co) gx += prm[o.kx + (int64_t)ci * (C / 2) + co] * dV[px * C + co];
+        float gy = 0.0f;
+        if (i + 1 < H)
+          for (int co = 0; co < C / 2; ++co)
+            gy += prm[o.ky + (int64_t)ci * (C / 2) + co] * dV[(p + W) * C + C / 2 + co];
+        dH[idx] = gx * bdact<ACT>(XA[idx]);   // d hx
+        t0[idx] += gy * bdact<ACT>(VA[idx]);  // d v'
+      }
+      __syncthreads();
+      // horizontal 1xk and vertical kxk: weights, biases
+      conv_bwd_weight(O + o.kh, O + o.bh, Hin, cs, dH, C, cin, C, 1, 3, H, W);
+      conv_bwd_weight(O + o.kv, O + o.bv, Vin, cs, t0, C, cin, C, 0, 9, H, W);
+      if (first) break;
+      // gradients w.r.t. the block inputs (= the previous block's outputs)
+      conv_bwd_data(t1, C, false, dH, C, C, C, prm + o.kh, 1, 3, H, W);
+      __syncthreads();
+      conv_bwd_data(dV, C, false, t0, C, C, C, prm + o.kv, 0, 9, H, W);
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+        float gh = t1[idx], gv = dV[idx];
+        if (ka) {  // first block of a residual pair: add the residual path
+          gh += dHr[idx];
+          gv += dVr[idx];
+        }
+        dH[idx] = gh;
+        dV[idx] = gv;
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && a.logpsi) {
+    a.logpsi[2 * b] = s_lp[0];
+    a.logpsi[2 * b + 1] = s_lp[1];
+  }
+}
+
+int64_t grad_work_floats(int HW, int C, int nb) { return (int64_t)HW + (int64_t)nb * 5 * HW * C; }
+
+}  // namespace fk
+
+using namespace fk;
+
+extern "C" {
+
+int64_t fk_convar_grad_workspace_bytes(int32_t H, int32_t W, int32_t depth, int32_t channels, int64_t batch) {
+  if (H < 1 || W < 1 || depth < 1 || channels < 2 || (channels % 2) || batch < 0) return -1;
+  return batch * grad_work_floats(H * W, channels, num_causal_blocks(depth)) * (int64_t)sizeof(float);
+}
+
+int fk_convar_grad(const float* params, int32_t H, int32_t W, int32_t depth, int32_t channels, int32_t activation,
+                   const int8_t* spins, int64_t batch, void* workspace, float* o_re, float* o_im, float* logpsi,
+                   void* stream) {
+  FK_REQUIRE(H >= 1 && W >= 1 && depth >= 1 && channels >= 2 && channels % 2 == 0, "fk_convar_grad: bad shape");
+  FK_REQUIRE(activation == FK_ACT_RELU || activation == FK_ACT_ELU, "fk_convar_grad: bad activation");
+  if (batch <= 0) return FK_OK;
+  FK_REQUIRE(params && spins && workspace && o_re && o_im, "fk_convar_grad: null pointer");
+  FK_REQUIRE(batch <= 0x7fffffff, "fk_convar_grad: batch too large");
+  BwdArgs a;
+  a.params = params;
+  a.H = H;
+  a.W = W;
+  a.HW = H * W;
+  a.C = channels;
+  a.nb = num_causal_blocks(depth);
+  a.batch = batch;
+  a.spins = spins;
+  a.work = reinterpret_cast<float*>(workspace);
+  a.work_floats = grad_work_floats(a.HW, channels, a.nb);
+  a.o_re = o_re;
+  a.o_im = o_im;
+  a.logpsi = logpsi;
+  // P = flat parameter count
+  const int64_t C = channels;
+  a.P = (9 * C + C + 3 * C + C + C * (C / 2) * 2 + C + 9 * C * C + C) +
+        (int64_t)(a.nb - 1) * (9 * C * C + C + 3 * C * C + C + C * (C / 2) * 2 + C + 9 * C * C + C) + 4 * C + 4;
+  const size_t lds = ((size_t)6 * a.HW * channels + 4 * a.HW) * sizeof(float);
+  FK_REQUIRE(lds <= 160 * 1024, "fk_convar_grad: lattice too large for the LDS gradient buffers");
+  hipStream_t s = (hipStream_t)stream;
+  if (activation == FK_ACT_ELU) {
+    FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(convar_grad_kernel<FK_ACT_ELU>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(convar_grad_kernel<FK_ACT_ELU>, dim3((unsigned)batch), dim3(256), lds, s, a);
+  } else {
+    FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(convar_grad_kernel<FK_ACT_RELU>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(convar_grad_kernel<FK_ACT_RELU>, dim3((unsigned)batch), dim3(256), lds, s, a);
+  }
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,242 @@
+// Stochastic reconfiguration on the implicit S = Re[(O-Obar)^H (O-Obar)]/B +
+// shift*I (optimizers/stochastic_reconfiguration/optimizer.py:69-92 and its
+// force :94-109), never forming S: O (B, P) is stored as two fp32 arrays
+// (re, im), the two matvecs read it once each (HBM-bound), products are FP32
+// and every reduction across a row or a vector is FP64.
+#include "fk_common.h"
+
+namespace fk {
+
+// mean over rows of O_re and O_im (column means Obar)
+__global__ void o_mean_kernel(const float* __restrict__ ore, const float* __restrict__ oim, int64_t B, int64_t P,
+                              float* __restrict__ mre, float* __restrict__ mim) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  double sr = 0.0, si = 0.0;
+  for (int64_t b = 0; b < B; ++b) {
+    sr += ore[b * P + k];
+    si += oim[b * P + k];
+  }
+  mre[k] = (float)(sr / (double)B);
+  mim[k] = (float)(si / (double)B);
+}
+
+__global__ __launch_bounds__(1024) void dot_f64_kernel(const float* __restrict__ a, const float* __restrict__ b,
+                                                       int64_t n, double* __restrict__ out) {
+  __shared__ double s[1024];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += (double)a[i] * (double)b[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void sum_f64_kernel(const double* __restrict__ in, int n, double* __restrict__ out) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += in[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// u_b = sum_k (O_bk - Obar_k) v_k = O_b.v - Obar.v (complex).  A workgroup
+// covers ROWS rows so each v_k is read once per ROWS rows; ov_c = Obar.v
+// (re, im) in FP64 is subtracted at the end.
+constexpr int OV_ROWS = 8;
+__global__ __launch_bounds__(256) void o_times_v_kernel(const float* __restrict__ ore, const float* __restrict__ oim,
+                                                        const double* __restrict__ ov_c, const float* __restrict__ v,
+                                                        int64_t B, int64_t P, float* __restrict__ u) {
+  __shared__ double s_r[OV_ROWS][256], s_i[OV_ROWS][256];
+  const int64_t b0 = (int64_t)blockIdx.x * OV_ROWS;
+  double dr[OV_ROWS], di[OV_ROWS];
+  float ar[OV_ROWS], ai[OV_ROWS];
+#pragma unroll
+  for (int r = 0; r < OV_ROWS; ++r) dr[r] = di[r] = 0.0, ar[r] = ai[r] = 0.0f;
+  int cnt = 0;
+  for (int64_t k = threadIdx.x; k < P; k += blockDim.x) {
+    const float vk = v[k];
+#pragma unroll
+    for (int r = 0; r < OV_ROWS; ++r) {
+      if (b0 + r < B) {
+        ar[r] += ore[(b0 + r) * P + k] * vk;
+        ai[r] += oim[(b0 + r) * P + k] * vk;
+      }
+    }
+    if (++cnt == 64) {  // fold fp32 partials into fp64 every 64 terms
+#pragma unroll
+      for (int r = 0; r < OV_ROWS; ++r) dr[r] += ar[r], di[r] += ai[r], ar[r] = ai[r] = 0.0f;
+      cnt = 0;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < OV_ROWS; ++r) {
+    s_r[r][threadIdx.x] = dr[r] + ar[r];
+    s_i[r][threadIdx.x] = di[r] + ai[r];
+  }
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int r = 0; r < OV_ROWS; ++r) {
+        s_r[r][threadIdx.x] += s_r[r][threadIdx.x + s];
+        s_i[r][threadIdx.x] += s_i[r][threadIdx.x + s];
+      }
+    }
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < OV_ROWS && b0 + threadIdx.x < B) {
+    const int r = threadIdx.x;
+    u[2 * (b0 + r)] = (float)(s_r[r][0] - ov_c[0]);
+    u[2 * (b0 + r) + 1] = (float)(s_i[r][0] - ov_c[1]);
+  }
+}
+
+__global__ __launch_bounds__(1024) void sum_u_kernel(const float* __restrict__ u, int64_t B, double* __restrict__ out) {
+  __shared__ double s_r[1024], s_i[1024];
+  double r = 0.0, i = 0.0;
+  for (int64_t b = threadIdx.x; b < B; b += blockDim.x) {
+    r += u[2 * b];
+    i += u[2 * b + 1];
+  }
+  s_r[threadIdx.x] = r;
+  s_i[threadIdx.x] = i;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      s_r[threadIdx.x] += s_r[threadIdx.x + s];
+      s_i[threadIdx.x] += s_i[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s_r[0];
+    out[1] = s_i[0];
+  }
+}
+
+// out_k = scale * Re[sum_b conj(O_bk - Obar_k) u_b] + shift * v_k
+__global__ void oh_times_u_kernel(const float* __restrict__ ore, const float* __restrict__ oim,
+                                  const float* __restrict__ mre, const float* __restrict__ mim,
+                                  const float* __restrict__ u, const double* __restrict__ usum, int64_t B, int64_t P,
+                                  double scale, double shift, const float* __restrict__ v, float* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  double acc = 0.0;
+  float part = 0.0f;
+  int cnt = 0;
+  for (int64_t b = 0; b < B; ++b) {
+    part += ore[b * P + k] * u[2 * b] + oim[b * P + k] * u[2 * b + 1];
+    if (++cnt == 64) {
+      acc += part;
+      part = 0.0f;
+      cnt = 0;
+    }
+  }
+  acc += part;
+  acc -= (double)mre[k] * usum[0] + (double)mim[k] * usum[1];
+  double r = scale * acc;
+  if (v) r += shift * (double)v[k];
+  out[k] = (float)r;
+}
+
+// y = a*x + y (fp32 vectors, fp64 coefficient)
+__global__ void axpy_kernel(double alpha, const float* __restrict__ x, float* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)((double)y[i] + alpha * (double)x[i]);
+}
+// p = r + beta*p
+__global__ void xpby_kernel(const float* __restrict__ r, double beta, float* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (float)((double)r[i] + beta * (double)p[i]);
+}
+
+}  // namespace fk
+
+using namespace fk;
+
+extern "C" {
+
+int fk_o_mean(const float* o_re, const float* o_im, int64_t batch, int64_t n_params, float* mean_re, float* mean_im,
+              void* stream) {
+  FK_REQUIRE(o_re && o_im && mean_re && mean_im, "fk_o_mean: null pointer");
+  FK_REQUIRE(batch > 0 && n_params > 0, "fk_o_mean: empty");
+  hipLaunchKernelGGL(o_mean_kernel, dim3(div_up(n_params, 256)), dim3(256), 0, (hipStream_t)stream, o_re, o_im, batch,
+                     n_params, mean_re, mean_im);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_o_times_v(const float* o_re, const float* o_im, const float* mean_re, const float* mean_im, const float* v,
+                 int64_t batch, int64_t n_params, float* u, double* scratch, void* stream) {
+  FK_REQUIRE(o_re && o_im && mean_re && mean_im && v && u && scratch, "fk_o_times_v: null pointer");
+  FK_REQUIRE(batch > 0 && n_params > 0, "fk_o_times_v: bad sizes");
+  hipStream_t s = (hipStream_t)stream;
+  // scratch: [0..255] partials, [256] Obar_re.v, [257] Obar_im.v
+  hipLaunchKernelGGL(dot_f64_kernel, dim3(256), dim3(1024), 0, s, mean_re, v, n_params, scratch);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, s, scratch, 256, scratch + 256);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(dot_f64_kernel, dim3(256), dim3(1024), 0, s, mean_im, v, n_params, scratch);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, s, scratch, 256, scratch + 257);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(o_times_v_kernel, dim3(div_up(batch, OV_ROWS)), dim3(256), 0, s, o_re, o_im, scratch + 256, v,
+                     batch, n_params, u);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_oh_times_u(const float* o_re, const float* o_im, const float* mean_re, const float* mean_im, const float* u,
+                  int64_t batch, int64_t n_params, double scale, double shift, const float* v, float* out,
+                  double* scratch2, void* stream) {
+  FK_REQUIRE(o_re && o_im && mean_re && mean_im && u && out && scratch2, "fk_oh_times_u: null pointer");
+  FK_REQUIRE(batch > 0 && n_params > 0, "fk_oh_times_u: bad sizes");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(sum_u_kernel, dim3(1), dim3(1024), 0, s, u, batch, scratch2);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(oh_times_u_kernel, dim3(div_up(n_params, 256)), dim3(256), 0, s, o_re, o_im, mean_re, mean_im, u,
+                     scratch2, batch, n_params, scale, shift, v, out);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int64_t fk_sr_scratch_doubles(void) { return 258; }
+
+int fk_dot_f64(const float* a, const float* b, int64_t n, double* scratch, double* out, void* stream) {
+  FK_REQUIRE(a && b && scratch && out, "fk_dot_f64: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const int blocks = 256;
+  hipLaunchKernelGGL(dot_f64_kernel, dim3(blocks), dim3(1024), 0, s, a, b, n, scratch);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, s, scratch, blocks, out);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_axpy(double alpha, const float* x, float* y, int64_t n, void* stream) {
+  FK_REQUIRE(x && y, "fk_axpy: null pointer");
+  if (n <= 0) return FK_OK;
+  hipLaunchKernelGGL(axpy_kernel, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, alpha, x, y, n);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_xpby(const float* r, double beta, float* p, int64_t n, void* stream) {
+  FK_REQUIRE(r && p, "fk_xpby: null pointer");
+  if (n <= 0) return FK_OK;
+  hipLaunchKernelGGL(xpby_kernel, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, r, beta, p, n);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+}  // extern "C"

@@ -246,3 +246,25 @@ class ConvNetAutoregressive2D:
 
     def count_params(self):
         return self.n_params
+
+    def per_sample_gradients(self, x, workspace=None, stream=None):
+        """O = d log psi / d theta per sample (machines/abstract_machine.py:24-28,
+        utils/jacobian.py:114-123) through fk_convar_grad: returns (o_re, o_im,
+        logpsi) with o_* (B, P) float32 in the flat parameter order and logpsi
+        (B, 2) float32 (FP32 forward of the gradient kernel)."""
+        if self.weights_normalization:
+            raise NotImplementedError('per-sample gradients are computed w.r.t. the effective kernels; '
+                                      'build the machine with weights_normalization=False')
+        _lib.require_gpu()
+        spins = self._spins(x)
+        B = spins.shape[0]
+        o_re = torch.empty((B, self.n_params), dtype=torch.float32, device=self.device)
+        o_im = torch.empty((B, self.n_params), dtype=torch.float32, device=self.device)
+        lp = torch.empty((B, 2), dtype=torch.float32, device=self.device)
+        nbytes = _lib.load().fk_convar_grad_workspace_bytes(self.H, self.W, self.depth, self.num_of_channels, B)
+        if workspace is None or workspace.numel() < nbytes:
+            workspace = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.device)
+        _lib.call('fk_convar_grad', _lib.ptr(self.params), self.H, self.W, self.depth, self.num_of_channels,
+                  _ACT[self.activation], _lib.ptr(spins), B, _lib.ptr(workspace), _lib.ptr(o_re), _lib.ptr(o_im),
+                  _lib.ptr(lp), _lib.stream_ptr(stream))
+        return o_re, o_im, lp

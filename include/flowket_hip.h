@@ -166,6 +166,52 @@ int fk_fast_sample(const void* wpacked, const float* aux, int32_t precision,
                    int32_t activation, int64_t batch, const double* uniforms,
                    void* workspace, int8_t* out, float* cond_logp, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Per-sample log-psi gradients O = d log psi / d theta (real parameters,
+ * complex log psi): Machine.predictions_jacobian (machines/abstract_machine.py
+ * :24-28) re-paired as complex O_k (optimizers/complex_values_optimizer.py:
+ * 8-9), per example as utils/jacobian.py:114-123.  params: flat FP32
+ * parameters (effective kernels); o_re / o_im (batch, P) FP32 in the flat
+ * order; logpsi (batch, 2) or NULL.  FP32 throughout.  `workspace` holds
+ * fk_convar_grad_workspace_bytes(...) bytes.
+ * ------------------------------------------------------------------------- */
+int64_t fk_convar_grad_workspace_bytes(int32_t H, int32_t W, int32_t depth,
+                                       int32_t channels, int64_t batch);
+int fk_convar_grad(const float* params, int32_t H, int32_t W, int32_t depth,
+                   int32_t channels, int32_t activation, const int8_t* spins,
+                   int64_t batch, void* workspace, float* o_re, float* o_im,
+                   float* logpsi, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Stochastic reconfiguration on the implicit S = Re[(O-Obar)^H (O-Obar)]/B
+ * + shift*I (optimizers/stochastic_reconfiguration/optimizer.py:69-92, force
+ * :94-109; CG of linear_equations.py:34-137 is driven by the host with these
+ * pieces).  O as fp32 (re, im) arrays (batch, n_params); products FP32,
+ * reductions FP64.
+ *   fk_o_mean:     Obar = column means.
+ *   fk_o_times_v:  u (batch, 2) = (O - Obar) v, v real (n_params).
+ *   fk_oh_times_u: out = scale * Re[(O - Obar)^H u] + shift * v (v may be
+ *                  NULL: no shift term); scratch2 holds 2 doubles.
+ *   fk_dot_f64:    *out = sum a_i b_i in FP64; scratch holds
+ *                  fk_sr_scratch_doubles() doubles (also the scratch of
+ *                  fk_o_times_v).
+ *   fk_axpy:       y += alpha x.   fk_xpby: p = r + beta p.
+ * ------------------------------------------------------------------------- */
+int fk_o_mean(const float* o_re, const float* o_im, int64_t batch,
+              int64_t n_params, float* mean_re, float* mean_im, void* stream);
+int fk_o_times_v(const float* o_re, const float* o_im, const float* mean_re,
+                 const float* mean_im, const float* v, int64_t batch,
+                 int64_t n_params, float* u, double* scratch, void* stream);
+int fk_oh_times_u(const float* o_re, const float* o_im, const float* mean_re,
+                  const float* mean_im, const float* u, int64_t batch,
+                  int64_t n_params, double scale, double shift, const float* v,
+                  float* out, double* scratch2, void* stream);
+int64_t fk_sr_scratch_doubles(void);
+int fk_dot_f64(const float* a, const float* b, int64_t n, double* scratch,
+               double* out, void* stream);
+int fk_axpy(double alpha, const float* x, float* y, int64_t n, void* stream);
+int fk_xpby(const float* r, double beta, float* p, int64_t n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
