@@ -5,9 +5,10 @@ C ABI in include/flowket_hip.h); this package is the Python host side that
 mirrors FlowKet's operator / machine / sampler / VMC classes.
 """
 from .operators import Operator, OperatorOnGrid, cube_shape, Heisenberg, Ising, J1J2, j1j2_edges  # noqa: F401
-from .machines import ConvNetAutoregressive2D  # noqa: F401
+from .machines import ConvNetAutoregressive2D, RBM  # noqa: F401
 from .observables import BaseObservable, Observable, energy_moments  # noqa: F401
-from .samplers import Sampler, AutoregressiveSampler, FastAutoregressiveSampler  # noqa: F401
+from .samplers import (Sampler, AutoregressiveSampler, FastAutoregressiveSampler,  # noqa: F401
+                       MetropolisHastingsSampler, MetropolisHastingsLocal, MetropolisHastingsExchange)
 from .optimization import VariationalMonteCarlo, MiniBatchGenerator, loss_for_energy_minimization  # noqa: F401
 from .optimizers import ComplexValuesStochasticReconfiguration, Adam, energy_gradient, conjugate_gradient  # noqa: F401
 

@@ -53,6 +53,11 @@ SIGNATURES = {
     'fk_dot_f64': (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
     'fk_axpy': (ctypes.c_int, [ctypes.c_double, _P, _P, _I64, _P]),
     'fk_xpby': (ctypes.c_int, [_P, ctypes.c_double, _P, _I64, _P]),
+    'fk_rbm_log_psi': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _I64, _I32, _P, _P, _P, _P]),
+    'fk_mh_rbm': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I64, _I64, _I64, _I64, _P,
+                                 _P, _P, _P, _P, _P, _P, ctypes.c_uint64, _P]),
+    'fk_mh_propose': (ctypes.c_int, [_P, _I64, _I32, _I32, _I32, _P, _P, _P, _P]),
+    'fk_mh_accept': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I32, _P, _P]),
 }
 
 _lib = None

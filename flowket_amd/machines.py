@@ -268,3 +268,92 @@ class ConvNetAutoregressive2D:
                   _ACT[self.activation], _lib.ptr(spins), B, _lib.ptr(workspace), _lib.ptr(o_re), _lib.ptr(o_im),
                   _lib.ptr(lp), _lib.stream_ptr(stream))
         return o_re, o_im, lp
+
+
+class RBM:
+    """Complex RBM (machines/rbm.py:12-48): log psi = sum_j a_j s_j +
+    sum_h lncosh(b_h + sum_j W_jh s_j), hidden units alpha * N
+    (multiply_units_by_input_dim, layers/complex/dense.py:27-28).  Each complex
+    parameter is stored as (real, imag) and used as real - 1j*imag
+    (layers/complex/base_layer.py:30-35); the flat parameter order is the
+    Keras weight order W_re, W_im, b_re, b_im, a_re, a_im.  Initial values are
+    N(0, stddev^2) for every real and imaginary part (rbm.py:23,
+    layers/complex/initializers.py:137-145) from a seeded generator."""
+
+    def __init__(self, keras_input_layer, alpha=1, stddev=1.0, use_float64_ops=False, seed=0, device=None,
+                 params=None):
+        if use_float64_ops:
+            raise NotImplementedError('the RBM kernels compute in FP32 (use_float64_ops=False)')
+        shape = getattr(keras_input_layer, 'shape', keras_input_layer)
+        shape = tuple(int(s) for s in shape if s is not None)
+        self.input_size = shape
+        self.input_shape = (None,) + shape
+        self.n_visible = int(np.prod(shape))
+        self.alpha = int(alpha)
+        self.n_hidden = self.alpha * self.n_visible
+        if device is None:
+            device = torch.device('cuda', torch.cuda.current_device()) if torch.cuda.is_available() else 'cpu'
+        self.device = torch.device(device)
+        N, M = self.n_visible, self.n_hidden
+        self.sizes = [N * M, N * M, M, M, N, N]
+        self.n_params = sum(self.sizes)
+        if params is None:
+            gen = torch.Generator().manual_seed(int(seed))
+            params = torch.randn(self.n_params, generator=gen, dtype=torch.float64) * stddev
+        self.set_params(params)
+
+    def set_params(self, params):
+        p = torch.as_tensor(params, dtype=torch.float32).reshape(-1)
+        if p.numel() != self.n_params:
+            raise ValueError(f'expected {self.n_params} parameters, got {p.numel()}')
+        self.params = p.to(self.device).contiguous()
+
+    def mark_params_updated(self):
+        pass
+
+    def parts(self):
+        """(w_re, w_im, b_re, b_im, a_re, a_im) views of the flat parameters."""
+        out, off = [], 0
+        for n in self.sizes:
+            out.append(self.params[off:off + n])
+            off += n
+        return out
+
+    def _spins(self, x):
+        if not isinstance(x, torch.Tensor):
+            x = torch.as_tensor(np.asarray(x))
+        x = x.to(self.device)
+        if x.dtype != torch.int8:
+            x = x.to(torch.int8)
+        return x.reshape(x.shape[0], self.n_visible).contiguous()
+
+    def forward(self, spins, pairs=None, m=None, m_dev=None, operator=None, stream=None):
+        if pairs is None:
+            m = spins.shape[0]
+        elif m is None:
+            m = pairs.shape[0]
+        out = torch.empty((m, 2), dtype=torch.float32, device=self.device)
+        kind, ca, cb = 0, None, None
+        if pairs is not None:
+            tb = operator.tables(self.device)
+            kind, ca, cb = operator.kind, tb['conn_a'], tb['conn_b']
+        w = [_lib.ptr(t) for t in self.parts()]
+        _lib.call('fk_rbm_log_psi', *w, self.n_visible, self.n_hidden, _lib.ptr(spins), _lib.ptr(pairs),
+                  _lib.ptr(m_dev), int(m), kind, _lib.ptr(ca), _lib.ptr(cb), _lib.ptr(out), _lib.stream_ptr(stream))
+        return out, None
+
+    def log_psi(self, x):
+        out, _ = self.forward(self._spins(x))
+        return torch.complex(out[:, 0], out[:, 1])
+
+    def predict(self, x, batch_size=None):
+        return self.log_psi(x)[:, None]
+
+    __call__ = predict
+
+    def log_psi_connected(self, spins, pairs, m, operator, m_dev=None, precision=None, stream=None):
+        out, _ = self.forward(spins, pairs=pairs, m=m, m_dev=m_dev, operator=operator, stream=stream)
+        return out
+
+    def count_params(self):
+        return self.n_params

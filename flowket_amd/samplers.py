@@ -133,3 +133,143 @@ class FastAutoregressiveSampler(AutoregressiveSampler):
                       _lib.ptr(cond[start:start + rows]) if cond is not None else None, s)
         self.last_cond_logp = cond
         return out.reshape((self.batch_size,) + self.input_size)
+
+
+def sum_correlations(correlations):
+    """Geyer pair truncation (samplers/metropolis_hastings.py:8-12)."""
+    for i in range(1, correlations.shape[0] // 2):
+        if correlations[2 * i - 1] + correlations[2 * i] < 0:
+            return correlations[:2 * i].sum()
+    return correlations.sum()
+
+
+class MetropolisHastingsSampler(Sampler):
+    """samplers/metropolis_hastings.py:15-89.  ``num_of_chains`` chains
+    (uniform +-1 start), per ``next``: (samples_per_chain // discard_ratio) *
+    unused_sampels burn-in sweeps, then (unused_sampels + 1) sweeps per
+    recorded sample; one sweep = one proposal per chain.  RBM machines run the
+    whole chain in one fused kernel (fk_mh_rbm, incremental theta updates);
+    other machines use fk_mh_propose / the machine's log psi / fk_mh_accept
+    per sweep.  ``randomness`` may inject (sites, moves, uniforms) arrays of
+    shape (sweeps, chains) for draw-by-draw parity."""
+
+    kind = 0
+
+    def __init__(self, machine, batch_size, num_of_chains=1, unused_sampels=0, discard_ratio=10, seed=None,
+                 mini_batch_size=None, **kwargs):
+        self.machine = _machine_of(machine)
+        super().__init__(self.machine.input_size, batch_size, mini_batch_size)
+        if self.batch_size % num_of_chains != 0:
+            raise Exception('Num of samplers must divide the batch size')
+        self.num_of_chains = num_of_chains
+        self.unused_sampels = unused_sampels
+        self.discard_ratio = discard_ratio
+        self.acceptance_ratio = 1.0
+        self.seed = int(seed) if seed is not None else int(np.random.randint(0, 2 ** 62))
+        self._calls = 0
+        self.randomness = None
+        dev = self.machine.device
+        g = torch.Generator().manual_seed(self.seed)
+        n = int(np.prod(self.input_size))
+        self.sample = (torch.randint(0, 2, (num_of_chains, n), generator=g) * 2 - 1).to(torch.int8).to(dev)
+        shape = self.input_size if len(self.input_size) == 2 else (1, n)
+        self.H, self.W = int(shape[0]), int(shape[1])
+
+    def _counts(self):
+        spc = self.batch_size // self.num_of_chains
+        burn = (spc // self.discard_ratio) * self.unused_sampels if self.discard_ratio > 0 else 0
+        return spc, burn
+
+    def __next__(self):
+        _lib.require_gpu()
+        spc, burn = self._counts()
+        n = self.sample.shape[1]
+        dev = self.machine.device
+        out = torch.empty((self.num_of_chains, spc, n), dtype=torch.int8, device=dev)
+        steps = burn + spc * (self.unused_sampels + 1)
+        from .machines import RBM
+        if isinstance(self.machine, RBM):
+            accepts = torch.zeros(self.num_of_chains, dtype=torch.int64, device=dev)
+            r_site = r_move = r_u = None
+            if self.randomness is not None:
+                r_site, r_move, r_u = [None if t is None else torch.as_tensor(t).to(dev).contiguous()
+                                       for t in self.randomness]
+            w = [_lib.ptr(t) for t in self.machine.parts()]
+            _lib.call('fk_mh_rbm', *w, n, self.machine.n_hidden, self.H, self.W, self.kind, self.num_of_chains,
+                      burn, spc, self.unused_sampels, _lib.ptr(self.sample), _lib.ptr(out), _lib.ptr(accepts), None,
+                      _lib.ptr(r_site), _lib.ptr(r_move), _lib.ptr(r_u),
+                      (self.seed * 1000003 + self._calls) & 0xFFFFFFFFFFFFFFFF, _lib.stream_ptr())
+            acc_total = int(accepts.sum().item())
+        else:
+            acc_total = self._generic(out, burn, spc, steps)
+        self._calls += 1
+        self.acceptance_ratio = acc_total / (self.batch_size * (self.unused_sampels + 1))
+        return out.reshape((self.batch_size,) + self.input_size)
+
+    def _draw(self, step, gen, dev):
+        C, n = self.num_of_chains, self.sample.shape[1]
+        if self.randomness is not None:
+            sites, moves, us = self.randomness
+            return (torch.as_tensor(sites[step]).to(dev).to(torch.int32).contiguous(),
+                    None if moves is None else torch.as_tensor(moves[step]).to(dev).to(torch.int32).contiguous(),
+                    torch.as_tensor(us[step], dtype=torch.float64).to(dev).contiguous())
+        site = torch.randint(0, n, (C,), generator=gen, device=dev, dtype=torch.int32)
+        move = torch.randint(0, 9, (C,), generator=gen, device=dev, dtype=torch.int32) if self.kind == 1 else None
+        u = torch.rand((C,), generator=gen, device=dev, dtype=torch.float64)
+        return site, move, u
+
+    def _generic(self, out, burn, spc, steps):
+        m = self.machine
+        dev = m.device
+        C, n = self.num_of_chains, self.sample.shape[1]
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(self.seed * 7919 + self._calls)
+        cur, _ = m.forward(self.sample)   # machine_updated (:39-40)
+        cur = cur.contiguous()
+        cand = torch.empty_like(self.sample)
+        accepted = torch.empty(C, dtype=torch.int32, device=dev)
+        acc_total = torch.zeros((), dtype=torch.int64, device=dev)
+        s = _lib.stream_ptr()
+        for step in range(steps):
+            site, move, u = self._draw(step, gen, dev)
+            _lib.call('fk_mh_propose', _lib.ptr(self.sample), C, self.H, self.W, self.kind, _lib.ptr(site),
+                      _lib.ptr(move), _lib.ptr(cand), s)
+            lc, _ = m.forward(cand)
+            _lib.call('fk_mh_accept', _lib.ptr(self.sample), _lib.ptr(cur), _lib.ptr(cand), _lib.ptr(lc),
+                      _lib.ptr(u), C, n, _lib.ptr(accepted), s)
+            if step >= burn:
+                acc_total += accepted.sum()
+                rs = step - burn
+                if (rs + 1) % (self.unused_sampels + 1) == 0:
+                    out[:, rs // (self.unused_sampels + 1)] = self.sample
+        return int(acc_total.item())
+
+    def calc_r_hat_value(self, estimated_values):
+        """Gelman-Rubin R-hat, variogram correlations and ESS
+        (samplers/metropolis_hastings.py:67-89)."""
+        v = np.asarray(estimated_values.cpu() if isinstance(estimated_values, torch.Tensor) else estimated_values)
+        spc = self.batch_size // self.num_of_chains
+        per = v.reshape(self.num_of_chains, spc)
+        cm = per.mean(axis=1)
+        mean = cm.mean()
+        between = (spc / (self.num_of_chains - 1.0)) * ((cm - mean) ** 2).sum()
+        if spc == 1:
+            return 1.0, between, 0.0, self.batch_size
+        within = ((per - cm[:, None]) ** 2).sum(axis=1).mean() / (spc - 1)
+        var = ((spc - 1) * within + between) / spc
+        r_hat = np.sqrt(var / within)
+        vt = np.array([((per[:, t + 1:] - per[:, :spc - t - 1]) ** 2).mean(axis=1).mean() for t in range(spc - 1)])
+        corr = 1 - vt / (2 * var)
+        cs = sum_correlations(corr)
+        return r_hat, var, cs, self.batch_size / (1 + 2 * cs)
+
+
+class MetropolisHastingsLocal(MetropolisHastingsSampler):
+    """Flip one uniformly chosen site per chain (:119-127)."""
+    kind = 0
+
+
+class MetropolisHastingsExchange(MetropolisHastingsSampler):
+    """Swap a uniformly chosen site with a neighbour at offset in
+    {-1, 0, 1}^2, periodic (:130-145)."""
+    kind = 1

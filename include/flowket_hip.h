@@ -212,6 +212,53 @@ int fk_dot_f64(const float* a, const float* b, int64_t n, double* scratch,
 int fk_axpy(double alpha, const float* x, float* y, int64_t n, void* stream);
 int fk_xpby(const float* r, double beta, float* p, int64_t n, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Complex RBM (machines/rbm.py:35-48): log psi = sum_j a_j s_j +
+ * sum_h lncosh(b_h + sum_j W_jh s_j) with each complex parameter stored as
+ * (real, imag) and used as real - 1j*imag (layers/complex/base_layer.py:
+ * 30-35).  W (n_visible, n_hidden) row-major.  Configurations as in
+ * fk_convar_forward (rows of spins, or (sample, connection) pairs with a
+ * connection table).  logpsi (m, 2) float.
+ * ------------------------------------------------------------------------- */
+int fk_rbm_log_psi(const float* w_re, const float* w_im, const float* b_re,
+                   const float* b_im, const float* a_re, const float* a_im,
+                   int32_t n_visible, int32_t n_hidden, const int8_t* spins,
+                   const int32_t* pairs, const int64_t* m_dev, int64_t m_count,
+                   int32_t conn_kind, const int32_t* conn_a,
+                   const int32_t* conn_b, float* logpsi, void* stream);
+
+/* Metropolis-Hastings chains of an RBM (samplers/metropolis_hastings.py:
+ * 15-145): kind 0 = MetropolisHastingsLocal (flip one site), 1 =
+ * MetropolisHastingsExchange (swap a site with a neighbour offset in
+ * {-1,0,1}^2, periodic).  Each chain runs n_burn sweeps, then records
+ * n_record samples every (unused + 1) sweeps into out (chains, n_record, N);
+ * one sweep = one proposal per chain, accepted iff
+ * exp(min(2 Re(logpsi' - logpsi), 0)) > U.  state (chains, N) is read and
+ * updated; accepts (chains) counts accepted proposals of the recorded sweeps;
+ * logpsi (chains, 2) final values or NULL.  Randomness: injected
+ * (steps, chains) arrays r_site (flat site), r_move ((dy+1)*3+(dx+1),
+ * exchange only) and r_u (uniforms), or, when r_u is NULL, a counter-based
+ * generator keyed by `seed`. */
+int fk_mh_rbm(const float* w_re, const float* w_im, const float* b_re,
+              const float* b_im, const float* a_re, const float* a_im,
+              int32_t n_visible, int32_t n_hidden, int32_t H, int32_t W,
+              int32_t kind, int64_t chains, int64_t n_burn, int64_t n_record,
+              int64_t unused, int8_t* state, int8_t* out, int64_t* accepts,
+              float* logpsi, const int32_t* r_site, const int32_t* r_move,
+              const double* r_u, uint64_t seed, void* stream);
+
+/* Machine-agnostic MH steps (MetropolisHastingsSymmetricProposal._sweep,
+ * samplers/metropolis_hastings.py:99-113): candidates from the chain states
+ * (kind / r_site / r_move as fk_mh_rbm, one entry per chain), then
+ * acceptance given the candidates' log psi; accepted (chains) int32. */
+int fk_mh_propose(const int8_t* state, int64_t chains, int32_t H, int32_t W,
+                  int32_t kind, const int32_t* r_site, const int32_t* r_move,
+                  int8_t* candidates, void* stream);
+int fk_mh_accept(int8_t* state, float* logpsi_cur, const int8_t* candidates,
+                 const float* logpsi_cand, const double* uniforms,
+                 int64_t chains, int32_t n_sites, int32_t* accepted,
+                 void* stream);
+
 #ifdef __cplusplus
 }
 #endif
