@@ -58,6 +58,12 @@ SIGNATURES = {
                                  _P, _P, _P, _P, _P, _P, ctypes.c_uint64, _P]),
     'fk_mh_propose': (ctypes.c_int, [_P, _I64, _I32, _I32, _I32, _P, _P, _P, _P]),
     'fk_mh_accept': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I32, _P, _P]),
+    'fk_mh_accept_independent': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _I64, _I32, _P, _P]),
+    'fk_made_log_prob': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P, _P]),
+    'fk_made_sample': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P, _P, _P]),
+    'fk_made_fit_workspace_floats': (_I64, [_I32, _I32, _I64]),
+    'fk_made_fit_step': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P,
+                                        ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I64, _P]),
 }
 
 _lib = None

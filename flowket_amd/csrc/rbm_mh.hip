@@ -337,6 +337,26 @@ __global__ void mh_accept_kernel(int8_t* __restrict__ state, float* __restrict__
   }
 }
 
+// Independent proposal with the exact Metropolis-Hastings ratio:
+// accept iff exp(min(2 Re(logpsi' - logpsi) + log q(x) - log q(x'), 0)) > U.
+__global__ void mh_accept_independent_kernel(int8_t* __restrict__ state, float* __restrict__ logpsi_cur,
+                                             float* __restrict__ logq_cur, const int8_t* __restrict__ cand,
+                                             const float* __restrict__ logpsi_cand, const float* __restrict__ logq_cand,
+                                             const double* __restrict__ u, int64_t chains, int N,
+                                             int32_t* __restrict__ accepted) {
+  const int64_t ch = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ch >= chains) return;
+  const float lr = fminf(2.0f * (logpsi_cand[2 * ch] - logpsi_cur[2 * ch]) + (logq_cur[ch] - logq_cand[ch]), 0.0f);
+  const bool acc = (double)expf(lr) > u[ch];
+  accepted[ch] = acc ? 1 : 0;
+  if (acc) {
+    for (int j = 0; j < N; ++j) state[ch * N + j] = cand[ch * N + j];
+    logpsi_cur[2 * ch] = logpsi_cand[2 * ch];
+    logpsi_cur[2 * ch + 1] = logpsi_cand[2 * ch + 1];
+    logq_cur[ch] = logq_cand[ch];
+  }
+}
+
 }  // namespace fk
 
 using namespace fk;
@@ -421,6 +441,19 @@ int fk_mh_accept(int8_t* state, float* logpsi_cur, const int8_t* candidates, con
   FK_REQUIRE(state && logpsi_cur && candidates && logpsi_cand && uniforms && accepted, "fk_mh_accept: null pointer");
   hipLaunchKernelGGL(mh_accept_kernel, dim3(div_up(chains, 256)), dim3(256), 0, (hipStream_t)stream, state,
                      logpsi_cur, candidates, logpsi_cand, uniforms, chains, n_sites, accepted);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_mh_accept_independent(int8_t* state, float* logpsi_cur, float* logq_cur, const int8_t* candidates,
+                             const float* logpsi_cand, const float* logq_cand, const double* uniforms, int64_t chains,
+                             int32_t n_sites, int32_t* accepted, void* stream) {
+  if (chains <= 0) return FK_OK;
+  FK_REQUIRE(state && logpsi_cur && logq_cur && candidates && logpsi_cand && logq_cand && uniforms && accepted,
+             "fk_mh_accept_independent: null pointer");
+  hipLaunchKernelGGL(mh_accept_independent_kernel, dim3(div_up(chains, 256)), dim3(256), 0, (hipStream_t)stream,
+                     state, logpsi_cur, logq_cur, candidates, logpsi_cand, logq_cand, uniforms, chains, n_sites,
+                     accepted);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }

@@ -259,6 +259,45 @@ int fk_mh_accept(int8_t* state, float* logpsi_cur, const int8_t* candidates,
                  int64_t chains, int32_t n_sites, int32_t* accepted,
                  void* stream);
 
+/* Independent-proposal acceptance with the exact ratio (no reference
+ * counterpart: MetropolisHastingsGlobal, samplers/metropolis_hastings.py:
+ * 148-157, treats any proposal as symmetric): accept iff
+ * exp(min(2 Re(logpsi' - logpsi) + log q(x) - log q(x'), 0)) > U. */
+int fk_mh_accept_independent(int8_t* state, float* logpsi_cur, float* logq_cur,
+                             const int8_t* candidates, const float* logpsi_cand,
+                             const float* logq_cand, const double* uniforms,
+                             int64_t chains, int32_t n_sites, int32_t* accepted,
+                             void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Learned autoregressive proposal (no reference counterpart): a masked MLP
+ * (MADE) n_sites -> hidden -> hidden -> n_sites logits, ReLU, BF16 weights
+ * (w1 (hidden, n_sites), w2t (hidden, hidden) = W2 transposed,
+ * w3 (n_sites, hidden)) and FP32 biases / arithmetic.  Site i is +1 with
+ * probability sigmoid(logit_i) given sites < i.
+ *   fk_made_log_prob: logq (batch) = log q(x).
+ *   fk_made_sample:   x ~ q from uniforms (batch, n_sites), with log q(x).
+ *   fk_made_fit_step: one full-batch Adam step of -mean log q(x) on the FP32
+ *                     master vector [w1 | b1 | w2t | b2 | w3 | b3] (b1, b2,
+ *                     b3 point into it), refreshing the BF16 copies;
+ *                     workspace holds fk_made_fit_workspace_floats floats.
+ * ------------------------------------------------------------------------- */
+int fk_made_log_prob(const void* w1, const float* b1, const void* w2t,
+                     const float* b2, const void* w3, const float* b3,
+                     int32_t n_sites, int32_t hidden, const int8_t* x,
+                     int64_t batch, float* logq, void* stream);
+int fk_made_sample(const void* w1, const float* b1, const void* w2t,
+                   const float* b2, const void* w3, const float* b3,
+                   int32_t n_sites, int32_t hidden, const double* uniforms,
+                   int64_t batch, int8_t* x, float* logq, void* stream);
+int64_t fk_made_fit_workspace_floats(int32_t n_sites, int32_t hidden,
+                                     int64_t batch);
+int fk_made_fit_step(float* master, float* adam_m, float* adam_v, void* w1,
+                     float* b1, void* w2t, float* b2, void* w3, float* b3,
+                     int32_t n_sites, int32_t hidden, const int8_t* x,
+                     int64_t batch, float* workspace, float lr, float beta1,
+                     float beta2, float eps, int64_t step, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,10 @@ Pinning (how each function is checked, see ``tests/test_oracle_*.py``):
                              :63-82) and detailed balance on 2^N enumerations.
   rbm.*                      lncosh identities (tests/test_tensorflow_complex_
                              numbers_ops.py:9-33) and brute-force sums.
+  made.*                     sum_x q(x) == 1 on all 2^N states, backprop vs
+                             central finite differences, sampler z-test
+                             (a learned proposal with no reference
+                             counterpart: pinned by mathematics only).
   sr.*                       dense pinv solve (tests/test_stochastic_
                              reconfiguration.py:19-72), explicit S.v.
 """

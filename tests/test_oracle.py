@@ -310,3 +310,72 @@ def test_real_parameter_sr_closed_form(rng):
     S2 = stacked.T @ stacked / B + 1e-3 * np.eye(P)
     assert np.allclose(S, S2)
     assert np.allclose(S @ x, F)
+
+
+# ---------------------------------------------------------------- learned proposal (MADE)
+def _made_params(rng, N, Hd, scale=0.7):
+    from oracle import made as omd
+    v = rng.normal(scale=scale, size=Hd * N + Hd + Hd * Hd + Hd + N * Hd + N)
+    return omd.unpack(v, N, Hd)
+
+
+@pytest.mark.parametrize('N,Hd', [(5, 7), (6, 16), (2, 3)])
+def test_made_normalised(rng, N, Hd):
+    """sum_x q(x) == 1 over all 2^N states: every mask is autoregressive."""
+    from oracle import made as omd
+    p = _made_params(rng, N, Hd)
+    states = oex.all_states((N,)).reshape(-1, N)
+    assert np.exp(omd.log_q(p, states)).sum() == pytest.approx(1.0, abs=1e-12)
+
+
+def test_made_mask_error_breaks_normalisation(rng):
+    """The normalisation pin is sensitive: letting logit o see hidden units of
+    degree o+1 (one off) makes sum_x q(x) != 1."""
+    from oracle import made as omd
+    N, Hd = 5, 8
+    p = list(_made_params(rng, N, Hd))
+    dh = omd.degrees(N, Hd)
+    v = rng.normal(size=(N, Hd))
+    p[4] = v * (dh[None, :] <= np.arange(N)[:, None] + 1)
+    states = oex.all_states((N,)).reshape(-1, N)
+    assert abs(np.exp(omd.log_q(p, states)).sum() - 1.0) > 1e-3
+
+
+def test_made_gradient_vs_finite_differences(rng):
+    from oracle import made as omd
+    N, Hd = 4, 6
+    p = [a.copy() for a in _made_params(rng, N, Hd)]
+    x = rng.choice([-1.0, 1.0], size=(7, N))
+    g = omd.grad_mean_nll(p, x)
+    m = omd.masks(N, Hd)
+    live = [m[0], np.ones(Hd), m[1], np.ones(Hd), m[2], np.ones(N)]
+    h = 1e-6
+    for a, ga, lv in zip(p, g, live):
+        for idx in np.ndindex(a.shape):
+            if lv[idx] == 0:
+                assert ga[idx] == 0
+                continue
+            old = a[idx]
+            a[idx] = old + h
+            fp = -omd.log_q(p, x).mean()
+            a[idx] = old - h
+            fm = -omd.log_q(p, x).mean()
+            a[idx] = old
+            assert ga[idx] == pytest.approx((fp - fm) / (2 * h), abs=1e-6)
+
+
+def test_made_sampler_distribution(rng):
+    """Samples drawn site by site follow q (z-statistic of
+    tests/test_samplers.py:63-82), and the returned log q matches log_q."""
+    from oracle import made as omd
+    N, Hd = 4, 8
+    p = _made_params(rng, N, Hd, scale=1.0)
+    n = 2 ** 14
+    x, lq = omd.sample(p, rng.random((n, N)))
+    assert np.allclose(lq, omd.log_q(p, x))
+    states = oex.all_states((N,)).reshape(-1, N)
+    q = np.exp(omd.log_q(p, states))
+    a = np.bincount(oex.binary_array_to_decimal_array(x), minlength=2 ** N)
+    b = np.bincount(rng.choice(2 ** N, size=n, p=q / q.sum()), minlength=2 ** N)
+    z = ((np.square(a - b) - a - b) / (a + b + 1e-20)).sum()
+    assert z <= np.sqrt(n)
