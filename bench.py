@@ -50,14 +50,19 @@ def parse():
                     help='only run two fused-forward launches over the connected configurations of N random '
                          'samples (a short program for rocprofv3 counter passes) and exit')
     ap.add_argument('--precision', default='bf16', choices=['bf16', 'fp32'])
+    ap.add_argument('--schedule', type=int, default=None,
+                    help='force a forward launch schedule (index into the compiled table) instead of autotuning')
     return ap.parse_args()
 
 
-def profile_forward(n, precision):
+def profile_forward(n, precision, schedule):
     import torch
     import flowket_amd as fk
+    from flowket_amd import _lib, autotune
     m = fk.ConvNetAutoregressive2D((H, W), depth=DEPTH, num_of_channels=CHANNELS, activation=ACT,
                                    weights_normalization=False, precision=precision, seed=1234)
+    if schedule is not None:
+        autotune.set_schedule(m, _lib.FK_PREC_BF16 if precision == 'bf16' else _lib.FK_PREC_FP32, schedule)
     op = fk.Heisenberg(hilbert_state_shape=(H, W), pbc=True)
     gen = torch.Generator().manual_seed(0)
     spins = (torch.randint(0, 2, (n, H * W), generator=gen) * 2 - 1).to(torch.int8).cuda()
@@ -66,7 +71,8 @@ def profile_forward(n, precision):
     for _ in range(2):
         m.log_psi_connected(spins, r['pairs'], total, op)
     torch.cuda.synchronize()
-    print(json.dumps({'profile_forward': {'samples': n, 'configurations': total, 'precision': precision}}))
+    print(json.dumps({'profile_forward': {'samples': n, 'configurations': total, 'precision': precision,
+                                          'schedule': schedule}}))
     return 0
 
 
@@ -104,7 +110,7 @@ def cpu_baseline(machine, spins_np, n_samples=8):
 def main():
     args = parse()
     if args.profile_forward:
-        return profile_forward(args.profile_forward, args.precision)
+        return profile_forward(args.profile_forward, args.precision, args.schedule)
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; multi-GPU sharding is a design note only '
                                    '(see DESIGN.md), --gpus must be 1'}))
@@ -170,6 +176,16 @@ def main():
             fwd_times.append((e0, e1))
         return offsets, mom
 
+    # ---- launch schedule of the fused forward: forced, or autotuned on this step
+    from flowket_amd import autotune
+    if args.schedule is not None:
+        autotune.set_schedule(m16, _lib.FK_PREC_BF16, args.schedule)
+        tune = {'schedule': args.schedule, 'forced': True}
+    else:
+        tune = autotune.tune_forward(m16, lambda: step(False), reps=2, persist=False)
+        tune = {'schedule': tune['schedule'], 'ms_per_schedule': [round(t, 2) for t in tune['ms']],
+                'schedules': tune['schedules']}
+
     for _ in range(args.warmup):
         step(False)
     torch.cuda.synchronize()
@@ -207,6 +223,7 @@ def main():
                      'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_BF16, 4), 'traffic': None,
                      'kernel': 'convar_fwd_kernel<bf16,32>', 'kernel_ms': round(fwd_ms, 3),
                      'flop_per_launch': flop_launch},
+        'forward_schedule': tune,
         'samples_per_s': sps,
         'connected_logpsi_per_s': round(n_conn * args.steps / elapsed, 1),
     }

@@ -12,6 +12,7 @@ from .samplers import (Sampler, AutoregressiveSampler, FastAutoregressiveSampler
 from .optimization import VariationalMonteCarlo, MiniBatchGenerator, loss_for_energy_minimization  # noqa: F401
 from .optimizers import ComplexValuesStochasticReconfiguration, Adam, energy_gradient, conjugate_gradient  # noqa: F401
 
+from . import autotune  # noqa: F401
 from .proposals import MADEProposal, MetropolisHastingsLearnedProposal  # noqa: F401
 
 __version__ = '0.1.0'

@@ -40,6 +40,10 @@ SIGNATURES = {
     'fk_pack_gather': (ctypes.c_int, [_P, _P, _I64, _I32, _P, _P]),
     'fk_convar_forward': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _I32,
                                          _P, _P, _P, _P, _P]),
+    'fk_convar_num_schedules': (_I32, [_I32, _I32]),
+    'fk_convar_schedule_info': (ctypes.c_int, [_I32, _I32, _I32, _P, _P]),
+    'fk_convar_set_schedule': (ctypes.c_int, [_I32, _I32, _I32, _I32, _I32]),
+    'fk_convar_get_schedule': (_I32, [_I32, _I32, _I32, _I32]),
     'fk_ar_draw_site': (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P]),
     'fk_fast_sampler_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I32, _I64]),
     'fk_fast_sample': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I64, _P, _P, _P, _P, _P]),

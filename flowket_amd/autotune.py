@@ -1,0 +1,112 @@
+"""Schedule autotuner for the fused ConvNetAutoregressive2D forward.
+
+Every (channels, precision) has a few compiled launch schedules (loop order x
+waves per workgroup, fk_convar_num_schedules).  ``tune_forward`` times each on
+the caller's own workload -- events recorded on the stream the kernels are
+launched on -- selects the fastest for that lattice (fk_convar_set_schedule,
+process wide) and records it in a JSON cache keyed by device, lattice,
+channels, precision, activation and depth, so later processes start from the
+tuned choice (``apply_cached``).  The cache path is $FLOWKET_AMD_TUNE_CACHE or
+~/.cache/flowket_amd/forward_schedules.json.  All schedules compute the same
+function; they differ only in FP32 summation order.
+"""
+import json
+import os
+import threading
+
+import torch
+
+from . import _lib
+
+_lock = threading.Lock()
+
+
+def cache_path():
+    return os.environ.get('FLOWKET_AMD_TUNE_CACHE',
+                          os.path.join(os.path.expanduser('~'), '.cache', 'flowket_amd', 'forward_schedules.json'))
+
+
+def _key(machine, prec):
+    dev = torch.cuda.get_device_name(machine.device)
+    return '|'.join(str(v) for v in (dev, machine.H, machine.W, machine.num_of_channels, prec, machine.activation,
+                                     machine.depth))
+
+
+def _read_cache():
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _write_cache(entries):
+    path = cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + '.tmp%d' % os.getpid()
+        with open(tmp, 'w') as f:
+            json.dump(entries, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only loses the cache, not the choice
+
+
+def schedules(channels, prec):
+    lib = _lib.load()
+    n = int(lib.fk_convar_num_schedules(channels, prec))
+    out = []
+    for i in range(n):
+        t, w = _lib.ctypes.c_int32(), _lib.ctypes.c_int32()
+        _lib.call('fk_convar_schedule_info', channels, prec, i, _lib.ctypes.byref(t), _lib.ctypes.byref(w))
+        out.append({'index': i, 'tile_outer': int(t.value), 'waves': int(w.value)})
+    return out
+
+
+def set_schedule(machine, prec, index):
+    _lib.call('fk_convar_set_schedule', machine.H, machine.W, machine.num_of_channels, prec, int(index))
+
+
+def get_schedule(machine, prec):
+    return int(_lib.load().fk_convar_get_schedule(machine.H, machine.W, machine.num_of_channels, prec))
+
+
+def apply_cached(machine, prec):
+    """Select the cached schedule for this machine/precision, if any."""
+    with _lock:
+        entry = _read_cache().get(_key(machine, prec))
+    if entry is not None and 0 <= int(entry['schedule']) < len(schedules(machine.num_of_channels, prec)):
+        set_schedule(machine, prec, int(entry['schedule']))
+        return int(entry['schedule'])
+    return None
+
+
+def tune_forward(machine, run, precision=None, reps=3, warmup=1, persist=True):
+    """Times ``run()`` (a closure that launches the machine's forward on the
+    current stream, e.g. one ragged local-energy evaluation) under every
+    schedule and keeps the fastest.  Returns {'schedule': i, 'ms': [...],
+    'schedules': [...]}."""
+    prec = _lib.FK_PREC_BF16 if (precision or machine.precision) == 'bf16' else _lib.FK_PREC_FP32
+    table = schedules(machine.num_of_channels, prec)
+    stream = torch.cuda.current_stream(machine.device)
+    times = []
+    for s in table:
+        set_schedule(machine, prec, s['index'])
+        for _ in range(warmup):
+            run()
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(reps):
+            run()
+        end.record(stream)
+        end.synchronize()
+        times.append(start.elapsed_time(end) / reps)
+    best = min(range(len(table)), key=lambda i: times[i])
+    set_schedule(machine, prec, table[best]['index'])
+    if persist:
+        with _lock:
+            entries = _read_cache()
+            entries[_key(machine, prec)] = {'schedule': table[best]['index'], 'ms': times}
+            _write_cache(entries)
+    return {'schedule': table[best]['index'], 'ms': times, 'schedules': table}

@@ -12,6 +12,9 @@
 // Zero padding, DownShift and RightShift are index arithmetic (tap validity
 // masks and shifted write targets), never materialised.
 // HBM traffic: H*W int8 spins in, 8 bytes out per configuration.
+#include <map>
+#include <mutex>
+
 #include "fk_common.h"
 #include "convar_layout.h"
 #include "convar_dev.h"
@@ -228,7 +231,12 @@ __device__ __forceinline__ void conv_tile(f32x4 (&acc)[MTO], const uint8_t* buf,
       for (int ks = 0; ks < KS; ++ks) acc[mt] = P::mfma(cw.w[t][mt][ks], b[t][ks], acc[mt]);
 }
 
-template <int PREC, int C, int MAXT, int ACT, int NWAVE>
+// SCHED 0: taps outer -- each tap's weights are loaded once and serve all the
+// wave's tiles (accumulators of every tile live in registers).
+// SCHED 1: tiles outer -- a 3x3 convolution's weights stay in registers for
+// the whole phase and the wave walks its tiles one by one (fits bf16 C=32 and
+// fp32 C=16).
+template <int PREC, int C, int MAXT, int ACT, int NWAVE, int SCHED>
 __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   using P = Prec<PREC>;
   using frag = typename P::frag;
@@ -237,10 +245,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   constexpr int KS = C / P::KW;
   constexpr int MTH = (C / 2 + 15) / 16;
   constexpr int PIXB = L::PIXB;
-  // bf16 C=32: a 3x3 convolution's weights fit in 72 VGPRs, so each wave
-  // loads them once per phase and walks its tiles; otherwise taps are the
-  // outer loop and every tap's weights serve all tiles of the wave.
-  constexpr bool TILE_OUTER = (PREC == FK_PREC_BF16 && C == 32);
+  constexpr bool TILE_OUTER = SCHED == 1;
   const ConvarLayout lay(C, PREC);
 
   extern __shared__ __align__(16) uint8_t lds[];
@@ -625,13 +630,42 @@ struct FwdPlan {
 
 static size_t fwd_lds_bytes(int npix, int pixb) { return (size_t)(npix + 1) * pixb * 5 + (size_t)npix * (4 + 8); }
 
+// One launch shape of the forward: loop order and waves per workgroup.
+struct FwdSchedule {
+  int sched, nwave;
+};
+
+// Schedules compiled for each (precision, channels); index 0 is the default.
+static int fwd_schedules(int C, int prec, const FwdSchedule** out) {
+  static const FwdSchedule tile_capable[] = {{0, 8}, {1, 4}, {1, 8}, {0, 4}};
+  static const FwdSchedule taps_only[] = {{0, 8}, {0, 4}};
+  if ((prec == FK_PREC_BF16 && C == 32) || (prec == FK_PREC_FP32 && C == 16)) {
+    *out = tile_capable;
+    return 4;
+  }
+  *out = taps_only;
+  return 2;
+}
+
+static std::mutex g_sched_mu;
+static std::map<uint64_t, int> g_sched;
+
+static uint64_t sched_key(int H, int W, int C, int prec) {
+  return ((uint64_t)(uint32_t)H << 40) | ((uint64_t)(uint32_t)W << 20) | ((uint64_t)(uint32_t)C << 4) |
+         (uint64_t)(prec & 15);
+}
+
+static int current_schedule(int H, int W, int C, int prec) {
+  std::lock_guard<std::mutex> g(g_sched_mu);
+  auto it = g_sched.find(sched_key(H, W, C, prec));
+  return it == g_sched.end() ? 0 : it->second;
+}
+
 // Largest tile of configurations whose five activation lattices fit in the
-// 160 KB LDS; bf16 C=32 runs one wave per SIMD (the whole register file per
-// wave: register-resident convolution weights), the other shapes two.
-static int plan_forward(int HW, int C, int prec, int64_t m, FwdPlan* plan) {
+// 160 KB LDS with at most 4 (8 waves) or 8 (4 waves) 16-pixel tiles per wave.
+static int plan_forward(int HW, int C, int prec, int NW, int64_t m, FwdPlan* plan) {
   const int pixb = C * (prec == FK_PREC_BF16 ? 2 : 4);
   const size_t cap = 160 * 1024;
-  const int NW = (prec == FK_PREC_BF16 && C == 32) ? 4 : 8;
   const int maxt_cap = (NW == 4) ? 8 : 4;
   int best = 0;
   for (int T = 1; T <= 256; ++T) {
@@ -652,11 +686,14 @@ static int plan_forward(int HW, int C, int prec, int64_t m, FwdPlan* plan) {
   return FK_OK;
 }
 
-template <int PREC, int C, int MAXT, int ACT, int NWAVE>
+template <int PREC, int C, int MAXT, int ACT, int NWAVE, int SCHED>
 static int launch_fwd(const FwdArgs& a, const FwdPlan& pl, hipStream_t s) {
-  auto kern = convar_fwd_kernel<PREC, C, MAXT, ACT, NWAVE>;
-  FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)pl.lds));
+  auto kern = convar_fwd_kernel<PREC, C, MAXT, ACT, NWAVE, SCHED>;
+  // raise the dynamic LDS limit once per instantiation (not a stream
+  // operation, so launches may later be captured into a graph)
+  static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  FK_CHECK_HIP(attr);
   const int64_t ngroups = (a.m_count + pl.T - 1) / pl.T;
   const int per_cu = (int)((160 * 1024) / pl.lds) > 0 ? (int)((160 * 1024) / pl.lds) : 1;
   int64_t grid = (int64_t)device_cu_count() * per_cu;
@@ -667,21 +704,23 @@ static int launch_fwd(const FwdArgs& a, const FwdPlan& pl, hipStream_t s) {
   return FK_OK;
 }
 
+template <int PREC, int C, int NW, int SCHED>
+static int launch_variant(const FwdArgs& a, const FwdPlan& pl, int act, hipStream_t s) {
+  constexpr int LO = NW == 4 ? 4 : 2, HI = NW == 4 ? 8 : 4;
+  if (pl.maxt <= LO)
+    return act == FK_ACT_ELU ? launch_fwd<PREC, C, LO, FK_ACT_ELU, NW, SCHED>(a, pl, s)
+                             : launch_fwd<PREC, C, LO, FK_ACT_RELU, NW, SCHED>(a, pl, s);
+  return act == FK_ACT_ELU ? launch_fwd<PREC, C, HI, FK_ACT_ELU, NW, SCHED>(a, pl, s)
+                           : launch_fwd<PREC, C, HI, FK_ACT_RELU, NW, SCHED>(a, pl, s);
+}
+
 template <int PREC, int C>
-static int dispatch_fwd_c(const FwdArgs& a, const FwdPlan& pl, int act, hipStream_t s) {
-  if constexpr (PREC == FK_PREC_BF16 && C == 32) {
-    if (pl.maxt <= 4)
-      return act == FK_ACT_ELU ? launch_fwd<PREC, C, 4, FK_ACT_ELU, 4>(a, pl, s)
-                               : launch_fwd<PREC, C, 4, FK_ACT_RELU, 4>(a, pl, s);
-    return act == FK_ACT_ELU ? launch_fwd<PREC, C, 8, FK_ACT_ELU, 4>(a, pl, s)
-                             : launch_fwd<PREC, C, 8, FK_ACT_RELU, 4>(a, pl, s);
-  } else {
-    if (pl.maxt == 2)
-      return act == FK_ACT_ELU ? launch_fwd<PREC, C, 2, FK_ACT_ELU, 8>(a, pl, s)
-                               : launch_fwd<PREC, C, 2, FK_ACT_RELU, 8>(a, pl, s);
-    return act == FK_ACT_ELU ? launch_fwd<PREC, C, 4, FK_ACT_ELU, 8>(a, pl, s)
-                             : launch_fwd<PREC, C, 4, FK_ACT_RELU, 8>(a, pl, s);
+static int dispatch_fwd_c(const FwdArgs& a, const FwdPlan& pl, const FwdSchedule& sc, int act, hipStream_t s) {
+  if constexpr ((PREC == FK_PREC_BF16 && C == 32) || (PREC == FK_PREC_FP32 && C == 16)) {
+    if (sc.sched == 1)
+      return sc.nwave == 4 ? launch_variant<PREC, C, 4, 1>(a, pl, act, s) : launch_variant<PREC, C, 8, 1>(a, pl, act, s);
   }
+  return sc.nwave == 4 ? launch_variant<PREC, C, 4, 0>(a, pl, act, s) : launch_variant<PREC, C, 8, 0>(a, pl, act, s);
 }
 
 bool convar_supported(int channels, int precision) {
@@ -802,8 +841,13 @@ int fk_convar_forward(const void* wpacked, const float* aux, int32_t precision, 
     return FK_ERR_UNSUPPORTED;
   }
   if (m_count <= 0) return FK_OK;
+  const FwdSchedule* table = nullptr;
+  const int nsched = fwd_schedules(channels, precision, &table);
+  int si = current_schedule(H, W, channels, precision);
+  if (si < 0 || si >= nsched) si = 0;
+  const FwdSchedule sc = table[si];
   FwdPlan pl;
-  if (plan_forward(H * W, channels, precision, m_count, &pl) != FK_OK) {
+  if (plan_forward(H * W, channels, precision, sc.nwave, m_count, &pl) != FK_OK) {
     set_error("ConvNetAutoregressive2D kernels: lattice %dx%d with %d channels does not fit in LDS", H, W, channels);
     return FK_ERR_UNSUPPORTED;
   }
@@ -827,12 +871,44 @@ int fk_convar_forward(const void* wpacked, const float* aux, int32_t precision, 
   a.cond = cond_logp;
   hipStream_t s = (hipStream_t)stream;
   if (precision == FK_PREC_BF16) {
-    if (channels == 32) return dispatch_fwd_c<FK_PREC_BF16, 32>(a, pl, activation, s);
-    return dispatch_fwd_c<FK_PREC_BF16, 64>(a, pl, activation, s);
+    if (channels == 32) return dispatch_fwd_c<FK_PREC_BF16, 32>(a, pl, sc, activation, s);
+    return dispatch_fwd_c<FK_PREC_BF16, 64>(a, pl, sc, activation, s);
   }
-  if (channels == 16) return dispatch_fwd_c<FK_PREC_FP32, 16>(a, pl, activation, s);
-  if (channels == 32) return dispatch_fwd_c<FK_PREC_FP32, 32>(a, pl, activation, s);
-  return dispatch_fwd_c<FK_PREC_FP32, 64>(a, pl, activation, s);
+  if (channels == 16) return dispatch_fwd_c<FK_PREC_FP32, 16>(a, pl, sc, activation, s);
+  if (channels == 32) return dispatch_fwd_c<FK_PREC_FP32, 32>(a, pl, sc, activation, s);
+  return dispatch_fwd_c<FK_PREC_FP32, 64>(a, pl, sc, activation, s);
+}
+
+int32_t fk_convar_num_schedules(int32_t channels, int32_t precision) {
+  if (!convar_supported(channels, precision)) return 0;
+  const FwdSchedule* t = nullptr;
+  return fwd_schedules(channels, precision, &t);
+}
+
+int fk_convar_schedule_info(int32_t channels, int32_t precision, int32_t schedule, int32_t* tile_outer,
+                            int32_t* waves) {
+  FK_REQUIRE(tile_outer && waves, "fk_convar_schedule_info: null pointer");
+  const int n = fk_convar_num_schedules(channels, precision);
+  FK_REQUIRE(schedule >= 0 && schedule < n, "fk_convar_schedule_info: schedule out of range");
+  const FwdSchedule* t = nullptr;
+  fwd_schedules(channels, precision, &t);
+  *tile_outer = t[schedule].sched;
+  *waves = t[schedule].nwave;
+  return FK_OK;
+}
+
+int fk_convar_set_schedule(int32_t H, int32_t W, int32_t channels, int32_t precision, int32_t schedule) {
+  const int n = fk_convar_num_schedules(channels, precision);
+  FK_REQUIRE(n > 0, "fk_convar_set_schedule: unsupported channels/precision");
+  FK_REQUIRE(schedule >= -1 && schedule < n, "fk_convar_set_schedule: schedule out of range");
+  std::lock_guard<std::mutex> g(g_sched_mu);
+  if (schedule < 0) g_sched.erase(sched_key(H, W, channels, precision));
+  else g_sched[sched_key(H, W, channels, precision)] = schedule;
+  return FK_OK;
+}
+
+int32_t fk_convar_get_schedule(int32_t H, int32_t W, int32_t channels, int32_t precision) {
+  return current_schedule(H, W, channels, precision);
 }
 
 }  // extern "C"

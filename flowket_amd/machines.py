@@ -177,6 +177,8 @@ class ConvNetAutoregressive2D:
             _lib.call('fk_convar_pack_maps', self.depth, self.num_of_channels, prec,
                       wmap.ctypes.data_as(_lib.ctypes.c_void_p), amap.ctypes.data_as(_lib.ctypes.c_void_p))
             self._maps[prec] = (torch.from_numpy(wmap).to(self.device), torch.from_numpy(amap).to(self.device))
+            from . import autotune
+            autotune.apply_cached(self, prec)
         wmap, amap = self._maps[prec]
         eff = self.effective_params().contiguous()
         wdtype = torch.bfloat16 if prec == _lib.FK_PREC_BF16 else torch.float32

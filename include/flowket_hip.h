@@ -140,6 +140,22 @@ int fk_convar_forward(const void* wpacked, const float* aux, int32_t precision,
                       const int32_t* conn_a, const int32_t* conn_b,
                       float* logpsi, float* cond_logp, void* stream);
 
+/* Launch schedules of fk_convar_forward (no counterpart in the reference:
+ * the knobs of this library's autotuner).  Each (channels, precision) has
+ * fk_convar_num_schedules() compiled schedules (loop order tile_outer 0/1,
+ * waves per workgroup); fk_convar_set_schedule selects one for a lattice
+ * H x W (-1 restores the default, index 0) for every later forward launch of
+ * the process; results are identical up to FP32 summation order.
+ * Host-only, thread-safe. */
+int32_t fk_convar_num_schedules(int32_t channels, int32_t precision);
+int fk_convar_schedule_info(int32_t channels, int32_t precision,
+                            int32_t schedule, int32_t* tile_outer,
+                            int32_t* waves);
+int fk_convar_set_schedule(int32_t H, int32_t W, int32_t channels,
+                           int32_t precision, int32_t schedule);
+int32_t fk_convar_get_schedule(int32_t H, int32_t W, int32_t channels,
+                               int32_t precision);
+
 /* Plain autoregressive draw at one site (deepar/samplers/autoregressive.py:
  * 42-44): spins[b, site] = +1 if exp(cond_logp[b, site, 0]) > u[b, site]
  * else -1, for all b; cond_logp (batch, n_sites, 2) float, u (batch, n_sites)

@@ -77,3 +77,28 @@ def test_unsupported_shapes_are_errors(lib):
     nw, na = ctypes.c_int64(), ctypes.c_int64()
     assert _lib.load().fk_convar_packed_sizes(3, 24, 1, ctypes.byref(nw), ctypes.byref(na)) == 3
     assert b'not implemented' in _lib.load().fk_last_error()
+
+
+def test_schedule_table(lib):
+    """Host-only schedule selection: table sizes, range checks, set/get/reset."""
+    from flowket_amd import _lib
+    L = _lib.load()
+    assert L.fk_convar_num_schedules(32, 1) == 4
+    assert L.fk_convar_num_schedules(16, 0) == 4
+    assert L.fk_convar_num_schedules(64, 1) == 2
+    assert L.fk_convar_num_schedules(24, 1) == 0
+    t, w = ctypes.c_int32(), ctypes.c_int32()
+    seen = set()
+    for i in range(4):
+        assert L.fk_convar_schedule_info(32, 1, i, ctypes.byref(t), ctypes.byref(w)) == 0
+        seen.add((t.value, w.value))
+    assert len(seen) == 4 and (0, 8) in seen
+    assert L.fk_convar_schedule_info(32, 1, 4, ctypes.byref(t), ctypes.byref(w)) == 1
+    assert L.fk_convar_set_schedule(10, 10, 32, 1, 4) == 1
+    assert b'out of range' in L.fk_last_error()
+    assert L.fk_convar_get_schedule(10, 10, 32, 1) == 0
+    assert L.fk_convar_set_schedule(10, 10, 32, 1, 2) == 0
+    assert L.fk_convar_get_schedule(10, 10, 32, 1) == 2
+    assert L.fk_convar_get_schedule(10, 9, 32, 1) == 0
+    assert L.fk_convar_set_schedule(10, 10, 32, 1, -1) == 0
+    assert L.fk_convar_get_schedule(10, 10, 32, 1) == 0

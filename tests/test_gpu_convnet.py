@@ -130,3 +130,58 @@ def test_connected_equals_dense(rng):
         dense = conn[pairs[:, 1], pairs[:, 0]]
         lp2, _ = m.forward(dense.contiguous())
         assert torch.equal(lp, lp2)
+
+
+@pytest.mark.parametrize('shape,depth,C,act,prec', [((10, 10), 4, 32, 'elu', 'bf16'), ((7, 9), 3, 32, 'relu', 'bf16'),
+                                                    ((6, 5), 3, 16, 'relu', 'fp32'), ((10, 10), 3, 64, 'elu', 'bf16'),
+                                                    ((4, 4), 2, 32, 'relu', 'fp32')])
+def test_every_schedule_matches_oracle(rng, shape, depth, C, act, prec):
+    """Each compiled launch schedule (loop order x waves per workgroup)
+    computes the same function, on plain rows and on ragged connected
+    configurations with the count read from device memory."""
+    import flowket_amd as fk
+    from flowket_amd import _lib, autotune
+    gpu_banner(f'every schedule shape={shape} depth={depth} C={C} {act} {prec}')
+    m = _machine(shape, depth, C, act, prec)
+    pc = _lib.FK_PREC_BF16 if prec == 'bf16' else _lib.FK_PREC_FP32
+    s = rng.choice([-1, 1], size=(45,) + shape).astype(np.int8)
+    ref = ocn.log_psi(_oracle_params(m), s, depth, activation=act)
+    op = fk.Heisenberg(hilbert_state_shape=shape, pbc=True)
+    sd = torch.as_tensor(s.reshape(45, -1)).cuda()
+    r = op.compact(sd)
+    B = sd.shape[0]
+    outs = []
+    try:
+        for sc in autotune.schedules(C, pc):
+            autotune.set_schedule(m, pc, sc['index'])
+            got = m.log_psi(s)
+            _check_logpsi(got, ref, FP32_RTOL if prec == 'fp32' else BF16_RTOL)
+            lp = m.log_psi_connected(sd, r['pairs'], r['pairs'].shape[0], op, m_dev=r['offsets'][B:])
+            outs.append(lp[:int(r['offsets'][-1])].clone())
+    finally:
+        autotune.set_schedule(m, pc, -1)
+    for o in outs[1:]:
+        assert torch.allclose(o, outs[0], rtol=1e-5, atol=1e-4)
+
+
+def test_autotune_selects_and_caches(rng, tmp_path, monkeypatch):
+    import flowket_amd as fk
+    from flowket_amd import _lib, autotune
+    gpu_banner('autotune 8x8 depth 3 C32 bf16')
+    monkeypatch.setenv('FLOWKET_AMD_TUNE_CACHE', str(tmp_path / 'tune.json'))
+    m = _machine((8, 8), 3, 32, 'relu', 'bf16')
+    op = fk.Heisenberg(hilbert_state_shape=(8, 8), pbc=True)
+    sd = torch.as_tensor(rng.choice([-1, 1], size=(256, 64)).astype(np.int8)).cuda()
+    obs = fk.Observable(op)
+    ref = obs.local_values(m, sd)
+    try:
+        res = autotune.tune_forward(m, lambda: obs.local_values(m, sd), reps=2)
+        assert len(res['ms']) == 4 and all(t > 0 for t in res['ms'])
+        assert autotune.get_schedule(m, _lib.FK_PREC_BF16) == res['schedule']
+        assert (tmp_path / 'tune.json').exists()
+        autotune.set_schedule(m, _lib.FK_PREC_BF16, -1)
+        assert autotune.apply_cached(m, _lib.FK_PREC_BF16) == res['schedule']
+        got = obs.local_values(m, sd)
+        assert torch.allclose(got, ref, rtol=1e-4, atol=1e-3)
+    finally:
+        autotune.set_schedule(m, _lib.FK_PREC_BF16, -1)
