@@ -13,6 +13,7 @@ from .optimization import VariationalMonteCarlo, MiniBatchGenerator, loss_for_en
 from .optimizers import ComplexValuesStochasticReconfiguration, Adam, energy_gradient, conjugate_gradient  # noqa: F401
 
 from . import autotune  # noqa: F401
+from .graphs import LocalEnergyGraph  # noqa: F401
 from .proposals import MADEProposal, MetropolisHastingsLearnedProposal  # noqa: F401
 
 __version__ = '0.1.0'

@@ -249,17 +249,21 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   const ConvarLayout lay(C, PREC);
 
   extern __shared__ __align__(16) uint8_t lds[];
+  // npix: pixels covered by the 16-pixel tiles; NPX = T*HW real pixels.
+  // Each buffer holds the NPX pixels plus one all-zero pixel (index NPX) that
+  // padding taps and the tile lanes past NPX read.
   const int npix = a.npix;
-  // each buffer has npix pixels plus one all-zero pixel (index npix) that
-  // padding taps read
-  const int BUFB = (npix + 1) * PIXB;
+  const int NPX = a.T * a.HW;
+  const int BUFB = (NPX + 1) * PIXB;
   uint8_t* V0 = lds;
   uint8_t* H0 = V0 + BUFB;
   uint8_t* V1 = H0 + BUFB;
   uint8_t* H1 = V1 + BUFB;
   uint8_t* CAT = H1 + BUFB;
   float* sp = reinterpret_cast<float*>(CAT + BUFB);
-  float2* part = reinterpret_cast<float2*>(sp + npix);
+  // per-site log-probabilities of the final block: V1 is neither read nor
+  // written by the final block, so they reuse its space
+  float2* part = reinterpret_cast<float2*>(V1);
 
   const int lane = threadIdx.x & 63;
   // wave index and tile count are wave-uniform: keep them in SGPRs so tile
@@ -298,13 +302,13 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
       const int dr = k / 4 - 2, dc = k % 4 - 2;
-      const int pp = ((m >> k) & 1u) ? p + dr * W + dc : npix;
+      const int pp = ((m >> k) & 1u) ? p + dr * W + dc : NPX;
       toff[u][k] = (uint32_t)L::chunk(pp, L::base_chunk(g));
     }
   }
   for (int z = threadIdx.x; z < 5 * (PIXB / 4); z += blockDim.x) {
     const int buf = z / (PIXB / 4), w4 = z - buf * (PIXB / 4);
-    reinterpret_cast<uint32_t*>(lds + buf * BUFB + npix * PIXB)[w4] = 0u;
+    reinterpret_cast<uint32_t*>(lds + buf * BUFB + NPX * PIXB)[w4] = 0u;
   }
 
   int64_t M = a.m_dev ? *a.m_dev : a.m_count;
@@ -315,11 +319,11 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
 
   for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     // ---- configurations -> LDS (float spins; connection applied on the fly)
-    for (int p = threadIdx.x; p < npix; p += blockDim.x) {
+    for (int p = threadIdx.x; p < NPX; p += blockDim.x) {
       float s = 0.0f;
       const int t = p / HW;
       const int64_t m = grp * T + t;
-      if (p < T * HW && m < M) {
+      if (m < M) {
         const int q = p - t * HW;
         int64_t b = m;
         int c = 0;
@@ -628,7 +632,9 @@ struct FwdPlan {
   size_t lds;
 };
 
-static size_t fwd_lds_bytes(int npix, int pixb) { return (size_t)(npix + 1) * pixb * 5 + (size_t)npix * (4 + 8); }
+// five activation buffers of P + 1 pixels and the float spins (the per-site
+// log-probabilities live in V1)
+static size_t fwd_lds_bytes(int P, int pixb) { return (size_t)(P + 1) * pixb * 5 + (size_t)P * 4; }
 
 // One launch shape of the forward: loop order and waves per workgroup.
 struct FwdSchedule {
@@ -670,7 +676,7 @@ static int plan_forward(int HW, int C, int prec, int NW, int64_t m, FwdPlan* pla
   int best = 0;
   for (int T = 1; T <= 256; ++T) {
     const int npix = ((T * HW + 15) / 16) * 16;
-    if (fwd_lds_bytes(npix, pixb) > cap) break;
+    if (fwd_lds_bytes(T * HW, pixb) > cap) break;
     const int per_wave = (npix / 16 + NW - 1) / NW;
     if (per_wave > maxt_cap) break;
     best = T;
@@ -679,7 +685,7 @@ static int plan_forward(int HW, int C, int prec, int NW, int64_t m, FwdPlan* pla
   if (best == 0) return FK_ERR_UNSUPPORTED;
   plan->T = best;
   plan->npix = ((best * HW + 15) / 16) * 16;
-  plan->lds = fwd_lds_bytes(plan->npix, pixb);
+  plan->lds = fwd_lds_bytes(best * HW, pixb);
   plan->nwave = NW;
   const int per_wave = (plan->npix / 16 + NW - 1) / NW;
   plan->maxt = per_wave <= 2 ? 2 : (per_wave <= 4 ? 4 : 8);
@@ -723,11 +729,7 @@ static int dispatch_fwd_c(const FwdArgs& a, const FwdPlan& pl, const FwdSchedule
   return sc.nwave == 4 ? launch_variant<PREC, C, 4, 0>(a, pl, act, s) : launch_variant<PREC, C, 8, 0>(a, pl, act, s);
 }
 
-bool convar_supported(int channels, int precision) {
-  if (precision == FK_PREC_BF16) return channels == 32 || channels == 64;
-  if (precision == FK_PREC_FP32) return channels == 16 || channels == 32 || channels == 64;
-  return false;
-}
+bool convar_supported(int channels, int precision) { return kernel_channels(channels, precision) > 0; }
 
 }  // namespace fk
 
@@ -742,7 +744,7 @@ int fk_convar_packed_sizes(int32_t depth, int32_t channels, int32_t precision, i
     set_error("ConvNetAutoregressive2D kernels: channels=%d precision=%d not implemented", channels, precision);
     return FK_ERR_UNSUPPORTED;
   }
-  const ConvarLayout lay(channels, precision);
+  const ConvarLayout lay(kernel_channels(channels, precision), precision);
   const int nb = num_causal_blocks(depth);
   *n_weights = lay.total_frags(nb) * 64 * lay.E;
   *n_aux = lay.aux_total(nb);
@@ -755,9 +757,11 @@ int fk_convar_pack_maps(int32_t depth, int32_t channels, int32_t precision, int3
   if (rc != FK_OK) return rc;
   FK_REQUIRE(wmap && amap, "fk_convar_pack_maps: null pointer");
   const int C = channels;
-  const ConvarLayout lay(C, precision);
+  const int CP = kernel_channels(channels, precision);
+  const ConvarLayout lay(CP, precision);
   const int nb = num_causal_blocks(depth);
   for (int64_t i = 0; i < nw; ++i) wmap[i] = -1;
+  for (int64_t i = 0; i < na; ++i) amap[i] = -1;
   // canonical offsets
   int64_t src = 0;
   struct Conv {
@@ -769,7 +773,14 @@ int fk_convar_pack_maps(int32_t depth, int32_t channels, int32_t precision, int3
     src = c.boff + cout;
     return c;
   };
-  auto fill = [&](int64_t frag_base, const Conv& cv, int mto) {
+  // kernel input channel -> channel of the C-channel network (-1: padding);
+  // `cat`: the input is the concatenation [x | y] whose y half starts at CP/2
+  auto real_cin = [&](int pc, const Conv& cv, bool cat) {
+    if (!cat) return pc < cv.cin ? pc : -1;
+    if (pc < CP / 2) return pc < C / 2 ? pc : -1;
+    return pc - CP / 2 < C / 2 ? C / 2 + (pc - CP / 2) : -1;
+  };
+  auto fill = [&](int64_t frag_base, const Conv& cv, int mto, bool cat) {
     const int ntap = cv.kh * cv.kw;
     for (int tap = 0; tap < ntap; ++tap)
       for (int mt = 0; mt < mto; ++mt)
@@ -778,11 +789,12 @@ int fk_convar_pack_maps(int32_t depth, int32_t channels, int32_t precision, int3
           for (int lane = 0; lane < 64; ++lane) {
             const int m = lane % 16, g = lane / 16;
             for (int e = 0; e < lay.E; ++e) {
-              const int cin = precision == FK_PREC_BF16 ? slot_channel_bf16(32 * ks + 8 * g + e)
-                                                        : slot_channel_f32(ks, g);
+              const int pc = precision == FK_PREC_BF16 ? slot_channel_bf16(32 * ks + 8 * g + e)
+                                                       : slot_channel_f32(ks, g);
+              const int cin = real_cin(pc, cv, cat);
               const int cout = 16 * mt + m;
               const int64_t idx = (frag * 64 + lane) * lay.E + e;
-              if (cout < cv.cout && cin < cv.cin)
+              if (cout < cv.cout && cin >= 0)
                 wmap[idx] = (int32_t)(cv.koff + ((int64_t)tap * cv.cin + cin) * cv.cout + cout);
             }
           }
@@ -797,23 +809,26 @@ int fk_convar_pack_maps(int32_t depth, int32_t channels, int32_t precision, int3
          cc = take(3, 3, C, C);
     const int64_t ab = lay.aux_bias(b);
     bias_copy(ab, v);
-    bias_copy(ab + C, h);
-    bias_copy(ab + 2 * C, x);
-    bias_copy(ab + 2 * C + C / 2, y);
-    bias_copy(ab + 3 * C, cc);
+    bias_copy(ab + CP, h);
+    bias_copy(ab + 2 * CP, x);
+    bias_copy(ab + 2 * CP + CP / 2, y);
+    bias_copy(ab + 3 * CP, cc);
     if (b == 0) {
-      for (int i = 0; i < 9 * C; ++i) amap[lay.aux_kv0(nb) + i] = (int32_t)(v.koff + i);
-      for (int i = 0; i < 3 * C; ++i) amap[lay.aux_kh0(nb) + i] = (int32_t)(h.koff + i);
+      // (3,3,1,C) and (1,3,1,C) kernels, [tap][channel] with stride CP
+      for (int t = 0; t < 9; ++t)
+        for (int c = 0; c < C; ++c) amap[lay.aux_kv0(nb) + t * CP + c] = (int32_t)(v.koff + t * C + c);
+      for (int t = 0; t < 3; ++t)
+        for (int c = 0; c < C; ++c) amap[lay.aux_kh0(nb) + t * CP + c] = (int32_t)(h.koff + t * C + c);
     } else {
-      fill(lay.off_v(b), v, lay.MT);
-      fill(lay.off_h(b), h, lay.MT);
+      fill(lay.off_v(b), v, lay.MT, false);
+      fill(lay.off_h(b), h, lay.MT, false);
     }
-    fill(lay.off_x(b), x, lay.MTH);
-    fill(lay.off_y(b), y, lay.MTH);
-    fill(lay.off_c(b), cc, lay.MT);
+    fill(lay.off_x(b), x, lay.MTH, false);
+    fill(lay.off_y(b), y, lay.MTH, false);
+    fill(lay.off_c(b), cc, lay.MT, true);
   }
   Conv head = take(1, 1, C, 4);
-  fill(lay.head_base(nb), head, 1);
+  fill(lay.head_base(nb), head, 1, false);
   for (int c = 0; c < 4; ++c) amap[lay.aux_head_bias(nb) + c] = (int32_t)(head.boff + c);
   return FK_OK;
 }
@@ -841,13 +856,14 @@ int fk_convar_forward(const void* wpacked, const float* aux, int32_t precision, 
     return FK_ERR_UNSUPPORTED;
   }
   if (m_count <= 0) return FK_OK;
+  const int CP = kernel_channels(channels, precision);
   const FwdSchedule* table = nullptr;
-  const int nsched = fwd_schedules(channels, precision, &table);
-  int si = current_schedule(H, W, channels, precision);
+  const int nsched = fwd_schedules(CP, precision, &table);
+  int si = current_schedule(H, W, CP, precision);
   if (si < 0 || si >= nsched) si = 0;
   const FwdSchedule sc = table[si];
   FwdPlan pl;
-  if (plan_forward(H * W, channels, precision, sc.nwave, m_count, &pl) != FK_OK) {
+  if (plan_forward(H * W, CP, precision, sc.nwave, m_count, &pl) != FK_OK) {
     set_error("ConvNetAutoregressive2D kernels: lattice %dx%d with %d channels does not fit in LDS", H, W, channels);
     return FK_ERR_UNSUPPORTED;
   }
@@ -871,18 +887,18 @@ int fk_convar_forward(const void* wpacked, const float* aux, int32_t precision, 
   a.cond = cond_logp;
   hipStream_t s = (hipStream_t)stream;
   if (precision == FK_PREC_BF16) {
-    if (channels == 32) return dispatch_fwd_c<FK_PREC_BF16, 32>(a, pl, sc, activation, s);
+    if (CP == 32) return dispatch_fwd_c<FK_PREC_BF16, 32>(a, pl, sc, activation, s);
     return dispatch_fwd_c<FK_PREC_BF16, 64>(a, pl, sc, activation, s);
   }
-  if (channels == 16) return dispatch_fwd_c<FK_PREC_FP32, 16>(a, pl, sc, activation, s);
-  if (channels == 32) return dispatch_fwd_c<FK_PREC_FP32, 32>(a, pl, sc, activation, s);
+  if (CP == 16) return dispatch_fwd_c<FK_PREC_FP32, 16>(a, pl, sc, activation, s);
+  if (CP == 32) return dispatch_fwd_c<FK_PREC_FP32, 32>(a, pl, sc, activation, s);
   return dispatch_fwd_c<FK_PREC_FP32, 64>(a, pl, sc, activation, s);
 }
 
 int32_t fk_convar_num_schedules(int32_t channels, int32_t precision) {
   if (!convar_supported(channels, precision)) return 0;
   const FwdSchedule* t = nullptr;
-  return fwd_schedules(channels, precision, &t);
+  return fwd_schedules(kernel_channels(channels, precision), precision, &t);
 }
 
 int fk_convar_schedule_info(int32_t channels, int32_t precision, int32_t schedule, int32_t* tile_outer,
@@ -891,7 +907,7 @@ int fk_convar_schedule_info(int32_t channels, int32_t precision, int32_t schedul
   const int n = fk_convar_num_schedules(channels, precision);
   FK_REQUIRE(schedule >= 0 && schedule < n, "fk_convar_schedule_info: schedule out of range");
   const FwdSchedule* t = nullptr;
-  fwd_schedules(channels, precision, &t);
+  fwd_schedules(kernel_channels(channels, precision), precision, &t);
   *tile_outer = t[schedule].sched;
   *waves = t[schedule].nwave;
   return FK_OK;
@@ -901,14 +917,15 @@ int fk_convar_set_schedule(int32_t H, int32_t W, int32_t channels, int32_t preci
   const int n = fk_convar_num_schedules(channels, precision);
   FK_REQUIRE(n > 0, "fk_convar_set_schedule: unsupported channels/precision");
   FK_REQUIRE(schedule >= -1 && schedule < n, "fk_convar_set_schedule: schedule out of range");
+  const int CP = kernel_channels(channels, precision);
   std::lock_guard<std::mutex> g(g_sched_mu);
-  if (schedule < 0) g_sched.erase(sched_key(H, W, channels, precision));
-  else g_sched[sched_key(H, W, channels, precision)] = schedule;
+  if (schedule < 0) g_sched.erase(sched_key(H, W, CP, precision));
+  else g_sched[sched_key(H, W, CP, precision)] = schedule;
   return FK_OK;
 }
 
 int32_t fk_convar_get_schedule(int32_t H, int32_t W, int32_t channels, int32_t precision) {
-  return current_schedule(H, W, channels, precision);
+  return current_schedule(H, W, kernel_channels(channels, precision), precision);
 }
 
 }  // extern "C"

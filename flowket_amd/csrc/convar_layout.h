@@ -17,6 +17,13 @@
 // Aux fp32 buffer: per block b at 4*C*b: bv(C) bh(C) bx(C/2) by(C/2) bc(C);
 // head bias (4) at 4*C*nb; block-0 vertical kernel (9*C, [tap][c]) and
 // horizontal kernel (3*C) after it.
+//
+// Channel padding: a network of C channels runs on the kernels compiled for
+// CP = kernel_channels(C) >= C (bf16: 32, 64; fp32: 16, 32, 64).  The layout
+// above is then taken with CP; channels >= C of every convolution and the
+// x/y halves past C/2 get zero weights and biases, so their activations stay
+// exactly zero and the result is the C-channel network's.  In the padded
+// concatenation [x | y] the y half starts at channel CP/2.
 #pragma once
 #include <cstdint>
 
@@ -72,6 +79,16 @@ FK_HD inline int slot_channel_bf16(int s) {
   return (s & ~31) | (((s >> 2) & 1) << 4) | (((s >> 3) & 3) << 2) | (s & 3);
 }
 FK_HD inline int slot_channel_f32(int t, int g) { return 16 * (t >> 2) + 4 * g + (t & 3); }
+
+// Kernel channel width for a C-channel network, 0 if none fits.
+inline int kernel_channels(int channels, int precision) {
+  if (channels < 2 || (channels & 1)) return 0;
+  if (precision != 0 && precision != 1) return 0;
+  const int widths[3] = {16, 32, 64};
+  for (int i = (precision == 1) ? 1 : 0; i < 3; ++i)
+    if (channels <= widths[i]) return widths[i];
+  return 0;
+}
 
 inline int num_causal_blocks(int depth) { return 2 + 2 * (depth > 2 ? depth - 2 : 0); }
 

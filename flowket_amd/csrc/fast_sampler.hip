@@ -409,9 +409,14 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
   }
 }
 
-bool fast_sampler_supported(int channels, int precision) {
-  return (precision == FK_PREC_BF16 || precision == FK_PREC_FP32) && (channels == 32 || channels == 64);
+// the fast sampler runs the 32- and 64-channel kernel widths (networks of
+// other even widths up to 64 are zero-padded to them, see kernel_channels;
+// the fp32 16-wide packing has no fast-sampler kernel)
+static int fast_channels(int channels, int precision) {
+  const int cp = kernel_channels(channels, precision);
+  return cp == 16 ? 0 : cp;
 }
+bool fast_sampler_supported(int channels, int precision) { return fast_channels(channels, precision) > 0; }
 
 int64_t fast_tile_bytes(int HW, int nb, int channels, int precision) {
   const int esz = precision == FK_PREC_BF16 ? 2 : 4;
@@ -439,7 +444,7 @@ int64_t fk_fast_sampler_workspace_bytes(int32_t H, int32_t W, int32_t depth, int
                                         int64_t batch) {
   if (H < 1 || W < 1 || depth < 1 || batch < 0 || !fast_sampler_supported(channels, precision)) return -1;
   const int64_t tiles = (batch + 15) / 16;
-  return tiles * fast_tile_bytes(H * W, num_causal_blocks(depth), channels, precision);
+  return tiles * fast_tile_bytes(H * W, num_causal_blocks(depth), fast_channels(channels, precision), precision);
 }
 
 int fk_fast_sample(const void* wpacked, const float* aux, int32_t precision, int32_t H, int32_t W, int32_t depth,
@@ -448,7 +453,8 @@ int fk_fast_sample(const void* wpacked, const float* aux, int32_t precision, int
   FK_REQUIRE(H >= 1 && W >= 1 && depth >= 1 && batch >= 0, "fk_fast_sample: bad shape");
   FK_REQUIRE(activation == FK_ACT_RELU || activation == FK_ACT_ELU, "fk_fast_sample: bad activation");
   if (!fast_sampler_supported(channels, precision)) {
-    set_error("fast sampler: channels=%d precision=%d not implemented (channels 32 or 64)", channels, precision);
+    set_error("fast sampler: channels=%d precision=%d not implemented (17..64 channels, bf16 2..64)", channels,
+              precision);
     return FK_ERR_UNSUPPORTED;
   }
   if (batch == 0) return FK_OK;
@@ -465,15 +471,16 @@ int fk_fast_sample(const void* wpacked, const float* aux, int32_t precision, int
   a.out = out;
   a.cond = cond_logp;
   a.cache = reinterpret_cast<uint8_t*>(workspace);
-  a.tile_bytes = fast_tile_bytes(H * W, a.nb, channels, precision);
+  const int CP = fast_channels(channels, precision);
+  a.tile_bytes = fast_tile_bytes(H * W, a.nb, CP, precision);
   const int64_t tiles = (batch + 15) / 16;
   FK_REQUIRE(tiles <= 0x7fffffff, "fk_fast_sample: batch too large");
   hipStream_t s = (hipStream_t)stream;
   if (precision == FK_PREC_BF16) {
-    if (channels == 32) return launch_fast<FK_PREC_BF16, 32>(a, activation, tiles, s);
+    if (CP == 32) return launch_fast<FK_PREC_BF16, 32>(a, activation, tiles, s);
     return launch_fast<FK_PREC_BF16, 64>(a, activation, tiles, s);
   }
-  if (channels == 32) return launch_fast<FK_PREC_FP32, 32>(a, activation, tiles, s);
+  if (CP == 32) return launch_fast<FK_PREC_FP32, 32>(a, activation, tiles, s);
   return launch_fast<FK_PREC_FP32, 64>(a, activation, tiles, s);
 }
 

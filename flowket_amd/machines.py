@@ -130,7 +130,11 @@ class ConvNetAutoregressive2D:
         p = torch.as_tensor(params, dtype=torch.float32).reshape(-1)
         if p.numel() != self.n_params:
             raise ValueError(f'expected {self.n_params} parameters, got {p.numel()}')
-        self.params = p.to(self.device).contiguous()
+        if getattr(self, 'params', None) is not None:
+            # in place: device pointers stay valid for captured graphs
+            self.params.copy_(p.to(self.device))
+        else:
+            self.params = p.to(self.device).clone().contiguous()
         if self.weights_normalization and not hasattr(self, 'wn_g'):
             self.wn_g = self._wn_initial_g(self.params.cpu())
         if self.weights_normalization:
@@ -163,10 +167,13 @@ class ConvNetAutoregressive2D:
         return torch.cat(parts)
 
     def _pack(self, precision):
+        """Packed (fragment-order) weight and side buffers of ``precision``.
+        They are allocated once per precision and repacked in place (on the
+        current stream) when the parameters changed, so a captured graph that
+        reads them sees every update."""
         prec = _PREC[precision]
-        key = (prec, self._version)
-        if key in self._packs:
-            return self._packs[key]
+        if self._packs.get(prec, (None, -1))[1] == self._version:
+            return self._packs[prec][0]
         _lib.require_gpu()
         if prec not in self._maps:
             nw, na = _lib.ctypes.c_int64(), _lib.ctypes.c_int64()
@@ -181,14 +188,17 @@ class ConvNetAutoregressive2D:
             autotune.apply_cached(self, prec)
         wmap, amap = self._maps[prec]
         eff = self.effective_params().contiguous()
-        wdtype = torch.bfloat16 if prec == _lib.FK_PREC_BF16 else torch.float32
-        wbuf = torch.empty(wmap.numel(), dtype=wdtype, device=self.device)
-        abuf = torch.empty(amap.numel(), dtype=torch.float32, device=self.device)
+        if prec in self._packs:
+            wbuf, abuf = self._packs[prec][0]
+        else:
+            wdtype = torch.bfloat16 if prec == _lib.FK_PREC_BF16 else torch.float32
+            wbuf = torch.empty(wmap.numel(), dtype=wdtype, device=self.device)
+            abuf = torch.empty(amap.numel(), dtype=torch.float32, device=self.device)
         s = _lib.stream_ptr()
         _lib.call('fk_pack_gather', _lib.ptr(eff), _lib.ptr(wmap), wmap.numel(), prec, _lib.ptr(wbuf), s)
         _lib.call('fk_pack_gather', _lib.ptr(eff), _lib.ptr(amap), amap.numel(), _lib.FK_PREC_FP32,
                   _lib.ptr(abuf), s)
-        self._packs = {key: (wbuf, abuf)}
+        self._packs[prec] = ((wbuf, abuf), self._version)
         return wbuf, abuf
 
     # ------------------------------------------------------------------ eval
@@ -308,7 +318,11 @@ class RBM:
         p = torch.as_tensor(params, dtype=torch.float32).reshape(-1)
         if p.numel() != self.n_params:
             raise ValueError(f'expected {self.n_params} parameters, got {p.numel()}')
-        self.params = p.to(self.device).contiguous()
+        if getattr(self, 'params', None) is not None:
+            # in place: device pointers stay valid for captured graphs
+            self.params.copy_(p.to(self.device))
+        else:
+            self.params = p.to(self.device).clone().contiguous()
 
     def mark_params_updated(self):
         pass

@@ -110,6 +110,11 @@ int fk_energy_moments(const double* eloc, int64_t batch, double* out,
  * Keras kernel layout (kh, kw, cin, cout)).  The kernels read them from two
  * packed device buffers (MFMA fragment order + an fp32 side buffer) produced
  * by fk_pack_gather with the maps from fk_convar_pack_maps.
+ *
+ * Channels: any even count up to 64 (bf16) / 2..64 (fp32).  The kernels are
+ * compiled for 32 and 64 (bf16) and 16, 32, 64 (fp32) channels; other widths
+ * run on the next wider kernel with zero weights and biases in the padding
+ * channels (exactly the narrower network's result, at the wider cost).
  * ------------------------------------------------------------------------- */
 
 /* Host-only.  Sizes (elements) of the packed buffers; returns FK_OK or
@@ -172,7 +177,8 @@ int fk_ar_draw_site(const float* cond_logp, const double* uniforms,
  * (batch, H*W) double.  Output spins (batch, H*W) int8; cond_logp (batch,
  * H*W, 2) float = log-probabilities of each site's conditional, or NULL.
  * `workspace` holds fk_fast_sampler_workspace_bytes(...) bytes (the
- * activation cache; contents need no initialisation).  Channels 32 or 64.
+ * activation cache; contents need no initialisation).  Channels 17..64
+ * (bf16: 2..64), run on the 32- or 64-channel kernel as above.
  * ------------------------------------------------------------------------- */
 int64_t fk_fast_sampler_workspace_bytes(int32_t H, int32_t W, int32_t depth,
                                         int32_t channels, int32_t precision,

@@ -58,7 +58,8 @@ def _maps(depth, C, prec):
     return wmap, amap
 
 
-@pytest.mark.parametrize('depth,C,prec', [(5, 16, 0), (4, 32, 0), (20, 32, 1), (3, 64, 1), (2, 32, 1), (1, 16, 0)])
+@pytest.mark.parametrize('depth,C,prec', [(5, 16, 0), (4, 32, 0), (20, 32, 1), (3, 64, 1), (2, 32, 1), (1, 16, 0),
+                                          (3, 48, 1), (2, 24, 0), (2, 8, 0), (3, 40, 1), (2, 6, 1)])
 def test_pack_maps_cover_every_parameter_once(lib, depth, C, prec):
     """Every network parameter appears in the packed buffers exactly once
     (weights of MFMA convolutions once in the fragment buffer, biases and the
@@ -75,8 +76,14 @@ def test_pack_maps_cover_every_parameter_once(lib, depth, C, prec):
 def test_unsupported_shapes_are_errors(lib):
     from flowket_amd import _lib
     nw, na = ctypes.c_int64(), ctypes.c_int64()
-    assert _lib.load().fk_convar_packed_sizes(3, 24, 1, ctypes.byref(nw), ctypes.byref(na)) == 3
-    assert b'not implemented' in _lib.load().fk_last_error()
+    for C, prec in ((25, 1), (66, 1), (72, 0), (0, 0)):
+        assert _lib.load().fk_convar_packed_sizes(3, C, prec, ctypes.byref(nw), ctypes.byref(na)) == 3
+        assert b'not implemented' in _lib.load().fk_last_error()
+    # padded widths share the wider kernel's packing
+    assert _lib.load().fk_convar_packed_sizes(3, 48, 1, ctypes.byref(nw), ctypes.byref(na)) == 0
+    n48 = nw.value
+    assert _lib.load().fk_convar_packed_sizes(3, 64, 1, ctypes.byref(nw), ctypes.byref(na)) == 0
+    assert nw.value == n48
 
 
 def test_schedule_table(lib):
@@ -86,7 +93,9 @@ def test_schedule_table(lib):
     assert L.fk_convar_num_schedules(32, 1) == 4
     assert L.fk_convar_num_schedules(16, 0) == 4
     assert L.fk_convar_num_schedules(64, 1) == 2
-    assert L.fk_convar_num_schedules(24, 1) == 0
+    assert L.fk_convar_num_schedules(24, 1) == 4
+    assert L.fk_convar_num_schedules(48, 1) == 2
+    assert L.fk_convar_num_schedules(25, 1) == 0
     t, w = ctypes.c_int32(), ctypes.c_int32()
     seen = set()
     for i in range(4):

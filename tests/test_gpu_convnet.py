@@ -62,6 +62,11 @@ CASES = [
     ((10, 10), 3, 64, 'elu', 'fp32'),
     ((10, 10), 3, 64, 'elu', 'bf16'),
     ((4, 4), 1, 32, 'relu', 'bf16'),
+    # widths without their own kernel: zero-padded to the next compiled one
+    ((10, 10), 3, 48, 'elu', 'bf16'),
+    ((6, 6), 2, 24, 'relu', 'fp32'),
+    ((5, 4), 2, 8, 'elu', 'fp32'),
+    ((4, 4), 2, 6, 'relu', 'bf16'),
 ]
 
 

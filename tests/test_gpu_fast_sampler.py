@@ -15,7 +15,9 @@ pytestmark = pytest.mark.gpu
                                                      ((4, 4), 2, 32, 'relu', 'fp32', 33),
                                                      ((10, 10), 4, 32, 'relu', 'bf16', 48),
                                                      ((5, 7), 3, 64, 'elu', 'fp32', 20),
-                                                     ((5, 7), 3, 64, 'elu', 'bf16', 20)])
+                                                     ((5, 7), 3, 64, 'elu', 'bf16', 20),
+                                                     ((6, 6), 3, 48, 'relu', 'bf16', 40),
+                                                     ((4, 5), 2, 24, 'elu', 'fp32', 30)])
 def test_fast_equals_plain_given_uniforms(rng, shape, depth, C, act, prec, B):
     """Same uniforms -> the same draws as AutoregressiveSampler (identical
     arithmetic), and the conditionals it drew from equal the full forward's."""
