@@ -46,9 +46,14 @@ def parse():
     ap.add_argument('--batch', type=int, default=BATCH)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline')
     ap.add_argument('--no-fp32', action='store_true', help='skip the FP32 plain-path comparison')
+    ap.add_argument('--no-sr', action='store_true', help='skip the SR step (per-sample gradients + CG) leg')
+    ap.add_argument('--no-mh', action='store_true', help='skip the Metropolis-Hastings RBM leg')
     ap.add_argument('--profile-forward', type=int, default=0, metavar='N',
                     help='only run two fused-forward launches over the connected configurations of N random '
                          'samples (a short program for rocprofv3 counter passes) and exit')
+    ap.add_argument('--profile-copy', action='store_true',
+                    help='only run two 256 MiB device-to-device copies (the calibration kernel for the '
+                         'FETCH_SIZE / WRITE_SIZE counters) and exit')
     ap.add_argument('--precision', default='bf16', choices=['bf16', 'fp32'])
     ap.add_argument('--schedule', type=int, default=None,
                     help='force a forward launch schedule (index into the compiled table) instead of autotuning')
@@ -76,10 +81,110 @@ def profile_forward(n, precision, schedule):
     return 0
 
 
+def profile_copy():
+    import torch
+    src = torch.ones(256 << 20, dtype=torch.uint8, device='cuda')
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({'profile_copy': {'bytes': src.numel()}}))
+    return 0
+
+
 def flops_per_forward():
     C = CHANNELS
     nb = 2 * DEPTH - 2
     return 2 * H * W * ((10 * C * C + 12 * C) + (nb - 1) * 22 * C * C + 4 * C)
+
+
+def bench_sr(machine, op, spins, shift=1e-3, tol=1e-6, max_iter=50):
+    """One SR step of the headline configuration: local energies (bf16 ragged
+    path), per-sample O_k of every sample (FP32 gradient kernel, O stored
+    B x P complex as two FP32 planes), force and CG on the implicit
+    S = Re[(O-Obar)^H (O-Obar)]/B + shift (FP32 products, FP64 reductions)."""
+    import torch
+    import flowket_amd as fk
+    from flowket_amd.optimizers import _OFactor, _complex_eloc_pairs, conjugate_gradient
+    B = spins.shape[0]
+    eloc = fk.Observable(op).local_values(machine, spins)
+    E = eloc.mean()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    ev[0].record()
+    of = _OFactor(machine, spins)
+    ev[1].record()
+    u = _complex_eloc_pairs(eloc, E)
+    force = of.oh(u, 1.0 / of.B)
+    x, it, res = conjugate_gradient(lambda v, out: of.s_matvec(v, shift, out), of.dot, force, tol=tol,
+                                    max_iter=max_iter)
+    ev[2].record()
+    # one S.v product on its own (two passes over O)
+    z = torch.empty_like(force)
+    ev[3].record()
+    of.s_matvec(force, shift, z)
+    ev[4].record()
+    torch.cuda.synchronize()
+    P = of.P
+    grad_ms = ev[0].elapsed_time(ev[1])
+    cg_ms = ev[1].elapsed_time(ev[2])
+    mv_ms = ev[3].elapsed_time(ev[4])
+    o_bytes = 2 * 4 * B * P
+    out = {'batch': B, 'params': P, 'grad_ms': round(grad_ms, 2),
+           'grad_tflops': round(B * 2 * flops_per_forward() / (grad_ms * 1e-3) / 1e12, 2),
+           'grad_o_write_gbs': round(o_bytes / (grad_ms * 1e-3) / 1e9, 1),
+           'cg_ms': round(cg_ms, 2), 'cg_iterations': it, 'cg_residual': res,
+           'cg_force_norm': float(torch.linalg.vector_norm(force.double()).item()),
+           'matvec_ms': round(mv_ms, 3), 'matvec_gbs': round(2 * o_bytes / (mv_ms * 1e-3) / 1e9, 1),
+           'matvec_hbm_frac': round(2 * o_bytes / (mv_ms * 1e-3) / PEAK_HBM, 4),
+           'step_ms': round(grad_ms + cg_ms, 2), 'o_gib': round(o_bytes / 2 ** 30, 2)}
+    del of
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_mh(seconds_burn=None):
+    """Metropolis-Hastings on the 8x8 periodic Heisenberg RBM (alpha 4,
+    stddev 0.01, 1024 chains, exchange moves): ESS/s of the plain chain and
+    of the chain with the learned MADE proposal (64 -> 256 -> 256 -> 64,
+    refit every 20 batches).  One recorded sample per 64 proposals per chain
+    (one sweep of the lattice); 500 sweeps of burn-in before timing."""
+    import torch
+    import flowket_amd as fk
+    m = fk.RBM((8, 8), alpha=4, stddev=0.01, seed=5)
+    op = fk.Heisenberg(hilbert_state_shape=(8, 8), pbc=True)
+    obs = fk.Observable(op)
+    chains, spc = 1024, 16
+    B = chains * spc
+    res = {}
+    for name, cls, kw in (('exchange', fk.MetropolisHastingsExchange, {}),
+                          ('exchange+learned', fk.MetropolisHastingsLearnedProposal,
+                           {'base_kind': 1, 'hidden': 256, 'refit_every': 20, 'fit_steps': 100})):
+        s = cls(m, B, num_of_chains=chains, unused_sampels=63, discard_ratio=spc, seed=3, **kw)
+        # burn-in: 500 sweeps of 64 proposals per chain
+        for _ in range(500 // (spc + 1) + 1):
+            next(s)
+        torch.cuda.synchronize()
+        ess, es, secs = [], [], 0.0
+        acc = []
+        for _ in range(4):
+            t0 = time.perf_counter()
+            x = next(s)
+            torch.cuda.synchronize()
+            secs += time.perf_counter() - t0
+            el = obs.local_values(m, x).real
+            r_hat, var, cs, e = s.calc_r_hat_value(el)
+            ess.append(float(e))
+            es.append(float(el.mean().item()))
+            acc.append(float(getattr(s, 'proposal_acceptance', s.acceptance_ratio)))
+        res[name] = {'ess_per_s': round(sum(ess) / secs, 1), 'ess_per_batch': round(float(np.mean(ess)), 1),
+                     'tau_int': round(B / float(np.mean(ess)), 2), 'batch_ms': round(1e3 * secs / 4, 2),
+                     'energy_per_site': round(float(np.mean(es)) / 64, 5),
+                     'energy_spread': round(float(np.std(es)) / 64, 5), 'acceptance': round(float(np.mean(acc)), 3)}
+        if hasattr(s, 'used_proposal'):
+            res[name]['used_proposal'] = bool(s.used_proposal)
+            res[name]['proposal_nll'] = s.last_nll
+    return res
 
 
 def cpu_baseline(machine, spins_np, n_samples=8):
@@ -109,6 +214,8 @@ def cpu_baseline(machine, spins_np, n_samples=8):
 
 def main():
     args = parse()
+    if args.profile_copy:
+        return profile_copy()
     if args.profile_forward:
         return profile_forward(args.profile_forward, args.precision, args.schedule)
     if args.gpus != 1:
@@ -246,6 +353,10 @@ def main():
                              'evaluations': nc * B, 'energy_per_site': e32.real / (H * W)}
         out['speedup_vs_fp32_plain'] = round(value / (B / dt32), 2)
 
+    if not args.no_sr:
+        out['vmc_sr_step'] = bench_sr(m16, op, spins)
+    if not args.no_mh:
+        out['mh_rbm_8x8'] = bench_mh()
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(m32, spins.cpu().numpy())
     print(json.dumps(out))

@@ -161,34 +161,50 @@ struct FwdArgs {
 
 // acc[u][mt] += sum over taps of W[tap][mt] x B(pixel + tap offset).
 // toff[u][k] is this lane's byte offset of k-step 0 at tap k = (dr+2)*4+(dc+2)
-// of tile u, pointing at the all-zero pixel for padding taps.
+// of tile u, pointing at the all-zero pixel for padding taps -- tiles past
+// the wave's count hold only such offsets, so every tile is computed without
+// a branch (their epilogues are skipped).
+// Software pipeline: the weights (read from L2) are requested WD taps ahead
+// and the B fragments of all tiles one tap ahead; scheduling fences keep the
+// compiler from sinking those requests next to their first use.
 template <int PREC, int C, int MAXT, int KIND, int MTO>
 __device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t* buf,
                                           const typename Prec<PREC>::frag* __restrict__ wg,
-                                          const uint32_t (&toff)[MAXT][12], int nt, int lane) {
+                                          const uint32_t (&toff)[MAXT][12], int lane) {
   using P = Prec<PREC>;
   using frag = typename P::frag;
   using L = Lds<PREC, C>;
   constexpr int KS = C / P::KW;
   constexpr int NTAP = Taps<KIND>::N;
-#pragma unroll
-  for (int tap = 0; tap < NTAP; ++tap) {
-    frag w[MTO][KS];
+  constexpr int WD = 2;  // weight prefetch distance (taps)
+  frag w[WD + 1][MTO][KS];
+  frag b[2][MAXT][KS];
+  auto load_w = [&](int tap, frag (&dst)[MTO][KS]) {
 #pragma unroll
     for (int mt = 0; mt < MTO; ++mt)
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) w[mt][ks] = wg[((tap * MTO + mt) * KS + ks) * 64 + lane];
+      for (int ks = 0; ks < KS; ++ks) dst[mt][ks] = wg[((tap * MTO + mt) * KS + ks) * 64 + lane];
+  };
+  auto load_b = [&](int tap, frag (&dst)[MAXT][KS]) {
 #pragma unroll
-    for (int u = 0; u < MAXT; ++u) {
-      if (u < nt) {
-        frag b[KS];
-        L::template load_bfrags_at<KS>(buf, toff[u][Taps<KIND>::bit(tap)], b);
+    for (int u = 0; u < MAXT; ++u) L::template load_bfrags_at<KS>(buf, toff[u][Taps<KIND>::bit(tap)], dst[u]);
+  };
 #pragma unroll
-        for (int mt = 0; mt < MTO; ++mt)
+  for (int t = 0; t < WD && t < NTAP; ++t) load_w(t, w[t]);
+  load_b(0, b[0]);
 #pragma unroll
-          for (int ks = 0; ks < KS; ++ks) acc[u][mt] = P::mfma(w[mt][ks], b[ks], acc[u][mt]);
-      }
-    }
+  for (int tap = 0; tap < NTAP; ++tap) {
+    if (tap + WD < NTAP) load_w(tap + WD, w[(tap + WD) % (WD + 1)]);
+    if (tap + 1 < NTAP) load_b(tap + 1, b[(tap + 1) & 1]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < MAXT; ++u)
+#pragma unroll
+      for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+          acc[u][mt] = P::mfma(w[tap % (WD + 1)][mt][ks], b[tap & 1][u][ks], acc[u][mt]);
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -490,7 +506,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
             for (int u = 0; u < MAXT; ++u)
               if (u < nt) valu_v(u, acc[u]);
           } else {
-            conv_pass<PREC, C, MAXT, 0, MT>(acc, Vin, wall + lay.off_v(blk) * 64, toff, nt, lane);
+            conv_pass<PREC, C, MAXT, 0, MT>(acc, Vin, wall + lay.off_v(blk) * 64, toff, lane);
           }
 #pragma unroll
           for (int u = 0; u < MAXT; ++u)
@@ -507,7 +523,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
             for (int u = 0; u < MAXT; ++u)
               if (u < nt) valu_h(u, acc[u]);
           } else {
-            conv_pass<PREC, C, MAXT, 1, MT>(acc, Hin, wall + lay.off_h(blk) * 64, toff, nt, lane);
+            conv_pass<PREC, C, MAXT, 1, MT>(acc, Hin, wall + lay.off_h(blk) * 64, toff, lane);
           }
 #pragma unroll
           for (int u = 0; u < MAXT; ++u)
@@ -584,7 +600,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
         for (int u = 0; u < MAXT; ++u)
 #pragma unroll
           for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + 3 * C + 16 * mt + 4 * g);
-        conv_pass<PREC, C, MAXT, 2, MT>(acc, CAT, wall + lay.off_c(blk) * 64, toff, nt, lane);
+        conv_pass<PREC, C, MAXT, 2, MT>(acc, CAT, wall + lay.off_c(blk) * 64, toff, lane);
 #pragma unroll
         for (int u = 0; u < MAXT; ++u)
           if (u < nt) epi_c(u, acc[u]);
