@@ -205,11 +205,11 @@ def cpu_baseline(machine, spins_np, n_samples=8):
     with threadpool_limits(limits=cores):
         t0 = time.perf_counter()
         conn, mel, use = oop.heisenberg_find_conn(s, pbc=True)
-        oob.local_values(lambda x: ocn.log_psi(params, x, DEPTH, activation=ACT), conn, mel, use)
+        eloc = oob.local_values(lambda x: ocn.log_psi(params, x, DEPTH, activation=ACT), conn, mel, use)
         dt = time.perf_counter() - t0
     return {'value': n_samples / dt, 'unit': 'E_loc evals/s', 'cores': cores, 'kind': 'port',
             'sample': f'{n_samples} configurations of the benchmark batch, numpy float64 oracle '
-                      f'(find_conn + unbalanced E_loc with {int(use.sum())} network forwards), {dt:.1f} s'}
+                      f'(find_conn + unbalanced E_loc with {int(use.sum())} network forwards), {dt:.1f} s'}, eloc
 
 
 def main():
@@ -357,8 +357,45 @@ def main():
         out['vmc_sr_step'] = bench_sr(m16, op, spins)
     if not args.no_mh:
         out['mh_rbm_8x8'] = bench_mh()
+    # ---- the same ragged step with fp16 MFMA (same speed class as bf16,
+    # finer rounding)
+    mh = fk.ConvNetAutoregressive2D((H, W), depth=DEPTH, num_of_channels=CHANNELS, activation=ACT,
+                                    weights_normalization=False, precision='fp16', params=m16.params)
+    elh = obs.local_values(mh, spins)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        obs.local_values(mh, spins)
+    torch.cuda.synchronize()
+    dth = (time.perf_counter() - t0) / args.steps
+    out['fp16'] = {'E_loc_evals_per_s': round(B / dth, 2), 'ms_per_step': round(dth * 1e3, 3),
+                   'speedup_vs_fp32_plain': round(B / dth / out['fp32_plain']['E_loc_evals_per_s'], 2)
+                   if 'fp32_plain' in out else None}
+
+    # ---- accuracy: the same samples through the FP32 ragged path, and the
+    # FP64 oracle on the CPU-baseline subset
+    el16 = obs.local_values(m16, spins)
+    el32 = obs.local_values(m32, spins)
+    e32 = el32.real.mean().item() / (H * W)
+    acc = {'energy_per_site_fp32': e32,
+           'eloc_std_per_site_fp32': el32.real.std().item() / (H * W),
+           'eloc_stat_err_per_site': el32.real.std().item() / (H * W) / np.sqrt(B)}
+    for name, el in (('bf16', el16), ('fp16', elh)):
+        e = el.real.mean().item() / (H * W)
+        acc[f'energy_per_site_{name}'] = e
+        acc[f'energy_per_site_err_{name}_vs_fp32'] = abs(e - e32)
+        acc[f'eloc_rms_rel_err_{name}_vs_fp32'] = float(((el - el32).abs().pow(2).mean().sqrt()
+                                                         / el32.abs().pow(2).mean().sqrt()).item())
     if not args.no_cpu:
-        out['cpu_baseline'] = cpu_baseline(m32, spins.cpu().numpy())
+        cb, el64 = cpu_baseline(m32, spins.cpu().numpy())
+        out['cpu_baseline'] = cb
+        n8 = len(el64)
+        for name, el in (('fp32', el32), ('bf16', el16), ('fp16', elh)):
+            g = el[:n8].cpu().numpy()
+            acc[f'eloc_max_rel_err_{name}_vs_fp64_oracle'] = float(np.abs(g - el64).max() / np.abs(el64).max())
+            acc[f'energy_per_site_err_{name}_vs_fp64_oracle_subset'] = float(
+                abs(el[:n8].real.mean().item() - el64.real.mean()) / (H * W))
+    out['accuracy'] = acc
     print(json.dumps(out))
     return 0
 

@@ -20,6 +20,7 @@ FK_ACT_RELU = 0
 FK_ACT_ELU = 1
 FK_PREC_FP32 = 0
 FK_PREC_BF16 = 1
+FK_PREC_FP16 = 2
 
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32

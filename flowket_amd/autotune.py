@@ -86,7 +86,8 @@ def tune_forward(machine, run, precision=None, reps=3, warmup=1, persist=True):
     current stream, e.g. one ragged local-energy evaluation) under every
     schedule and keeps the fastest.  Returns {'schedule': i, 'ms': [...],
     'schedules': [...]}."""
-    prec = _lib.FK_PREC_BF16 if (precision or machine.precision) == 'bf16' else _lib.FK_PREC_FP32
+    from .machines import _PREC
+    prec = _PREC[precision or machine.precision]
     table = schedules(machine.num_of_channels, prec)
     stream = torch.cuda.current_stream(machine.device)
     times = []

@@ -9,32 +9,65 @@ namespace fk {
 template <int PREC>
 struct Prec;
 
+// 16-bit precisions (bf16, fp16) share the layout: 8 elements per lane per
+// fragment, 32 input-channel slots per k-step.
 template <>
 struct Prec<FK_PREC_BF16> {
+  using elem = __bf16;
   using frag = bf16x8;
+  using quad = bf16x4;
+  static constexpr bool H16 = true;
   static constexpr int ESZ = 2;
   static constexpr int KW = 32;
   __device__ static __forceinline__ f32x4 mfma(frag a, frag b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
-  __device__ static __forceinline__ frag zero() {
-    frag z;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) z[e] = (__bf16)0.0f;
-    return z;
+};
+
+template <>
+struct Prec<FK_PREC_FP16> {
+  using elem = _Float16;
+  using frag = f16x8;
+  using quad = f16x4;
+  static constexpr bool H16 = true;
+  static constexpr int ESZ = 2;
+  static constexpr int KW = 32;
+  __device__ static __forceinline__ f32x4 mfma(frag a, frag b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
   }
 };
 
 template <>
 struct Prec<FK_PREC_FP32> {
   using frag = float;
+  static constexpr bool H16 = false;
   static constexpr int ESZ = 4;
   static constexpr int KW = 4;
   __device__ static __forceinline__ f32x4 mfma(frag a, frag b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
-  __device__ static __forceinline__ frag zero() { return 0.0f; }
 };
+
+// 4 fp32 values -> one quad of 16-bit elements and back
+template <int PREC>
+__device__ __forceinline__ typename Prec<PREC>::quad to_quad(f32x4 v) {
+  using E = typename Prec<PREC>::elem;
+  typename Prec<PREC>::quad h;
+  h[0] = (E)v.x;
+  h[1] = (E)v.y;
+  h[2] = (E)v.z;
+  h[3] = (E)v.w;
+  return h;
+}
+template <int PREC>
+__device__ __forceinline__ f32x4 from_quad(typename Prec<PREC>::quad h) {
+  f32x4 v;
+  v.x = (float)h[0];
+  v.y = (float)h[1];
+  v.z = (float)h[2];
+  v.w = (float)h[3];
+  return v;
+}
 
 template <int ACT>
 __device__ __forceinline__ float act1(float x) {
@@ -55,18 +88,19 @@ __device__ __forceinline__ f32x4 act4(f32x4 v) {
 // 1x1 contraction, in registers.
 template <int PREC, int MT, int KS>
 __device__ __forceinline__ void d_to_b(const f32x4 (&d)[MT], typename Prec<PREC>::frag (&b)[KS]) {
-  if constexpr (PREC == FK_PREC_BF16) {
+  if constexpr (Prec<PREC>::H16) {
+    using E = typename Prec<PREC>::elem;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      bf16x8 f;
-      f[0] = (__bf16)d[2 * ks].x;
-      f[1] = (__bf16)d[2 * ks].y;
-      f[2] = (__bf16)d[2 * ks].z;
-      f[3] = (__bf16)d[2 * ks].w;
-      f[4] = (__bf16)d[2 * ks + 1].x;
-      f[5] = (__bf16)d[2 * ks + 1].y;
-      f[6] = (__bf16)d[2 * ks + 1].z;
-      f[7] = (__bf16)d[2 * ks + 1].w;
+      typename Prec<PREC>::frag f;
+      f[0] = (E)d[2 * ks].x;
+      f[1] = (E)d[2 * ks].y;
+      f[2] = (E)d[2 * ks].z;
+      f[3] = (E)d[2 * ks].w;
+      f[4] = (E)d[2 * ks + 1].x;
+      f[5] = (E)d[2 * ks + 1].y;
+      f[6] = (E)d[2 * ks + 1].z;
+      f[7] = (E)d[2 * ks + 1].w;
       b[ks] = f;
     }
   } else {

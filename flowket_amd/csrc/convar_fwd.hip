@@ -39,7 +39,7 @@ struct Lds {
   }
   // byte offset of the 4-channel quad q (channels 4q..4q+3) of pixel p
   __device__ static __forceinline__ int quad(int p, int q) {
-    if constexpr (PREC == FK_PREC_BF16) {
+    if constexpr (Prec<PREC>::H16) {
       const int ks = q >> 3, w = q & 7;
       return chunk(p, 4 * ks + (w & 3)) + ((w >> 2) << 3);
     } else {
@@ -47,26 +47,15 @@ struct Lds {
     }
   }
   __device__ static __forceinline__ void store_quad(uint8_t* buf, int p, int q, f32x4 v) {
-    if constexpr (PREC == FK_PREC_BF16) {
-      bf16x4 h;
-      h[0] = (__bf16)v.x;
-      h[1] = (__bf16)v.y;
-      h[2] = (__bf16)v.z;
-      h[3] = (__bf16)v.w;
-      *reinterpret_cast<bf16x4*>(buf + quad(p, q)) = h;
+    if constexpr (Prec<PREC>::H16) {
+      *reinterpret_cast<typename Prec<PREC>::quad*>(buf + quad(p, q)) = to_quad<PREC>(v);
     } else {
       *reinterpret_cast<f32x4*>(buf + quad(p, q)) = v;
     }
   }
   __device__ static __forceinline__ f32x4 load_quad(const uint8_t* buf, int p, int q) {
-    if constexpr (PREC == FK_PREC_BF16) {
-      bf16x4 h = *reinterpret_cast<const bf16x4*>(buf + quad(p, q));
-      f32x4 v;
-      v.x = (float)h[0];
-      v.y = (float)h[1];
-      v.z = (float)h[2];
-      v.w = (float)h[3];
-      return v;
+    if constexpr (Prec<PREC>::H16) {
+      return from_quad<PREC>(*reinterpret_cast<const typename Prec<PREC>::quad*>(buf + quad(p, q)));
     } else {
       return *reinterpret_cast<const f32x4*>(buf + quad(p, q));
     }
@@ -74,14 +63,15 @@ struct Lds {
   // Chunk of k-step 0 for lane group g (the address that precomputed tap
   // offsets point at); further k-steps / chunks are reached by XOR.
   __device__ static __forceinline__ int base_chunk(int g) {
-    return PREC == FK_PREC_BF16 ? g : g * (C / 16);
+    return Prec<PREC>::H16 ? g : g * (C / 16);
   }
   template <int KS>
   __device__ static __forceinline__ void load_bfrags_at(const uint8_t* buf, uint32_t off,
                                                         typename Prec<PREC>::frag (&b)[KS]) {
-    if constexpr (PREC == FK_PREC_BF16) {
+    if constexpr (Prec<PREC>::H16) {
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) b[ks] = *reinterpret_cast<const bf16x8*>(buf + (off ^ (64u * ks)));
+      for (int ks = 0; ks < KS; ++ks)
+        b[ks] = *reinterpret_cast<const typename Prec<PREC>::frag*>(buf + (off ^ (64u * ks)));
     } else {
 #pragma unroll
       for (int j = 0; j < KS / 4; ++j) {
@@ -96,30 +86,19 @@ struct Lds {
   // byte offset of M-tile mt's quad of this lane from the lane's own
   // k-step-0 chunk offset
   __device__ static __forceinline__ uint32_t self_quad(uint32_t off, int mt) {
-    if constexpr (PREC == FK_PREC_BF16) return (off ^ (64u * (mt >> 1))) + 8u * (mt & 1);
+    if constexpr (Prec<PREC>::H16) return (off ^ (64u * (mt >> 1))) + 8u * (mt & 1);
     else return off ^ (16u * mt);
   }
   __device__ static __forceinline__ void store_quad_at(uint8_t* buf, uint32_t off, f32x4 v) {
-    if constexpr (PREC == FK_PREC_BF16) {
-      bf16x4 h;
-      h[0] = (__bf16)v.x;
-      h[1] = (__bf16)v.y;
-      h[2] = (__bf16)v.z;
-      h[3] = (__bf16)v.w;
-      *reinterpret_cast<bf16x4*>(buf + off) = h;
+    if constexpr (Prec<PREC>::H16) {
+      *reinterpret_cast<typename Prec<PREC>::quad*>(buf + off) = to_quad<PREC>(v);
     } else {
       *reinterpret_cast<f32x4*>(buf + off) = v;
     }
   }
   __device__ static __forceinline__ f32x4 load_quad_at(const uint8_t* buf, uint32_t off) {
-    if constexpr (PREC == FK_PREC_BF16) {
-      bf16x4 h = *reinterpret_cast<const bf16x4*>(buf + off);
-      f32x4 v;
-      v.x = (float)h[0];
-      v.y = (float)h[1];
-      v.z = (float)h[2];
-      v.w = (float)h[3];
-      return v;
+    if constexpr (Prec<PREC>::H16) {
+      return from_quad<PREC>(*reinterpret_cast<const typename Prec<PREC>::quad*>(buf + off));
     } else {
       return *reinterpret_cast<const f32x4*>(buf + off);
     }
@@ -128,9 +107,10 @@ struct Lds {
   template <int KS>
   __device__ static __forceinline__ void load_bfrags(const uint8_t* buf, int p, int g,
                                                      typename Prec<PREC>::frag (&b)[KS]) {
-    if constexpr (PREC == FK_PREC_BF16) {
+    if constexpr (Prec<PREC>::H16) {
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) b[ks] = *reinterpret_cast<const bf16x8*>(buf + chunk(p, 4 * ks + g));
+      for (int ks = 0; ks < KS; ++ks)
+        b[ks] = *reinterpret_cast<const typename Prec<PREC>::frag*>(buf + chunk(p, 4 * ks + g));
     } else {
 #pragma unroll
       for (int j = 0; j < KS / 4; ++j) {
@@ -639,6 +619,7 @@ __global__ void pack_gather_kernel(const float* __restrict__ src, const int32_t*
   const int32_t k = map[i];
   const float v = k >= 0 ? src[k] : 0.0f;
   if (prec == FK_PREC_BF16) reinterpret_cast<__bf16*>(dst)[i] = (__bf16)v;
+  else if (prec == FK_PREC_FP16) reinterpret_cast<_Float16*>(dst)[i] = (_Float16)v;
   else reinterpret_cast<float*>(dst)[i] = v;
 }
 
@@ -686,7 +667,7 @@ static int current_schedule(int H, int W, int C, int prec) {
 // Largest tile of configurations whose five activation lattices fit in the
 // 160 KB LDS with at most 4 (8 waves) or 8 (4 waves) 16-pixel tiles per wave.
 static int plan_forward(int HW, int C, int prec, int NW, int64_t m, FwdPlan* plan) {
-  const int pixb = C * (prec == FK_PREC_BF16 ? 2 : 4);
+  const int pixb = C * (prec == FK_PREC_FP32 ? 4 : 2);
   const size_t cap = 160 * 1024;
   const int maxt_cap = (NW == 4) ? 8 : 4;
   int best = 0;
@@ -805,7 +786,7 @@ int fk_convar_pack_maps(int32_t depth, int32_t channels, int32_t precision, int3
           for (int lane = 0; lane < 64; ++lane) {
             const int m = lane % 16, g = lane / 16;
             for (int e = 0; e < lay.E; ++e) {
-              const int pc = precision == FK_PREC_BF16 ? slot_channel_bf16(32 * ks + 8 * g + e)
+              const int pc = precision != FK_PREC_FP32 ? slot_channel_bf16(32 * ks + 8 * g + e)
                                                        : slot_channel_f32(ks, g);
               const int cin = real_cin(pc, cv, cat);
               const int cout = 16 * mt + m;
@@ -851,7 +832,8 @@ int fk_convar_pack_maps(int32_t depth, int32_t channels, int32_t precision, int3
 
 int fk_pack_gather(const float* params, const int32_t* map, int64_t n, int32_t precision, void* dst, void* stream) {
   FK_REQUIRE(params && map && dst, "fk_pack_gather: null pointer");
-  FK_REQUIRE(precision == FK_PREC_FP32 || precision == FK_PREC_BF16, "fk_pack_gather: bad precision");
+  FK_REQUIRE(precision == FK_PREC_FP32 || precision == FK_PREC_BF16 || precision == FK_PREC_FP16,
+             "fk_pack_gather: bad precision");
   if (n <= 0) return FK_OK;
   hipLaunchKernelGGL(pack_gather_kernel, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, params, map, n,
                      precision, dst);
@@ -905,6 +887,10 @@ int fk_convar_forward(const void* wpacked, const float* aux, int32_t precision, 
   if (precision == FK_PREC_BF16) {
     if (CP == 32) return dispatch_fwd_c<FK_PREC_BF16, 32>(a, pl, sc, activation, s);
     return dispatch_fwd_c<FK_PREC_BF16, 64>(a, pl, sc, activation, s);
+  }
+  if (precision == FK_PREC_FP16) {
+    if (CP == 32) return dispatch_fwd_c<FK_PREC_FP16, 32>(a, pl, sc, activation, s);
+    return dispatch_fwd_c<FK_PREC_FP16, 64>(a, pl, sc, activation, s);
   }
   if (CP == 16) return dispatch_fwd_c<FK_PREC_FP32, 16>(a, pl, sc, activation, s);
   if (CP == 32) return dispatch_fwd_c<FK_PREC_FP32, 32>(a, pl, sc, activation, s);

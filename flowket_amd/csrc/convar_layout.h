@@ -19,7 +19,7 @@
 // horizontal kernel (3*C) after it.
 //
 // Channel padding: a network of C channels runs on the kernels compiled for
-// CP = kernel_channels(C) >= C (bf16: 32, 64; fp32: 16, 32, 64).  The layout
+// CP = kernel_channels(C) >= C (bf16/fp16: 32, 64; fp32: 16, 32, 64).  The layout
 // above is then taken with CP; channels >= C of every convolution and the
 // x/y halves past C/2 get zero weights and biases, so their activations stay
 // exactly zero and the result is the C-channel network's.  In the padded
@@ -40,8 +40,8 @@ struct ConvarLayout {
   FK_HD ConvarLayout(int channels, int precision) {
     C = channels;
     prec = precision;
-    E = (prec == 1) ? 8 : 1;       // elements per lane per fragment
-    KW = (prec == 1) ? 32 : 4;     // input-channel slots per MFMA k-step
+    E = (prec != 0) ? 8 : 1;       // elements per lane per fragment
+    KW = (prec != 0) ? 32 : 4;     // input-channel slots per MFMA k-step
     MT = C / 16;                   // 16-row output tiles of a C-output conv
     KS = C / KW;                   // k-steps per tap
     MTH = (C / 2 + 15) / 16;       // output tiles of a C/2-output 1x1 conv
@@ -83,9 +83,9 @@ FK_HD inline int slot_channel_f32(int t, int g) { return 16 * (t >> 2) + 4 * g +
 // Kernel channel width for a C-channel network, 0 if none fits.
 inline int kernel_channels(int channels, int precision) {
   if (channels < 2 || (channels & 1)) return 0;
-  if (precision != 0 && precision != 1) return 0;
+  if (precision < 0 || precision > 2) return 0;
   const int widths[3] = {16, 32, 64};
-  for (int i = (precision == 1) ? 1 : 0; i < 3; ++i)
+  for (int i = (precision != 0) ? 1 : 0; i < 3; ++i)
     if (channels <= widths[i]) return widths[i];
   return 0;
 }

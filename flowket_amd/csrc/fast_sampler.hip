@@ -27,8 +27,9 @@ namespace fk {
 template <int PREC, int C>
 struct LaneData;
 
-template <int C>
-struct LaneData<FK_PREC_BF16, C> {
+// 16-bit precisions (bf16, fp16): 8 bytes per quad
+template <int PREC, int C>
+struct LaneData {
   static constexpr int MT = C / 16;
   static constexpr int LB = MT * 8;  // bytes per lane
   uint2 q[MT];
@@ -56,28 +57,18 @@ struct LaneData<FK_PREC_BF16, C> {
       *reinterpret_cast<uint4*>(p + 16 * k) = make_uint4(q[2 * k].x, q[2 * k].y, q[2 * k + 1].x, q[2 * k + 1].y);
   }
   __device__ __forceinline__ void set(int m, f32x4 v) {
-    bf16x4 h;
-    h[0] = (__bf16)v.x;
-    h[1] = (__bf16)v.y;
-    h[2] = (__bf16)v.z;
-    h[3] = (__bf16)v.w;
+    typename Prec<PREC>::quad h = to_quad<PREC>(v);
     q[m] = *reinterpret_cast<uint2*>(&h);
   }
   __device__ __forceinline__ f32x4 get(int m) const {
-    bf16x4 h = *reinterpret_cast<const bf16x4*>(&q[m]);
-    f32x4 v;
-    v.x = (float)h[0];
-    v.y = (float)h[1];
-    v.z = (float)h[2];
-    v.w = (float)h[3];
-    return v;
+    return from_quad<PREC>(*reinterpret_cast<const typename Prec<PREC>::quad*>(&q[m]));
   }
   template <int KS>
-  __device__ __forceinline__ void frags(bf16x8 (&b)[KS]) const {
+  __device__ __forceinline__ void frags(typename Prec<PREC>::frag (&b)[KS]) const {
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       uint4 v = make_uint4(q[2 * ks].x, q[2 * ks].y, q[2 * ks + 1].x, q[2 * ks + 1].y);
-      b[ks] = *reinterpret_cast<bf16x8*>(&v);
+      b[ks] = *reinterpret_cast<typename Prec<PREC>::frag*>(&v);
     }
   }
 };
@@ -419,7 +410,7 @@ static int fast_channels(int channels, int precision) {
 bool fast_sampler_supported(int channels, int precision) { return fast_channels(channels, precision) > 0; }
 
 int64_t fast_tile_bytes(int HW, int nb, int channels, int precision) {
-  const int esz = precision == FK_PREC_BF16 ? 2 : 4;
+  const int esz = precision == FK_PREC_FP32 ? 4 : 2;
   const int64_t pb = 16LL * channels * esz;
   return (int64_t)(3 * nb - 2) * HW * pb + (int64_t)HW * 16 * 4;
 }
@@ -453,7 +444,7 @@ int fk_fast_sample(const void* wpacked, const float* aux, int32_t precision, int
   FK_REQUIRE(H >= 1 && W >= 1 && depth >= 1 && batch >= 0, "fk_fast_sample: bad shape");
   FK_REQUIRE(activation == FK_ACT_RELU || activation == FK_ACT_ELU, "fk_fast_sample: bad activation");
   if (!fast_sampler_supported(channels, precision)) {
-    set_error("fast sampler: channels=%d precision=%d not implemented (17..64 channels, bf16 2..64)", channels,
+    set_error("fast sampler: channels=%d precision=%d not implemented (17..64 channels, bf16/fp16 2..64)", channels,
               precision);
     return FK_ERR_UNSUPPORTED;
   }
@@ -479,6 +470,10 @@ int fk_fast_sample(const void* wpacked, const float* aux, int32_t precision, int
   if (precision == FK_PREC_BF16) {
     if (CP == 32) return launch_fast<FK_PREC_BF16, 32>(a, activation, tiles, s);
     return launch_fast<FK_PREC_BF16, 64>(a, activation, tiles, s);
+  }
+  if (precision == FK_PREC_FP16) {
+    if (CP == 32) return launch_fast<FK_PREC_FP16, 32>(a, activation, tiles, s);
+    return launch_fast<FK_PREC_FP16, 64>(a, activation, tiles, s);
   }
   if (CP == 32) return launch_fast<FK_PREC_FP32, 32>(a, activation, tiles, s);
   return launch_fast<FK_PREC_FP32, 64>(a, activation, tiles, s);

@@ -37,6 +37,8 @@ void set_error(const char* fmt, ...);
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
 inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 

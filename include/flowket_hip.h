@@ -37,9 +37,12 @@ extern "C" {
 #define FK_ACT_RELU 0
 #define FK_ACT_ELU 1
 
-/* Arithmetic of the fused network kernels. */
+/* Arithmetic of the fused network kernels: FP32 MFMA, or 16-bit MFMA
+ * (bf16 / fp16 weights and activations) with FP32 accumulation.  fp16 runs at
+ * the bf16 rate with 8x finer rounding; its range is |x| < 65504. */
 #define FK_PREC_FP32 0
 #define FK_PREC_BF16 1
+#define FK_PREC_FP16 2
 
 int fk_version(void);
 const char* fk_last_error(void);

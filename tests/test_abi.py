@@ -59,7 +59,8 @@ def _maps(depth, C, prec):
 
 
 @pytest.mark.parametrize('depth,C,prec', [(5, 16, 0), (4, 32, 0), (20, 32, 1), (3, 64, 1), (2, 32, 1), (1, 16, 0),
-                                          (3, 48, 1), (2, 24, 0), (2, 8, 0), (3, 40, 1), (2, 6, 1)])
+                                          (3, 48, 1), (2, 24, 0), (2, 8, 0), (3, 40, 1), (2, 6, 1),
+                                          (3, 32, 2), (2, 48, 2)])
 def test_pack_maps_cover_every_parameter_once(lib, depth, C, prec):
     """Every network parameter appears in the packed buffers exactly once
     (weights of MFMA convolutions once in the fragment buffer, biases and the
@@ -96,6 +97,8 @@ def test_schedule_table(lib):
     assert L.fk_convar_num_schedules(24, 1) == 4
     assert L.fk_convar_num_schedules(48, 1) == 2
     assert L.fk_convar_num_schedules(25, 1) == 0
+    assert L.fk_convar_num_schedules(32, 2) == 2
+    assert L.fk_convar_num_schedules(32, 3) == 0
     t, w = ctypes.c_int32(), ctypes.c_int32()
     seen = set()
     for i in range(4):

@@ -13,6 +13,8 @@ pytestmark = pytest.mark.gpu
 # of the FP32 and BF16 paths).
 FP32_RTOL = 1e-5
 BF16_RTOL = 2e-2
+FP16_RTOL = 3e-3
+RTOL = {'fp32': FP32_RTOL, 'bf16': BF16_RTOL, 'fp16': FP16_RTOL}
 
 
 def _oracle_params(machine):
@@ -67,6 +69,10 @@ CASES = [
     ((6, 6), 2, 24, 'relu', 'fp32'),
     ((5, 4), 2, 8, 'elu', 'fp32'),
     ((4, 4), 2, 6, 'relu', 'bf16'),
+    # fp16 MFMA
+    ((10, 10), 4, 32, 'elu', 'fp16'),
+    ((7, 9), 3, 64, 'relu', 'fp16'),
+    ((6, 5), 2, 20, 'elu', 'fp16'),
 ]
 
 
@@ -78,10 +84,10 @@ def test_forward_matches_oracle(rng, shape, depth, C, act, prec):
     s = rng.choice([-1, 1], size=(B,) + shape).astype(np.int8)
     got = m.log_psi(s)
     ref = ocn.log_psi(_oracle_params(m), s, depth, activation=act)
-    _check_logpsi(got, ref, FP32_RTOL if prec == 'fp32' else BF16_RTOL)
+    _check_logpsi(got, ref, RTOL[prec])
     cond = m.conditional_log_probs(s).cpu().numpy()
     rcond = ocn.conditional_log_probs(_oracle_params(m), s, depth, activation=act)
-    tol = 1e-4 if prec == 'fp32' else 5e-2
+    tol = {'fp32': 1e-4, 'bf16': 5e-2, 'fp16': 8e-3}[prec]
     assert np.abs(cond - rcond).max() < tol
 
 
@@ -139,7 +145,8 @@ def test_connected_equals_dense(rng):
 
 @pytest.mark.parametrize('shape,depth,C,act,prec', [((10, 10), 4, 32, 'elu', 'bf16'), ((7, 9), 3, 32, 'relu', 'bf16'),
                                                     ((6, 5), 3, 16, 'relu', 'fp32'), ((10, 10), 3, 64, 'elu', 'bf16'),
-                                                    ((4, 4), 2, 32, 'relu', 'fp32')])
+                                                    ((4, 4), 2, 32, 'relu', 'fp32'),
+                                                    ((8, 7), 3, 32, 'elu', 'fp16')])
 def test_every_schedule_matches_oracle(rng, shape, depth, C, act, prec):
     """Each compiled launch schedule (loop order x waves per workgroup)
     computes the same function, on plain rows and on ragged connected
@@ -148,7 +155,8 @@ def test_every_schedule_matches_oracle(rng, shape, depth, C, act, prec):
     from flowket_amd import _lib, autotune
     gpu_banner(f'every schedule shape={shape} depth={depth} C={C} {act} {prec}')
     m = _machine(shape, depth, C, act, prec)
-    pc = _lib.FK_PREC_BF16 if prec == 'bf16' else _lib.FK_PREC_FP32
+    from flowket_amd.machines import _PREC
+    pc = _PREC[prec]
     s = rng.choice([-1, 1], size=(45,) + shape).astype(np.int8)
     ref = ocn.log_psi(_oracle_params(m), s, depth, activation=act)
     op = fk.Heisenberg(hilbert_state_shape=shape, pbc=True)
@@ -160,7 +168,7 @@ def test_every_schedule_matches_oracle(rng, shape, depth, C, act, prec):
         for sc in autotune.schedules(C, pc):
             autotune.set_schedule(m, pc, sc['index'])
             got = m.log_psi(s)
-            _check_logpsi(got, ref, FP32_RTOL if prec == 'fp32' else BF16_RTOL)
+            _check_logpsi(got, ref, RTOL[prec])
             lp = m.log_psi_connected(sd, r['pairs'], r['pairs'].shape[0], op, m_dev=r['offsets'][B:])
             outs.append(lp[:int(r['offsets'][-1])].clone())
     finally:
@@ -190,3 +198,15 @@ def test_autotune_selects_and_caches(rng, tmp_path, monkeypatch):
         assert torch.allclose(got, ref, rtol=1e-4, atol=1e-3)
     finally:
         autotune.set_schedule(m, _lib.FK_PREC_BF16, -1)
+
+
+def test_fp16_closer_to_fp32_than_bf16(rng):
+    """On the same network and configurations the fp16 path's log psi is
+    closer to the FP32 result than the bf16 path's (finer rounding)."""
+    gpu_banner('fp16 vs bf16 accuracy 10x10 depth 6 C32')
+    m = _machine((10, 10), 6, 32, 'relu', 'fp32', bias_scale=0.2)
+    s = rng.choice([-1, 1], size=(64, 10, 10)).astype(np.int8)
+    ref = m.log_psi(s).real.double()
+    e16 = (m.log_psi(s, precision='fp16').real.double() - ref).abs().max().item()
+    eb16 = (m.log_psi(s, precision='bf16').real.double() - ref).abs().max().item()
+    assert e16 < eb16 / 3, (e16, eb16)

@@ -17,7 +17,9 @@ pytestmark = pytest.mark.gpu
                                                      ((5, 7), 3, 64, 'elu', 'fp32', 20),
                                                      ((5, 7), 3, 64, 'elu', 'bf16', 20),
                                                      ((6, 6), 3, 48, 'relu', 'bf16', 40),
-                                                     ((4, 5), 2, 24, 'elu', 'fp32', 30)])
+                                                     ((4, 5), 2, 24, 'elu', 'fp32', 30),
+                                                     ((10, 10), 3, 32, 'relu', 'fp16', 40),
+                                                     ((5, 6), 2, 64, 'elu', 'fp16', 17)])
 def test_fast_equals_plain_given_uniforms(rng, shape, depth, C, act, prec, B):
     """Same uniforms -> the same draws as AutoregressiveSampler (identical
     arithmetic), and the conditionals it drew from equal the full forward's."""
