@@ -22,6 +22,10 @@
 namespace fk {
 
 // ---------------------------------------------------------------------------
+// all-zero pixels after each activation buffer (one per bank placement of
+// the 8-pixel swizzle period)
+constexpr int ZERO_PIX = 8;
+
 // LDS addressing.  A pixel holds C channels as 16-byte chunks; chunk indices
 // are XOR-swizzled with the pixel index so that 16 lanes reading the same
 // chunk of 16 consecutive pixels hit distinct banks.
@@ -103,6 +107,27 @@ struct Lds {
       return *reinterpret_cast<const f32x4*>(buf + off);
     }
   }
+  // 16-bit precisions: the quads of M-tiles 2j and 2j+1 share the 16-byte
+  // chunk at off ^ 64j, so they are written and read as one access.
+  __device__ static __forceinline__ void store_pair_at(uint8_t* buf, uint32_t off, f32x4 lo, f32x4 hi) {
+    using Q = typename Prec<PREC>::quad;
+    const Q a = to_quad<PREC>(lo), b = to_quad<PREC>(hi);
+    typename Prec<PREC>::frag f;
+    f[0] = a[0];
+    f[1] = a[1];
+    f[2] = a[2];
+    f[3] = a[3];
+    f[4] = b[0];
+    f[5] = b[1];
+    f[6] = b[2];
+    f[7] = b[3];
+    *reinterpret_cast<typename Prec<PREC>::frag*>(buf + off) = f;
+  }
+  __device__ static __forceinline__ void load_pair_at(const uint8_t* buf, uint32_t off, f32x4& lo, f32x4& hi) {
+    const typename Prec<PREC>::frag f = *reinterpret_cast<const typename Prec<PREC>::frag*>(buf + off);
+    lo = f32x4{(float)f[0], (float)f[1], (float)f[2], (float)f[3]};
+    hi = f32x4{(float)f[4], (float)f[5], (float)f[6], (float)f[7]};
+  }
   // All B fragments (one per k-step) of pixel p for lane group g.
   template <int KS>
   __device__ static __forceinline__ void load_bfrags(const uint8_t* buf, int p, int g,
@@ -139,26 +164,42 @@ struct FwdArgs {
   float* cond;
 };
 
+// Per-tile LDS byte offsets (from the buffer start) of the 12 causal taps
+// k = (dr+2)*4 + (dc+2) of this lane's pixel, two 16-bit offsets per
+// register: every activation buffer is below 64 KB.
+template <int MAXT>
+struct TapOff {
+  uint32_t v[MAXT][6];
+  __device__ __forceinline__ uint32_t get(int u, int k) const {
+    return (k & 1) ? (v[u][k >> 1] >> 16) : (v[u][k >> 1] & 0xffffu);
+  }
+};
+
 // acc[u][mt] += sum over taps of W[tap][mt] x B(pixel + tap offset).
-// toff[u][k] is this lane's byte offset of k-step 0 at tap k = (dr+2)*4+(dc+2)
-// of tile u, pointing at the all-zero pixel for padding taps -- tiles past
+// toff.get(u, k) is this lane's byte offset of k-step 0 at tap k of tile u,
+// pointing at the all-zero pixel for padding taps -- tiles past
 // the wave's count hold only such offsets, so every tile is computed without
 // a branch (their epilogues are skipped).
 // Software pipeline: the weights (read from L2) are requested WD taps ahead
-// and the B fragments of all tiles one tap ahead; scheduling fences keep the
-// compiler from sinking those requests next to their first use.
+// and the B fragments of all tiles BD taps ahead (depths bounded by the
+// registers they take); scheduling fences keep the compiler from sinking
+// those requests next to their first use.
 template <int PREC, int C, int MAXT, int KIND, int MTO>
 __device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t* buf,
                                           const typename Prec<PREC>::frag* __restrict__ wg,
-                                          const uint32_t (&toff)[MAXT][12], int lane) {
+                                          const TapOff<MAXT>& toff, int lane) {
   using P = Prec<PREC>;
   using frag = typename P::frag;
   using L = Lds<PREC, C>;
   constexpr int KS = C / P::KW;
   constexpr int NTAP = Taps<KIND>::N;
-  constexpr int WD = 2;  // weight prefetch distance (taps)
+  // prefetch depth bounded by the registers a tap's fragments take
+  constexpr int FR = (int)(sizeof(frag) / 4);
+  constexpr int WREGS = MTO * KS * FR, BREGS = MAXT * KS * FR;
+  constexpr int WD = WREGS <= 16 ? 2 : (WREGS <= 32 ? 1 : 0);  // weight prefetch distance (taps)
+  constexpr int BD = BREGS <= 32 ? 1 : 0;                       // B prefetch distance (taps)
   frag w[WD + 1][MTO][KS];
-  frag b[2][MAXT][KS];
+  frag b[BD + 1][MAXT][KS];
   auto load_w = [&](int tap, frag (&dst)[MTO][KS]) {
 #pragma unroll
     for (int mt = 0; mt < MTO; ++mt)
@@ -167,15 +208,19 @@ __device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t
   };
   auto load_b = [&](int tap, frag (&dst)[MAXT][KS]) {
 #pragma unroll
-    for (int u = 0; u < MAXT; ++u) L::template load_bfrags_at<KS>(buf, toff[u][Taps<KIND>::bit(tap)], dst[u]);
+    for (int u = 0; u < MAXT; ++u) L::template load_bfrags_at<KS>(buf, toff.get(u, Taps<KIND>::bit(tap)), dst[u]);
   };
 #pragma unroll
   for (int t = 0; t < WD && t < NTAP; ++t) load_w(t, w[t]);
-  load_b(0, b[0]);
+  if constexpr (BD > 0) load_b(0, b[0]);
 #pragma unroll
   for (int tap = 0; tap < NTAP; ++tap) {
     if (tap + WD < NTAP) load_w(tap + WD, w[(tap + WD) % (WD + 1)]);
-    if (tap + 1 < NTAP) load_b(tap + 1, b[(tap + 1) & 1]);
+    if constexpr (BD > 0) {
+      if (tap + 1 < NTAP) load_b(tap + 1, b[(tap + 1) % (BD + 1)]);
+    } else {
+      load_b(tap, b[0]);
+    }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < MAXT; ++u)
@@ -183,7 +228,7 @@ __device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t
       for (int mt = 0; mt < MTO; ++mt)
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks)
-          acc[u][mt] = P::mfma(w[tap % (WD + 1)][mt][ks], b[tap & 1][u][ks], acc[u][mt]);
+          acc[u][mt] = P::mfma(w[tap % (WD + 1)][mt][ks], b[tap % (BD + 1)][u][ks], acc[u][mt]);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -209,16 +254,16 @@ struct ConvW {
 
 // One 16-pixel tile of a convolution with register-resident weights: every
 // tap's B fragments are read from LDS first, then the MFMAs run back to back.
-template <int PREC, int C, int KIND, int MTO>
+template <int PREC, int C, int KIND, int MTO, int MAXT>
 __device__ __forceinline__ void conv_tile(f32x4 (&acc)[MTO], const uint8_t* buf, const ConvW<PREC, C, KIND, MTO>& cw,
-                                          const uint32_t (&to)[12]) {
+                                          const TapOff<MAXT>& to, int u) {
   using P = Prec<PREC>;
   using L = Lds<PREC, C>;
   constexpr int KS = C / P::KW;
   constexpr int N = Taps<KIND>::N;
   typename P::frag b[N][KS];
 #pragma unroll
-  for (int t = 0; t < N; ++t) L::template load_bfrags_at<KS>(buf, to[Taps<KIND>::bit(t)], b[t]);
+  for (int t = 0; t < N; ++t) L::template load_bfrags_at<KS>(buf, to.get(u, Taps<KIND>::bit(t)), b[t]);
 #pragma unroll
   for (int t = 0; t < N; ++t)
 #pragma unroll
@@ -246,17 +291,19 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
 
   extern __shared__ __align__(16) uint8_t lds[];
   // npix: pixels covered by the 16-pixel tiles; NPX = T*HW real pixels.
-  // Each buffer holds the NPX pixels plus one all-zero pixel (index NPX) that
-  // padding taps and the tile lanes past NPX read.
+  // Each buffer holds the NPX pixels plus ZERO_PIX all-zero pixels that padding
+  // taps and the tile lanes past NPX read: a lane whose tap target q falls
+  // outside reads zero pixel NPX + ((q - NPX) & 7), which has the bank
+  // placement q itself would have, so border tiles stay conflict-free.
   const int npix = a.npix;
   const int NPX = a.T * a.HW;
-  const int BUFB = (NPX + 1) * PIXB;
+  const int BUFB = (NPX + ZERO_PIX) * PIXB;
   uint8_t* V0 = lds;
   uint8_t* H0 = V0 + BUFB;
   uint8_t* V1 = H0 + BUFB;
   uint8_t* H1 = V1 + BUFB;
   uint8_t* CAT = H1 + BUFB;
-  float* sp = reinterpret_cast<float*>(CAT + BUFB);
+  int8_t* sp = reinterpret_cast<int8_t*>(CAT + BUFB);
   // per-site log-probabilities of the final block: V1 is neither read nor
   // written by the final block, so they reuse its space
   float2* part = reinterpret_cast<float2*>(V1);
@@ -273,7 +320,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
 
   int pix[MAXT], pi[MAXT], pj[MAXT];
   uint32_t msk[MAXT];
-  uint32_t toff[MAXT][12];
+  TapOff<MAXT> toff;
   bool inp[MAXT];
 #pragma unroll
   for (int u = 0; u < MAXT; ++u) {
@@ -296,14 +343,21 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
     msk[u] = m;
     inp[u] = in;
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      const int dr = k / 4 - 2, dc = k % 4 - 2;
-      const int pp = ((m >> k) & 1u) ? p + dr * W + dc : NPX;
-      toff[u][k] = (uint32_t)L::chunk(pp, L::base_chunk(g));
+    for (int k2 = 0; k2 < 6; ++k2) {
+      uint32_t two = 0;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = 2 * k2 + h;
+        const int dr = k / 4 - 2, dc = k % 4 - 2;
+        const int q = p + dr * W + dc;
+        const int pp = ((m >> k) & 1u) ? q : NPX + ((q - NPX) & (ZERO_PIX - 1));
+        two |= (uint32_t)L::chunk(pp, L::base_chunk(g)) << (16 * h);
+      }
+      toff.v[u][k2] = two;
     }
   }
-  for (int z = threadIdx.x; z < 5 * (PIXB / 4); z += blockDim.x) {
-    const int buf = z / (PIXB / 4), w4 = z - buf * (PIXB / 4);
+  for (int z = threadIdx.x; z < 5 * ZERO_PIX * (PIXB / 4); z += blockDim.x) {
+    const int buf = z / (ZERO_PIX * PIXB / 4), w4 = z - buf * (ZERO_PIX * PIXB / 4);
     reinterpret_cast<uint32_t*>(lds + buf * BUFB + NPX * PIXB)[w4] = 0u;
   }
 
@@ -314,9 +368,9 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   const frag* wall = reinterpret_cast<const frag*>(a.w);
 
   for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-    // ---- configurations -> LDS (float spins; connection applied on the fly)
+    // ---- configurations -> LDS (int8 spins; connection applied on the fly)
     for (int p = threadIdx.x; p < NPX; p += blockDim.x) {
-      float s = 0.0f;
+      int8_t s = 0;
       const int t = p / HW;
       const int64_t m = grp * T + t;
       if (m < M) {
@@ -339,7 +393,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
             else if (q == cb) v = row[ca];
           }
         }
-        s = (float)v;
+        s = v;
       }
       sp[p] = s;
     }
@@ -368,7 +422,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
           const bool valid = (msk[u] >> Taps<0>::bit(tap)) & 1u;
-          const float s = valid ? sp[pix[u] + Taps<0>::dr(tap) * W + Taps<0>::dc(tap)] : 0.0f;
+          const float s = valid ? (float)sp[pix[u] + Taps<0>::dr(tap) * W + Taps<0>::dc(tap)] : 0.0f;
 #pragma unroll
           for (int mt = 0; mt < MT; ++mt) {
             const f32x4 k = *reinterpret_cast<const f32x4*>(kv + tap * C + 16 * mt + 4 * g);
@@ -381,7 +435,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
 #pragma unroll
         for (int tap = 0; tap < 3; ++tap) {
           const bool valid = (msk[u] >> Taps<1>::bit(tap)) & 1u;
-          const float s = valid ? sp[pix[u] + Taps<1>::dc(tap)] : 0.0f;
+          const float s = valid ? (float)sp[pix[u] + Taps<1>::dc(tap)] : 0.0f;
 #pragma unroll
           for (int mt = 0; mt < MT; ++mt) {
             const f32x4 k = *reinterpret_cast<const f32x4*>(kh + tap * C + 16 * mt + 4 * g);
@@ -394,13 +448,28 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
       auto epi_v = [&](int u, const f32x4 (&acc)[MT]) {
         f32x4 av[MT];
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          av[mt] = act4<ACT>(acc[mt]);
-          if (inp[u]) {
-            const uint32_t so = L::self_quad(toff[u][10], mt);
-            if (kind == BK_FIRST) L::store_quad_at(V0, so, av[mt]);
-            else if (kind == BK_A) L::store_quad_at(V1, so, av[mt]);
-            else if (kind == BK_B) L::store_quad_at(V0, so, act4<ACT>(L::load_quad_at(V0, so) + acc[mt]));
+        for (int mt = 0; mt < MT; ++mt) av[mt] = act4<ACT>(acc[mt]);
+        if (inp[u]) {
+          if constexpr (Prec<PREC>::H16) {
+#pragma unroll
+            for (int j = 0; j < MT / 2; ++j) {
+              const uint32_t so = toff.get(u, 10) ^ (64u * j);
+              if (kind == BK_FIRST) L::store_pair_at(V0, so, av[2 * j], av[2 * j + 1]);
+              else if (kind == BK_A) L::store_pair_at(V1, so, av[2 * j], av[2 * j + 1]);
+              else if (kind == BK_B) {
+                f32x4 rl, rh;
+                L::load_pair_at(V0, so, rl, rh);
+                L::store_pair_at(V0, so, act4<ACT>(rl + acc[2 * j]), act4<ACT>(rh + acc[2 * j + 1]));
+              }
+            }
+          } else {
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+              const uint32_t so = L::self_quad(toff.get(u, 10), mt);
+              if (kind == BK_FIRST) L::store_quad_at(V0, so, av[mt]);
+              else if (kind == BK_A) L::store_quad_at(V1, so, av[mt]);
+              else if (kind == BK_B) L::store_quad_at(V0, so, act4<ACT>(L::load_quad_at(V0, so) + acc[mt]));
+            }
           }
         }
         frag bf[KS];
@@ -454,7 +523,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
 #pragma unroll
               for (int mt = 0; mt < MT; ++mt) acc[mt] = bias4(bias + 16 * mt + 4 * g);
               if (kind == BK_FIRST) valu_v(u, acc);
-              else conv_tile<PREC, C, 0, MT>(acc, Vin, wv, toff[u]);
+              else conv_tile<PREC, C, 0, MT>(acc, Vin, wv, toff, u);
               epi_v(u, acc);
             }
           }
@@ -469,7 +538,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
 #pragma unroll
               for (int mt = 0; mt < MT; ++mt) acch[mt] = bias4(bias + C + 16 * mt + 4 * g);
               if (kind == BK_FIRST) valu_h(u, acch);
-              else conv_tile<PREC, C, 1, MT>(acch, Hin, wh, toff[u]);
+              else conv_tile<PREC, C, 1, MT>(acch, Hin, wh, toff, u);
               epi_h(u, acch);
             }
           }
@@ -523,12 +592,26 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
       auto epi_c = [&](int u, const f32x4 (&acc)[MT]) {
         if (kind != BK_FINAL) {
           if (inp[u]) {
+            if constexpr (Prec<PREC>::H16) {
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-              const uint32_t so = L::self_quad(toff[u][10], mt);
-              if (kind == BK_FIRST) L::store_quad_at(H0, so, act4<ACT>(acc[mt]));
-              else if (kind == BK_A) L::store_quad_at(H1, so, act4<ACT>(acc[mt]));
-              else L::store_quad_at(H0, so, act4<ACT>(L::load_quad_at(H0, so) + acc[mt]));
+              for (int j = 0; j < MT / 2; ++j) {
+                const uint32_t so = toff.get(u, 10) ^ (64u * j);
+                if (kind == BK_FIRST) L::store_pair_at(H0, so, act4<ACT>(acc[2 * j]), act4<ACT>(acc[2 * j + 1]));
+                else if (kind == BK_A) L::store_pair_at(H1, so, act4<ACT>(acc[2 * j]), act4<ACT>(acc[2 * j + 1]));
+                else {
+                  f32x4 rl, rh;
+                  L::load_pair_at(H0, so, rl, rh);
+                  L::store_pair_at(H0, so, act4<ACT>(rl + acc[2 * j]), act4<ACT>(rh + acc[2 * j + 1]));
+                }
+              }
+            } else {
+#pragma unroll
+              for (int mt = 0; mt < MT; ++mt) {
+                const uint32_t so = L::self_quad(toff.get(u, 10), mt);
+                if (kind == BK_FIRST) L::store_quad_at(H0, so, act4<ACT>(acc[mt]));
+                else if (kind == BK_A) L::store_quad_at(H1, so, act4<ACT>(acc[mt]));
+                else L::store_quad_at(H0, so, act4<ACT>(L::load_quad_at(H0, so) + acc[mt]));
+              }
             }
           }
           return;
@@ -547,8 +630,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
           // z = (Re z0, Re z1, Im z0, Im z1) for spin values (+1, -1)
           const float mx = fmaxf(z.x, z.y);
           const float lse = mx + 0.5f * log1pf(expf(-2.0f * fabsf(z.x - z.y)));
-          const float s = sp[pix[u]];
-          const bool up = s > 0.0f;
+          const bool up = sp[pix[u]] > 0;
           part[pix[u]] = make_float2((up ? z.x : z.y) - lse, up ? z.z : z.w);
           if (a.cond) {
             const int t = pix[u] / HW;
@@ -570,7 +652,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
             f32x4 acc[MT];
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) acc[mt] = bias4(bias + 3 * C + 16 * mt + 4 * g);
-            conv_tile<PREC, C, 2, MT>(acc, CAT, wc, toff[u]);
+            conv_tile<PREC, C, 2, MT>(acc, CAT, wc, toff, u);
             epi_c(u, acc);
           }
         }
@@ -629,9 +711,9 @@ struct FwdPlan {
   size_t lds;
 };
 
-// five activation buffers of P + 1 pixels and the float spins (the per-site
-// log-probabilities live in V1)
-static size_t fwd_lds_bytes(int P, int pixb) { return (size_t)(P + 1) * pixb * 5 + (size_t)P * 4; }
+// five activation buffers of P + ZERO_PIX pixels and the int8 spins (the
+// per-site log-probabilities live in V1)
+static size_t fwd_lds_bytes(int P, int pixb) { return (size_t)(P + ZERO_PIX) * pixb * 5 + (size_t)P; }
 
 // One launch shape of the forward: loop order and waves per workgroup.
 struct FwdSchedule {
