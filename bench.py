@@ -36,6 +36,12 @@ ACT = 'relu'
 PEAK_BF16 = 2.5e15      # MI355X dense BF16 MFMA FLOP/s
 PEAK_FP32 = 157.3e12    # MI355X FP32 FLOP/s
 PEAK_HBM = 8.0e12       # MI355X HBM3E bytes/s
+# HBM bytes per connected configuration moved by the fused forward, measured
+# with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over `bench.py --profile-forward
+# 1024` (103,432 configurations), each scaled by the reading of a 256 MiB
+# device copy (`bench.py --profile-copy`: FETCH x 2.0, WRITE x 1.0):
+# (8,953.7 KiB x 2 + 1,292.9 KiB) / 103,432 = 190 B.
+FWD_HBM_BYTES_PER_CONFIG = 190.1
 
 
 def parse():
@@ -327,7 +333,8 @@ def main():
                    'connected_evaluations_per_step': n_conn, 'params': m16.n_params},
         'energy_per_site': float(mom[0]) / (H * W),
         'roofline': {'bound': 'mfma', 'achieved': round(achieved / 1e12, 2), 'peak': PEAK_BF16 / 1e12,
-                     'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_BF16, 4), 'traffic': None,
+                     'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_BF16, 4),
+                     'traffic': round(FWD_HBM_BYTES_PER_CONFIG * n_conn),
                      'kernel': 'convar_fwd_kernel<bf16,32>', 'kernel_ms': round(fwd_ms, 3),
                      'flop_per_launch': flop_launch},
         'forward_schedule': tune,
