@@ -81,3 +81,48 @@ def test_autoregressive_sampler_matches_oracle_rule(rng):
     ok = margin > 1e-6
     assert ok.mean() > 0.95
     assert np.array_equal(got[ok], ref[ok])
+
+
+def test_reduction_kernels_on_reference_fixture():
+    """The E_loc reductions (ragged fk_local_energy over the compacted use
+    mask, and fk_local_energy_dense over every slot) on the reference's
+    unbalanced/balanced fixture (tests/test_variational.py:18-46) with the
+    log psi of a linear complex wave function."""
+    from flowket_amd import _lib
+    from tests.test_oracle import reference_fixture
+    gpu_banner('E_loc reduction kernels on the reference fixture')
+    conn, mel, use, w = reference_fixture(np.random.default_rng(3))
+    nc, B, n = conn.shape
+    lp = (conn.reshape(nc * B, n) @ w).astype(np.complex64)          # (nc*B,)
+    ref = oob.local_values_unbalanced(lambda x: (x.reshape(x.shape[0], -1) @ w).astype(np.complex64),
+                                      conn, mel, use)
+    # dense: every slot
+    lpd = torch.view_as_real(torch.as_tensor(lp)).contiguous().cuda()
+    # the operators of the path have real matrix elements: the kernels carry
+    # float64 mel, so the fixture's real parts are used
+    mel_re = torch.as_tensor(np.ascontiguousarray(mel.real)).cuda()
+    out = torch.empty((B, 2), dtype=torch.float64, device='cuda')
+    _lib.call('fk_local_energy_dense', _lib.ptr(lpd), _lib.ptr(mel_re), nc, B, _lib.ptr(out), _lib.stream_ptr())
+    got_dense = out[:, 0].cpu().numpy() + 1j * out[:, 1].cpu().numpy()
+    ref_re = oob.local_values_balanced(lambda x: (x.reshape(x.shape[0], -1) @ w).astype(np.complex64), conn,
+                                       mel.real)
+    assert np.allclose(got_dense, ref_re, rtol=1e-6)
+    # ragged: compact the use mask, evaluate only used slots
+    use_d = torch.as_tensor(use.astype(np.uint8)).contiguous().cuda()
+    offsets = torch.empty(B + 1, dtype=torch.int64, device='cuda')
+    pairs = torch.empty((nc * B, 2), dtype=torch.int32, device='cuda')
+    _lib.call('fk_compact_connections', _lib.ptr(use_d), B, nc, _lib.ptr(offsets), _lib.ptr(pairs), None,
+              _lib.stream_ptr())
+    M = int(offsets[-1])
+    assert M == int(use.sum())
+    pr = pairs[:M].cpu().numpy()
+    lpr = torch.view_as_real(torch.as_tensor(lp.reshape(nc, B)[pr[:, 1], pr[:, 0]])).contiguous().cuda()
+    _lib.call('fk_local_energy', _lib.ptr(offsets), _lib.ptr(pairs), _lib.ptr(lpr), _lib.ptr(mel_re), B,
+              _lib.ptr(out), _lib.stream_ptr())
+    got = out[:, 0].cpu().numpy() + 1j * out[:, 1].cpu().numpy()
+    ref_u = oob.local_values_unbalanced(lambda x: (x.reshape(x.shape[0], -1) @ w).astype(np.complex64), conn,
+                                        mel.real, use)
+    assert np.allclose(got, ref_u, rtol=1e-6)
+    # the fixture's point: balanced and unbalanced agree on the mean
+    assert np.allclose(np.mean(ref), np.mean(oob.local_values_balanced(
+        lambda x: (x.reshape(x.shape[0], -1) @ w).astype(np.complex64), conn, mel)))

@@ -20,19 +20,27 @@ def test_heisenberg_4x4_pbc_ground_energy():
     assert e == pytest.approx(-44.9139328337, abs=1e-8)
 
 
-def test_ising_ground_energies_published():
-    """examples/basic_autoregressive_2d.py:39 and
-    examples/basic_autoregressive_monte_carlo_gradient.py:41."""
-    e2 = oex.ground_energy(lambda s: oop.ising_find_conn(s, h=3.0, pbc=False), (4, 4))
-    assert e2 == pytest.approx(-50.18662388277671, abs=1e-9)
-    e1 = oex.ground_energy(lambda s: oop.ising_find_conn(s, h=3.0, pbc=False), (16,))
-    assert e1 == pytest.approx(-49.257706531889006, abs=1e-9)
+def _golden():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), 'golden', 'reference_fixtures.json')) as f:
+        return json.load(f)
 
 
-def test_j1j2_4x4_obc_ground_energy_published():
-    """examples/j1j2_2d_monte_carlo_4.py:43."""
-    e = oex.ground_energy(lambda s: oop.j1j2_find_conn(s, j2=0.5, pbc=False), (4, 4))
-    assert e == pytest.approx(-30.022227800323677, abs=1e-9)
+@pytest.mark.parametrize('k', range(3))
+def test_ground_energies_published(k):
+    """Ground energies the reference's examples print as the target
+    (examples/basic_autoregressive_2d.py:39,
+    examples/basic_autoregressive_monte_carlo_gradient.py:41,
+    examples/j1j2_2d_monte_carlo_4.py:43), by ED of the oracle's matrix
+    elements."""
+    g = _golden()['ground_energies'][k]
+    shape = tuple(g['shape'])
+    if g['model'] == 'ising':
+        fc = lambda s: oop.ising_find_conn(s, h=g['h'], j=g['j'], pbc=g['pbc'])  # noqa: E731
+    else:
+        fc = lambda s: oop.j1j2_find_conn(s, j2=g['j2'], pbc=g['pbc'])  # noqa: E731
+    assert oex.ground_energy(fc, shape) == pytest.approx(g['energy'], abs=1e-9)
 
 
 def test_heisenberg_chain_bethe_limit():
@@ -76,17 +84,23 @@ def test_j1j2_edge_count():
 
 
 # ---------------------------------------------------------------- E_loc
-def test_unbalanced_equals_balanced_reference_fixture():
+def reference_fixture(rng):
     """tests/test_variational.py:18-46 fixture (3 samples x 7 spins, complex
-    mel, use mask) with a linear complex wave function."""
-    rng = np.random.default_rng(0)
-    sample = np.array([[1, 1, 1, -1, -1, -1, -1], [1, 1, 1, -1, 1, -1, -1], [1, -1, 1, 1, -1, -1, -1]])
-    conn = rng.choice([-1, 1], size=(5, 3, 7)).astype(np.float64)
+    mel, use mask): (conn (5, 3, 7), mel (5, 3), use (5, 3), linear complex
+    wave-function weights w (7,))."""
+    g = _golden()['unbalanced_vs_balanced_local_energy']
+    sample = np.array(g['sample'])
+    nc = g['num_connections']
+    conn = rng.choice([-1, 1], size=(nc,) + sample.shape).astype(np.float64)
     conn[0] = sample
-    mel = np.array([[2.0, 7j + 8, 0.0, 0.0, 3], [0.0, 0.0, 0.0, 0.0, -1.0], [5.0, 3j, 0.0, -2, 9]]).T
-    use = np.array([[True, True, False, False, True], [True, False, False, False, True],
-                    [True, True, False, True, True]]).T
-    w = rng.normal(size=7) + 1j * rng.normal(size=7)
+    mel = (np.array(g['hamiltonian_values_re']) + 1j * np.array(g['hamiltonian_values_im'])).T
+    use = np.array(g['all_use_conn']).T
+    w = rng.normal(size=sample.shape[1]) + 1j * rng.normal(size=sample.shape[1])
+    return conn, mel, use, w
+
+
+def test_unbalanced_equals_balanced_reference_fixture():
+    conn, mel, use, w = reference_fixture(np.random.default_rng(0))
 
     def wf(x):
         return (x.reshape(x.shape[0], -1) @ w).astype(np.complex64)
