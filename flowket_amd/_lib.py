@@ -41,6 +41,9 @@ SIGNATURES = {
     'fk_pack_gather': (ctypes.c_int, [_P, _P, _I64, _I32, _P, _P]),
     'fk_convar_forward': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _I32,
                                          _P, _P, _P, _P, _P]),
+    'fk_convar_forward_ws': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _I32,
+                                            _P, _P, _P, _P, _P, _I64, _P]),
+    'fk_convar_forward_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32]),
     'fk_convar_num_schedules': (_I32, [_I32, _I32]),
     'fk_convar_schedule_info': (ctypes.c_int, [_I32, _I32, _I32, _P, _P]),
     'fk_convar_set_schedule': (ctypes.c_int, [_I32, _I32, _I32, _I32, _I32]),

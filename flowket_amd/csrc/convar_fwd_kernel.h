@@ -1,0 +1,753 @@
+// Fused ConvNetAutoregressive2D forward kernel template (machines/
+// conv_net_autoregressive_2D.py:33-74 + machines/abstract_machine.py:31-71),
+// shared by convar_fwd.hip (lattices resident in LDS) and convar_fwd_large.hip
+// (the horizontal-stack lattices in a per-workgroup global scratch, for
+// lattices whose five buffers exceed the LDS).
+//
+// One workgroup evaluates a tile of T configurations through the whole network
+// (buffers V0, V1, CAT, H0, H1 of T*H*W pixels x C channels).  Each
+// convolution is an implicit GEMM on MFMA: A = weights (16 output channels x k
+// input slots), B = activations (k slots x 16 pixels) read at the tap offset,
+// D = 16 channels x 16 pixels.  The 1x1 convolutions (x, y, head) take their B
+// operand straight from the registers of the preceding convolution's D
+// fragment: input slots are numbered so that a D fragment *is* a B fragment
+// (see convar_layout.h).  Zero padding, DownShift and RightShift are index
+// arithmetic (tap validity masks and shifted write targets), never
+// materialised.
+#pragma once
+#include "fk_common.h"
+#include "convar_layout.h"
+#include "convar_dev.h"
+
+namespace fk {
+
+// ---------------------------------------------------------------------------
+// all-zero pixels after each activation buffer (one per bank placement of
+// the 8-pixel swizzle period)
+constexpr int ZERO_PIX = 8;
+
+// LDS addressing.  A pixel holds C channels as 16-byte chunks; chunk indices
+// are XOR-swizzled with the pixel index so that 16 lanes reading the same
+// chunk of 16 consecutive pixels hit distinct banks.
+template <int PREC, int C>
+struct Lds {
+  static constexpr int ESZ = Prec<PREC>::ESZ;
+  static constexpr int PIXB = C * ESZ;
+  static constexpr int CPP = PIXB / 16;  // chunks per pixel
+  static constexpr int ROWB = 128;       // bytes per LDS bank row (32 banks x 4 B)
+  static constexpr int PPR = PIXB >= ROWB ? 1 : ROWB / PIXB;
+  static constexpr bool SWZ = (CPP & (CPP - 1)) == 0;
+  __device__ static __forceinline__ int chunk(int p, int c) {
+    const int sw = SWZ ? ((p / PPR) & (CPP - 1)) : 0;
+    return p * PIXB + ((c ^ sw) << 4);
+  }
+  // byte offset of the 4-channel quad q (channels 4q..4q+3) of pixel p
+  __device__ static __forceinline__ int quad(int p, int q) {
+    if constexpr (Prec<PREC>::H16) {
+      const int ks = q >> 3, w = q & 7;
+      return chunk(p, 4 * ks + (w & 3)) + ((w >> 2) << 3);
+    } else {
+      return chunk(p, (q & 3) * (C / 16) + (q >> 2));
+    }
+  }
+  __device__ static __forceinline__ void store_quad(uint8_t* buf, int p, int q, f32x4 v) {
+    if constexpr (Prec<PREC>::H16) {
+      *reinterpret_cast<typename Prec<PREC>::quad*>(buf + quad(p, q)) = to_quad<PREC>(v);
+    } else {
+      *reinterpret_cast<f32x4*>(buf + quad(p, q)) = v;
+    }
+  }
+  __device__ static __forceinline__ f32x4 load_quad(const uint8_t* buf, int p, int q) {
+    if constexpr (Prec<PREC>::H16) {
+      return from_quad<PREC>(*reinterpret_cast<const typename Prec<PREC>::quad*>(buf + quad(p, q)));
+    } else {
+      return *reinterpret_cast<const f32x4*>(buf + quad(p, q));
+    }
+  }
+  // Chunk of k-step 0 for lane group g (the address that precomputed tap
+  // offsets point at); further k-steps / chunks are reached by XOR.
+  __device__ static __forceinline__ int base_chunk(int g) {
+    return Prec<PREC>::H16 ? g : g * (C / 16);
+  }
+  template <int KS>
+  __device__ static __forceinline__ void load_bfrags_at(const uint8_t* buf, uint32_t off,
+                                                        typename Prec<PREC>::frag (&b)[KS]) {
+    if constexpr (Prec<PREC>::H16) {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+        b[ks] = *reinterpret_cast<const typename Prec<PREC>::frag*>(buf + (off ^ (64u * ks)));
+    } else {
+#pragma unroll
+      for (int j = 0; j < KS / 4; ++j) {
+        f32x4 v = *reinterpret_cast<const f32x4*>(buf + (off ^ (16u * j)));
+        b[4 * j + 0] = v.x;
+        b[4 * j + 1] = v.y;
+        b[4 * j + 2] = v.z;
+        b[4 * j + 3] = v.w;
+      }
+    }
+  }
+  // byte offset of M-tile mt's quad of this lane from the lane's own
+  // k-step-0 chunk offset
+  __device__ static __forceinline__ uint32_t self_quad(uint32_t off, int mt) {
+    if constexpr (Prec<PREC>::H16) return (off ^ (64u * (mt >> 1))) + 8u * (mt & 1);
+    else return off ^ (16u * mt);
+  }
+  __device__ static __forceinline__ void store_quad_at(uint8_t* buf, uint32_t off, f32x4 v) {
+    if constexpr (Prec<PREC>::H16) {
+      *reinterpret_cast<typename Prec<PREC>::quad*>(buf + off) = to_quad<PREC>(v);
+    } else {
+      *reinterpret_cast<f32x4*>(buf + off) = v;
+    }
+  }
+  __device__ static __forceinline__ f32x4 load_quad_at(const uint8_t* buf, uint32_t off) {
+    if constexpr (Prec<PREC>::H16) {
+      return from_quad<PREC>(*reinterpret_cast<const typename Prec<PREC>::quad*>(buf + off));
+    } else {
+      return *reinterpret_cast<const f32x4*>(buf + off);
+    }
+  }
+  // 16-bit precisions: the quads of M-tiles 2j and 2j+1 share the 16-byte
+  // chunk at off ^ 64j, so they are written and read as one access.
+  __device__ static __forceinline__ void store_pair_at(uint8_t* buf, uint32_t off, f32x4 lo, f32x4 hi) {
+    using Q = typename Prec<PREC>::quad;
+    const Q a = to_quad<PREC>(lo), b = to_quad<PREC>(hi);
+    typename Prec<PREC>::frag f;
+    f[0] = a[0];
+    f[1] = a[1];
+    f[2] = a[2];
+    f[3] = a[3];
+    f[4] = b[0];
+    f[5] = b[1];
+    f[6] = b[2];
+    f[7] = b[3];
+    *reinterpret_cast<typename Prec<PREC>::frag*>(buf + off) = f;
+  }
+  __device__ static __forceinline__ void load_pair_at(const uint8_t* buf, uint32_t off, f32x4& lo, f32x4& hi) {
+    const typename Prec<PREC>::frag f = *reinterpret_cast<const typename Prec<PREC>::frag*>(buf + off);
+    lo = f32x4{(float)f[0], (float)f[1], (float)f[2], (float)f[3]};
+    hi = f32x4{(float)f[4], (float)f[5], (float)f[6], (float)f[7]};
+  }
+  // All B fragments (one per k-step) of pixel p for lane group g.
+  template <int KS>
+  __device__ static __forceinline__ void load_bfrags(const uint8_t* buf, int p, int g,
+                                                     typename Prec<PREC>::frag (&b)[KS]) {
+    if constexpr (Prec<PREC>::H16) {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+        b[ks] = *reinterpret_cast<const typename Prec<PREC>::frag*>(buf + chunk(p, 4 * ks + g));
+    } else {
+#pragma unroll
+      for (int j = 0; j < KS / 4; ++j) {
+        f32x4 v = *reinterpret_cast<const f32x4*>(buf + chunk(p, g * (C / 16) + j));
+        b[4 * j + 0] = v.x;
+        b[4 * j + 1] = v.y;
+        b[4 * j + 2] = v.z;
+        b[4 * j + 3] = v.w;
+      }
+    }
+  }
+};
+
+struct FwdArgs {
+  const void* w;
+  const float* aux;
+  int H, W, HW, nblocks, T, npix;
+  const int8_t* spins;
+  const int32_t* pairs;
+  const int64_t* m_dev;
+  int64_t m_count;
+  int conn_kind;
+  const int32_t* conn_a;
+  const int32_t* conn_b;
+  float* logpsi;
+  float* cond;
+  uint8_t* hscr;  // HG kernels: 2 lattices per workgroup of global scratch
+};
+
+// Per-tile LDS byte offsets (from the buffer start) of the 12 causal taps
+// k = (dr+2)*4 + (dc+2) of this lane's pixel, two 16-bit offsets per
+// register: every activation buffer is below 64 KB.
+template <int MAXT>
+struct TapOff {
+  uint32_t v[MAXT][6];
+  __device__ __forceinline__ uint32_t get(int u, int k) const {
+    return (k & 1) ? (v[u][k >> 1] >> 16) : (v[u][k >> 1] & 0xffffu);
+  }
+};
+
+// acc[u][mt] += sum over taps of W[tap][mt] x B(pixel + tap offset).
+// toff.get(u, k) is this lane's byte offset of k-step 0 at tap k of tile u,
+// pointing at the all-zero pixel for padding taps -- tiles past
+// the wave's count hold only such offsets, so every tile is computed without
+// a branch (their epilogues are skipped).
+// Software pipeline: the weights (read from L2) are requested WD taps ahead
+// and the B fragments of all tiles BD taps ahead (depths bounded by the
+// registers they take); scheduling fences keep the compiler from sinking
+// those requests next to their first use.
+template <int PREC, int C, int MAXT, int KIND, int MTO>
+__device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t* buf,
+                                          const typename Prec<PREC>::frag* __restrict__ wg,
+                                          const TapOff<MAXT>& toff, int lane) {
+  using P = Prec<PREC>;
+  using frag = typename P::frag;
+  using L = Lds<PREC, C>;
+  constexpr int KS = C / P::KW;
+  constexpr int NTAP = Taps<KIND>::N;
+  // prefetch depth bounded by the registers a tap's fragments take
+  constexpr int FR = (int)(sizeof(frag) / 4);
+  constexpr int WREGS = MTO * KS * FR, BREGS = MAXT * KS * FR;
+  constexpr int WD = WREGS <= 16 ? 2 : (WREGS <= 32 ? 1 : 0);  // weight prefetch distance (taps)
+  constexpr int BD = BREGS <= 32 ? 1 : 0;                       // B prefetch distance (taps)
+  frag w[WD + 1][MTO][KS];
+  frag b[BD + 1][MAXT][KS];
+  auto load_w = [&](int tap, frag (&dst)[MTO][KS]) {
+#pragma unroll
+    for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) dst[mt][ks] = wg[((tap * MTO + mt) * KS + ks) * 64 + lane];
+  };
+  auto load_b = [&](int tap, frag (&dst)[MAXT][KS]) {
+#pragma unroll
+    for (int u = 0; u < MAXT; ++u) L::template load_bfrags_at<KS>(buf, toff.get(u, Taps<KIND>::bit(tap)), dst[u]);
+  };
+#pragma unroll
+  for (int t = 0; t < WD && t < NTAP; ++t) load_w(t, w[t]);
+  if constexpr (BD > 0) load_b(0, b[0]);
+#pragma unroll
+  for (int tap = 0; tap < NTAP; ++tap) {
+    if (tap + WD < NTAP) load_w(tap + WD, w[(tap + WD) % (WD + 1)]);
+    if constexpr (BD > 0) {
+      if (tap + 1 < NTAP) load_b(tap + 1, b[(tap + 1) % (BD + 1)]);
+    } else {
+      load_b(tap, b[0]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < MAXT; ++u)
+#pragma unroll
+      for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+          acc[u][mt] = P::mfma(w[tap % (WD + 1)][mt][ks], b[tap % (BD + 1)][u][ks], acc[u][mt]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+__device__ __forceinline__ f32x4 bias4(const float* b) { return *reinterpret_cast<const f32x4*>(b); }
+
+// Register-resident weights of one convolution (all taps) for the tile-outer
+// schedule.
+template <int PREC, int C, int KIND, int MTO>
+struct ConvW {
+  static constexpr int KS = C / Prec<PREC>::KW;
+  static constexpr int N = Taps<KIND>::N;
+  typename Prec<PREC>::frag w[N][MTO][KS];
+  __device__ __forceinline__ void load(const typename Prec<PREC>::frag* __restrict__ wg, int lane) {
+#pragma unroll
+    for (int t = 0; t < N; ++t)
+#pragma unroll
+      for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) w[t][mt][ks] = wg[((t * MTO + mt) * KS + ks) * 64 + lane];
+  }
+};
+
+// One 16-pixel tile of a convolution with register-resident weights: every
+// tap's B fragments are read from LDS first, then the MFMAs run back to back.
+template <int PREC, int C, int KIND, int MTO, int MAXT>
+__device__ __forceinline__ void conv_tile(f32x4 (&acc)[MTO], const uint8_t* buf, const ConvW<PREC, C, KIND, MTO>& cw,
+                                          const TapOff<MAXT>& to, int u) {
+  using P = Prec<PREC>;
+  using L = Lds<PREC, C>;
+  constexpr int KS = C / P::KW;
+  constexpr int N = Taps<KIND>::N;
+  typename P::frag b[N][KS];
+#pragma unroll
+  for (int t = 0; t < N; ++t) L::template load_bfrags_at<KS>(buf, to.get(u, Taps<KIND>::bit(t)), b[t]);
+#pragma unroll
+  for (int t = 0; t < N; ++t)
+#pragma unroll
+    for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) acc[mt] = P::mfma(cw.w[t][mt][ks], b[t][ks], acc[mt]);
+}
+
+// SCHED 0: taps outer -- each tap's weights are loaded once and serve all the
+// wave's tiles (accumulators of every tile live in registers).
+// SCHED 1: tiles outer -- a 3x3 convolution's weights stay in registers for
+// the whole phase and the wave walks its tiles one by one (fits bf16 C=32 and
+// fp32 C=16).
+// HG: the horizontal-stack lattices H0, H1 live in global scratch (a.hscr,
+// two lattices per workgroup, L2-resident) instead of LDS.
+template <int PREC, int C, int MAXT, int ACT, int NWAVE, int SCHED, bool HG>
+__global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
+  using P = Prec<PREC>;
+  using frag = typename P::frag;
+  using L = Lds<PREC, C>;
+  constexpr int MT = C / 16;
+  constexpr int KS = C / P::KW;
+  constexpr int MTH = (C / 2 + 15) / 16;
+  constexpr int PIXB = L::PIXB;
+  constexpr bool TILE_OUTER = SCHED == 1;
+  const ConvarLayout lay(C, PREC);
+
+  extern __shared__ __align__(16) uint8_t lds[];
+  // npix: pixels covered by the 16-pixel tiles; NPX = T*HW real pixels.
+  // Each buffer holds the NPX pixels plus ZERO_PIX all-zero pixels that padding
+  // taps and the tile lanes past NPX read: a lane whose tap target q falls
+  // outside reads zero pixel NPX + ((q - NPX) & 7), which has the bank
+  // placement q itself would have, so border tiles stay conflict-free.
+  const int npix = a.npix;
+  const int NPX = a.T * a.HW;
+  const int BUFB = (NPX + ZERO_PIX) * PIXB;
+  uint8_t* V0 = lds;
+  uint8_t* V1 = V0 + BUFB;
+  uint8_t* CAT = V1 + BUFB;
+  uint8_t* H0;
+  uint8_t* H1;
+  int8_t* sp;
+  if constexpr (HG) {
+    H0 = a.hscr + (int64_t)blockIdx.x * 2 * BUFB;
+    H1 = H0 + BUFB;
+    sp = reinterpret_cast<int8_t*>(CAT + BUFB);
+  } else {
+    H0 = CAT + BUFB;
+    H1 = H0 + BUFB;
+    sp = reinterpret_cast<int8_t*>(H1 + BUFB);
+  }
+  // per-site log-probabilities of the final block: V1 is neither read nor
+  // written by the final block, so they reuse its space
+  float2* part = reinterpret_cast<float2*>(V1);
+
+  const int lane = threadIdx.x & 63;
+  // wave index and tile count are wave-uniform: keep them in SGPRs so tile
+  // predicates are scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int NW = blockDim.x >> 6;
+  const int n16 = lane & 15, g = lane >> 4;
+  const int H = a.H, W = a.W, HW = a.HW, T = a.T;
+  const int ntiles = npix >> 4;
+  const int nt = (ntiles > wave) ? (ntiles - wave + NW - 1) / NW : 0;
+
+  int pix[MAXT], pi[MAXT], pj[MAXT];
+  uint32_t msk[MAXT];
+  TapOff<MAXT> toff;
+  bool inp[MAXT];
+#pragma unroll
+  for (int u = 0; u < MAXT; ++u) {
+    const int p = (wave + u * NW) * 16 + n16;
+    const bool in = (u < nt) && (p < T * HW);
+    const int t = in ? p / HW : 0;
+    const int rem = in ? p - t * HW : 0;
+    const int i = rem / W, j = rem - (rem / W) * W;
+    uint32_t m = 0;
+    if (in) {
+#pragma unroll
+      for (int dr = -2; dr <= 0; ++dr)
+#pragma unroll
+        for (int dc = -2; dc <= 1; ++dc)
+          if (i + dr >= 0 && j + dc >= 0 && j + dc < W) m |= 1u << ((dr + 2) * 4 + (dc + 2));
+    }
+    pix[u] = p;
+    pi[u] = i;
+    pj[u] = j;
+    msk[u] = m;
+    inp[u] = in;
+#pragma unroll
+    for (int k2 = 0; k2 < 6; ++k2) {
+      uint32_t two = 0;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = 2 * k2 + h;
+        const int dr = k / 4 - 2, dc = k % 4 - 2;
+        const int q = p + dr * W + dc;
+        const int pp = ((m >> k) & 1u) ? q : NPX + ((q - NPX) & (ZERO_PIX - 1));
+        two |= (uint32_t)L::chunk(pp, L::base_chunk(g)) << (16 * h);
+      }
+      toff.v[u][k2] = two;
+    }
+  }
+  for (int z = threadIdx.x; z < 3 * ZERO_PIX * (PIXB / 4); z += blockDim.x) {
+    const int buf = z / (ZERO_PIX * PIXB / 4), w4 = z - buf * (ZERO_PIX * PIXB / 4);
+    reinterpret_cast<uint32_t*>(lds + buf * BUFB + NPX * PIXB)[w4] = 0u;
+  }
+  for (int z = threadIdx.x; z < ZERO_PIX * (PIXB / 4); z += blockDim.x) {
+    reinterpret_cast<uint32_t*>(H0 + NPX * PIXB)[z] = 0u;
+    reinterpret_cast<uint32_t*>(H1 + NPX * PIXB)[z] = 0u;
+  }
+
+  int64_t M = a.m_dev ? *a.m_dev : a.m_count;
+  if (M > a.m_count) M = a.m_count;
+  const int64_t ngroups = (M + T - 1) / T;
+  const int nb = a.nblocks;
+  const frag* wall = reinterpret_cast<const frag*>(a.w);
+
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    // ---- configurations -> LDS (int8 spins; connection applied on the fly)
+    for (int p = threadIdx.x; p < NPX; p += blockDim.x) {
+      int8_t s = 0;
+      const int t = p / HW;
+      const int64_t m = grp * T + t;
+      if (m < M) {
+        const int q = p - t * HW;
+        int64_t b = m;
+        int c = 0;
+        if (a.pairs) {
+          b = a.pairs[2 * m];
+          c = a.pairs[2 * m + 1];
+        }
+        const int8_t* row = a.spins + b * HW;
+        int8_t v = row[q];
+        if (c > 0) {
+          const int ca = a.conn_a[c];
+          if (a.conn_kind == FK_CONN_FLIP) {
+            if (q == ca) v = (int8_t)-v;
+          } else if (ca >= 0) {
+            const int cb = a.conn_b[c];
+            if (q == ca) v = row[cb];
+            else if (q == cb) v = row[ca];
+          }
+        }
+        s = v;
+      }
+      sp[p] = s;
+    }
+    __syncthreads();
+
+    for (int blk = 0; blk < nb; ++blk) {
+      const int kind = block_kind(blk, nb);
+      const uint8_t* Vin = (kind == BK_B) ? V1 : V0;
+      const uint8_t* Hin = (kind == BK_B) ? H1 : H0;
+      const float* bias = a.aux + lay.aux_bias(blk);
+      const float* by = bias + 2 * C + C / 2;
+      const float* bx = bias + 2 * C;
+
+      frag wy[MTH][KS], wx[MTH][KS];
+#pragma unroll
+      for (int mh = 0; mh < MTH; ++mh)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          wy[mh][ks] = wall[(lay.off_y(blk) + mh * KS + ks) * 64 + lane];
+          wx[mh][ks] = wall[(lay.off_x(blk) + mh * KS + ks) * 64 + lane];
+        }
+
+      // -- block-0 convolutions on the 1-channel input (VALU)
+      auto valu_v = [&](int u, f32x4 (&acc)[MT]) {
+        const float* kv = a.aux + lay.aux_kv0(nb);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+          const bool valid = (msk[u] >> Taps<0>::bit(tap)) & 1u;
+          const float s = valid ? (float)sp[pix[u] + Taps<0>::dr(tap) * W + Taps<0>::dc(tap)] : 0.0f;
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            const f32x4 k = *reinterpret_cast<const f32x4*>(kv + tap * C + 16 * mt + 4 * g);
+            acc[mt] += k * s;
+          }
+        }
+      };
+      auto valu_h = [&](int u, f32x4 (&acc)[MT]) {
+        const float* kh = a.aux + lay.aux_kh0(nb);
+#pragma unroll
+        for (int tap = 0; tap < 3; ++tap) {
+          const bool valid = (msk[u] >> Taps<1>::bit(tap)) & 1u;
+          const float s = valid ? (float)sp[pix[u] + Taps<1>::dc(tap)] : 0.0f;
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            const f32x4 k = *reinterpret_cast<const f32x4*>(kh + tap * C + 16 * mt + 4 * g);
+            acc[mt] += k * s;
+          }
+        }
+      };
+      // -- epilogue of the vertical kxk: vertical output + y = act(1x1(
+      //    DownShift(act(v')))) written one row down
+      auto epi_v = [&](int u, const f32x4 (&acc)[MT]) {
+        f32x4 av[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) av[mt] = act4<ACT>(acc[mt]);
+        if (inp[u]) {
+          if constexpr (Prec<PREC>::H16) {
+#pragma unroll
+            for (int j = 0; j < MT / 2; ++j) {
+              const uint32_t so = toff.get(u, 10) ^ (64u * j);
+              if (kind == BK_FIRST) L::store_pair_at(V0, so, av[2 * j], av[2 * j + 1]);
+              else if (kind == BK_A) L::store_pair_at(V1, so, av[2 * j], av[2 * j + 1]);
+              else if (kind == BK_B) {
+                f32x4 rl, rh;
+                L::load_pair_at(V0, so, rl, rh);
+                L::store_pair_at(V0, so, act4<ACT>(rl + acc[2 * j]), act4<ACT>(rh + acc[2 * j + 1]));
+              }
+            }
+          } else {
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+              const uint32_t so = L::self_quad(toff.get(u, 10), mt);
+              if (kind == BK_FIRST) L::store_quad_at(V0, so, av[mt]);
+              else if (kind == BK_A) L::store_quad_at(V1, so, av[mt]);
+              else if (kind == BK_B) L::store_quad_at(V0, so, act4<ACT>(L::load_quad_at(V0, so) + acc[mt]));
+            }
+          }
+        }
+        frag bf[KS];
+        d_to_b<PREC, MT, KS>(av, bf);
+        const bool down = pi[u] + 1 < H;
+        const int tgt = down ? pix[u] + W : pix[u] - (H - 1) * W;
+#pragma unroll
+        for (int mh = 0; mh < MTH; ++mh) {
+          const f32x4 b0 = bias4(by + 16 * mh + 4 * g);
+          f32x4 y = b0;
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) y = P::mfma(wy[mh][ks], bf[ks], y);
+          y = act4<ACT>(down ? y : b0);
+          if (inp[u] && 16 * mh + 4 * g < C / 2) L::store_quad(CAT, tgt, C / 8 + 4 * mh + g, y);
+        }
+      };
+      // -- epilogue of the horizontal 1xk: x = act(1x1(act(.))) into CAT (the
+      //    final block's RightShift folded into the write target)
+      auto epi_h = [&](int u, const f32x4 (&acc)[MT]) {
+        f32x4 ax[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) ax[mt] = act4<ACT>(acc[mt]);
+        frag bf[KS];
+        d_to_b<PREC, MT, KS>(ax, bf);
+        bool keep = true;
+        int tgt = pix[u];
+        if (kind == BK_FINAL) {
+          keep = pj[u] + 1 < W;
+          tgt = keep ? pix[u] + 1 : pix[u] - (W - 1);
+        }
+#pragma unroll
+        for (int mh = 0; mh < MTH; ++mh) {
+          const f32x4 b0 = bias4(bx + 16 * mh + 4 * g);
+          f32x4 x = b0;
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) x = P::mfma(wx[mh][ks], bf[ks], x);
+          x = act4<ACT>(keep ? x : b0);
+          if (inp[u] && 16 * mh + 4 * g < C / 2) L::store_quad(CAT, tgt, 4 * mh + g, x);
+        }
+      };
+
+      // ================= phase VH: vertical stack + horizontal 1xk =========
+      if constexpr (TILE_OUTER) {
+        {
+          ConvW<PREC, C, 0, MT> wv;
+          if (kind != BK_FIRST) wv.load(wall + lay.off_v(blk) * 64, lane);
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u) {
+            if (u < nt) {
+              f32x4 acc[MT];
+#pragma unroll
+              for (int mt = 0; mt < MT; ++mt) acc[mt] = bias4(bias + 16 * mt + 4 * g);
+              if (kind == BK_FIRST) valu_v(u, acc);
+              else conv_tile<PREC, C, 0, MT>(acc, Vin, wv, toff, u);
+              epi_v(u, acc);
+            }
+          }
+        }
+        {
+          ConvW<PREC, C, 1, MT> wh;
+          if (kind != BK_FIRST) wh.load(wall + lay.off_h(blk) * 64, lane);
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u) {
+            if (u < nt) {
+              f32x4 acch[MT];
+#pragma unroll
+              for (int mt = 0; mt < MT; ++mt) acch[mt] = bias4(bias + C + 16 * mt + 4 * g);
+              if (kind == BK_FIRST) valu_h(u, acch);
+              else conv_tile<PREC, C, 1, MT>(acch, Hin, wh, toff, u);
+              epi_h(u, acch);
+            }
+          }
+        }
+      } else {
+        {
+          f32x4 acc[MAXT][MT];
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + 16 * mt + 4 * g);
+          if (kind == BK_FIRST) {
+#pragma unroll
+            for (int u = 0; u < MAXT; ++u)
+              if (u < nt) valu_v(u, acc[u]);
+          } else {
+            conv_pass<PREC, C, MAXT, 0, MT>(acc, Vin, wall + lay.off_v(blk) * 64, toff, lane);
+          }
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u)
+            if (u < nt) epi_v(u, acc[u]);
+        }
+        {
+          f32x4 acc[MAXT][MT];
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + C + 16 * mt + 4 * g);
+          if (kind == BK_FIRST) {
+#pragma unroll
+            for (int u = 0; u < MAXT; ++u)
+              if (u < nt) valu_h(u, acc[u]);
+          } else {
+            conv_pass<PREC, C, MAXT, 1, MT>(acc, Hin, wall + lay.off_h(blk) * 64, toff, lane);
+          }
+#pragma unroll
+          for (int u = 0; u < MAXT; ++u)
+            if (u < nt) epi_h(u, acc[u]);
+        }
+      }
+      __syncthreads();
+
+      // ================= phase H3: kxk on the concatenation ================
+      frag whd[KS];
+      f32x4 hb = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (kind == BK_FINAL) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) whd[ks] = wall[(lay.head_base(nb) + ks) * 64 + lane];
+        hb = load_bias4(a.aux + lay.aux_head_bias(nb), 4 * g, 4);
+      }
+      auto epi_c = [&](int u, const f32x4 (&acc)[MT]) {
+        if (kind != BK_FINAL) {
+          if (inp[u]) {
+            if constexpr (Prec<PREC>::H16) {
+#pragma unroll
+              for (int j = 0; j < MT / 2; ++j) {
+                const uint32_t so = toff.get(u, 10) ^ (64u * j);
+                if (kind == BK_FIRST) L::store_pair_at(H0, so, act4<ACT>(acc[2 * j]), act4<ACT>(acc[2 * j + 1]));
+                else if (kind == BK_A) L::store_pair_at(H1, so, act4<ACT>(acc[2 * j]), act4<ACT>(acc[2 * j + 1]));
+                else {
+                  f32x4 rl, rh;
+                  L::load_pair_at(H0, so, rl, rh);
+                  L::store_pair_at(H0, so, act4<ACT>(rl + acc[2 * j]), act4<ACT>(rh + acc[2 * j + 1]));
+                }
+              }
+            } else {
+#pragma unroll
+              for (int mt = 0; mt < MT; ++mt) {
+                const uint32_t so = L::self_quad(toff.get(u, 10), mt);
+                if (kind == BK_FIRST) L::store_quad_at(H0, so, act4<ACT>(acc[mt]));
+                else if (kind == BK_A) L::store_quad_at(H1, so, act4<ACT>(acc[mt]));
+                else L::store_quad_at(H0, so, act4<ACT>(L::load_quad_at(H0, so) + acc[mt]));
+              }
+            }
+          }
+          return;
+        }
+        // head: 1x1 C->4 on act(h'), complex logits, 2-way log-normalise,
+        // gather at the actual spin (abstract_machine.py:34-37,54-57)
+        f32x4 ah[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) ah[mt] = act4<ACT>(acc[mt]);
+        frag bf[KS];
+        d_to_b<PREC, MT, KS>(ah, bf);
+        f32x4 z = hb;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) z = P::mfma(whd[ks], bf[ks], z);
+        if (g == 0 && inp[u]) {
+          // z = (Re z0, Re z1, Im z0, Im z1) for spin values (+1, -1)
+          const float mx = fmaxf(z.x, z.y);
+          const float lse = mx + 0.5f * log1pf(expf(-2.0f * fabsf(z.x - z.y)));
+          const bool up = sp[pix[u]] > 0;
+          part[pix[u]] = make_float2((up ? z.x : z.y) - lse, up ? z.z : z.w);
+          if (a.cond) {
+            const int t = pix[u] / HW;
+            const int64_t m = grp * T + t;
+            if (m < M) {
+              float* o = a.cond + (m * HW + (pix[u] - t * HW)) * 2;
+              o[0] = 2.0f * (z.x - lse);
+              o[1] = 2.0f * (z.y - lse);
+            }
+          }
+        }
+      };
+      if constexpr (TILE_OUTER) {
+        ConvW<PREC, C, 2, MT> wc;
+        wc.load(wall + lay.off_c(blk) * 64, lane);
+#pragma unroll
+        for (int u = 0; u < MAXT; ++u) {
+          if (u < nt) {
+            f32x4 acc[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) acc[mt] = bias4(bias + 3 * C + 16 * mt + 4 * g);
+            conv_tile<PREC, C, 2, MT>(acc, CAT, wc, toff, u);
+            epi_c(u, acc);
+          }
+        }
+      } else {
+        f32x4 acc[MAXT][MT];
+#pragma unroll
+        for (int u = 0; u < MAXT; ++u)
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + 3 * C + 16 * mt + 4 * g);
+        conv_pass<PREC, C, MAXT, 2, MT>(acc, CAT, wall + lay.off_c(blk) * 64, toff, lane);
+#pragma unroll
+        for (int u = 0; u < MAXT; ++u)
+          if (u < nt) epi_c(u, acc[u]);
+      }
+      __syncthreads();
+    }
+
+    // ---- per-configuration site sums (deterministic tree per wave)
+    for (int t = wave; t < T; t += NW) {
+      float sr = 0.0f, si = 0.0f;
+      for (int q = lane; q < HW; q += 64) {
+        const float2 v = part[t * HW + q];
+        sr += v.x;
+        si += v.y;
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off, 64);
+        si += __shfl_xor(si, off, 64);
+      }
+      const int64_t m = grp * T + t;
+      if (lane == 0 && m < M) {
+        a.logpsi[2 * m] = sr;
+        a.logpsi[2 * m + 1] = si;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+struct FwdPlan {
+  int T, npix, maxt, nwave;
+  size_t lds;
+};
+
+// persistent grid: CU count x workgroups per CU, at most one per group
+inline int64_t fwd_grid(const FwdArgs& a, const FwdPlan& pl) {
+  const int64_t ngroups = (a.m_count + pl.T - 1) / pl.T;
+  const int per_cu = (int)((160 * 1024) / pl.lds) > 0 ? (int)((160 * 1024) / pl.lds) : 1;
+  int64_t grid = (int64_t)device_cu_count() * per_cu;
+  if (grid > ngroups) grid = ngroups;
+  if (grid < 1) grid = 1;
+  return grid;
+}
+
+template <int PREC, int C, int MAXT, int ACT, int NWAVE, int SCHED, bool HG>
+int launch_fwd(const FwdArgs& a, const FwdPlan& pl, hipStream_t s) {
+  auto kern = convar_fwd_kernel<PREC, C, MAXT, ACT, NWAVE, SCHED, HG>;
+  // raise the dynamic LDS limit once per instantiation (not a stream
+  // operation, so launches may later be captured into a graph)
+  static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  FK_CHECK_HIP(attr);
+  hipLaunchKernelGGL(kern, dim3((unsigned)fwd_grid(a, pl)), dim3(64 * NWAVE), pl.lds, s, a);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+template <int PREC, int C, int NW, int SCHED, bool HG>
+int launch_variant(const FwdArgs& a, const FwdPlan& pl, int act, hipStream_t s) {
+  constexpr int LO = NW == 4 ? 4 : 2, HI = NW == 4 ? 8 : 4;
+  if (pl.maxt <= LO)
+    return act == FK_ACT_ELU ? launch_fwd<PREC, C, LO, FK_ACT_ELU, NW, SCHED, HG>(a, pl, s)
+                             : launch_fwd<PREC, C, LO, FK_ACT_RELU, NW, SCHED, HG>(a, pl, s);
+  return act == FK_ACT_ELU ? launch_fwd<PREC, C, HI, FK_ACT_ELU, NW, SCHED, HG>(a, pl, s)
+                           : launch_fwd<PREC, C, HI, FK_ACT_RELU, NW, SCHED, HG>(a, pl, s);
+}
+
+// the HG dispatcher (convar_fwd_large.hip)
+int dispatch_fwd_hg(const FwdArgs& a, const FwdPlan& pl, int prec, int cp, int act, hipStream_t s);
+
+}  // namespace fk

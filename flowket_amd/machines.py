@@ -90,6 +90,7 @@ class ConvNetAutoregressive2D:
         self.set_params(params)
         self._packs = {}
         self._maps = {}
+        self._fws = {}
         self.conditional_log_probs_model = _ConditionalLogProbsModel(self)
 
     # ------------------------------------------------------------------ params
@@ -230,10 +231,24 @@ class ConvNetAutoregressive2D:
         if pairs is not None:
             tb = operator.tables(self.device)
             kind, ca, cb = operator.kind, tb['conn_a'], tb['conn_b']
-        _lib.call('fk_convar_forward', _lib.ptr(wbuf), _lib.ptr(abuf), prec, self.H, self.W, self.depth,
+        ws = self._forward_workspace(prec)
+        _lib.call('fk_convar_forward_ws', _lib.ptr(wbuf), _lib.ptr(abuf), prec, self.H, self.W, self.depth,
                   self.num_of_channels, _ACT[self.activation], _lib.ptr(spins), _lib.ptr(pairs), _lib.ptr(m_dev),
-                  int(m), kind, _lib.ptr(ca), _lib.ptr(cb), _lib.ptr(out), _lib.ptr(cond), _lib.stream_ptr(stream))
+                  int(m), kind, _lib.ptr(ca), _lib.ptr(cb), _lib.ptr(out), _lib.ptr(cond), _lib.ptr(ws),
+                  0 if ws is None else ws.numel(), _lib.stream_ptr(stream))
         return out, cond
+
+    def _forward_workspace(self, prec):
+        """Global scratch of the forward for lattices whose activations exceed
+        the LDS (allocated once per precision; None when not needed)."""
+        key = (prec, _lib.load().fk_convar_get_schedule(self.H, self.W, self.num_of_channels, prec))
+        if key not in self._fws:
+            n = int(_lib.load().fk_convar_forward_workspace_bytes(self.H, self.W, self.num_of_channels, prec))
+            if n < 0:
+                raise _lib.FlowketHipError(f'ConvNetAutoregressive2D kernels: {self.H}x{self.W} lattice with '
+                                           f'{self.num_of_channels} channels is not supported')
+            self._fws[key] = torch.empty(n, dtype=torch.uint8, device=self.device) if n > 0 else None
+        return self._fws[key]
 
     def log_psi(self, x, precision=None):
         """(B,) complex64 log psi."""

@@ -148,6 +148,28 @@ int fk_convar_forward(const void* wpacked, const float* aux, int32_t precision,
                       const int32_t* conn_a, const int32_t* conn_b,
                       float* logpsi, float* cond_logp, void* stream);
 
+/* Same as fk_convar_forward, for every lattice the kernels support: when the
+ * five activation lattices of one configuration exceed the LDS (e.g. 20x20
+ * with 64 bf16 channels), the horizontal-stack lattices are kept in
+ * `workspace`, a caller-owned device buffer of at least
+ * fk_convar_forward_workspace_bytes(H, W, channels, precision) bytes (0 when
+ * everything fits in LDS; then workspace may be NULL).  Calls that share a
+ * workspace must be ordered on one stream.  fk_convar_forward returns
+ * FK_ERR_ARG for such lattices.  (No counterpart in the reference: its
+ * Keras convolutions have no on-chip memory limit.) */
+int fk_convar_forward_ws(const void* wpacked, const float* aux, int32_t precision,
+                         int32_t H, int32_t W, int32_t depth, int32_t channels,
+                         int32_t activation, const int8_t* spins,
+                         const int32_t* pairs, const int64_t* m_dev,
+                         int64_t m_count, int32_t conn_kind,
+                         const int32_t* conn_a, const int32_t* conn_b,
+                         float* logpsi, float* cond_logp, void* workspace,
+                         int64_t workspace_bytes, void* stream);
+/* Host-only (queries the current device's CU count).  Bytes of workspace
+ * fk_convar_forward_ws needs, 0 if none, -1 if the lattice is unsupported. */
+int64_t fk_convar_forward_workspace_bytes(int32_t H, int32_t W,
+                                          int32_t channels, int32_t precision);
+
 /* Launch schedules of fk_convar_forward (no counterpart in the reference:
  * the knobs of this library's autotuner).  Each (channels, precision) has
  * fk_convar_num_schedules() compiled schedules (loop order tile_outer 0/1,

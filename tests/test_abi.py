@@ -114,3 +114,13 @@ def test_schedule_table(lib):
     assert L.fk_convar_get_schedule(10, 9, 32, 1) == 0
     assert L.fk_convar_set_schedule(10, 10, 32, 1, -1) == 0
     assert L.fk_convar_get_schedule(10, 10, 32, 1) == 0
+
+
+def test_forward_workspace_query_lds_resident(lib):
+    """Host-only: lattices whose activations fit in LDS need no workspace;
+    unsupported widths report -1."""
+    from flowket_amd import _lib
+    L = _lib.load()
+    assert L.fk_convar_forward_workspace_bytes(10, 10, 32, 1) == 0
+    assert L.fk_convar_forward_workspace_bytes(4, 4, 16, 0) == 0
+    assert L.fk_convar_forward_workspace_bytes(10, 10, 66, 1) == -1

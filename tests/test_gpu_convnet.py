@@ -69,6 +69,11 @@ CASES = [
     ((6, 6), 2, 24, 'relu', 'fp32'),
     ((5, 4), 2, 8, 'elu', 'fp32'),
     ((4, 4), 2, 6, 'relu', 'bf16'),
+    # lattices whose five activation buffers exceed the LDS: the horizontal
+    # stack lives in global scratch
+    ((16, 16), 3, 64, 'relu', 'bf16'),
+    ((20, 20), 2, 64, 'elu', 'fp16'),
+    ((16, 16), 2, 32, 'relu', 'fp32'),
     # fp16 MFMA
     ((10, 10), 4, 32, 'elu', 'fp16'),
     ((7, 9), 3, 64, 'relu', 'fp16'),
@@ -210,3 +215,34 @@ def test_fp16_closer_to_fp32_than_bf16(rng):
     e16 = (m.log_psi(s, precision='fp16').real.double() - ref).abs().max().item()
     eb16 = (m.log_psi(s, precision='bf16').real.double() - ref).abs().max().item()
     assert e16 < eb16 / 3, (e16, eb16)
+
+
+def test_global_scratch_forward_connected_and_workspace(rng):
+    """20x20 / 64 channels (the largest workload's lattice): the ragged
+    connected evaluation equals the evaluation of the materialised
+    configurations, the C ABI without a workspace refuses the lattice with
+    FK_ERR_ARG, and the workspace query is consistent."""
+    import flowket_amd as fk
+    from flowket_amd import _lib
+    gpu_banner('global-scratch forward 20x20 C64 bf16 connected')
+    m = _machine((20, 20), 2, 64, 'relu', 'bf16')
+    n = int(_lib.load().fk_convar_forward_workspace_bytes(20, 20, 64, _lib.FK_PREC_BF16))
+    assert n > 0
+    assert int(_lib.load().fk_convar_forward_workspace_bytes(10, 10, 32, _lib.FK_PREC_BF16)) == 0
+    op = fk.Heisenberg(hilbert_state_shape=(20, 20), pbc=True)
+    B = 6
+    s = torch.as_tensor(rng.choice([-1, 1], size=(B, 400)).astype(np.int8)).cuda()
+    r = op.compact(s)
+    total = int(r['offsets'][-1])
+    lp = m.log_psi_connected(s, r['pairs'], total, op)
+    conn = op.find_conn_device(s)['conn']
+    pairs = r['pairs'][:total].long()
+    dense = conn[pairs[:, 1], pairs[:, 0]]
+    lp2, _ = m.forward(dense.contiguous())
+    assert torch.equal(lp, lp2)
+    wbuf, abuf = m._pack('bf16')
+    out = torch.empty((B, 2), dtype=torch.float32, device='cuda')
+    rc = _lib.load().fk_convar_forward(_lib.ptr(wbuf), _lib.ptr(abuf), _lib.FK_PREC_BF16, 20, 20, 2, 64,
+                                       _lib.FK_ACT_RELU, _lib.ptr(s), None, None, B, 0, None, None,
+                                       _lib.ptr(out), None, _lib.stream_ptr())
+    assert rc == 1 and b'workspace' in _lib.load().fk_last_error()
