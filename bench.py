@@ -54,6 +54,7 @@ def parse():
     ap.add_argument('--no-fp32', action='store_true', help='skip the FP32 plain-path comparison')
     ap.add_argument('--no-sr', action='store_true', help='skip the SR step (per-sample gradients + CG) leg')
     ap.add_argument('--no-mh', action='store_true', help='skip the Metropolis-Hastings RBM leg')
+    ap.add_argument('--no-large', action='store_true', help='skip the 20x20 depth-24 64-channel leg')
     ap.add_argument('--profile-forward', type=int, default=0, metavar='N',
                     help='only run two fused-forward launches over the connected configurations of N random '
                          'samples (a short program for rocprofv3 counter passes) and exit')
@@ -191,6 +192,50 @@ def bench_mh(seconds_burn=None):
             res[name]['used_proposal'] = bool(s.used_proposal)
             res[name]['proposal_nll'] = s.last_nll
     return res
+
+
+def bench_large(n_samples=1024, n_eloc=64):
+    """The largest configuration's network and lattice: 20x20 periodic
+    Heisenberg, depth 24, 64 channels, bf16.  Fast sampling of n_samples
+    configurations and the ragged local energies of the first n_eloc (a
+    bounded slice of the 16384-sample step), with the forward's MFMA roofline
+    fraction (its horizontal stack lives in global scratch: the five 20x20x64
+    lattices exceed the LDS)."""
+    import torch
+    import flowket_amd as fk
+    Hl = Wl = 20
+    depth, C = 24, 64
+    m = fk.ConvNetAutoregressive2D((Hl, Wl), depth=depth, num_of_channels=C, activation=ACT,
+                                   weights_normalization=False, precision='bf16', seed=99)
+    op = fk.Heisenberg(hilbert_state_shape=(Hl, Wl), pbc=True)
+    fs = fk.FastAutoregressiveSampler(m, n_samples, seed=5)
+    next(fs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x = next(fs)
+    torch.cuda.synchronize()
+    sps = n_samples / (time.perf_counter() - t0)
+    spins = x.reshape(n_samples, Hl * Wl)[:n_eloc].contiguous()
+    obs = fk.Observable(op)
+    obs.local_values(m, spins)
+    r = op.compact(spins)
+    total = int(r['offsets'][-1])
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    el = obs.local_values(m, spins)
+    e1.record()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    fwd = 2 * Hl * Wl * ((10 * C * C + 12 * C) + (2 * depth - 3) * 22 * C * C + 4 * C)
+    return {'workload': '20x20 periodic Heisenberg, depth 24, 64 channels, bf16 (horizontal stack in global scratch)',
+            'params': m.n_params, 'fast_samples_per_s': round(sps, 1), 'eloc_samples': n_eloc,
+            'connected_configurations': total, 'E_loc_evals_per_s': round(n_eloc / dt, 2),
+            'connected_logpsi_per_s': round(total / dt, 1),
+            'forward_tflops': round(total * fwd / dt / 1e12, 1),
+            'forward_frac_of_bf16_peak': round(total * fwd / dt / PEAK_BF16, 4),
+            'energy_per_site': float(el.real.mean().item()) / (Hl * Wl)}
 
 
 def cpu_baseline(machine, spins_np, n_samples=8):
@@ -364,6 +409,8 @@ def main():
         out['vmc_sr_step'] = bench_sr(m16, op, spins)
     if not args.no_mh:
         out['mh_rbm_8x8'] = bench_mh()
+    if not args.no_large:
+        out['large_20x20_d24_c64'] = bench_large()
     # ---- the same ragged step with fp16 MFMA (same speed class as bf16,
     # finer rounding)
     mh = fk.ConvNetAutoregressive2D((H, W), depth=DEPTH, num_of_channels=CHANNELS, activation=ACT,
