@@ -162,8 +162,10 @@ __device__ void conv_bwd_weight(float* dK, float* db, const float* in, int cs_in
 // Per-sample workspace layout (floats): spins HW, then per block
 // VA (act(v')), XA (act(hx)), CAT (x2|y2), VO (vertical output), HO
 // (horizontal output), each HW*C.
+// (the spins region is padded to 4 floats so that every tensor is 16-byte
+// aligned)
 __device__ __forceinline__ float* ws_tensor(float* ws, int HW, int C, int blk, int which) {
-  return ws + HW + ((int64_t)blk * 5 + which) * HW * C;
+  return ws + ((HW + 3) & ~3) + ((int64_t)blk * 5 + which) * HW * C;
 }
 enum { T_VA = 0, T_XA = 1, T_CAT = 2, T_VO = 3, T_HO = 4 };
 
@@ -407,7 +409,357 @@ __global__ __launch_bounds__(256) void convar_grad_kernel(BwdArgs a) {
   }
 }
 
-int64_t grad_work_floats(int HW, int C, int nb) { return (int64_t)HW + (int64_t)nb * 5 * HW * C; }
+// ===========================================================================
+// MFMA form (channels a multiple of 32): the same forward / backward
+// schedule, with every C-channel convolution, its data adjoint and its
+// per-sample weight gradient an implicit GEMM on v_mfma_f32_16x16x4f32
+// (FP32 in, FP32 accumulate).  Operands stream from the per-sample workspace
+// (L1/L2 resident); tap validity comes from per-lattice source tables in LDS.
+// MFMA 16x16x4 lane roles: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// D[i][j] in lane j + 16*(i/4), register i%4.
+
+// source-table rows: vertical taps 0-8, horizontal 9-11, h3 12-20, x1x1 21,
+// x1x1 behind the RightShift (final block) 22, y1x1 behind the DownShift 23
+constexpr int GT_V = 0, GT_H = 9, GT_C = 12, GT_X = 21, GT_XF = 22, GT_Y = 23, GT_N = 24;
+
+__device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Each wave walks 2x2 blocks of 16x16 D tiles; the K loop is (outer, inner)
+// so that tap and channel-quad indices need no division.
+template <class FA, class FB, class FS>
+__device__ __forceinline__ void wave_gemm(int mtiles, int ntiles, int kouter, int kinner, FA fa, FB fb, FS fs) {
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int mb = (mtiles + 1) >> 1, nbk = (ntiles + 1) >> 1;
+  for (int blk = wave; blk < mb * nbk; blk += nw) {
+    const int mt0 = 2 * (blk / nbk), nt0 = 2 * (blk % nbk);
+    const bool m1 = mt0 + 1 < mtiles, n1 = nt0 + 1 < ntiles;
+    f32x4 c00 = {0.f, 0.f, 0.f, 0.f}, c01 = c00, c10 = c00, c11 = c00;
+    for (int t = 0; t < kouter; ++t) {
+#pragma unroll 4
+      for (int k = 0; k < kinner; ++k) {
+        const float a0 = fa(mt0, t, k), a1 = m1 ? fa(mt0 + 1, t, k) : 0.0f;
+        const float b0 = fb(nt0, t, k), b1 = n1 ? fb(nt0 + 1, t, k) : 0.0f;
+        c00 = mfma4(a0, b0, c00);
+        c01 = mfma4(a0, b1, c01);
+        c10 = mfma4(a1, b0, c10);
+        c11 = mfma4(a1, b1, c11);
+      }
+    }
+    fs(mt0, nt0, c00);
+    if (n1) fs(mt0, nt0 + 1, c01);
+    if (m1) fs(mt0 + 1, nt0, c10);
+    if (m1 && n1) fs(mt0 + 1, nt0 + 1, c11);
+  }
+}
+
+// out[p][co_off + co] = bias[co] + sum_{tap, ci} K[tap][ci][co] in[src(tap, p)][ci]
+__device__ void mconv_fwd(float* out, int cs_out, int co_off, const float* in, int cs_in, int cin, int cout,
+                          const float* K, const float* bias, const int16_t* tab, int ntap, int HW) {
+  const int lane = threadIdx.x & 63, l16 = lane & 15, kk = lane >> 4;
+  wave_gemm(
+      cout / 16, (HW + 15) / 16, ntap, cin / 4,
+      [&](int mt, int t, int s) { return K[(int64_t)(t * cin + 4 * s + kk) * cout + 16 * mt + l16]; },
+      [&](int nt, int t, int s) {
+        const int p = 16 * nt + l16;
+        const int src = p < HW ? tab[t * HW + p] : -1;
+        return src >= 0 ? in[(int64_t)src * cs_in + 4 * s + kk] : 0.0f;
+      },
+      [&](int mt, int nt, f32x4 c) {
+        const int p = 16 * nt + l16;
+        if (p >= HW) return;
+        const int co = 16 * mt + 4 * kk;
+        const f32x4 b = *reinterpret_cast<const f32x4*>(bias + co);
+        *reinterpret_cast<f32x4*>(out + (int64_t)p * cs_out + co_off + co) = c + b;
+      });
+}
+
+// din[q][ci] = sum_{tap, co} K[tap][ci][co] dout[inv(tap, q)][co_off + co]
+__device__ void mconv_bwd_data(float* din, int cs_in, const float* dout, int cs_out, int co_off, int cin, int cout,
+                               const float* K, const int16_t* inv, int ntap, int HW) {
+  const int lane = threadIdx.x & 63, l16 = lane & 15, kk = lane >> 4;
+  wave_gemm(
+      cin / 16, (HW + 15) / 16, ntap, cout / 4,
+      [&](int mt, int t, int s) { return K[(int64_t)(t * cin + 16 * mt + l16) * cout + 4 * s + kk]; },
+      [&](int nt, int t, int s) {
+        const int q = 16 * nt + l16;
+        const int src = q < HW ? inv[t * HW + q] : -1;
+        return src >= 0 ? dout[(int64_t)src * cs_out + co_off + 4 * s + kk] : 0.0f;
+      },
+      [&](int mt, int nt, f32x4 c) {
+        const int q = 16 * nt + l16;
+        if (q >= HW) return;
+        *reinterpret_cast<f32x4*>(din + (int64_t)q * cs_in + 16 * mt + 4 * kk) = c;
+      });
+}
+
+// dK[tap][ci][co] = sum_p in[src(tap, p)][ci] dout[p][co_off + co]  (D[co][ci]),
+// every tap's 2x2 tile blocks shared out over the waves; db[co] = sum_p dout.
+__device__ void mconv_bwd_weight(float* dK, float* db, const float* in, int cs_in, const float* dout, int cs_out,
+                                 int co_off, int cin, int cout, const int16_t* tab, int ntap, int HW) {
+  const int lane = threadIdx.x & 63, l16 = lane & 15, kk = lane >> 4;
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int mtiles = cout / 16, ntiles = cin / 16;
+  const int mb = (mtiles + 1) >> 1, nbk = (ntiles + 1) >> 1;
+  const int ks = (HW + 3) / 4;
+  for (int blk = wave; blk < ntap * mb * nbk; blk += nw) {
+    const int t = blk / (mb * nbk), r = blk - t * (mb * nbk);
+    const int mt0 = 2 * (r / nbk), nt0 = 2 * (r % nbk);
+    const bool m1 = mt0 + 1 < mtiles, n1 = nt0 + 1 < ntiles;
+    const int16_t* tt = tab + t * HW;
+    f32x4 c00 = {0.f, 0.f, 0.f, 0.f}, c01 = c00, c10 = c00, c11 = c00;
+#pragma unroll 4
+    for (int k = 0; k < ks; ++k) {
+      const int p = 4 * k + kk;
+      const bool pin = p < HW;
+      const float* dp = dout + (int64_t)p * cs_out + co_off + l16;
+      const float a0 = pin ? dp[16 * mt0] : 0.0f;
+      const float a1 = (pin && m1) ? dp[16 * mt0 + 16] : 0.0f;
+      const int src = pin ? tt[p] : -1;
+      const float* ip = in + (int64_t)src * cs_in + l16;
+      const float b0 = src >= 0 ? ip[16 * nt0] : 0.0f;
+      const float b1 = (src >= 0 && n1) ? ip[16 * nt0 + 16] : 0.0f;
+      c00 = mfma4(a0, b0, c00);
+      c01 = mfma4(a0, b1, c01);
+      c10 = mfma4(a1, b0, c10);
+      c11 = mfma4(a1, b1, c11);
+    }
+    auto st = [&](int mt, int nt, f32x4 c) {
+      const int ci = 16 * nt + l16, co = 16 * mt + 4 * kk;
+      *reinterpret_cast<f32x4*>(dK + ((int64_t)t * cin + ci) * cout + co) = c;
+    };
+    st(mt0, nt0, c00);
+    if (n1) st(mt0, nt0 + 1, c01);
+    if (m1) st(mt0 + 1, nt0, c10);
+    if (m1 && n1) st(mt0 + 1, nt0 + 1, c11);
+  }
+  for (int co = threadIdx.x; co < cout; co += blockDim.x) {
+    float acc = 0.0f;
+    for (int p = 0; p < HW; ++p) acc += dout[(int64_t)p * cs_out + co_off + co];
+    db[co] = acc;
+  }
+}
+
+// Per-sample workspace of the MFMA kernel: the scalar kernel's layout plus six
+// gradient tensors (dV, dH, dVr, dHr, t0, t1) of HW*C floats.
+__device__ __forceinline__ float* ws_grad(float* ws, int HW, int C, int nb, int which) {
+  return ws + ((HW + 3) & ~3) + ((int64_t)nb * 5 + which) * HW * C;
+}
+
+template <int ACT>
+__global__ __launch_bounds__(256) void convar_grad_mfma_kernel(BwdArgs a) {
+  extern __shared__ int16_t tabs[];  // GT_N forward rows, then GT_N inverse rows, of HW entries
+  __shared__ float s_lp[2];
+  const int H = a.H, W = a.W, HW = a.HW, C = a.C, nb = a.nb;
+  const int F = nb - 1;
+  const int64_t b = blockIdx.x;
+  if (b >= a.batch) return;
+  float* ws = a.work + (int64_t)blockIdx.x * a.work_floats;
+  int16_t* ftab = tabs;
+  int16_t* itab = tabs + GT_N * HW;
+  float* dz = reinterpret_cast<float*>(tabs + 2 * GT_N * HW + (2 * GT_N * HW & 1));
+  float* dV = ws_grad(ws, HW, C, nb, 0);
+  float* dH = ws_grad(ws, HW, C, nb, 1);
+  float* dVr = ws_grad(ws, HW, C, nb, 2);
+  float* dHr = ws_grad(ws, HW, C, nb, 3);
+  float* t0 = ws_grad(ws, HW, C, nb, 4);
+  float* t1 = ws_grad(ws, HW, C, nb, 5);
+  const float* prm = a.params;
+
+  // source tables (and their inverses) of every tap / shift
+  for (int idx = threadIdx.x; idx < GT_N * HW; idx += blockDim.x) itab[idx] = -1;
+  for (int idx = threadIdx.x; idx < GT_N * HW; idx += blockDim.x) {
+    const int row = idx / HW, p = idx - row * HW;
+    const int i = p / W, j = p - (p / W) * W;
+    int dr = 0, dc = 0;
+    bool ok = true;
+    if (row < GT_H) tap_rc(0, row - GT_V, dr, dc);
+    else if (row < GT_C) tap_rc(1, row - GT_H, dr, dc);
+    else if (row < GT_X) tap_rc(2, row - GT_C, dr, dc);
+    else if (row == GT_XF) dc = -1;
+    else if (row == GT_Y) dr = -1;
+    const int r = i + dr, c = j + dc;
+    ok = r >= 0 && r < H && c >= 0 && c < W;
+    ftab[idx] = ok ? (int16_t)(r * W + c) : (int16_t)-1;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < GT_N * HW; idx += blockDim.x) {
+    const int row = idx / HW, p = idx - row * HW;
+    const int src = ftab[idx];
+    if (src >= 0) itab[row * HW + src] = (int16_t)p;
+  }
+
+  // ------------------------------------------------------------ forward
+  float* sp = ws;
+  for (int q = threadIdx.x; q < HW; q += blockDim.x) sp[q] = (float)a.spins[b * HW + q];
+  __syncthreads();
+  for (int blk = 0; blk < nb; ++blk) {
+    const ParamOffsets o = param_offsets(blk, C);
+    const bool first = blk == 0, fin = blk == F;
+    const bool kb = !first && !fin && ((blk & 1) == 0);
+    const float* Vin = first ? sp : ws_tensor(ws, HW, C, blk - 1, T_VO);
+    const float* Hin = first ? sp : ws_tensor(ws, HW, C, blk - 1, T_HO);
+    float* VA = ws_tensor(ws, HW, C, blk, T_VA);
+    float* XA = ws_tensor(ws, HW, C, blk, T_XA);
+    float* CAT = ws_tensor(ws, HW, C, blk, T_CAT);
+    float* VO = ws_tensor(ws, HW, C, blk, T_VO);
+    float* HO = ws_tensor(ws, HW, C, blk, T_HO);
+    if (first) {
+      conv_fwd(t0, C, Vin, 1, 1, C, prm + o.kv, prm + o.bv, 0, 9, H, W);
+      conv_fwd(t1, C, Hin, 1, 1, C, prm + o.kh, prm + o.bh, 1, 3, H, W);
+    } else {
+      mconv_fwd(t0, C, 0, Vin, C, C, C, prm + o.kv, prm + o.bv, ftab + GT_V * HW, 9, HW);
+      mconv_fwd(t1, C, 0, Hin, C, C, C, prm + o.kh, prm + o.bh, ftab + GT_H * HW, 3, HW);
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+      const float v = t0[idx];
+      VA[idx] = bact<ACT>(v);
+      XA[idx] = bact<ACT>(t1[idx]);
+      if (!fin) VO[idx] = kb ? bact<ACT>(ws_tensor(ws, HW, C, blk - 2, T_VO)[idx] + v) : bact<ACT>(v);
+    }
+    __syncthreads();
+    mconv_fwd(CAT, C, 0, XA, C, C, C / 2, prm + o.kx, prm + o.bx, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW);
+    mconv_fwd(CAT, C, C / 2, VA, C, C, C / 2, prm + o.ky, prm + o.by, ftab + GT_Y * HW, 1, HW);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) CAT[idx] = bact<ACT>(CAT[idx]);
+    __syncthreads();
+    mconv_fwd(t0, C, 0, CAT, C, C, C, prm + o.kc, prm + o.bc, ftab + GT_C * HW, 9, HW);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+      const float h = t0[idx];
+      HO[idx] = kb ? bact<ACT>(ws_tensor(ws, HW, C, blk - 2, T_HO)[idx] + h) : bact<ACT>(h);
+    }
+    __syncthreads();
+  }
+
+  // head and its cotangents (as in the scalar kernel)
+  const int64_t ho = head_offset(nb, C);
+  const float* HF = ws_tensor(ws, HW, C, F, T_HO);
+  if (threadIdx.x == 0) s_lp[0] = s_lp[1] = 0.0f;
+  __syncthreads();
+  for (int part = 0; part < 2; ++part) {
+    for (int p = threadIdx.x; p < HW; p += blockDim.x) {
+      float z[4];
+      for (int k = 0; k < 4; ++k) {
+        float acc = prm[ho + 4 * C + k];
+        for (int ci = 0; ci < C; ++ci) acc += prm[ho + ci * 4 + k] * HF[p * C + ci];
+        z[k] = acc;
+      }
+      const float mx = fmaxf(z[0], z[1]);
+      const float e0 = expf(2.0f * (z[0] - mx)), e1 = expf(2.0f * (z[1] - mx));
+      const float p0 = e0 / (e0 + e1), p1 = e1 / (e0 + e1);
+      const bool up = sp[p] > 0.0f;
+      if (part == 0) {
+        dz[p * 4 + 0] = (up ? 1.0f : 0.0f) - p0;
+        dz[p * 4 + 1] = (up ? 0.0f : 1.0f) - p1;
+        dz[p * 4 + 2] = 0.0f;
+        dz[p * 4 + 3] = 0.0f;
+        const float lse = mx + 0.5f * logf(e0 + e1);
+        atomicAdd(&s_lp[0], (up ? z[0] : z[1]) - lse);
+        atomicAdd(&s_lp[1], up ? z[2] : z[3]);
+      } else {
+        dz[p * 4 + 0] = 0.0f;
+        dz[p * 4 + 1] = 0.0f;
+        dz[p * 4 + 2] = up ? 1.0f : 0.0f;
+        dz[p * 4 + 3] = up ? 0.0f : 1.0f;
+      }
+    }
+    __syncthreads();
+    float* O = (part == 0 ? a.o_re : a.o_im) + b * a.P;
+    for (int idx = threadIdx.x; idx < 4 * C + 4; idx += blockDim.x) {
+      float acc = 0.0f;
+      if (idx < 4 * C) {
+        const int ci = idx / 4, k = idx - (idx / 4) * 4;
+        for (int p = 0; p < HW; ++p) acc += HF[p * C + ci] * dz[p * 4 + k];
+      } else {
+        for (int p = 0; p < HW; ++p) acc += dz[p * 4 + idx - 4 * C];
+      }
+      O[ho + idx] = acc;
+    }
+    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+      const int p = idx / C, ci = idx - (idx / C) * C;
+      float g = 0.0f;
+      for (int k = 0; k < 4; ++k) g += prm[ho + ci * 4 + k] * dz[p * 4 + k];
+      dH[idx] = g;
+      dV[idx] = 0.0f;
+      dVr[idx] = 0.0f;
+      dHr[idx] = 0.0f;
+    }
+    __syncthreads();
+
+    for (int blk = F; blk >= 0; --blk) {
+      const ParamOffsets o = param_offsets(blk, C);
+      const bool first = blk == 0, fin = blk == F;
+      const bool kb = !first && !fin && ((blk & 1) == 0);
+      const bool ka = !first && !fin && ((blk & 1) == 1);
+      const float* Vin = first ? sp : ws_tensor(ws, HW, C, blk - 1, T_VO);
+      const float* Hin = first ? sp : ws_tensor(ws, HW, C, blk - 1, T_HO);
+      const float* VA = ws_tensor(ws, HW, C, blk, T_VA);
+      const float* XA = ws_tensor(ws, HW, C, blk, T_XA);
+      const float* CAT = ws_tensor(ws, HW, C, blk, T_CAT);
+      const float* VO = ws_tensor(ws, HW, C, blk, T_VO);
+      const float* HO = ws_tensor(ws, HW, C, blk, T_HO);
+      for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+        const float gh = dH[idx] * bdact<ACT>(HO[idx]);
+        t1[idx] = gh;
+        const float gv = fin ? 0.0f : dV[idx] * bdact<ACT>(VO[idx]);
+        t0[idx] = gv;
+        if (kb) {
+          dVr[idx] = gv;
+          dHr[idx] = gh;
+        }
+      }
+      __syncthreads();
+      mconv_bwd_weight(O + o.kc, O + o.bc, CAT, C, t1, C, 0, C, C, ftab + GT_C * HW, 9, HW);
+      mconv_bwd_data(dV, C, t1, C, 0, C, C, prm + o.kc, itab + GT_C * HW, 9, HW);
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) dV[idx] *= bdact<ACT>(CAT[idx]);
+      __syncthreads();
+      const int xr = fin ? GT_XF : GT_X;
+      mconv_bwd_weight(O + o.kx, O + o.bx, XA, C, dV, C, 0, C, C / 2, ftab + xr * HW, 1, HW);
+      mconv_bwd_weight(O + o.ky, O + o.by, VA, C, dV, C, C / 2, C, C / 2, ftab + GT_Y * HW, 1, HW);
+      mconv_bwd_data(dH, C, dV, C, 0, C, C / 2, prm + o.kx, itab + xr * HW, 1, HW);
+      mconv_bwd_data(t1, C, dV, C, C / 2, C, C / 2, prm + o.ky, itab + GT_Y * HW, 1, HW);
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+        dH[idx] *= bdact<ACT>(XA[idx]);
+        t0[idx] += t1[idx] * bdact<ACT>(VA[idx]);
+      }
+      __syncthreads();
+      if (first) {
+        conv_bwd_weight(O + o.kh, O + o.bh, Hin, 1, dH, C, 1, C, 1, 3, H, W);
+        conv_bwd_weight(O + o.kv, O + o.bv, Vin, 1, t0, C, 1, C, 0, 9, H, W);
+        break;
+      }
+      mconv_bwd_weight(O + o.kh, O + o.bh, Hin, C, dH, C, 0, C, C, ftab + GT_H * HW, 3, HW);
+      mconv_bwd_weight(O + o.kv, O + o.bv, Vin, C, t0, C, 0, C, C, ftab + GT_V * HW, 9, HW);
+      mconv_bwd_data(t1, C, dH, C, 0, C, C, prm + o.kh, itab + GT_H * HW, 3, HW);
+      mconv_bwd_data(dV, C, t0, C, 0, C, C, prm + o.kv, itab + GT_V * HW, 9, HW);
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+        float gh = t1[idx], gv = dV[idx];
+        if (ka) {
+          gh += dHr[idx];
+          gv += dVr[idx];
+        }
+        dH[idx] = gh;
+        dV[idx] = gv;
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && a.logpsi) {
+    a.logpsi[2 * b] = s_lp[0];
+    a.logpsi[2 * b + 1] = s_lp[1];
+  }
+}
+
+int64_t grad_work_floats(int HW, int C, int nb) {
+  return (int64_t)((HW + 3) & ~3) + (((int64_t)nb * 5 + 6) * HW * C + 3) / 4 * 4;
+}
 
 }  // namespace fk
 
@@ -446,9 +798,25 @@ int fk_convar_grad(const float* params, int32_t H, int32_t W, int32_t depth, int
   const int64_t C = channels;
   a.P = (9 * C + C + 3 * C + C + C * (C / 2) * 2 + C + 9 * C * C + C) +
         (int64_t)(a.nb - 1) * (9 * C * C + C + 3 * C * C + C + C * (C / 2) * 2 + C + 9 * C * C + C) + 4 * C + 4;
+  hipStream_t s = (hipStream_t)stream;
+  if (channels % 32 == 0) {
+    // MFMA kernel: LDS holds the tap source tables and the head cotangents
+    const size_t lds = (size_t)2 * GT_N * a.HW * sizeof(int16_t) + 4 + (size_t)4 * a.HW * sizeof(float);
+    FK_REQUIRE(lds <= 160 * 1024 && a.HW <= 32767, "fk_convar_grad: lattice too large");
+    if (activation == FK_ACT_ELU) {
+      FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(convar_grad_mfma_kernel<FK_ACT_ELU>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(convar_grad_mfma_kernel<FK_ACT_ELU>, dim3((unsigned)batch), dim3(256), lds, s, a);
+    } else {
+      FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(convar_grad_mfma_kernel<FK_ACT_RELU>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(convar_grad_mfma_kernel<FK_ACT_RELU>, dim3((unsigned)batch), dim3(256), lds, s, a);
+    }
+    FK_CHECK_LAUNCH();
+    return FK_OK;
+  }
   const size_t lds = ((size_t)6 * a.HW * channels + 4 * a.HW) * sizeof(float);
   FK_REQUIRE(lds <= 160 * 1024, "fk_convar_grad: lattice too large for the LDS gradient buffers");
-  hipStream_t s = (hipStream_t)stream;
   if (activation == FK_ACT_ELU) {
     FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(convar_grad_kernel<FK_ACT_ELU>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -116,3 +116,81 @@ def test_vmc_adam_lowers_energy_4x4():
         opt.step(configs, vmc.current_local_energy, vmc.current_energy)
     assert np.mean(energies[-5:]) < np.mean(energies[:5]) - 1.0
     assert np.mean(energies[-5:]) > -44.9139328337 - 0.5  # variational bound (statistical slack)
+
+
+def _torch_log_psi(flat, shapes, spins, depth, act_name):
+    """float64 torch.nn.functional restatement of the network (the same
+    restatement the oracle tests pin against oracle.convnet) on a flat
+    parameter tensor, for autograd Jacobians."""
+    import torch.nn.functional as F
+    params, off = [], 0
+    for ks, bs in shapes:
+        nk, nbias = int(np.prod(ks)), int(np.prod(bs))
+        params.append((flat[off:off + nk].reshape(ks), flat[off + nk:off + nk + nbias]))
+        off += nk + nbias
+    act = F.relu if act_name == 'relu' else F.elu
+
+    def conv(x, kb, pad):
+        k, b = kb
+        (pt, pb), (pl, pr) = pad
+        return F.conv2d(F.pad(x, (pl, pr, pt, pb)), k.permute(3, 2, 0, 1), b)
+
+    def block(v, h, layers, mask=False):
+        v_out = conv(v, layers[0], ((2, 0), (1, 1)))
+        x = act(conv(h, layers[1], ((0, 0), (2, 0))))
+        if mask:
+            x = F.pad(x, (1, 0, 0, 0))[..., :-1]
+        x = conv(x, layers[2], ((0, 0), (0, 0)))
+        y = F.pad(act(v_out), (0, 0, 1, 0))[..., :-1, :]
+        y = conv(y, layers[3], ((0, 0), (0, 0)))
+        return v_out, conv(torch.cat([act(x), act(y)], dim=1), layers[4], ((2, 0), (2, 0)))
+
+    x = torch.as_tensor(spins, dtype=torch.float64)[:, None]
+    nbk = ocn.num_causal_blocks(depth)
+    it = iter(range(0, 5 * nbk, 5))
+    i0 = next(it)
+    v, h = block(x, x, params[i0:i0 + 5])
+    v, h = act(v), act(h)
+    for _ in range(depth - 2):
+        vi, hi = v, h
+        i0 = next(it)
+        v, h = block(v, h, params[i0:i0 + 5])
+        v, h = act(v), act(h)
+        i0 = next(it)
+        v, h = block(v, h, params[i0:i0 + 5])
+        v, h = act(vi + v), act(hi + h)
+    i0 = next(it)
+    _, h = block(v, h, params[i0:i0 + 5], mask=True)
+    z = conv(act(h), params[-1], ((0, 0), (0, 0))).permute(0, 2, 3, 1)      # (B, H, W, 4)
+    re, im = z[..., 0:2], z[..., 2:4]
+    re = re - 0.5 * torch.logsumexp(2 * re, dim=-1, keepdim=True)
+    idx = torch.as_tensor((1 - np.asarray(spins, dtype=np.int64)) // 2)[..., None]
+    return torch.gather(re, -1, idx)[..., 0].sum(dim=(1, 2)), torch.gather(im, -1, idx)[..., 0].sum(dim=(1, 2))
+
+
+@pytest.mark.parametrize('shape,depth,C,act', [((5, 6), 3, 32, 'elu'), ((4, 4), 2, 64, 'relu'),
+                                              ((3, 7), 4, 32, 'relu')])
+def test_mfma_per_sample_gradients_vs_autograd(rng, shape, depth, C, act):
+    """The MFMA gradient kernel (widths that are multiples of 32) against
+    float64 autograd per-sample Jacobians of the torch restatement, itself
+    checked against the oracle's log psi."""
+    gpu_banner(f'MFMA per-sample gradients shape={shape} depth={depth} C={C} {act}')
+    import flowket_amd as fk
+    m = fk.ConvNetAutoregressive2D(shape, depth=depth, num_of_channels=C, activation=act,
+                                   weights_normalization=False, precision='fp32', seed=6)
+    g = torch.Generator().manual_seed(7)
+    m.set_params(m.params.cpu() * (1.0 + 0.3 * torch.randn(m.n_params, generator=g)))
+    B = 3
+    s = rng.choice([-1, 1], size=(B,) + shape).astype(np.int8)
+    o_re, o_im, lp = m.per_sample_gradients(s)
+    flat = m.params.double().cpu().clone().requires_grad_(True)
+    re, im = _torch_log_psi(flat, m.shapes, s, depth, act)
+    ref = ocn.log_psi(_oracle_params(m), s, depth, activation=act)
+    assert np.abs(re.detach().numpy() - ref.real).max() < 1e-9
+    J_re = torch.stack([torch.autograd.grad(re[b], flat, retain_graph=True)[0] for b in range(B)]).numpy()
+    J_im = torch.stack([torch.autograd.grad(im[b], flat, retain_graph=True)[0] for b in range(B)]).numpy()
+    got_re, got_im = o_re.double().cpu().numpy(), o_im.double().cpu().numpy()
+    scale = max(np.abs(J_re).max(), np.abs(J_im).max())
+    assert np.abs(got_re - J_re).max() / scale < 1e-4
+    assert np.abs(got_im - J_im).max() / scale < 1e-4
+    assert np.abs(lp[:, 0].cpu().numpy() - ref.real).max() < 1e-4 * (np.abs(ref.real).max() + 1)
