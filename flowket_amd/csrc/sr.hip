@@ -51,7 +51,8 @@ __global__ __launch_bounds__(256) void sum_f64_kernel(const double* __restrict__
 
 // u_b = sum_k (O_bk - Obar_k) v_k = O_b.v - Obar.v (complex).  A workgroup
 // covers ROWS rows so each v_k is read once per ROWS rows; ov_c = Obar.v
-// (re, im) in FP64 is subtracted at the end.
+// (re, im) in FP64 is subtracted at the end.  With P a multiple of 4 the rows
+// are streamed as float4.
 constexpr int OV_ROWS = 8;
 __global__ __launch_bounds__(256) void o_times_v_kernel(const float* __restrict__ ore, const float* __restrict__ oim,
                                                         const double* __restrict__ ov_c, const float* __restrict__ v,
@@ -63,19 +64,41 @@ __global__ __launch_bounds__(256) void o_times_v_kernel(const float* __restrict_
 #pragma unroll
   for (int r = 0; r < OV_ROWS; ++r) dr[r] = di[r] = 0.0, ar[r] = ai[r] = 0.0f;
   int cnt = 0;
-  for (int64_t k = threadIdx.x; k < P; k += blockDim.x) {
-    const float vk = v[k];
+  if ((P & 3) == 0) {
+    const int64_t P4 = P >> 2;
+    const f32x4* v4 = reinterpret_cast<const f32x4*>(v);
+    for (int64_t k = threadIdx.x; k < P4; k += blockDim.x) {
+      const f32x4 vk = v4[k];
 #pragma unroll
-    for (int r = 0; r < OV_ROWS; ++r) {
-      if (b0 + r < B) {
-        ar[r] += ore[(b0 + r) * P + k] * vk;
-        ai[r] += oim[(b0 + r) * P + k] * vk;
+      for (int r = 0; r < OV_ROWS; ++r) {
+        if (b0 + r < B) {
+          const f32x4 x = reinterpret_cast<const f32x4*>(ore + (b0 + r) * P)[k];
+          const f32x4 y = reinterpret_cast<const f32x4*>(oim + (b0 + r) * P)[k];
+          ar[r] += x.x * vk.x + x.y * vk.y + x.z * vk.z + x.w * vk.w;
+          ai[r] += y.x * vk.x + y.y * vk.y + y.z * vk.z + y.w * vk.w;
+        }
+      }
+      if (++cnt == 16) {  // fold fp32 partials into fp64 every 64 terms
+#pragma unroll
+        for (int r = 0; r < OV_ROWS; ++r) dr[r] += ar[r], di[r] += ai[r], ar[r] = ai[r] = 0.0f;
+        cnt = 0;
       }
     }
-    if (++cnt == 64) {  // fold fp32 partials into fp64 every 64 terms
+  } else {
+    for (int64_t k = threadIdx.x; k < P; k += blockDim.x) {
+      const float vk = v[k];
 #pragma unroll
-      for (int r = 0; r < OV_ROWS; ++r) dr[r] += ar[r], di[r] += ai[r], ar[r] = ai[r] = 0.0f;
-      cnt = 0;
+      for (int r = 0; r < OV_ROWS; ++r) {
+        if (b0 + r < B) {
+          ar[r] += ore[(b0 + r) * P + k] * vk;
+          ai[r] += oim[(b0 + r) * P + k] * vk;
+        }
+      }
+      if (++cnt == 64) {
+#pragma unroll
+        for (int r = 0; r < OV_ROWS; ++r) dr[r] += ar[r], di[r] += ai[r], ar[r] = ai[r] = 0.0f;
+        cnt = 0;
+      }
     }
   }
 #pragma unroll
@@ -124,12 +147,42 @@ __global__ __launch_bounds__(1024) void sum_u_kernel(const float* __restrict__ u
   }
 }
 
-// out_k = scale * Re[sum_b conj(O_bk - Obar_k) u_b] + shift * v_k
+// out_k = scale * Re[sum_b conj(O_bk - Obar_k) u_b] + shift * v_k; each
+// thread owns 4 consecutive columns (float4 rows) when P is a multiple of 4.
 __global__ void oh_times_u_kernel(const float* __restrict__ ore, const float* __restrict__ oim,
                                   const float* __restrict__ mre, const float* __restrict__ mim,
                                   const float* __restrict__ u, const double* __restrict__ usum, int64_t B, int64_t P,
                                   double scale, double shift, const float* __restrict__ v, float* __restrict__ out) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if ((P & 3) == 0) {
+    const int64_t P4 = P >> 2;
+    if (t >= P4) return;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    f32x4 part = {0.f, 0.f, 0.f, 0.f};
+    int cnt = 0;
+#pragma unroll 4
+    for (int64_t b = 0; b < B; ++b) {
+      const f32x4 x = reinterpret_cast<const f32x4*>(ore + b * P)[t];
+      const f32x4 y = reinterpret_cast<const f32x4*>(oim + b * P)[t];
+      const float ur = u[2 * b], ui = u[2 * b + 1];
+      part += x * ur + y * ui;
+      if (++cnt == 64) {
+        acc[0] += part.x, acc[1] += part.y, acc[2] += part.z, acc[3] += part.w;
+        part = f32x4{0.f, 0.f, 0.f, 0.f};
+        cnt = 0;
+      }
+    }
+    acc[0] += part.x, acc[1] += part.y, acc[2] += part.z, acc[3] += part.w;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int64_t k = 4 * t + c;
+      double r = scale * (acc[c] - ((double)mre[k] * usum[0] + (double)mim[k] * usum[1]));
+      if (v) r += shift * (double)v[k];
+      out[k] = (float)r;
+    }
+    return;
+  }
+  const int64_t k = t;
   if (k >= P) return;
   double acc = 0.0;
   float part = 0.0f;
@@ -204,7 +257,8 @@ int fk_oh_times_u(const float* o_re, const float* o_im, const float* mean_re, co
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(sum_u_kernel, dim3(1), dim3(1024), 0, s, u, batch, scratch2);
   FK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(oh_times_u_kernel, dim3(div_up(n_params, 256)), dim3(256), 0, s, o_re, o_im, mean_re, mean_im, u,
+  const int64_t cols = (n_params & 3) == 0 ? n_params / 4 : n_params;
+  hipLaunchKernelGGL(oh_times_u_kernel, dim3(div_up(cols, 256)), dim3(256), 0, s, o_re, o_im, mean_re, mean_im, u,
                      scratch2, batch, n_params, scale, shift, v, out);
   FK_CHECK_LAUNCH();
   return FK_OK;

@@ -194,3 +194,33 @@ def test_mfma_per_sample_gradients_vs_autograd(rng, shape, depth, C, act):
     assert np.abs(got_re - J_re).max() / scale < 1e-4
     assert np.abs(got_im - J_im).max() / scale < 1e-4
     assert np.abs(lp[:, 0].cpu().numpy() - ref.real).max() < 1e-4 * (np.abs(ref.real).max() + 1)
+
+
+@pytest.mark.parametrize('B,P', [(37, 1000), (37, 1001), (5, 4)])
+def test_sr_products_c_abi(rng, B, P):
+    """fk_o_times_v / fk_oh_times_u on random O against dense float64 algebra,
+    on both the float4 (P % 4 == 0) and the scalar column paths."""
+    from flowket_amd import _lib
+    gpu_banner(f'SR products B={B} P={P}')
+    O = rng.normal(size=(B, P)) + 1j * rng.normal(size=(B, P))
+    v = rng.normal(size=P)
+    ore = torch.as_tensor(O.real, dtype=torch.float32).contiguous().cuda()
+    oim = torch.as_tensor(O.imag, dtype=torch.float32).contiguous().cuda()
+    mre, mim = torch.empty(P, device='cuda'), torch.empty(P, device='cuda')
+    _lib.call('fk_o_mean', _lib.ptr(ore), _lib.ptr(oim), B, P, _lib.ptr(mre), _lib.ptr(mim), _lib.stream_ptr())
+    scratch = torch.empty(int(_lib.load().fk_sr_scratch_doubles()), dtype=torch.float64, device='cuda')
+    vd = torch.as_tensor(v, dtype=torch.float32).cuda()
+    u = torch.empty((B, 2), dtype=torch.float32, device='cuda')
+    _lib.call('fk_o_times_v', _lib.ptr(ore), _lib.ptr(oim), _lib.ptr(mre), _lib.ptr(mim), _lib.ptr(vd), B, P,
+              _lib.ptr(u), _lib.ptr(scratch), _lib.stream_ptr())
+    Oc = O.astype(np.complex64).astype(np.complex128)
+    Oc = Oc - Oc.mean(axis=0)
+    ref_u = Oc @ v.astype(np.float32).astype(np.float64)
+    got_u = u[:, 0].double().cpu().numpy() + 1j * u[:, 1].double().cpu().numpy()
+    assert np.abs(got_u - ref_u).max() < 1e-4 * np.abs(ref_u).max()
+    out = torch.empty(P, device='cuda')
+    _lib.call('fk_oh_times_u', _lib.ptr(ore), _lib.ptr(oim), _lib.ptr(mre), _lib.ptr(mim), _lib.ptr(u), B, P,
+              0.5, 0.25, _lib.ptr(vd), _lib.ptr(out), _lib.ptr(scratch), _lib.stream_ptr())
+    uu = got_u
+    ref = 0.5 * np.real(np.conj(Oc).T @ uu) + 0.25 * v.astype(np.float32)
+    assert np.abs(out.double().cpu().numpy() - ref).max() < 1e-4 * np.abs(ref).max()
