@@ -141,6 +141,11 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
   float* spins = reinterpret_cast<float*>(base + (int64_t)(3 * nb - 2) * BUF);  // [HW][16]
 
   auto wfrag = [&](int64_t fragidx) { return wall[fragidx * 64 + lane]; };
+  // batch every load of a block step ahead of its MFMAs when a tap's
+  // fragments are small (bf16/fp16 C=32); wider nets load tap by tap
+  constexpr bool BATCH = MT * KS * (int)(sizeof(frag) / 4) <= 8;
+  constexpr int BT = BATCH ? 9 : 1;
+  constexpr int UNR = BATCH ? 9 : 1;  // tap loops unrolled only with batched loads
 
   // row 0 of every block's concatenation: y = act(1x1(0) + b) (DownShift zeros)
   for (int l = 0; l <= F; ++l) {
@@ -264,22 +269,44 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
                 hacc[m] += *reinterpret_cast<const f32x4*>(kh + t * C + 16 * m + 4 * g) * sv;
             }
           } else {
+            // every load of this step is issued before the first MFMA: the
+            // step is a chain of dependent MFMAs, so serialised L2 round trips
+            // would dominate it
+            frag wh[BATCH ? 3 : 1][MT][KS];
+            LD hd[BATCH ? 2 : 1];
+            if constexpr (BATCH) {
+#pragma unroll
+              for (int t = 0; t < 3; ++t)
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+#pragma unroll
+                  for (int ks = 0; ks < KS; ++ks) wh[t][m][ks] = wfrag(lay.off_h(l) + (t * MT + m) * KS + ks);
+#pragma unroll
+              for (int t = 0; t < 2; ++t)
+                if (j + t - 2 >= 0) hd[t].load(HAp(l, i * W + j + t - 2));
+            }
+#pragma unroll UNR
             for (int t = 0; t < 3; ++t) {
               const int c = j + t - 2;
               if (c < 0) continue;
               frag b[KS];
               if (t == 2) {
                 d_to_b<PREC, MT, KS>(hcur, b);
+              } else if constexpr (BATCH) {
+                hd[t].template frags<KS>(b);
               } else {
-                LD hd;
-                hd.load(HAp(l, i * W + c));
-                hd.template frags<KS>(b);
+                hd[0].load(HAp(l, i * W + c));
+                hd[0].template frags<KS>(b);
               }
 #pragma unroll
               for (int m = 0; m < MT; ++m)
 #pragma unroll
-                for (int ks = 0; ks < KS; ++ks)
-                  hacc[m] = P::mfma(wfrag(lay.off_h(l) + (t * MT + m) * KS + ks), b[ks], hacc[m]);
+                for (int ks = 0; ks < KS; ++ks) {
+                  frag wk;
+                  if constexpr (BATCH) wk = wh[t][m][ks];
+                  else wk = wfrag(lay.off_h(l) + (t * MT + m) * KS + ks);
+                  hacc[m] = P::mfma(wk, b[ks], hacc[m]);
+                }
             }
           }
           f32x4 ax[MT];
@@ -289,6 +316,21 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           d_to_b<PREC, MT, KS>(ax, bxf);
           LD cur2;
           cur2.load_quads(CATp(l, q), MTH, MT);
+          frag wc[BT][MT][KS];
+          LD cd[BATCH ? 8 : 1];
+          if constexpr (BATCH) {
+#pragma unroll
+            for (int t = 0; t < 9; ++t)
+#pragma unroll
+              for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) wc[t][m][ks] = wfrag(lay.off_c(l) + (t * MT + m) * KS + ks);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+              const int r = i + t / 3 - 2, c = j + t % 3 - 2;
+              if (r >= 0 && c >= 0) cd[t].load(CATp(l, r * W + c));
+            }
+          }
 #pragma unroll
           for (int m = 0; m < MTH; ++m) {
             f32x4 x = load_bias4(bl + 2 * C, 16 * m + 4 * g, C / 2);
@@ -300,19 +342,28 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           f32x4 cacc[MT];
 #pragma unroll
           for (int m = 0; m < MT; ++m) cacc[m] = load_bias4(bl + 3 * C, 16 * m + 4 * g, C);
+#pragma unroll UNR
           for (int t = 0; t < 9; ++t) {
             const int r = i + t / 3 - 2, c = j + t % 3 - 2;
             if (r < 0 || c < 0) continue;
-            LD cd;
-            if (t == 8) cd = cur2;
-            else cd.load(CATp(l, r * W + c));
             frag b[KS];
-            cd.template frags<KS>(b);
+            if (t == 8) {
+              cur2.template frags<KS>(b);
+            } else if constexpr (BATCH) {
+              cd[t].template frags<KS>(b);
+            } else {
+              cd[0].load(CATp(l, r * W + c));
+              cd[0].template frags<KS>(b);
+            }
 #pragma unroll
             for (int m = 0; m < MT; ++m)
 #pragma unroll
-              for (int ks = 0; ks < KS; ++ks)
-                cacc[m] = P::mfma(wfrag(lay.off_c(l) + (t * MT + m) * KS + ks), b[ks], cacc[m]);
+              for (int ks = 0; ks < KS; ++ks) {
+                frag wk;
+                if constexpr (BATCH) wk = wc[t][m][ks];
+                else wk = wfrag(lay.off_c(l) + (t * MT + m) * KS + ks);
+                cacc[m] = P::mfma(wk, b[ks], cacc[m]);
+              }
           }
           LD hn;
           if (kind == BK_A) {
@@ -336,6 +387,18 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
     for (int l = 0; l <= F; ++l) {
       const int kind = block_kind(l, nb);
       const float* bl = a.aux + lay.aux_bias(l);
+      // this block's vertical and y weights serve the whole row
+      frag wv[BT][MT][KS];
+      if constexpr (BATCH) {
+        if (l > 0) {
+#pragma unroll
+          for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+              for (int ks = 0; ks < KS; ++ks) wv[t][m][ks] = wfrag(lay.off_v(l) + (t * MT + m) * KS + ks);
+        }
+      }
       for (int j = 0; j < W; ++j) {
         const int q = i * W + j;
         f32x4 acc[MT];
@@ -351,18 +414,34 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
             for (int m = 0; m < MT; ++m) acc[m] += *reinterpret_cast<const f32x4*>(kv + t * C + 16 * m + 4 * g) * sv;
           }
         } else {
+          LD vd[BT];
+          if constexpr (BATCH) {
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+              const int r = i + t / 3 - 2, c = j + t % 3 - 1;
+              if (r >= 0 && c >= 0 && c < W) vd[t].load(VAp(l, r * W + c));
+            }
+          }
+#pragma unroll UNR
           for (int t = 0; t < 9; ++t) {
             const int r = i + t / 3 - 2, c = j + t % 3 - 1;
             if (r < 0 || c < 0 || c >= W) continue;
-            LD vd;
-            vd.load(VAp(l, r * W + c));
             frag b[KS];
-            vd.template frags<KS>(b);
+            if constexpr (BATCH) {
+              vd[t].template frags<KS>(b);
+            } else {
+              vd[0].load(VAp(l, r * W + c));
+              vd[0].template frags<KS>(b);
+            }
 #pragma unroll
             for (int m = 0; m < MT; ++m)
 #pragma unroll
-              for (int ks = 0; ks < KS; ++ks)
-                acc[m] = P::mfma(wfrag(lay.off_v(l) + (t * MT + m) * KS + ks), b[ks], acc[m]);
+              for (int ks = 0; ks < KS; ++ks) {
+                frag wk;
+                if constexpr (BATCH) wk = wv[t][m][ks];
+                else wk = wfrag(lay.off_v(l) + (t * MT + m) * KS + ks);
+                acc[m] = P::mfma(wk, b[ks], acc[m]);
+              }
           }
         }
         f32x4 av[MT];
