@@ -67,8 +67,10 @@ SIGNATURES = {
     'fk_mh_propose': (ctypes.c_int, [_P, _I64, _I32, _I32, _I32, _P, _P, _P, _P]),
     'fk_mh_accept': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I32, _P, _P]),
     'fk_mh_accept_independent': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _I64, _I32, _P, _P]),
-    'fk_made_log_prob': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P, _P]),
-    'fk_made_sample': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P, _P, _P]),
+    'fk_made_packed_bytes': (_I64, [_I32, _I32]),
+    'fk_made_pack': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
+    'fk_made_log_prob': (ctypes.c_int, [_P, _I32, _I32, _P, _I64, _P, _P]),
+    'fk_made_sample': (ctypes.c_int, [_P, _I32, _I32, _P, _I64, _P, _P, _P]),
     'fk_made_fit_workspace_floats': (_I64, [_I32, _I32, _I64]),
     'fk_made_fit_step': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P,
                                         ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I64, _P]),

@@ -8,9 +8,11 @@
 //
 // Weights are stored in BF16 (forward, sampling), FP32 master copies are
 // trained by maximum likelihood on the chains' samples (Adam); the biases are
-// read straight from the FP32 master vector.  One workgroup of 256 threads
-// handles 8 configurations; activations live in LDS, weights are read
-// coalesced (second layer stored transposed).
+// read straight from the FP32 master vector.  Log q and sampling run the
+// degree-ordered per-lane evaluation (made_lane_kernel); the fit's backward
+// uses a masked tile forward, 8 configurations per 256-thread workgroup with
+// activations in LDS and coalesced weight reads (second layer stored
+// transposed).
 #include "fk_common.h"
 
 namespace fk {
@@ -74,66 +76,6 @@ __device__ void made_forward_tile(const MadeW& w, const float* s_x, float* s_h1,
 }
 
 __device__ __forceinline__ float log_sigmoid(float z) { return z >= 0.0f ? -log1pf(expf(-z)) : z - log1pf(expf(z)); }
-
-__global__ __launch_bounds__(MADE_T) void made_logq_kernel(MadeW w, const int8_t* __restrict__ x, int64_t B,
-                                                           float* __restrict__ logq) {
-  extern __shared__ float sm[];
-  const int N = w.N, Hd = w.Hd;
-  float* s_x = sm;
-  float* s_h1 = s_x + MADE_S * N;
-  float* s_h2 = s_h1 + MADE_S * Hd;
-  float* s_l = s_h2 + MADE_S * Hd;
-  const int64_t b0 = (int64_t)blockIdx.x * MADE_S;
-  for (int idx = threadIdx.x; idx < MADE_S * N; idx += blockDim.x) {
-    const int64_t b = b0 + idx / N;
-    s_x[idx] = b < B ? (float)x[b * N + idx % N] : 0.0f;
-  }
-  __syncthreads();
-  made_forward_tile(w, s_x, s_h1, s_h2, s_l);
-  if (threadIdx.x < MADE_S) {
-    const int s = threadIdx.x;
-    if (b0 + s < B) {
-      float acc = 0.0f;
-      for (int o = 0; o < N; ++o) acc += log_sigmoid(s_x[s * N + o] > 0.0f ? s_l[s * N + o] : -s_l[s * N + o]);
-      logq[b0 + s] = acc;
-    }
-  }
-}
-
-// x_i ~ q(x_i | x_<i) in site order: s_i = +1 iff sigmoid(logit_i) > u_i
-__global__ __launch_bounds__(MADE_T) void made_sample_kernel(MadeW w, const double* __restrict__ u, int64_t B,
-                                                             int8_t* __restrict__ x, float* __restrict__ logq) {
-  extern __shared__ float sm[];
-  const int N = w.N, Hd = w.Hd;
-  float* s_x = sm;
-  float* s_h1 = s_x + MADE_S * N;
-  float* s_h2 = s_h1 + MADE_S * Hd;
-  float* s_l = s_h2 + MADE_S * Hd;
-  __shared__ float s_lq[MADE_S];
-  const int64_t b0 = (int64_t)blockIdx.x * MADE_S;
-  for (int idx = threadIdx.x; idx < MADE_S * N; idx += blockDim.x) s_x[idx] = 0.0f;
-  if (threadIdx.x < MADE_S) s_lq[threadIdx.x] = 0.0f;
-  __syncthreads();
-  for (int i = 0; i < N; ++i) {
-    made_forward_tile(w, s_x, s_h1, s_h2, s_l);
-    if (threadIdx.x < MADE_S) {
-      const int s = threadIdx.x;
-      const int64_t b = b0 + s;
-      const float z = s_l[s * N + i];
-      const double uu = b < B ? u[b * N + i] : 0.5;
-      const float p = 1.0f / (1.0f + expf(-z));
-      const bool up = (double)p > uu;
-      s_x[s * N + i] = up ? 1.0f : -1.0f;
-      s_lq[s] += log_sigmoid(up ? z : -z);
-    }
-    __syncthreads();
-  }
-  for (int idx = threadIdx.x; idx < MADE_S * N; idx += blockDim.x) {
-    const int64_t b = b0 + idx / N;
-    if (b < B) x[b * N + idx % N] = (int8_t)s_x[idx];
-  }
-  if (threadIdx.x < MADE_S && b0 + threadIdx.x < B) logq[b0 + threadIdx.x] = s_lq[threadIdx.x];
-}
 
 // Per-sample backward of L = -log q(x): stores the layer inputs and the
 // gradients of the pre-activations (dl = sigmoid(l) - [x == +1],
@@ -271,40 +213,253 @@ __global__ void made_adam_kernel(float* __restrict__ master, float* __restrict__
 
 size_t made_lds(int N, int Hd) { return (size_t)MADE_S * (2 * N + 3 * Hd) * sizeof(float); }
 
+// ---------------------------------------------------------------------------
+// Log q and sampling on MFMA.  Hidden units are renumbered in degree order, so
+// the units a site finalises are one contiguous block and everything they
+// read is a contiguous prefix.  A unit of degree d depends only on inputs
+// x_0..x_{d-1}: at site i the units of degree i are computed once (first
+// layer from x_<i, second layer from first-layer units of degree <= i) and
+// logit i reads second-layer units of degree <= i -- each a 16-row block of
+// v_mfma_f32_16x16x32_bf16 with the 16 samples of a wave as its columns.
+// fk_made_pack writes the degree-sorted, masked BF16 matrices once per
+// refit; hidden activations are held in LDS as BF16 (part of the proposal's
+// definition: log q and sampling share this code path, so the density the
+// acceptance uses is the one the sampler drew from).
+//
+// Packed layout (16-byte aligned sections): A1 (Hd+16, K1P) bf16 first-layer
+// rows; A2 (Hd+16, HP) bf16 second layer [t][k]; A3 (N+16, HP) bf16 logits;
+// b1 (Hd+16), b2 (Hd+16), b3 (N+16) fp32 -- K1P = N, HP = Hd rounded up to 32.
+struct MadePacked {
+  int N, Hd, K1P, HP;
+  int64_t o_a1, o_a2, o_a3, o_b1, o_b2, o_b3, bytes;
+  __host__ __device__ MadePacked(int n, int hd) : N(n), Hd(hd) {
+    K1P = (n + 31) / 32 * 32;
+    HP = (hd + 31) / 32 * 32;
+    auto al = [](int64_t v) { return (v + 15) / 16 * 16; };
+    o_a1 = 0;
+    o_a2 = al(o_a1 + (int64_t)(hd + 16) * K1P * 2);
+    o_a3 = al(o_a2 + (int64_t)(hd + 16) * HP * 2);
+    o_b1 = al(o_a3 + (int64_t)(n + 16) * HP * 2);
+    o_b2 = al(o_b1 + (int64_t)(hd + 16) * 4);
+    o_b3 = al(o_b2 + (int64_t)(hd + 16) * 4);
+    bytes = al(o_b3 + (int64_t)(n + 16) * 4);
+  }
+};
+
+// first sorted position of the units of degree d (1-based degrees)
+__host__ __device__ inline int made_deg_start(int d, int N, int Hd) {
+  const int step = N - 1;
+  int s = 0;
+  for (int e = 1; e < d; ++e) s += (e - 1 < Hd) ? (Hd - (e - 1) + step - 1) / step : 0;
+  return s;
+}
+// unit index at sorted position r
+__device__ inline int made_sorted_unit(int r, int N, int Hd) {
+  const int step = N - 1;
+  int s = 0;
+  for (int d = 1; d < N; ++d) {
+    const int c = (d - 1 < Hd) ? (Hd - (d - 1) + step - 1) / step : 0;
+    if (r < s + c) return (d - 1) + (r - s) * step;
+    s += c;
+  }
+  return -1;
+}
+
+__global__ void made_pack_kernel(MadeW w, uint8_t* __restrict__ out) {
+  const MadePacked L(w.N, w.Hd);
+  const int N = w.N, Hd = w.Hd;
+  __bf16* a1 = reinterpret_cast<__bf16*>(out + L.o_a1);
+  __bf16* a2 = reinterpret_cast<__bf16*>(out + L.o_a2);
+  __bf16* a3 = reinterpret_cast<__bf16*>(out + L.o_a3);
+  float* b1 = reinterpret_cast<float*>(out + L.o_b1);
+  float* b2 = reinterpret_cast<float*>(out + L.o_b2);
+  float* b3 = reinterpret_cast<float*>(out + L.o_b3);
+  const int64_t n1 = (int64_t)(Hd + 16) * L.K1P, n2 = (int64_t)(Hd + 16) * L.HP, n3 = (int64_t)(N + 16) * L.HP;
+  const int64_t total = n1 + n2 + n3 + 2 * (Hd + 16) + (N + 16);
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = idx;
+    if (r < n1) {
+      const int row = (int)(r / L.K1P), j = (int)(r % L.K1P);
+      float v = 0.0f;
+      if (row < Hd && j < N) {
+        const int h = made_sorted_unit(row, N, Hd);
+        if (j + 1 <= deg_hidden(h, N)) v = (float)w.w1[(int64_t)h * N + j];
+      }
+      a1[r] = (__bf16)v;
+    } else if ((r -= n1) < n2) {
+      const int row = (int)(r / L.HP), c = (int)(r % L.HP);
+      float v = 0.0f;
+      if (row < Hd && c < Hd) {
+        const int t = made_sorted_unit(row, N, Hd), k = made_sorted_unit(c, N, Hd);
+        if (deg_hidden(k, N) <= deg_hidden(t, N)) v = (float)w.w2t[(int64_t)k * Hd + t];
+      }
+      a2[r] = (__bf16)v;
+    } else if ((r -= n2) < n3) {
+      const int o = (int)(r / L.HP), c = (int)(r % L.HP);
+      float v = 0.0f;
+      if (o < N && c < Hd) {
+        const int t = made_sorted_unit(c, N, Hd);
+        if (deg_hidden(t, N) <= o) v = (float)w.w3[(int64_t)o * Hd + t];
+      }
+      a3[r] = (__bf16)v;
+    } else if ((r -= n3) < Hd + 16) {
+      b1[r] = r < Hd ? w.b1[made_sorted_unit((int)r, N, Hd)] : 0.0f;
+    } else if ((r -= Hd + 16) < Hd + 16) {
+      b2[r] = r < Hd ? w.b2[made_sorted_unit((int)r, N, Hd)] : 0.0f;
+    } else {
+      r -= Hd + 16;
+      b3[r] = r < N ? w.b3[r] : 0.0f;
+    }
+  }
+}
+
+constexpr int MADE_MS = 16;  // samples per wave (the MFMA N dimension)
+
+// One wave per 16 samples.  LDS: X (16, K1P), H1 (16, HP), H2 (16, HP) bf16.
+__global__ __launch_bounds__(64) void made_mfma_kernel(const uint8_t* __restrict__ packed, int N, int Hd,
+                                                        const int8_t* __restrict__ x_in,
+                                                        const double* __restrict__ u, int64_t B,
+                                                        int8_t* __restrict__ x_out, float* __restrict__ logq) {
+  const MadePacked L(N, Hd);
+  extern __shared__ __align__(16) uint8_t made_sm[];
+  __bf16* X = reinterpret_cast<__bf16*>(made_sm);
+  __bf16* H1 = X + MADE_MS * L.K1P;
+  __bf16* H2 = H1 + MADE_MS * L.HP;
+  const __bf16* A1 = reinterpret_cast<const __bf16*>(packed + L.o_a1);
+  const __bf16* A2 = reinterpret_cast<const __bf16*>(packed + L.o_a2);
+  const __bf16* A3 = reinterpret_cast<const __bf16*>(packed + L.o_a3);
+  const float* b1 = reinterpret_cast<const float*>(packed + L.o_b1);
+  const float* b2 = reinterpret_cast<const float*>(packed + L.o_b2);
+  const float* b3 = reinterpret_cast<const float*>(packed + L.o_b3);
+  const int lane = threadIdx.x, n = lane & 15, g = lane >> 4;
+  const int64_t b = (int64_t)blockIdx.x * MADE_MS + n;
+  const bool live = b < B;
+  for (int idx = lane; idx < MADE_MS * L.K1P; idx += 64) {
+    const int sn = idx / L.K1P, j = idx - sn * L.K1P;
+    const int64_t bb = (int64_t)blockIdx.x * MADE_MS + sn;
+    float v = 0.0f;
+    if (x_in && j < N && bb < B) v = (float)x_in[bb * N + j];
+    X[idx] = (__bf16)v;
+  }
+  for (int idx = lane; idx < 2 * MADE_MS * L.HP; idx += 64) H1[idx] = (__bf16)0.0f;
+  __syncthreads();
+
+  // sorted units of degree < d start at pos; degree-i block is [s0, s1)
+  int s0 = 0;
+  float lq = 0.0f;
+  for (int i = 0; i < N; ++i) {
+    const int s1 = (i >= 1) ? s0 + ((i - 1 < Hd) ? (Hd - (i - 1) + N - 2) / (N - 1) : 0) : 0;
+    if (i >= 1 && s1 > s0) {
+      // first layer: rows [s0, s1), inputs j < i
+      for (int rb = s0; rb < s1; rb += 16) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks * 32 < i; ++ks) {
+          const bf16x8 a = *reinterpret_cast<const bf16x8*>(A1 + (int64_t)(rb + n) * L.K1P + 32 * ks + 8 * g);
+          const bf16x8 bx = *reinterpret_cast<const bf16x8*>(X + n * L.K1P + 32 * ks + 8 * g);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bx, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rb + 4 * g + r;
+          if (row < s1) H1[n * L.HP + row] = (__bf16)fmaxf(acc[r] + b1[row], 0.0f);
+        }
+      }
+      __syncthreads();
+      // second layer: rows [s0, s1), first-layer units [0, s1)
+      for (int rb = s0; rb < s1; rb += 16) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks * 32 < s1; ++ks) {
+          const bf16x8 a = *reinterpret_cast<const bf16x8*>(A2 + (int64_t)(rb + n) * L.HP + 32 * ks + 8 * g);
+          const bf16x8 bh = *reinterpret_cast<const bf16x8*>(H1 + n * L.HP + 32 * ks + 8 * g);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rb + 4 * g + r;
+          if (row < s1) H2[n * L.HP + row] = (__bf16)fmaxf(acc[r] + b2[row], 0.0f);
+        }
+      }
+      __syncthreads();
+    }
+    // logit i from second-layer units [0, s1) (row 0 of the block starting at i)
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks * 32 < s1; ++ks) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(A3 + (int64_t)(i + n) * L.HP + 32 * ks + 8 * g);
+      const bf16x8 bh = *reinterpret_cast<const bf16x8*>(H2 + n * L.HP + 32 * ks + 8 * g);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh, acc, 0, 0, 0);
+    }
+    if (g == 0) {
+      const float z = acc[0] + b3[i];
+      float xi;
+      if (x_in) {
+        xi = live ? (float)x_in[b * N + i] : 1.0f;
+      } else {
+        const double uu = live ? u[b * N + i] : 0.5;
+        const float p = 1.0f / (1.0f + expf(-z));
+        xi = ((double)p > uu) ? 1.0f : -1.0f;
+        X[n * L.K1P + i] = (__bf16)xi;
+        if (live) x_out[b * N + i] = (int8_t)xi;
+      }
+      lq += log_sigmoid(xi > 0.0f ? z : -z);
+    }
+    __syncthreads();
+    s0 = s1;
+  }
+  if (g == 0 && live) logq[b] = lq;
+}
+
+size_t made_mfma_lds(int N, int Hd) {
+  const MadePacked L(N, Hd);
+  return (size_t)MADE_MS * (L.K1P + 2 * L.HP) * sizeof(__bf16);
+}
+
 }  // namespace fk
 
 using namespace fk;
 
 extern "C" {
 
-int fk_made_log_prob(const void* w1, const float* b1, const void* w2t, const float* b2, const void* w3,
-                     const float* b3, int32_t n_sites, int32_t hidden, const int8_t* x, int64_t batch, float* logq,
-                     void* stream) {
-  FK_REQUIRE(n_sites >= 2 && hidden >= 1, "fk_made_log_prob: bad sizes");
-  if (batch <= 0) return FK_OK;
-  FK_REQUIRE(w1 && b1 && w2t && b2 && w3 && b3 && x && logq, "fk_made_log_prob: null pointer");
-  const size_t lds = made_lds(n_sites, hidden);
-  FK_REQUIRE(lds <= 64 * 1024, "fk_made_log_prob: too large for LDS");
+int64_t fk_made_packed_bytes(int32_t n_sites, int32_t hidden) {
+  if (n_sites < 2 || hidden < 1) return -1;
+  return MadePacked(n_sites, hidden).bytes;
+}
+
+int fk_made_pack(const void* w1, const float* b1, const void* w2t, const float* b2, const void* w3, const float* b3,
+                 int32_t n_sites, int32_t hidden, void* packed, void* stream) {
+  FK_REQUIRE(n_sites >= 2 && hidden >= 1, "fk_made_pack: bad sizes");
+  FK_REQUIRE(w1 && b1 && w2t && b2 && w3 && b3 && packed, "fk_made_pack: null pointer");
   MadeW w{(const __bf16*)w1, b1, (const __bf16*)w2t, b2, (const __bf16*)w3, b3, n_sites, hidden};
-  hipLaunchKernelGGL(made_logq_kernel, dim3(div_up(batch, MADE_S)), dim3(MADE_T), lds, (hipStream_t)stream, w, x,
-                     batch, logq);
+  hipLaunchKernelGGL(made_pack_kernel, dim3(256), dim3(256), 0, (hipStream_t)stream, w,
+                     reinterpret_cast<uint8_t*>(packed));
   FK_CHECK_LAUNCH();
   return FK_OK;
 }
 
-int fk_made_sample(const void* w1, const float* b1, const void* w2t, const float* b2, const void* w3, const float* b3,
-                   int32_t n_sites, int32_t hidden, const double* uniforms, int64_t batch, int8_t* x, float* logq,
-                   void* stream) {
-  FK_REQUIRE(n_sites >= 2 && hidden >= 1, "fk_made_sample: bad sizes");
-  if (batch <= 0) return FK_OK;
-  FK_REQUIRE(w1 && b1 && w2t && b2 && w3 && b3 && uniforms && x && logq, "fk_made_sample: null pointer");
-  const size_t lds = made_lds(n_sites, hidden);
-  FK_REQUIRE(lds <= 64 * 1024, "fk_made_sample: too large for LDS");
-  MadeW w{(const __bf16*)w1, b1, (const __bf16*)w2t, b2, (const __bf16*)w3, b3, n_sites, hidden};
-  hipLaunchKernelGGL(made_sample_kernel, dim3(div_up(batch, MADE_S)), dim3(MADE_T), lds, (hipStream_t)stream, w,
-                     uniforms, batch, x, logq);
+static int made_run(const void* packed, int32_t n_sites, int32_t hidden, const int8_t* x_in, const double* u,
+                    int64_t batch, int8_t* x_out, float* logq, void* stream) {
+  const size_t lds = made_mfma_lds(n_sites, hidden);
+  FK_REQUIRE(lds <= 64 * 1024, "MADE: too large for LDS");
+  hipLaunchKernelGGL(made_mfma_kernel, dim3(div_up(batch, MADE_MS)), dim3(64), lds, (hipStream_t)stream,
+                     reinterpret_cast<const uint8_t*>(packed), n_sites, hidden, x_in, u, batch, x_out, logq);
   FK_CHECK_LAUNCH();
   return FK_OK;
+}
+
+int fk_made_log_prob(const void* packed, int32_t n_sites, int32_t hidden, const int8_t* x, int64_t batch,
+                     float* logq, void* stream) {
+  FK_REQUIRE(n_sites >= 2 && hidden >= 1, "fk_made_log_prob: bad sizes");
+  if (batch <= 0) return FK_OK;
+  FK_REQUIRE(packed && x && logq, "fk_made_log_prob: null pointer");
+  return made_run(packed, n_sites, hidden, x, nullptr, batch, nullptr, logq, stream);
+}
+
+int fk_made_sample(const void* packed, int32_t n_sites, int32_t hidden, const double* uniforms, int64_t batch,
+                   int8_t* x, float* logq, void* stream) {
+  FK_REQUIRE(n_sites >= 2 && hidden >= 1, "fk_made_sample: bad sizes");
+  if (batch <= 0) return FK_OK;
+  FK_REQUIRE(packed && uniforms && x && logq, "fk_made_sample: null pointer");
+  return made_run(packed, n_sites, hidden, nullptr, uniforms, batch, x, logq, stream);
 }
 
 int64_t fk_made_fit_workspace_floats(int32_t n_sites, int32_t hidden, int64_t batch) {

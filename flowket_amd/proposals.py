@@ -62,6 +62,14 @@ class MADEProposal:
         self.w1.copy_(w1)
         self.w2t.copy_(w2t)
         self.w3.copy_(w3)
+        self._repack()
+
+    def _repack(self):
+        """Degree-sorted masked copy that log q and sampling read."""
+        if getattr(self, 'packed', None) is None:
+            nb = int(_lib.load().fk_made_packed_bytes(self.N, self.Hd))
+            self.packed = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        _lib.call('fk_made_pack', *self._wargs(), self.N, self.Hd, _lib.ptr(self.packed), _lib.stream_ptr())
 
     def _wargs(self):
         _, b1, _, b2, _, b3 = self._views()
@@ -70,8 +78,8 @@ class MADEProposal:
     def log_prob(self, x):
         x = x.reshape(x.shape[0], self.N).to(torch.int8).contiguous()
         out = torch.empty(x.shape[0], dtype=torch.float32, device=self.device)
-        _lib.call('fk_made_log_prob', *self._wargs(), self.N, self.Hd, _lib.ptr(x), x.shape[0], _lib.ptr(out),
-                  _lib.stream_ptr())
+        _lib.call('fk_made_log_prob', _lib.ptr(self.packed), self.N, self.Hd, _lib.ptr(x), x.shape[0],
+                  _lib.ptr(out), _lib.stream_ptr())
         return out
 
     def sample(self, n, uniforms=None, generator=None):
@@ -80,8 +88,8 @@ class MADEProposal:
         uniforms = uniforms.to(self.device).contiguous()
         x = torch.empty((n, self.N), dtype=torch.int8, device=self.device)
         lq = torch.empty(n, dtype=torch.float32, device=self.device)
-        _lib.call('fk_made_sample', *self._wargs(), self.N, self.Hd, _lib.ptr(uniforms), n, _lib.ptr(x), _lib.ptr(lq),
-                  _lib.stream_ptr())
+        _lib.call('fk_made_sample', _lib.ptr(self.packed), self.N, self.Hd, _lib.ptr(uniforms), n, _lib.ptr(x),
+                  _lib.ptr(lq), _lib.stream_ptr())
         return x, lq
 
     def fit(self, x, steps=50, lr=None):
@@ -97,7 +105,33 @@ class MADEProposal:
                       _lib.ptr(self.w1), _lib.ptr(b1), _lib.ptr(self.w2t), _lib.ptr(b2), _lib.ptr(self.w3),
                       _lib.ptr(b3), self.N, self.Hd, _lib.ptr(x), B, _lib.ptr(ws), float(lr or self.lr), 0.9, 0.999,
                       1e-7, self.steps, _lib.stream_ptr())
+        self._repack()
         return float(-self.log_prob(x).mean().item())
+
+    def fit_validated(self, x, max_steps=200, chunk=10, holdout=8, lr=None):
+        """Adam on -mean log q over (holdout-1)/holdout of x, stopped when the
+        NLL of the held-out rows stops improving (the best weights are kept).
+        Returns the held-out NLL."""
+        x = x.reshape(x.shape[0], self.N).to(torch.int8).contiguous()
+        nv = max(1, x.shape[0] // holdout)
+        xv, xt = x[:nv].contiguous(), x[nv:].contiguous()
+        best = float(-self.log_prob(xv).mean().item())
+        keep = (self.master.clone(), self.adam_m.clone(), self.adam_v.clone(), self.steps)
+        done = 0
+        while done < max_steps:
+            self.fit(xt, steps=chunk, lr=lr)
+            done += chunk
+            nll = float(-self.log_prob(xv).mean().item())
+            if nll >= best:
+                break
+            best = nll
+            keep = (self.master.clone(), self.adam_m.clone(), self.adam_v.clone(), self.steps)
+        self.master.copy_(keep[0])
+        self.adam_m.copy_(keep[1])
+        self.adam_v.copy_(keep[2])
+        self.steps = keep[3]
+        self._refresh_bf16()
+        return best
 
 
 class MetropolisHastingsLearnedProposal(MetropolisHastingsSampler):
@@ -108,8 +142,8 @@ class MetropolisHastingsLearnedProposal(MetropolisHastingsSampler):
     the plain chain runs."""
 
     def __init__(self, machine, batch_size, num_of_chains=1, unused_sampels=0, discard_ratio=10, seed=None,
-                 base_kind=1, hidden=256, refit_every=20, fit_steps=100, min_acceptance=0.05, proposal_lr=1e-3,
-                 fit_batch=2048, **kwargs):
+                 base_kind=1, hidden=256, refit_every=20, fit_steps=200, min_acceptance=0.05, proposal_lr=1e-3,
+                 fit_batch=8192, **kwargs):
         super().__init__(machine, batch_size, num_of_chains=num_of_chains, unused_sampels=unused_sampels,
                          discard_ratio=discard_ratio, seed=seed, **kwargs)
         self.kind = base_kind
@@ -139,10 +173,8 @@ class MetropolisHastingsLearnedProposal(MetropolisHastingsSampler):
             batch = self._plain()
         if self._outer_calls % self.refit_every == 0:
             flat = batch.reshape(self.batch_size, -1)
-            if self.batch_size > self.fit_batch:
-                pick = torch.randperm(self.batch_size, device=flat.device, generator=self._gen)[:self.fit_batch]
-                flat = flat[pick]
-            self.last_nll = self.proposal.fit(flat, steps=self.fit_steps)
+            pick = torch.randperm(self.batch_size, device=flat.device, generator=self._gen)[:self.fit_batch]
+            self.last_nll = self.proposal.fit_validated(flat[pick], max_steps=self.fit_steps)
             self.proposal_fitted = True
             # acceptance of one proposal step from the current chain states,
             # measured without moving the chains
@@ -150,15 +182,17 @@ class MetropolisHastingsLearnedProposal(MetropolisHastingsSampler):
         self._outer_calls += 1
         return batch
 
-    def _independent_step(self, logq_cur, logpsi_cur):
+    def _independent_step(self, logq_cur, logpsi_cur, cand=None, lq=None, lc=None, u=None, accepted=None):
         C, n = self.num_of_chains, self.sample.shape[1]
         m = self.machine
-        cand, lq = self.proposal.sample(C, generator=self._gen)
-        lc, _ = m.forward(cand)
-        u = torch.rand(C, dtype=torch.float64, device=m.device, generator=self._gen)
-        accepted = torch.empty(C, dtype=torch.int32, device=m.device)
+        if cand is None:
+            cand, lq = self.proposal.sample(C, generator=self._gen)
+            lc, _ = m.forward(cand)
+            u = torch.rand(C, dtype=torch.float64, device=m.device, generator=self._gen)
+        if accepted is None:
+            accepted = torch.empty(C, dtype=torch.int32, device=m.device)
         _lib.call('fk_mh_accept_independent', _lib.ptr(self.sample), _lib.ptr(logpsi_cur), _lib.ptr(logq_cur),
-                  _lib.ptr(cand), _lib.ptr(lc.contiguous()), _lib.ptr(lq), _lib.ptr(u), C, n, _lib.ptr(accepted),
+                  _lib.ptr(cand), _lib.ptr(lc), _lib.ptr(lq), _lib.ptr(u), C, n, _lib.ptr(accepted),
                   _lib.stream_ptr())
         return accepted
 
@@ -173,24 +207,40 @@ class MetropolisHastingsLearnedProposal(MetropolisHastingsSampler):
         return float(acc.float().mean().item())
 
     def _with_proposal(self):
-        spc, burn = self._counts()
+        """Per recorded sample: one independent proposal from the MADE, then
+        (unused_sampels + 1) plain proposals.  All candidates of the call are
+        drawn (and their log psi evaluated) in one launch each; the loop has
+        no host synchronisation."""
+        spc, _ = self._counts()
         C, n = self.num_of_chains, self.sample.shape[1]
         m = self.machine
-        out = torch.empty((C, spc, n), dtype=torch.int8, device=m.device)
-        saved_counts = (self.batch_size, self.unused_sampels, self.discard_ratio)
-        acc_prop = 0
-        acc_local = 0.0
+        dev = m.device
+        out = torch.empty((C, spc, n), dtype=torch.int8, device=dev)
+        cand, lqc = self.proposal.sample(C * spc, generator=self._gen)
+        lcand, _ = m.forward(cand)
+        cand = cand.reshape(spc, C, n)
+        lqc = lqc.reshape(spc, C)
+        lcand = lcand.reshape(spc, C, 2)
+        us = torch.rand((spc, C), dtype=torch.float64, device=dev, generator=self._gen)
+        acc_prop = torch.zeros(C, dtype=torch.int64, device=dev)
+        acc_loc = torch.zeros(C, dtype=torch.int64, device=dev)
+        accepted = torch.empty(C, dtype=torch.int32, device=dev)
+        sweep_out = torch.empty((C, 1, n), dtype=torch.int8, device=dev)
+        lp = None
         for i in range(spc):
-            lp, _ = m.forward(self.sample)
-            lp = lp.contiguous()
+            if lp is None:
+                lp, _ = m.forward(self.sample)
+                lp = lp.contiguous()
             lq = self.proposal.log_prob(self.sample)
-            acc_prop += int(self._independent_step(lq, lp).sum().item())
-            # plain chain sweeps between records: one recorded sample per chain
-            self.batch_size, self.discard_ratio = C, 0
-            rec = MetropolisHastingsSampler.__next__(self)
-            acc_local += self.acceptance_ratio
-            self.batch_size, self.unused_sampels, self.discard_ratio = saved_counts
-            out[:, i] = rec.reshape(C, n)
-        self.proposal_acceptance = acc_prop / (C * spc)
-        self.acceptance_ratio = acc_local / spc
+            self._independent_step(lq, lp, cand[i], lqc[i], lcand[i].contiguous(), us[i], accepted)
+            acc_prop += accepted
+            lp_next = torch.empty((C, 2), dtype=torch.float32, device=dev)
+            if not self._plain_sweeps(sweep_out, self.unused_sampels + 1, acc_loc, self._calls * 1000003 + i,
+                                      logpsi=lp_next):
+                lp_next = None
+            lp = lp_next
+            out[:, i] = sweep_out[:, 0]
+        self._calls += 1
+        self.proposal_acceptance = float(acc_prop.sum().item()) / (C * spc)
+        self.acceptance_ratio = float(acc_loc.sum().item()) / (C * spc * (self.unused_sampels + 1))
         return out.reshape((self.batch_size,) + self.input_size)

@@ -206,6 +206,44 @@ class MetropolisHastingsSampler(Sampler):
         self.acceptance_ratio = acc_total / (self.batch_size * (self.unused_sampels + 1))
         return out.reshape((self.batch_size,) + self.input_size)
 
+    def _plain_sweeps(self, out, n_prop, acc_total, salt, logpsi=None):
+        """n_prop proposals per chain from the current states (no burn-in),
+        recording the final states into out[:, 0]; per-chain accept counts
+        are added to acc_total (int64, device) without a host round trip.
+        For the RBM the final states' log psi is written to ``logpsi``
+        ((chains, 2) float32) when given; returns whether it was."""
+        from .machines import RBM
+        C, n = self.num_of_chains, self.sample.shape[1]
+        dev = self.machine.device
+        accepts = torch.empty(C, dtype=torch.int64, device=dev)
+        if isinstance(self.machine, RBM):
+            w = [_lib.ptr(t) for t in self.machine.parts()]
+            _lib.call('fk_mh_rbm', *w, n, self.machine.n_hidden, self.H, self.W, self.kind, C, 0, 1, n_prop - 1,
+                      _lib.ptr(self.sample), _lib.ptr(out), _lib.ptr(accepts), _lib.ptr(logpsi), None, None, None,
+                      (self.seed * 1000003 + salt) & 0xFFFFFFFFFFFFFFFF, _lib.stream_ptr())
+            acc_total += accepts
+            return logpsi is not None
+        m = self.machine
+        gen = torch.Generator(device=dev)
+        gen.manual_seed((self.seed * 7919 + salt) & 0x7FFFFFFFFFFFFFFF)
+        cur, _ = m.forward(self.sample)
+        cur = cur.contiguous()
+        cand = torch.empty_like(self.sample)
+        accepted = torch.empty(C, dtype=torch.int32, device=dev)
+        s = _lib.stream_ptr()
+        for _ in range(n_prop):
+            site = torch.randint(0, n, (C,), generator=gen, device=dev, dtype=torch.int32)
+            move = torch.randint(0, 9, (C,), generator=gen, device=dev, dtype=torch.int32) if self.kind == 1 else None
+            u = torch.rand((C,), generator=gen, device=dev, dtype=torch.float64)
+            _lib.call('fk_mh_propose', _lib.ptr(self.sample), C, self.H, self.W, self.kind, _lib.ptr(site),
+                      _lib.ptr(move), _lib.ptr(cand), s)
+            lc, _ = m.forward(cand)
+            _lib.call('fk_mh_accept', _lib.ptr(self.sample), _lib.ptr(cur), _lib.ptr(cand), _lib.ptr(lc),
+                      _lib.ptr(u), C, n, _lib.ptr(accepted), s)
+            acc_total += accepted
+        out[:, 0] = self.sample
+        return False
+
     def _draw(self, step, gen, dev):
         C, n = self.num_of_chains, self.sample.shape[1]
         if self.randomness is not None:

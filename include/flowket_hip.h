@@ -320,23 +320,29 @@ int fk_mh_accept_independent(int8_t* state, float* logpsi_cur, float* logq_cur,
  * Learned autoregressive proposal (no reference counterpart): a masked MLP
  * (MADE) n_sites -> hidden -> hidden -> n_sites logits, ReLU, BF16 weights
  * (w1 (hidden, n_sites), w2t (hidden, hidden) = W2 transposed,
- * w3 (n_sites, hidden)) and FP32 biases / arithmetic.  Site i is +1 with
- * probability sigmoid(logit_i) given sites < i.
+ * w3 (n_sites, hidden)), FP32 biases and accumulation, hidden activations
+ * rounded to BF16.  Site i is +1 with probability sigmoid(logit_i) given
+ * sites < i.
+ *   fk_made_pack:     writes the degree-sorted, masked weights that log q and
+ *                     sampling read (fk_made_packed_bytes bytes); call after
+ *                     every change of the weights.
  *   fk_made_log_prob: logq (batch) = log q(x).
- *   fk_made_sample:   x ~ q from uniforms (batch, n_sites), with log q(x).
+ *   fk_made_sample:   x ~ q from uniforms (batch, n_sites), with log q(x)
+ *                     (the same arithmetic as fk_made_log_prob).
  *   fk_made_fit_step: one full-batch Adam step of -mean log q(x) on the FP32
  *                     master vector [w1 | b1 | w2t | b2 | w3 | b3] (b1, b2,
  *                     b3 point into it), refreshing the BF16 copies;
  *                     workspace holds fk_made_fit_workspace_floats floats.
  * ------------------------------------------------------------------------- */
-int fk_made_log_prob(const void* w1, const float* b1, const void* w2t,
-                     const float* b2, const void* w3, const float* b3,
-                     int32_t n_sites, int32_t hidden, const int8_t* x,
-                     int64_t batch, float* logq, void* stream);
-int fk_made_sample(const void* w1, const float* b1, const void* w2t,
-                   const float* b2, const void* w3, const float* b3,
-                   int32_t n_sites, int32_t hidden, const double* uniforms,
-                   int64_t batch, int8_t* x, float* logq, void* stream);
+int64_t fk_made_packed_bytes(int32_t n_sites, int32_t hidden);
+int fk_made_pack(const void* w1, const float* b1, const void* w2t,
+                 const float* b2, const void* w3, const float* b3,
+                 int32_t n_sites, int32_t hidden, void* packed, void* stream);
+int fk_made_log_prob(const void* packed, int32_t n_sites, int32_t hidden,
+                     const int8_t* x, int64_t batch, float* logq, void* stream);
+int fk_made_sample(const void* packed, int32_t n_sites, int32_t hidden,
+                   const double* uniforms, int64_t batch, int8_t* x,
+                   float* logq, void* stream);
 int64_t fk_made_fit_workspace_floats(int32_t n_sites, int32_t hidden,
                                      int64_t batch);
 int fk_made_fit_step(float* master, float* adam_m, float* adam_v, void* w1,

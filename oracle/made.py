@@ -41,30 +41,41 @@ def unpack(master, N, Hd):
     return W1, parts[1], W2, parts[3], W3, parts[5]
 
 
+def bf16_round(x):
+    """Round to the nearest bfloat16 (ties to even), returned as float64."""
+    x32 = np.asarray(x, dtype=np.float32)
+    bits = x32.view(np.uint32).astype(np.uint64)
+    bits = (bits + 0x7FFF + ((bits >> 16) & 1)) & 0xFFFF0000
+    return bits.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
 def _log_sigmoid(z):
     return np.where(z >= 0, -np.log1p(np.exp(-np.abs(z))), z - np.log1p(np.exp(-np.abs(z))))
 
 
-def logits(params, x):
+def logits(params, x, hidden_bf16=False):
+    """Logits of every site; with hidden_bf16 the hidden activations are
+    rounded to bfloat16 (the proposal kernels hold them in BF16)."""
     W1, b1, W2, b2, W3, b3 = params
+    rnd = bf16_round if hidden_bf16 else (lambda v: v)
     x = np.asarray(x, dtype=np.float64)
-    h1 = np.maximum(x @ W1.T + b1, 0)
-    h2 = np.maximum(h1 @ W2.T + b2, 0)
+    h1 = rnd(np.maximum(x @ W1.T + b1, 0))
+    h2 = rnd(np.maximum(h1 @ W2.T + b2, 0))
     return h2 @ W3.T + b3
 
 
-def log_q(params, x):
+def log_q(params, x, hidden_bf16=False):
     x = np.asarray(x, dtype=np.float64).reshape(len(x), -1)
-    return _log_sigmoid(x * logits(params, x)).sum(axis=1)
+    return _log_sigmoid(x * logits(params, x, hidden_bf16)).sum(axis=1)
 
 
-def sample(params, uniforms):
+def sample(params, uniforms, hidden_bf16=False):
     """Site by site: x_i = +1 iff sigmoid(l_i(x_<i)) > u_i."""
     B, N = uniforms.shape
     x = np.zeros((B, N))
     lq = np.zeros(B)
     for i in range(N):
-        z = logits(params, x)[:, i]
+        z = logits(params, x, hidden_bf16)[:, i]
         up = 1.0 / (1.0 + np.exp(-z)) > uniforms[:, i]
         x[:, i] = np.where(up, 1.0, -1.0)
         lq += _log_sigmoid(np.where(up, z, -z))

@@ -48,8 +48,10 @@ def test_made_log_prob_vs_oracle(rng, N, Hd):
     q = _proposal(N, Hd, seed=2, scale=0.3)
     x = rng.choice([-1, 1], size=(77, N)).astype(np.int8)
     got = q.log_prob(torch.as_tensor(x).cuda()).double().cpu().numpy()
-    ref = omd.log_q(_oracle_params(q), x)
-    assert np.abs(got - ref).max() < 1e-4 * (np.abs(ref).max() + 1)
+    ref = omd.log_q(_oracle_params(q), x, hidden_bf16=True)
+    # FP32 accumulation order may move a hidden value across a bf16 rounding
+    # boundary: a few 1e-3 of a logit at most
+    assert np.abs(got - ref).max() < 2e-3 * (np.abs(ref).max() + 1)
 
 
 def test_made_normalised_on_gpu():
@@ -69,12 +71,13 @@ def test_made_sample_vs_oracle(rng, N, Hd):
     u = rng.random((B, N))
     x, lq = q.sample(B, uniforms=torch.as_tensor(u).cuda())
     x = x.cpu().numpy()
-    ref_x, ref_lq = omd.sample(_oracle_params(q), u)
+    ref_x, ref_lq = omd.sample(_oracle_params(q), u, hidden_bf16=True)
     same = np.all(x == ref_x, axis=1)
-    assert same.mean() > 0.98, same.mean()
-    assert np.abs(lq.double().cpu().numpy()[same] - ref_lq[same]).max() < 1e-4 * (np.abs(ref_lq).max() + 1)
-    # the returned log q is the log q of the returned sample
-    assert torch.allclose(lq, q.log_prob(torch.as_tensor(x).cuda()), atol=1e-4)
+    assert same.mean() > 0.95, same.mean()
+    assert np.abs(lq.double().cpu().numpy()[same] - ref_lq[same]).max() < 2e-3 * (np.abs(ref_lq).max() + 1)
+    # the returned log q is the log q of the returned sample, bit for bit
+    # (sampling and log q share the arithmetic)
+    assert torch.equal(lq, q.log_prob(torch.as_tensor(x).cuda()))
 
 
 def test_made_sample_distribution():
@@ -84,7 +87,7 @@ def test_made_sample_distribution():
     n = 2 ** 15
     x, _ = q.sample(n, generator=torch.Generator(device='cuda').manual_seed(3))
     states = oex.all_states((N,)).reshape(-1, N)
-    p = np.exp(omd.log_q(_oracle_params(q), states))
+    p = np.exp(q.log_prob(torch.as_tensor(states.astype(np.int8)).cuda()).double().cpu().numpy())
     rng = np.random.default_rng(5)
     a = np.bincount(oex.binary_array_to_decimal_array(x.cpu().numpy()), minlength=2 ** N)
     b = np.bincount(rng.choice(2 ** N, size=n, p=p / p.sum()), minlength=2 ** N)

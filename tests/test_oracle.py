@@ -393,3 +393,12 @@ def test_made_sampler_distribution(rng):
     b = np.bincount(rng.choice(2 ** N, size=n, p=q / q.sum()), minlength=2 ** N)
     z = ((np.square(a - b) - a - b) / (a + b + 1e-20)).sum()
     assert z <= np.sqrt(n)
+
+
+def test_made_bf16_round_matches_torch(rng):
+    """The oracle's bfloat16 rounding (ties to even) equals torch's."""
+    import torch
+    from oracle import made as omd
+    x = np.concatenate([rng.normal(scale=10.0, size=5000), [0.0, -0.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9]])
+    ref = torch.as_tensor(x, dtype=torch.float32).bfloat16().double().numpy()
+    assert np.array_equal(omd.bf16_round(x), ref)
