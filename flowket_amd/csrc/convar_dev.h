@@ -14,6 +14,7 @@ struct Prec;
 template <>
 struct Prec<FK_PREC_BF16> {
   using elem = __bf16;
+  using pair = bf16x2;
   using frag = bf16x8;
   using quad = bf16x4;
   static constexpr bool H16 = true;
@@ -27,6 +28,7 @@ struct Prec<FK_PREC_BF16> {
 template <>
 struct Prec<FK_PREC_FP16> {
   using elem = _Float16;
+  using pair = f16x2;
   using frag = f16x8;
   using quad = f16x4;
   static constexpr bool H16 = true;
@@ -69,9 +71,13 @@ __device__ __forceinline__ f32x4 from_quad(typename Prec<PREC>::quad h) {
   return v;
 }
 
+// ReLU on the bit pattern: a float with the sign bit set reads as a negative
+// int32, so max(bits, 0) sends every negative value (and -0) to +0 and keeps
+// the rest -- one integer max, where fmaxf on an MFMA result also costs a
+// NaN-quieting max of the input with itself.
 template <int ACT>
 __device__ __forceinline__ float act1(float x) {
-  if constexpr (ACT == FK_ACT_RELU) return fmaxf(x, 0.0f);
+  if constexpr (ACT == FK_ACT_RELU) return __int_as_float(max(__float_as_int(x), 0));
   else return x > 0.0f ? x : expm1f(x);
 }
 template <int ACT>
@@ -82,6 +88,41 @@ __device__ __forceinline__ f32x4 act4(f32x4 v) {
   r.z = act1<ACT>(v.z);
   r.w = act1<ACT>(v.w);
   return r;
+}
+
+// Two fp32 values -> one dword of two 16-bit elements (round to nearest even).
+template <int PREC>
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+  const f32x2 v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, typename Prec<PREC>::pair));
+}
+// ReLU of two packed 16-bit floats: the sign is the top bit of bf16 and fp16
+// alike, so a packed signed 16-bit max with 0 is the activation.
+__device__ __forceinline__ uint32_t relu2(uint32_t w) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, w), (s16x2)0));
+}
+// act(a), act(b) rounded to 16 bits.  For ReLU the rounding comes first:
+// round-to-nearest is monotone and maps 0 to 0, so relu(round(x)) ==
+// round(relu(x)) for every x (both give +0 for x <= 0), at half the
+// instructions.
+template <int PREC, int ACT>
+__device__ __forceinline__ uint32_t act_pack2(float a, float b) {
+  if constexpr (ACT == FK_ACT_RELU) return relu2(pack2<PREC>(a, b));
+  else return pack2<PREC>(act1<ACT>(a), act1<ACT>(b));
+}
+// act(v) as one quad of 16-bit elements
+template <int PREC, int ACT>
+__device__ __forceinline__ typename Prec<PREC>::quad act_quad(f32x4 v) {
+  const u32x2 w = {act_pack2<PREC, ACT>(v.x, v.y), act_pack2<PREC, ACT>(v.z, v.w)};
+  return __builtin_bit_cast(typename Prec<PREC>::quad, w);
+}
+// act of two D fragments as one 16-bit B fragment (lo -> elements 0..3, hi
+// -> 4..7: the order of d_to_b and of a stored pair of quads)
+template <int PREC, int ACT>
+__device__ __forceinline__ typename Prec<PREC>::frag act_frag(f32x4 lo, f32x4 hi) {
+  const u32x4 w = {act_pack2<PREC, ACT>(lo.x, lo.y), act_pack2<PREC, ACT>(lo.z, lo.w),
+                   act_pack2<PREC, ACT>(hi.x, hi.y), act_pack2<PREC, ACT>(hi.z, hi.w)};
+  return __builtin_bit_cast(typename Prec<PREC>::frag, w);
 }
 
 // D fragments (MT tiles of 4 channels per lane) -> B fragments of the next

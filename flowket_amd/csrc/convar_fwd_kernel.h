@@ -57,6 +57,15 @@ struct Lds {
       *reinterpret_cast<f32x4*>(buf + quad(p, q)) = v;
     }
   }
+  // 16-bit precisions: an already packed quad / fragment
+  template <typename Q>
+  __device__ static __forceinline__ void store_quad_raw(uint8_t* buf, int p, int q, Q v) {
+    *reinterpret_cast<Q*>(buf + quad(p, q)) = v;
+  }
+  template <typename F>
+  __device__ static __forceinline__ void store_frag_at(uint8_t* buf, uint32_t off, F v) {
+    *reinterpret_cast<F*>(buf + off) = v;
+  }
   __device__ static __forceinline__ f32x4 load_quad(const uint8_t* buf, int p, int q) {
     if constexpr (Prec<PREC>::H16) {
       return from_quad<PREC>(*reinterpret_cast<const typename Prec<PREC>::quad*>(buf + quad(p, q)));
@@ -108,21 +117,7 @@ struct Lds {
     }
   }
   // 16-bit precisions: the quads of M-tiles 2j and 2j+1 share the 16-byte
-  // chunk at off ^ 64j, so they are written and read as one access.
-  __device__ static __forceinline__ void store_pair_at(uint8_t* buf, uint32_t off, f32x4 lo, f32x4 hi) {
-    using Q = typename Prec<PREC>::quad;
-    const Q a = to_quad<PREC>(lo), b = to_quad<PREC>(hi);
-    typename Prec<PREC>::frag f;
-    f[0] = a[0];
-    f[1] = a[1];
-    f[2] = a[2];
-    f[3] = a[3];
-    f[4] = b[0];
-    f[5] = b[1];
-    f[6] = b[2];
-    f[7] = b[3];
-    *reinterpret_cast<typename Prec<PREC>::frag*>(buf + off) = f;
-  }
+  // chunk at off ^ 64j, so they are read as one access.
   __device__ static __forceinline__ void load_pair_at(const uint8_t* buf, uint32_t off, f32x4& lo, f32x4& hi) {
     const typename Prec<PREC>::frag f = *reinterpret_cast<const typename Prec<PREC>::frag*>(buf + off);
     lo = f32x4{(float)f[0], (float)f[1], (float)f[2], (float)f[3]};
@@ -431,6 +426,21 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
           wy[mh][ks] = wall[(lay.off_y(blk) + mh * KS + ks) * 64 + lane];
           wx[mh][ks] = wall[(lay.off_x(blk) + mh * KS + ks) * 64 + lane];
         }
+      // biases of the two 1x1 convolutions, read once per block (not per
+      // tile, where each read would stall the epilogue on its load), and --
+      // for 16-bit precisions -- their activations packed, which is what
+      // the DownShift row and the RightShift column receive
+      f32x4 by0[MTH], bx0[MTH];
+      typename Prec<PREC == FK_PREC_FP32 ? FK_PREC_BF16 : PREC>::quad aby[MTH], abx[MTH];
+#pragma unroll
+      for (int mh = 0; mh < MTH; ++mh) {
+        by0[mh] = bias4(by + 16 * mh + 4 * g);
+        bx0[mh] = bias4(bx + 16 * mh + 4 * g);
+        if constexpr (P::H16) {
+          aby[mh] = act_quad<PREC, ACT>(by0[mh]);
+          abx[mh] = act_quad<PREC, ACT>(bx0[mh]);
+        }
+      }
 
       // -- block-0 convolutions on the 1-channel input (VALU)
       auto valu_v = [&](int u, f32x4 (&acc)[MT]) {
@@ -462,23 +472,30 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
       // -- epilogue of the vertical kxk: vertical output + y = act(1x1(
       //    DownShift(act(v')))) written one row down
       auto epi_v = [&](int u, const f32x4 (&acc)[MT]) {
-        f32x4 av[MT];
+        frag bf[KS];
+        if constexpr (P::H16) {
+          // act(v') packed once: the B operand of the y 1x1 and, in the
+          // first and A blocks, the stored vertical output
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) av[mt] = act4<ACT>(acc[mt]);
-        if (inp[u]) {
-          if constexpr (Prec<PREC>::H16) {
+          for (int ks = 0; ks < KS; ++ks) bf[ks] = act_frag<PREC, ACT>(acc[2 * ks], acc[2 * ks + 1]);
+          if (inp[u]) {
 #pragma unroll
-            for (int j = 0; j < MT / 2; ++j) {
+            for (int j = 0; j < KS; ++j) {
               const uint32_t so = toff.get(u, 10) ^ (64u * j);
-              if (kind == BK_FIRST) L::store_pair_at(V0, so, av[2 * j], av[2 * j + 1]);
-              else if (kind == BK_A) L::store_pair_at(V1, so, av[2 * j], av[2 * j + 1]);
+              if (kind == BK_FIRST) L::store_frag_at(V0, so, bf[j]);
+              else if (kind == BK_A) L::store_frag_at(V1, so, bf[j]);
               else if (kind == BK_B) {
                 f32x4 rl, rh;
                 L::load_pair_at(V0, so, rl, rh);
-                L::store_pair_at(V0, so, act4<ACT>(rl + acc[2 * j]), act4<ACT>(rh + acc[2 * j + 1]));
+                L::store_frag_at(V0, so, act_frag<PREC, ACT>(rl + acc[2 * j], rh + acc[2 * j + 1]));
               }
             }
-          } else {
+          }
+        } else {
+          f32x4 av[MT];
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) av[mt] = act4<ACT>(acc[mt]);
+          if (inp[u]) {
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
               const uint32_t so = L::self_quad(toff.get(u, 10), mt);
@@ -487,29 +504,34 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
               else if (kind == BK_B) L::store_quad_at(V0, so, act4<ACT>(L::load_quad_at(V0, so) + acc[mt]));
             }
           }
+          d_to_b<PREC, MT, KS>(av, bf);
         }
-        frag bf[KS];
-        d_to_b<PREC, MT, KS>(av, bf);
         const bool down = pi[u] + 1 < H;
         const int tgt = down ? pix[u] + W : pix[u] - (H - 1) * W;
 #pragma unroll
         for (int mh = 0; mh < MTH; ++mh) {
-          const f32x4 b0 = bias4(by + 16 * mh + 4 * g);
-          f32x4 y = b0;
+          f32x4 y = by0[mh];
 #pragma unroll
           for (int ks = 0; ks < KS; ++ks) y = P::mfma(wy[mh][ks], bf[ks], y);
-          y = act4<ACT>(down ? y : b0);
-          if (inp[u] && 16 * mh + 4 * g < C / 2) L::store_quad(CAT, tgt, C / 8 + 4 * mh + g, y);
+          if (inp[u] && 16 * mh + 4 * g < C / 2) {
+            if constexpr (P::H16) L::store_quad_raw(CAT, tgt, C / 8 + 4 * mh + g, down ? act_quad<PREC, ACT>(y) : aby[mh]);
+            else L::store_quad(CAT, tgt, C / 8 + 4 * mh + g, act4<ACT>(down ? y : by0[mh]));
+          }
         }
       };
       // -- epilogue of the horizontal 1xk: x = act(1x1(act(.))) into CAT (the
       //    final block's RightShift folded into the write target)
       auto epi_h = [&](int u, const f32x4 (&acc)[MT]) {
-        f32x4 ax[MT];
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) ax[mt] = act4<ACT>(acc[mt]);
         frag bf[KS];
-        d_to_b<PREC, MT, KS>(ax, bf);
+        if constexpr (P::H16) {
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) bf[ks] = act_frag<PREC, ACT>(acc[2 * ks], acc[2 * ks + 1]);
+        } else {
+          f32x4 ax[MT];
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) ax[mt] = act4<ACT>(acc[mt]);
+          d_to_b<PREC, MT, KS>(ax, bf);
+        }
         bool keep = true;
         int tgt = pix[u];
         if (kind == BK_FINAL) {
@@ -518,12 +540,13 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
         }
 #pragma unroll
         for (int mh = 0; mh < MTH; ++mh) {
-          const f32x4 b0 = bias4(bx + 16 * mh + 4 * g);
-          f32x4 x = b0;
+          f32x4 x = bx0[mh];
 #pragma unroll
           for (int ks = 0; ks < KS; ++ks) x = P::mfma(wx[mh][ks], bf[ks], x);
-          x = act4<ACT>(keep ? x : b0);
-          if (inp[u] && 16 * mh + 4 * g < C / 2) L::store_quad(CAT, tgt, 4 * mh + g, x);
+          if (inp[u] && 16 * mh + 4 * g < C / 2) {
+            if constexpr (P::H16) L::store_quad_raw(CAT, tgt, 4 * mh + g, keep ? act_quad<PREC, ACT>(x) : abx[mh]);
+            else L::store_quad(CAT, tgt, 4 * mh + g, act4<ACT>(keep ? x : bx0[mh]));
+          }
         }
       };
 
@@ -612,12 +635,12 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
 #pragma unroll
               for (int j = 0; j < MT / 2; ++j) {
                 const uint32_t so = toff.get(u, 10) ^ (64u * j);
-                if (kind == BK_FIRST) L::store_pair_at(H0, so, act4<ACT>(acc[2 * j]), act4<ACT>(acc[2 * j + 1]));
-                else if (kind == BK_A) L::store_pair_at(H1, so, act4<ACT>(acc[2 * j]), act4<ACT>(acc[2 * j + 1]));
+                if (kind == BK_FIRST) L::store_frag_at(H0, so, act_frag<PREC, ACT>(acc[2 * j], acc[2 * j + 1]));
+                else if (kind == BK_A) L::store_frag_at(H1, so, act_frag<PREC, ACT>(acc[2 * j], acc[2 * j + 1]));
                 else {
                   f32x4 rl, rh;
                   L::load_pair_at(H0, so, rl, rh);
-                  L::store_pair_at(H0, so, act4<ACT>(rl + acc[2 * j]), act4<ACT>(rh + acc[2 * j + 1]));
+                  L::store_frag_at(H0, so, act_frag<PREC, ACT>(rl + acc[2 * j], rh + acc[2 * j + 1]));
                 }
               }
             } else {
@@ -634,11 +657,16 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
         }
         // head: 1x1 C->4 on act(h'), complex logits, 2-way log-normalise,
         // gather at the actual spin (abstract_machine.py:34-37,54-57)
-        f32x4 ah[MT];
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) ah[mt] = act4<ACT>(acc[mt]);
         frag bf[KS];
-        d_to_b<PREC, MT, KS>(ah, bf);
+        if constexpr (P::H16) {
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) bf[ks] = act_frag<PREC, ACT>(acc[2 * ks], acc[2 * ks + 1]);
+        } else {
+          f32x4 ah[MT];
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) ah[mt] = act4<ACT>(acc[mt]);
+          d_to_b<PREC, MT, KS>(ah, bf);
+        }
         f32x4 z = hb;
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) z = P::mfma(whd[ks], bf[ks], z);
