@@ -171,29 +171,64 @@ struct TapOff {
   }
 };
 
+// Prefetch depth of a convolution's weights (taps ahead), bounded by the
+// registers one tap's fragments take.
+template <int PREC, int C, int MTO>
+struct WeightDepth {
+  static constexpr int KS = C / Prec<PREC>::KW;
+  static constexpr int FR = (int)(sizeof(typename Prec<PREC>::frag) / 4);
+  static constexpr int WREGS = MTO * KS * FR;
+  static constexpr int WD = WREGS <= 16 ? 2 : (WREGS <= 32 ? 1 : 0);
+};
+
+// The head of a convolution pass -- its bias and the weights of its first WD
+// taps -- requested one phase early (during the previous phase's epilogue,
+// across the barrier), so that a pass starts without waiting on L2.
+template <int PREC, int C, int MTO>
+struct PassHead {
+  using frag = typename Prec<PREC>::frag;
+  static constexpr int KS = C / Prec<PREC>::KW;
+  static constexpr int WD = WeightDepth<PREC, C, MTO>::WD;
+  frag w[WD > 0 ? WD : 1][MTO][KS];
+  f32x4 bias[MTO];
+  __device__ __forceinline__ void load_bias(const float* b, int g) {
+#pragma unroll
+    for (int mt = 0; mt < MTO; ++mt) bias[mt] = *reinterpret_cast<const f32x4*>(b + 16 * mt + 4 * g);
+  }
+  __device__ __forceinline__ void load(const frag* __restrict__ wg, const float* b, int lane, int g) {
+#pragma unroll
+    for (int t = 0; t < WD; ++t)
+#pragma unroll
+      for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) w[t][mt][ks] = wg[((t * MTO + mt) * KS + ks) * 64 + lane];
+    load_bias(b, g);
+  }
+};
+
 // acc[u][mt] += sum over taps of W[tap][mt] x B(pixel + tap offset).
 // toff.get(u, k) is this lane's byte offset of k-step 0 at tap k of tile u,
 // pointing at the all-zero pixel for padding taps -- tiles past
 // the wave's count hold only such offsets, so every tile is computed without
 // a branch (their epilogues are skipped).
 // Software pipeline: the weights (read from L2) are requested WD taps ahead
-// and the B fragments of all tiles BD taps ahead (depths bounded by the
-// registers they take); scheduling fences keep the compiler from sinking
-// those requests next to their first use.
+// -- the first WD taps already by the caller, in hd -- and the B fragments
+// of all tiles BD taps ahead (depths bounded by the registers they take);
+// scheduling fences keep the compiler from sinking those requests next to
+// their first use.
 template <int PREC, int C, int MAXT, int KIND, int MTO>
 __device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t* buf,
                                           const typename Prec<PREC>::frag* __restrict__ wg,
-                                          const TapOff<MAXT>& toff, int lane) {
+                                          const TapOff<MAXT>& toff, int lane, const PassHead<PREC, C, MTO>& hd) {
   using P = Prec<PREC>;
   using frag = typename P::frag;
   using L = Lds<PREC, C>;
   constexpr int KS = C / P::KW;
   constexpr int NTAP = Taps<KIND>::N;
-  // prefetch depth bounded by the registers a tap's fragments take
   constexpr int FR = (int)(sizeof(frag) / 4);
-  constexpr int WREGS = MTO * KS * FR, BREGS = MAXT * KS * FR;
-  constexpr int WD = WREGS <= 16 ? 2 : (WREGS <= 32 ? 1 : 0);  // weight prefetch distance (taps)
-  constexpr int BD = BREGS <= 32 ? 1 : 0;                       // B prefetch distance (taps)
+  constexpr int BREGS = MAXT * KS * FR;
+  constexpr int WD = WeightDepth<PREC, C, MTO>::WD;  // weight prefetch distance (taps)
+  constexpr int BD = BREGS <= 32 ? 1 : 0;             // B prefetch distance (taps)
   frag w[WD + 1][MTO][KS];
   frag b[BD + 1][MAXT][KS];
   auto load_w = [&](int tap, frag (&dst)[MTO][KS]) {
@@ -207,7 +242,11 @@ __device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t
     for (int u = 0; u < MAXT; ++u) L::template load_bfrags_at<KS>(buf, toff.get(u, Taps<KIND>::bit(tap)), dst[u]);
   };
 #pragma unroll
-  for (int t = 0; t < WD && t < NTAP; ++t) load_w(t, w[t]);
+  for (int t = 0; t < WD && t < NTAP; ++t)
+#pragma unroll
+    for (int mt = 0; mt < MTO; ++mt)
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) w[t][mt][ks] = hd.w[t][mt][ks];
   if constexpr (BD > 0) load_b(0, b[0]);
 #pragma unroll
   for (int tap = 0; tap < NTAP; ++tap) {
@@ -377,6 +416,11 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   const int64_t ngroups = (M + T - 1) / T;
   const int nb = a.nblocks;
   const frag* wall = reinterpret_cast<const frag*>(a.w);
+
+  // head of the next vertical pass (block 0 convolves the 1-channel input on
+  // VALU: bias only)
+  PassHead<PREC, C, MT> hv;
+  hv.load_bias(a.aux + lay.aux_bias(0), g);
 
   for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     // ---- configurations -> LDS (int8 spins; connection applied on the fly)
@@ -551,6 +595,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
       };
 
       // ================= phase VH: vertical stack + horizontal 1xk =========
+      PassHead<PREC, C, MT> hc;  // head of this block's h3 pass
       if constexpr (TILE_OUTER) {
         {
           ConvW<PREC, C, 0, MT> wv;
@@ -583,18 +628,21 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
           }
         }
       } else {
+        PassHead<PREC, C, MT> hh;
         {
           f32x4 acc[MAXT][MT];
 #pragma unroll
           for (int u = 0; u < MAXT; ++u)
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + 16 * mt + 4 * g);
+            for (int mt = 0; mt < MT; ++mt) acc[u][mt] = hv.bias[mt];
           if (kind == BK_FIRST) {
 #pragma unroll
             for (int u = 0; u < MAXT; ++u)
               if (u < nt) valu_v(u, acc[u]);
+            hh.load_bias(bias + C, g);
           } else {
-            conv_pass<PREC, C, MAXT, 0, MT>(acc, Vin, wall + lay.off_v(blk) * 64, toff, lane);
+            conv_pass<PREC, C, MAXT, 0, MT>(acc, Vin, wall + lay.off_v(blk) * 64, toff, lane, hv);
+            hh.load(wall + lay.off_h(blk) * 64, bias + C, lane, g);
           }
 #pragma unroll
           for (int u = 0; u < MAXT; ++u)
@@ -605,14 +653,15 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
 #pragma unroll
           for (int u = 0; u < MAXT; ++u)
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + C + 16 * mt + 4 * g);
+            for (int mt = 0; mt < MT; ++mt) acc[u][mt] = hh.bias[mt];
           if (kind == BK_FIRST) {
 #pragma unroll
             for (int u = 0; u < MAXT; ++u)
               if (u < nt) valu_h(u, acc[u]);
           } else {
-            conv_pass<PREC, C, MAXT, 1, MT>(acc, Hin, wall + lay.off_h(blk) * 64, toff, lane);
+            conv_pass<PREC, C, MAXT, 1, MT>(acc, Hin, wall + lay.off_h(blk) * 64, toff, lane, hh);
           }
+          hc.load(wall + lay.off_c(blk) * 64, bias + 3 * C, lane, g);
 #pragma unroll
           for (int u = 0; u < MAXT; ++u)
             if (u < nt) epi_h(u, acc[u]);
@@ -705,8 +754,11 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
 #pragma unroll
         for (int u = 0; u < MAXT; ++u)
 #pragma unroll
-          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = bias4(bias + 3 * C + 16 * mt + 4 * g);
-        conv_pass<PREC, C, MAXT, 2, MT>(acc, CAT, wall + lay.off_c(blk) * 64, toff, lane);
+          for (int mt = 0; mt < MT; ++mt) acc[u][mt] = hc.bias[mt];
+        conv_pass<PREC, C, MAXT, 2, MT>(acc, CAT, wall + lay.off_c(blk) * 64, toff, lane, hc);
+        // the next block's vertical pass (block 0 of the next group: bias only)
+        if (blk + 1 < nb) hv.load(wall + lay.off_v(blk + 1) * 64, a.aux + lay.aux_bias(blk + 1), lane, g);
+        else hv.load_bias(a.aux + lay.aux_bias(0), g);
 #pragma unroll
         for (int u = 0; u < MAXT; ++u)
           if (u < nt) epi_c(u, acc[u]);
