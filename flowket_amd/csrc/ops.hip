@@ -2,6 +2,7 @@
 // Integer/byte and small-FP64 work: HBM/latency bound, no MFMA.
 #include "fk_common.h"
 
+#include <algorithm>
 #include <cstdarg>
 #include <string>
 
@@ -25,38 +26,46 @@ int device_cu_count() {
   return cus;
 }
 
-// One workgroup per sample.  Connection slot c of sample b follows
-// HeisenbergFindConn.calc_conn_and_mel (operators/heisenberg.py:84-121),
-// Ising.find_conn (operators/ising.py:17-46) or the J1J2 bond operator
-// (operators/j1j2.py:37-55 through operators/netket_operator.py:46-66).
-__global__ __launch_bounds__(256) void find_conn_kernel(
-    const int8_t* __restrict__ spins, int64_t batch, int n_sites, int kind,
+// Connection slot c of sample b follows HeisenbergFindConn.calc_conn_and_mel
+// (operators/heisenberg.py:84-121), Ising.find_conn (operators/ising.py:17-46)
+// or the J1J2 bond operator (operators/j1j2.py:37-55 through
+// operators/netket_operator.py:46-66).
+// One workgroup per tile of up to 64 consecutive samples, one lane per
+// sample: for each connection slot c a wave writes mel[c][b0..] and
+// use[c][b0..] as one contiguous run (the slot-major layout of the dense
+// contract), the slots shared out over the waves.
+// Diagonal element: sequential double accumulation in table order per sample
+// (exact for the integer/dyadic couplings of the reference models, and the
+// same order as the reference's loop otherwise).
+__global__ __launch_bounds__(512) void find_conn_kernel(
+    const int8_t* __restrict__ spins, int64_t batch, int n_sites, int tile, int kind,
     const int32_t* __restrict__ conn_a, const int32_t* __restrict__ conn_b,
     const double* __restrict__ conn_mel, int n_conn,
     const int32_t* __restrict__ diag_a, const int32_t* __restrict__ diag_b,
     const double* __restrict__ diag_j, int n_diag,
     int8_t* __restrict__ conn_out, double* __restrict__ mel_out,
     uint8_t* __restrict__ use_out, int32_t* __restrict__ n_used_out) {
-  extern __shared__ int8_t s_spin[];  // n_sites spins + per-connection use flags
-  int8_t* s_use = s_spin + ((n_sites + 15) & ~15);
-  __shared__ int s_count[4];
-  const int64_t b = blockIdx.x;
-  const int tid = threadIdx.x;
-  for (int q = tid; q < n_sites; q += blockDim.x) s_spin[q] = spins[b * n_sites + q];
+  extern __shared__ int8_t s_spin[];  // tile rows of n_sites spins
+  __shared__ int s_cnt[8][64];
+  const int64_t b0 = (int64_t)blockIdx.x * tile;
+  const int nb = (int)((batch - b0) < tile ? (batch - b0) : tile);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, NW = blockDim.x >> 6;
+  for (int q = tid; q < nb * n_sites; q += blockDim.x) s_spin[q] = spins[b0 * n_sites + q];
   __syncthreads();
-
-  // Diagonal element: sequential double accumulation in table order on one
-  // thread (exact for the integer/dyadic couplings of the reference models).
-  if (tid == 0) {
+  const bool ok = lane < nb;
+  const int8_t* row = s_spin + (ok ? lane : 0) * n_sites;
+  const int64_t b = b0 + lane;
+  int local = 0;
+  if (wave == 0) {
     double d = 0.0;
-    for (int k = 0; k < n_diag; ++k)
-      d += diag_j[k] * (double)s_spin[diag_a[k]] * (double)s_spin[diag_b[k]];
-    mel_out[b] = d;
-    use_out[b] = 1;
-    s_use[0] = 1;
+    for (int k = 0; k < n_diag; ++k) d += diag_j[k] * (double)row[diag_a[k]] * (double)row[diag_b[k]];
+    if (ok) {
+      mel_out[b] = d;
+      use_out[b] = 1;
+    }
+    local = 1;
   }
-  int local = (tid == 0) ? 1 : 0;
-  for (int c = 1 + tid; c < n_conn; c += blockDim.x) {
+  for (int c = 1 + wave; c < n_conn; c += NW) {
     const int a = conn_a[c];
     int u = 0;
     double m = 0.0;
@@ -65,63 +74,78 @@ __global__ __launch_bounds__(256) void find_conn_kernel(
       m = u ? conn_mel[c] : 0.0;
     } else if (a >= 0) {
       const int bb = conn_b[c];
-      const int anti = s_spin[a] != s_spin[bb];
+      const int anti = row[a] != row[bb];
       m = anti ? conn_mel[c] : 0.0;
       u = anti && (m != 0.0);
     }
-    mel_out[(int64_t)c * batch + b] = m;
-    use_out[(int64_t)c * batch + b] = (uint8_t)u;
-    s_use[c] = (int8_t)u;
+    if (ok) {
+      mel_out[(int64_t)c * batch + b] = m;
+      use_out[(int64_t)c * batch + b] = (uint8_t)u;
+    }
     local += u;
   }
-  // used count (integers: any order is exact)
-  for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
-  if ((tid & 63) == 0) s_count[tid >> 6] = local;
+  s_cnt[wave][lane] = local;
   __syncthreads();
-  if (tid == 0 && n_used_out) {
+  if (wave == 0 && ok && n_used_out) {
     int t = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_count[w];
+    for (int w = 0; w < NW; ++w) t += s_cnt[w][lane];
     n_used_out[b] = t;
   }
   if (!conn_out) return;
-  // Dense connected configurations, slot-major (n_conn, batch, n_sites).
-  const int64_t total = (int64_t)n_conn * n_sites;
-  for (int64_t idx = tid; idx < total; idx += blockDim.x) {
-    const int c = (int)(idx / n_sites);
-    const int q = (int)(idx - (int64_t)c * n_sites);
-    int8_t v = s_spin[q];
-    if (c > 0) {
-      const int a = conn_a[c];
-      if (kind == FK_CONN_FLIP) {
-        if (q == a) v = (int8_t)-v;
-      } else if (a >= 0) {
-        const int bb = conn_b[c];
-        if (kind == FK_CONN_SWAP_ZERO_PAD && !s_use[c]) {
+  // Dense connected configurations, slot-major (n_conn, batch, n_sites): the
+  // tile's rows of one slot are one contiguous run.
+  const int run = nb * n_sites;
+  for (int c = 0; c < n_conn; ++c) {
+    const int a = c > 0 ? conn_a[c] : -1;
+    const int bb = (a >= 0 && kind != FK_CONN_FLIP) ? conn_b[c] : -1;
+    const bool has_mel = c > 0 && conn_mel[c] != 0.0;
+    int8_t* dst = conn_out + ((int64_t)c * batch + b0) * n_sites;
+    for (int idx = tid; idx < run; idx += blockDim.x) {
+      const int l = idx / n_sites, q = idx - l * n_sites;
+      const int8_t* r = s_spin + l * n_sites;
+      int8_t v = r[q];
+      if (c > 0) {
+        if (kind == FK_CONN_FLIP) {
+          if (q == a) v = (int8_t)-v;
+        } else if (a >= 0) {
+          if (kind == FK_CONN_SWAP_ZERO_PAD && !(r[a] != r[bb] && has_mel)) {
+            v = 0;
+          } else {
+            if (q == a) v = r[bb];
+            else if (q == bb) v = r[a];
+          }
+        } else if (kind == FK_CONN_SWAP_ZERO_PAD) {
           v = 0;
-        } else {
-          if (q == a) v = s_spin[bb];
-          else if (q == bb) v = s_spin[a];
         }
-      } else if (kind == FK_CONN_SWAP_ZERO_PAD) {
-        v = 0;
       }
+      dst[idx] = v;
     }
-    conn_out[((int64_t)c * batch + b) * n_sites + q] = v;
   }
 }
 
 __global__ void zero_i64_kernel(int64_t* p) { *p = 0; }
 
-// counts[b] -> offsets[b+1]; one thread per sample reads use[c][b] (coalesced
-// over b for each c).
-__global__ void count_used_kernel(const uint8_t* __restrict__ use, int64_t batch, int n_conn,
-                                  int64_t* __restrict__ offsets) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= batch) return;
-  int64_t t = 0;
-  for (int c = 0; c < n_conn; ++c) t += use[(int64_t)c * batch + b] ? 1 : 0;
-  offsets[b + 1] = t;
-  if (b == 0) offsets[0] = 0;
+// counts[b] -> offsets[b+1]: a workgroup per 64 samples, lane = sample, the
+// slots shared out over the waves (64-byte coalesced reads of use[c][b0..]),
+// partial counts summed through LDS.
+__global__ __launch_bounds__(256) void count_used_kernel(const uint8_t* __restrict__ use, int64_t batch, int n_conn,
+                                                         int64_t* __restrict__ offsets) {
+  __shared__ int s_cnt[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, NW = blockDim.x >> 6;
+  const int64_t b = (int64_t)blockIdx.x * 64 + lane;
+  int t = 0;
+  if (b < batch) {
+#pragma unroll 4
+    for (int c = wave; c < n_conn; c += NW) t += use[(int64_t)c * batch + b] ? 1 : 0;
+  }
+  s_cnt[wave][lane] = t;
+  __syncthreads();
+  if (wave == 0 && b < batch) {
+    int64_t n = 0;
+    for (int w = 0; w < NW; ++w) n += s_cnt[w][lane];
+    offsets[b + 1] = n;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) offsets[0] = 0;
 }
 
 // In-place inclusive scan of offsets[1..batch] by a single workgroup.
@@ -157,12 +181,29 @@ __global__ __launch_bounds__(1024) void scan_offsets_kernel(int64_t* __restrict_
   }
 }
 
-__global__ void write_pairs_kernel(const uint8_t* __restrict__ use, int64_t batch, int n_conn,
-                                   const int64_t* __restrict__ offsets, int32_t* __restrict__ pairs) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// pairs of sample b at offsets[b]...: a workgroup per 64 samples, lane =
+// sample, wave w takes the slot range [w*n_conn/NW, (w+1)*n_conn/NW); each
+// wave counts its range, an exclusive prefix over the waves places it, and
+// the pairs come out in (sample, slot) order.
+__global__ __launch_bounds__(256) void write_pairs_kernel(const uint8_t* __restrict__ use, int64_t batch, int n_conn,
+                                                          const int64_t* __restrict__ offsets,
+                                                          int32_t* __restrict__ pairs) {
+  __shared__ int s_cnt[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, NW = blockDim.x >> 6;
+  const int64_t b = (int64_t)blockIdx.x * 64 + lane;
+  const int c0 = (int)((int64_t)n_conn * wave / NW), c1 = (int)((int64_t)n_conn * (wave + 1) / NW);
+  int t = 0;
+  if (b < batch) {
+#pragma unroll 4
+    for (int c = c0; c < c1; ++c) t += use[(int64_t)c * batch + b] ? 1 : 0;
+  }
+  s_cnt[wave][lane] = t;
+  __syncthreads();
   if (b >= batch) return;
   int64_t k = offsets[b];
-  for (int c = 0; c < n_conn; ++c) {
+  for (int w = 0; w < wave; ++w) k += s_cnt[w][lane];
+#pragma unroll 4
+  for (int c = c0; c < c1; ++c) {
     if (use[(int64_t)c * batch + b]) {
       pairs[2 * k] = (int32_t)b;
       pairs[2 * k + 1] = c;
@@ -181,16 +222,22 @@ __device__ __forceinline__ void cexp_diff(float dr, float di, double& er, double
   ei = (double)(m * s);
 }
 
-__global__ void local_energy_kernel(const int64_t* __restrict__ offsets, const int32_t* __restrict__ pairs,
-                                    const float* __restrict__ logpsi, const double* __restrict__ mel,
-                                    int64_t batch, double* __restrict__ eloc) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per sample, lanes over its used connections (k0 is slot 0, the
+// sample itself); the FP64 partial sums are reduced across the wave in a
+// fixed order.
+__global__ __launch_bounds__(256) void local_energy_kernel(const int64_t* __restrict__ offsets,
+                                                           const int32_t* __restrict__ pairs,
+                                                           const float* __restrict__ logpsi,
+                                                           const double* __restrict__ mel, int64_t batch,
+                                                           double* __restrict__ eloc) {
+  const int lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (b >= batch) return;
   const int64_t k0 = offsets[b], k1 = offsets[b + 1];
   double accr = 0.0, acci = 0.0;
   if (k1 > k0) {
     const float r0 = logpsi[2 * k0], i0 = logpsi[2 * k0 + 1];
-    for (int64_t k = k0; k < k1; ++k) {
+    for (int64_t k = k0 + lane; k < k1; k += 64) {
       const int c = pairs[2 * k + 1];
       double er, ei;
       cexp_diff(logpsi[2 * k] - r0, logpsi[2 * k + 1] - i0, er, ei);
@@ -199,8 +246,15 @@ __global__ void local_energy_kernel(const int64_t* __restrict__ offsets, const i
       acci += m * ei;
     }
   }
-  eloc[2 * b] = accr;
-  eloc[2 * b + 1] = acci;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    accr += __shfl_xor(accr, off, 64);
+    acci += __shfl_xor(acci, off, 64);
+  }
+  if (lane == 0) {
+    eloc[2 * b] = accr;
+    eloc[2 * b + 1] = acci;
+  }
 }
 
 __global__ void local_energy_dense_kernel(const float* __restrict__ logpsi, const double* __restrict__ mel,
@@ -295,11 +349,12 @@ int fk_find_conn(const int8_t* spins, int64_t batch, int32_t n_sites, int32_t ki
   FK_REQUIRE(n_diag == 0 || (diag_a && diag_b && diag_j), "fk_find_conn: null diagonal table");
   if (batch == 0) return FK_OK;
   FK_REQUIRE(batch <= 0x7fffffff, "fk_find_conn: batch too large");
-  const size_t lds = ((n_sites + 15) & ~15) + ((n_conn + 15) & ~15);
-  FK_REQUIRE(lds <= 65536, "fk_find_conn: table too large for LDS");
-  hipLaunchKernelGGL(find_conn_kernel, dim3((unsigned)batch), dim3(256), lds, (hipStream_t)stream, spins, batch,
-                     n_sites, kind, conn_a, conn_b, conn_mel, n_conn, diag_a, diag_b, diag_j, n_diag, conn_out,
-                     mel_out, use_out, n_used_out);
+  // tile of samples per workgroup: their spin rows fill at most 64 KB of LDS
+  const int tile = (int)std::min<int64_t>(64, 65536 / n_sites);
+  const size_t lds = (size_t)tile * n_sites;
+  hipLaunchKernelGGL(find_conn_kernel, dim3((unsigned)div_up(batch, tile)), dim3(512), lds, (hipStream_t)stream,
+                     spins, batch, n_sites, tile, kind, conn_a, conn_b, conn_mel, n_conn, diag_a, diag_b, diag_j,
+                     n_diag, conn_out, mel_out, use_out, n_used_out);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }
@@ -322,11 +377,11 @@ int fk_compact_connections(const uint8_t* use, int64_t batch, int32_t n_conn, in
   }
   FK_REQUIRE(use && pairs, "fk_compact_connections: null pointer");
   const int tb = 256;
-  hipLaunchKernelGGL(count_used_kernel, dim3(div_up(batch, tb)), dim3(tb), 0, s, use, batch, n_conn, offsets);
+  hipLaunchKernelGGL(count_used_kernel, dim3(div_up(batch, 64)), dim3(tb), 0, s, use, batch, n_conn, offsets);
   FK_CHECK_LAUNCH();
   hipLaunchKernelGGL(scan_offsets_kernel, dim3(1), dim3(1024), 0, s, offsets, batch);
   FK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(write_pairs_kernel, dim3(div_up(batch, tb)), dim3(tb), 0, s, use, batch, n_conn, offsets,
+  hipLaunchKernelGGL(write_pairs_kernel, dim3(div_up(batch, 64)), dim3(tb), 0, s, use, batch, n_conn, offsets,
                      pairs);
   FK_CHECK_LAUNCH();
   return FK_OK;
@@ -336,7 +391,7 @@ int fk_local_energy(const int64_t* offsets, const int32_t* pairs, const float* l
                     int64_t batch, double* eloc_out, void* stream) {
   FK_REQUIRE(offsets && pairs && logpsi && mel && eloc_out, "fk_local_energy: null pointer");
   if (batch <= 0) return FK_OK;
-  hipLaunchKernelGGL(local_energy_kernel, dim3(div_up(batch, 256)), dim3(256), 0, (hipStream_t)stream, offsets,
+  hipLaunchKernelGGL(local_energy_kernel, dim3(div_up(batch, 4)), dim3(256), 0, (hipStream_t)stream, offsets,
                      pairs, logpsi, mel, batch, eloc_out);
   FK_CHECK_LAUNCH();
   return FK_OK;
