@@ -188,7 +188,9 @@ template <int PREC, int C, int MTO>
 struct PassHead {
   using frag = typename Prec<PREC>::frag;
   static constexpr int KS = C / Prec<PREC>::KW;
-  static constexpr int WD = WeightDepth<PREC, C, MTO>::WD;
+  // taps whose weights travel in the head: the 64-channel variants have no
+  // registers to spare across an epilogue, so their passes fetch their own
+  static constexpr int WD = C <= 32 ? WeightDepth<PREC, C, MTO>::WD : 0;
   frag w[WD > 0 ? WD : 1][MTO][KS];
   f32x4 bias[MTO];
   __device__ __forceinline__ void load_bias(const float* b, int g) {
@@ -212,10 +214,10 @@ struct PassHead {
 // the wave's count hold only such offsets, so every tile is computed without
 // a branch (their epilogues are skipped).
 // Software pipeline: the weights (read from L2) are requested WD taps ahead
-// -- the first WD taps already by the caller, in hd -- and the B fragments
-// of all tiles BD taps ahead (depths bounded by the registers they take);
-// scheduling fences keep the compiler from sinking those requests next to
-// their first use.
+// -- the first taps already by the caller, in hd, where the head carries
+// them -- and the B fragments of all tiles BD taps ahead (depths bounded by
+// the registers they take); scheduling fences keep the compiler from sinking
+// those requests next to their first use.
 template <int PREC, int C, int MAXT, int KIND, int MTO>
 __device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t* buf,
                                           const typename Prec<PREC>::frag* __restrict__ wg,
@@ -242,11 +244,16 @@ __device__ __forceinline__ void conv_pass(f32x4 (&acc)[MAXT][MTO], const uint8_t
     for (int u = 0; u < MAXT; ++u) L::template load_bfrags_at<KS>(buf, toff.get(u, Taps<KIND>::bit(tap)), dst[u]);
   };
 #pragma unroll
-  for (int t = 0; t < WD && t < NTAP; ++t)
+  for (int t = 0; t < WD && t < NTAP; ++t) {
+    if (t < PassHead<PREC, C, MTO>::WD) {
 #pragma unroll
-    for (int mt = 0; mt < MTO; ++mt)
+      for (int mt = 0; mt < MTO; ++mt)
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) w[t][mt][ks] = hd.w[t][mt][ks];
+        for (int ks = 0; ks < KS; ++ks) w[t][mt][ks] = hd.w[t][mt][ks];
+    } else {
+      load_w(t, w[t]);
+    }
+  }
   if constexpr (BD > 0) load_b(0, b[0]);
 #pragma unroll
   for (int tap = 0; tap < NTAP; ++tap) {
