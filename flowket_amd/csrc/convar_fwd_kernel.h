@@ -357,9 +357,11 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
     H1 = H0 + BUFB;
     sp = reinterpret_cast<int8_t*>(H1 + BUFB);
   }
-  // per-site log-probabilities of the final block: V1 is neither read nor
-  // written by the final block, so they reuse its space
+  // per-site log-probabilities (and, when asked for, the conditionals) of
+  // the final block: V1 is neither read nor written by the final block, so
+  // they reuse its space
   float2* part = reinterpret_cast<float2*>(V1);
+  float2* cpart = part + NPX;
 
   const int lane = threadIdx.x & 63;
   // wave index and tile count are wave-uniform: keep them in SGPRs so tile
@@ -430,34 +432,37 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   hv.load_bias(a.aux + lay.aux_bias(0), g);
 
   for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-    // ---- configurations -> LDS (int8 spins; connection applied on the fly)
-    for (int p = threadIdx.x; p < NPX; p += blockDim.x) {
-      int8_t s = 0;
-      const int t = p / HW;
+    // ---- configurations -> LDS (int8 spins; connection applied on the fly):
+    //      one wave per configuration, so the (sample, connection) lookup is
+    //      wave-uniform and the lanes read the spin row
+    for (int t = wave; t < T; t += NW) {
       const int64_t m = grp * T + t;
-      if (m < M) {
-        const int q = p - t * HW;
-        int64_t b = m;
-        int c = 0;
-        if (a.pairs) {
-          b = a.pairs[2 * m];
-          c = a.pairs[2 * m + 1];
-        }
-        const int8_t* row = a.spins + b * HW;
-        int8_t v = row[q];
-        if (c > 0) {
-          const int ca = a.conn_a[c];
+      const bool live = m < M;
+      int64_t b = m;
+      int c = 0;
+      if (live && a.pairs) {
+        b = a.pairs[2 * m];
+        c = a.pairs[2 * m + 1];
+      }
+      int ca = -1, cb = -1;
+      if (live && c > 0) {
+        ca = a.conn_a[c];
+        if (a.conn_kind != FK_CONN_FLIP && ca >= 0) cb = a.conn_b[c];
+      }
+      const int8_t* row = a.spins + (live ? b : 0) * HW;
+      for (int q = lane; q < HW; q += 64) {
+        int8_t v = 0;
+        if (live) {
+          v = row[q];
           if (a.conn_kind == FK_CONN_FLIP) {
             if (q == ca) v = (int8_t)-v;
           } else if (ca >= 0) {
-            const int cb = a.conn_b[c];
             if (q == ca) v = row[cb];
             else if (q == cb) v = row[ca];
           }
         }
-        s = v;
+        sp[t * HW + q] = v;
       }
-      sp[p] = s;
     }
     __syncthreads();
 
@@ -732,15 +737,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
           const float lse = mx + 0.5f * log1pf(expf(-2.0f * fabsf(z.x - z.y)));
           const bool up = sp[pix[u]] > 0;
           part[pix[u]] = make_float2((up ? z.x : z.y) - lse, up ? z.z : z.w);
-          if (a.cond) {
-            const int t = pix[u] / HW;
-            const int64_t m = grp * T + t;
-            if (m < M) {
-              float* o = a.cond + (m * HW + (pix[u] - t * HW)) * 2;
-              o[0] = 2.0f * (z.x - lse);
-              o[1] = 2.0f * (z.y - lse);
-            }
-          }
+          if (a.cond) cpart[pix[u]] = make_float2(2.0f * (z.x - lse), 2.0f * (z.y - lse));
         }
       };
       if constexpr (TILE_OUTER) {
@@ -790,6 +787,14 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
       if (lane == 0 && m < M) {
         a.logpsi[2 * m] = sr;
         a.logpsi[2 * m + 1] = si;
+      }
+      if (a.cond && m < M) {
+        for (int q = lane; q < HW; q += 64) {
+          const float2 c = cpart[t * HW + q];
+          float* o = a.cond + (m * HW + q) * 2;
+          o[0] = c.x;
+          o[1] = c.y;
+        }
       }
     }
     __syncthreads();

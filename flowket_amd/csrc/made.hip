@@ -154,46 +154,89 @@ __global__ __launch_bounds__(MADE_T) void made_backward_kernel(MadeW w, const in
   }
 }
 
-// Masked gradients (mean over the batch) and one Adam step on the FP32
-// master weights; the BF16 copies are refreshed.  One thread per parameter
-// of the packed master vector [w1 (Hd*N) | b1 | w2t (Hd*Hd) | b2 | w3 (N*Hd) | b3].
+// Weight gradients as tiled products over the batch: part[r][c] =
+// sum_{b in this split} A[b][r] * Bm[b][c] (Bm = +-1 spins when xs is given,
+// ones when both are null: the bias sums).  A 64x64 output tile per
+// workgroup, 4x4 per thread, the batch in chunks of 16 staged through LDS;
+// blockIdx.z is the batch split, whose partial sums are added in split
+// order by the Adam step (deterministic).
+constexpr int WG_TILE = 64, WG_KC = 16;
+__global__ __launch_bounds__(256) void made_wgrad_kernel(const float* __restrict__ A, int lda,
+                                                         const float* __restrict__ Bm, const int8_t* __restrict__ xs,
+                                                         int ldb, int rows, int cols, int64_t B, int64_t per_split,
+                                                         float* __restrict__ out, int64_t split_stride) {
+  __shared__ float sa[WG_KC][WG_TILE];
+  __shared__ float sb[WG_KC][WG_TILE];
+  const int r0 = blockIdx.y * WG_TILE, c0 = blockIdx.x * WG_TILE;
+  const int64_t k0 = (int64_t)blockIdx.z * per_split;
+  const int64_t k1 = (k0 + per_split < B) ? k0 + per_split : B;
+  const int tid = threadIdx.x, tr = tid / 16, tc = tid % 16;
+  float acc[4][4] = {};
+  for (int64_t kb = k0; kb < k1; kb += WG_KC) {
+    for (int idx = tid; idx < WG_KC * WG_TILE; idx += 256) {
+      const int kk = idx / WG_TILE, j = idx % WG_TILE;
+      const int64_t b = kb + kk;
+      const bool kin = b < k1;
+      sa[kk][j] = (kin && r0 + j < rows) ? A[b * lda + r0 + j] : 0.0f;
+      float v = 0.0f;
+      if (kin && c0 + j < cols) v = xs ? (float)xs[b * ldb + c0 + j] : (Bm ? Bm[b * ldb + c0 + j] : 1.0f);
+      sb[kk][j] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < WG_KC; ++kk) {
+      float av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        av[i] = sa[kk][tr + 16 * i];
+        bv[i] = sb[kk][tc + 16 * i];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+    }
+    __syncthreads();
+  }
+  float* o = out + blockIdx.z * split_stride;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = r0 + tr + 16 * i, c = c0 + tc + 16 * j;
+      if (r < rows && c < cols) o[(int64_t)r * cols + c] = acc[i][j];
+    }
+}
+
+// Masked gradients (mean over the batch, from the split partial sums) and
+// one Adam step on the FP32 master weights; the BF16 copies are refreshed.
+// One thread per parameter of the packed master vector
+// [w1 (Hd*N) | b1 | w2t (Hd*Hd) | b2 | w3 (N*Hd) | b3].
 __global__ void made_adam_kernel(float* __restrict__ master, float* __restrict__ m1, float* __restrict__ m2,
                                  __bf16* __restrict__ w1, __bf16* __restrict__ w2t, __bf16* __restrict__ w3,
-                                 const int8_t* __restrict__ x, const float* __restrict__ g_h1,
-                                 const float* __restrict__ g_h2, const float* __restrict__ g_dl,
-                                 const float* __restrict__ g_dh2, const float* __restrict__ g_dh1, int64_t B, int N,
-                                 int Hd, float lr, float beta1, float beta2, float eps, float bc1, float bc2) {
+                                 const float* __restrict__ part, int splits, int64_t B, int N, int Hd, float lr,
+                                 float beta1, float beta2, float eps, float bc1, float bc2) {
   const int64_t n1 = (int64_t)Hd * N, n2 = (int64_t)Hd * Hd, n3 = (int64_t)N * Hd;
   const int64_t total = n1 + Hd + n2 + Hd + n3 + N;
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= total) return;
-  double g = 0.0;
   bool masked = false;
   int64_t r = k;
   if (r < n1) {  // w1[h][i]
     const int h = (int)(r / N), i = (int)(r % N);
     masked = !(i + 1 <= deg_hidden(h, N));
-    if (!masked)
-      for (int64_t b = 0; b < B; ++b) g += (double)g_dh1[b * Hd + h] * (double)x[b * N + i];
   } else if ((r -= n1) < Hd) {  // b1
-    for (int64_t b = 0; b < B; ++b) g += g_dh1[b * Hd + r];
   } else if ((r -= Hd) < n2) {  // w2t[kk][t] = W2[t][kk]
     const int kk = (int)(r / Hd), t = (int)(r % Hd);
     masked = deg_hidden(kk, N) > deg_hidden(t, N);
-    if (!masked)
-      for (int64_t b = 0; b < B; ++b) g += (double)g_dh2[b * Hd + t] * (double)g_h1[b * Hd + kk];
   } else if ((r -= n2) < Hd) {  // b2
-    for (int64_t b = 0; b < B; ++b) g += g_dh2[b * Hd + r];
   } else if ((r -= Hd) < n3) {  // w3[o][h]
     const int o = (int)(r / Hd), h = (int)(r % Hd);
     masked = deg_hidden(h, N) > o;
-    if (!masked)
-      for (int64_t b = 0; b < B; ++b) g += (double)g_dl[b * N + o] * (double)g_h2[b * Hd + h];
-  } else {  // b3
-    r -= n3;
-    for (int64_t b = 0; b < B; ++b) g += g_dl[b * N + r];
   }
-  float gf = masked ? 0.0f : (float)(g / (double)B);
+  double g = 0.0;
+  for (int sidx = 0; sidx < splits; ++sidx) g += part[(int64_t)sidx * total + k];
+  const float gf = masked ? 0.0f : (float)(g / (double)B);
   const float mm = beta1 * m1[k] + (1.0f - beta1) * gf;
   const float vv = beta2 * m2[k] + (1.0f - beta2) * gf * gf;
   m1[k] = mm;
@@ -462,8 +505,14 @@ int fk_made_sample(const void* packed, int32_t n_sites, int32_t hidden, const do
   return made_run(packed, n_sites, hidden, nullptr, uniforms, batch, x, logq, stream);
 }
 
+// batch samples per split of the weight-gradient products
+static constexpr int64_t MADE_SPLIT = 512;
+
 int64_t fk_made_fit_workspace_floats(int32_t n_sites, int32_t hidden, int64_t batch) {
-  return batch * (3LL * hidden + hidden + n_sites) + 2;
+  const int64_t Hd = hidden, N = n_sites;
+  const int64_t total = Hd * N + Hd + Hd * Hd + Hd + N * Hd + N;
+  const int64_t splits = (batch + MADE_SPLIT - 1) / MADE_SPLIT;
+  return batch * (3LL * hidden + hidden + n_sites) + splits * total + 2;
 }
 
 int fk_made_fit_step(float* master, float* adam_m, float* adam_v, void* w1, float* b1, void* w2t, float* b2, void* w3,
@@ -486,10 +535,27 @@ int fk_made_fit_step(float* master, float* adam_m, float* adam_v, void* w1, floa
                      g_h2, g_dl, g_dh2, g_dh1);
   FK_CHECK_LAUNCH();
   const int64_t total = Hd * N + Hd + Hd * Hd + Hd + N * Hd + N;
+  const int splits = (int)((batch + MADE_SPLIT - 1) / MADE_SPLIT);
+  FK_REQUIRE(splits <= 65535, "fk_made_fit_step: batch too large");
+  float* part = g_dl + batch * N;
+  // per-split partial sums, laid out like the master vector
+  auto wgrad = [&](const float* A, int lda, const float* Bm, const int8_t* xs, int ldb, int rows, int cols,
+                   int64_t off) {
+    hipLaunchKernelGGL(made_wgrad_kernel, dim3(div_up(cols, WG_TILE), div_up(rows, WG_TILE), splits), dim3(256),
+                       0, s, A, lda, Bm, xs, ldb, rows, cols, batch, MADE_SPLIT, part + off, total);
+  };
+  const int64_t o_b1 = Hd * N, o_w2 = o_b1 + Hd, o_b2 = o_w2 + Hd * Hd, o_w3 = o_b2 + Hd, o_b3 = o_w3 + N * Hd;
+  wgrad(g_dh1, (int)Hd, nullptr, x, (int)N, (int)Hd, (int)N, 0);                 // w1[h][i]: dh1^T x
+  wgrad(g_dh1, (int)Hd, nullptr, nullptr, 1, (int)Hd, 1, o_b1);                  // b1
+  wgrad(g_h1, (int)Hd, g_dh2, nullptr, (int)Hd, (int)Hd, (int)Hd, o_w2);         // w2t[kk][t]: h1^T dh2
+  wgrad(g_dh2, (int)Hd, nullptr, nullptr, 1, (int)Hd, 1, o_b2);                  // b2
+  wgrad(g_dl, (int)N, g_h2, nullptr, (int)Hd, (int)N, (int)Hd, o_w3);            // w3[o][h]: dl^T h2
+  wgrad(g_dl, (int)N, nullptr, nullptr, 1, (int)N, 1, o_b3);                     // b3
+  FK_CHECK_LAUNCH();
   const float bc1 = 1.0f - powf(beta1, (float)step), bc2 = 1.0f - powf(beta2, (float)step);
   hipLaunchKernelGGL(made_adam_kernel, dim3(div_up(total, 256)), dim3(256), 0, s, master, adam_m, adam_v,
-                     (__bf16*)w1, (__bf16*)w2t, (__bf16*)w3, x, g_h1, g_h2, g_dl, g_dh2, g_dh1, batch, n_sites,
-                     hidden, lr, beta1, beta2, eps, bc1, bc2);
+                     (__bf16*)w1, (__bf16*)w2t, (__bf16*)w3, part, splits, batch, n_sites, hidden, lr, beta1, beta2,
+                     eps, bc1, bc2);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }

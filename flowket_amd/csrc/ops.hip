@@ -37,7 +37,7 @@ int device_cu_count() {
 // Diagonal element: sequential double accumulation in table order per sample
 // (exact for the integer/dyadic couplings of the reference models, and the
 // same order as the reference's loop otherwise).
-__global__ __launch_bounds__(512) void find_conn_kernel(
+__global__ __launch_bounds__(1024) void find_conn_kernel(
     const int8_t* __restrict__ spins, int64_t batch, int n_sites, int tile, int kind,
     const int32_t* __restrict__ conn_a, const int32_t* __restrict__ conn_b,
     const double* __restrict__ conn_mel, int n_conn,
@@ -45,27 +45,44 @@ __global__ __launch_bounds__(512) void find_conn_kernel(
     const double* __restrict__ diag_j, int n_diag,
     int8_t* __restrict__ conn_out, double* __restrict__ mel_out,
     uint8_t* __restrict__ use_out, int32_t* __restrict__ n_used_out) {
-  extern __shared__ int8_t s_spin[];  // tile rows of n_sites spins
-  __shared__ int s_cnt[8][64];
+  // LDS: the diagonal bond table (site pairs, couplings), then the tile's
+  // spin rows
+  extern __shared__ __align__(16) uint8_t fc_sm[];
+  int2* s_dab = reinterpret_cast<int2*>(fc_sm);
+  double* s_dj = reinterpret_cast<double*>(fc_sm + 8 * (size_t)n_diag);
+  int8_t* s_spin = reinterpret_cast<int8_t*>(fc_sm + 16 * (size_t)n_diag);
+  __shared__ int s_cnt[16][64];
   const int64_t b0 = (int64_t)blockIdx.x * tile;
   const int nb = (int)((batch - b0) < tile ? (batch - b0) : tile);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, NW = blockDim.x >> 6;
+  for (int k = tid; k < n_diag; k += blockDim.x) {
+    s_dab[k] = make_int2(diag_a[k], diag_b[k]);
+    s_dj[k] = diag_j[k];
+  }
   for (int q = tid; q < nb * n_sites; q += blockDim.x) s_spin[q] = spins[b0 * n_sites + q];
   __syncthreads();
   const bool ok = lane < nb;
   const int8_t* row = s_spin + (ok ? lane : 0) * n_sites;
   const int64_t b = b0 + lane;
   int local = 0;
+  // wave 0: the diagonal; the other waves (all of them in a one-wave
+  // workgroup): the off-diagonal slots
   if (wave == 0) {
     double d = 0.0;
-    for (int k = 0; k < n_diag; ++k) d += diag_j[k] * (double)row[diag_a[k]] * (double)row[diag_b[k]];
+#pragma unroll 8
+    for (int k = 0; k < n_diag; ++k) {
+      const int2 ab = s_dab[k];
+      d += s_dj[k] * (double)row[ab.x] * (double)row[ab.y];
+    }
     if (ok) {
       mel_out[b] = d;
       use_out[b] = 1;
     }
     local = 1;
   }
-  for (int c = 1 + wave; c < n_conn; c += NW) {
+  const int w0 = NW > 1 ? 1 : 0, nw = NW > 1 ? NW - 1 : 1;
+#pragma unroll 4
+  for (int c = 1 + wave - w0; wave >= w0 && c < n_conn; c += nw) {
     const int a = conn_a[c];
     int u = 0;
     double m = 0.0;
@@ -349,10 +366,12 @@ int fk_find_conn(const int8_t* spins, int64_t batch, int32_t n_sites, int32_t ki
   FK_REQUIRE(n_diag == 0 || (diag_a && diag_b && diag_j), "fk_find_conn: null diagonal table");
   if (batch == 0) return FK_OK;
   FK_REQUIRE(batch <= 0x7fffffff, "fk_find_conn: batch too large");
-  // tile of samples per workgroup: their spin rows fill at most 64 KB of LDS
-  const int tile = (int)std::min<int64_t>(64, 65536 / n_sites);
-  const size_t lds = (size_t)tile * n_sites;
-  hipLaunchKernelGGL(find_conn_kernel, dim3((unsigned)div_up(batch, tile)), dim3(512), lds, (hipStream_t)stream,
+  // tile of samples per workgroup: the diagonal table and their spin rows
+  // fill at most 64 KB of LDS
+  FK_REQUIRE(n_diag <= 2048 && 16 * (int64_t)n_diag + n_sites <= 65536, "fk_find_conn: table too large for LDS");
+  const int tile = (int)std::min<int64_t>(64, (65536 - 16 * (int64_t)n_diag) / n_sites);
+  const size_t lds = 16 * (size_t)n_diag + (size_t)tile * n_sites;
+  hipLaunchKernelGGL(find_conn_kernel, dim3((unsigned)div_up(batch, tile)), dim3(1024), lds, (hipStream_t)stream,
                      spins, batch, n_sites, tile, kind, conn_a, conn_b, conn_mel, n_conn, diag_a, diag_b, diag_j,
                      n_diag, conn_out, mel_out, use_out, n_used_out);
   FK_CHECK_LAUNCH();
