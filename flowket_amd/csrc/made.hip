@@ -157,40 +157,47 @@ __global__ __launch_bounds__(MADE_T) void made_backward_kernel(MadeW w, const in
 // Weight gradients as tiled products over the batch: part[r][c] =
 // sum_{b in this split} A[b][r] * Bm[b][c] (Bm = +-1 spins when xs is given,
 // ones when both are null: the bias sums).  A 64x64 output tile per
-// workgroup, 4x4 per thread, the batch in chunks of 16 staged through LDS;
-// blockIdx.z is the batch split, whose partial sums are added in split
-// order by the Adam step (deterministic).
-constexpr int WG_TILE = 64, WG_KC = 16;
+// workgroup, a 4x4 block per thread, the batch in chunks of 64 staged
+// through LDS; blockIdx.z is the batch split, whose partial sums are added
+// in split order by the Adam step (deterministic).
+constexpr int WG_TILE = 64, WG_KC = 64;
 __global__ __launch_bounds__(256) void made_wgrad_kernel(const float* __restrict__ A, int lda,
                                                          const float* __restrict__ Bm, const int8_t* __restrict__ xs,
                                                          int ldb, int rows, int cols, int64_t B, int64_t per_split,
                                                          float* __restrict__ out, int64_t split_stride) {
-  __shared__ float sa[WG_KC][WG_TILE];
-  __shared__ float sb[WG_KC][WG_TILE];
+  __shared__ __align__(16) float sa[WG_KC][WG_TILE];
+  __shared__ __align__(16) float sb[WG_KC][WG_TILE];
   const int r0 = blockIdx.y * WG_TILE, c0 = blockIdx.x * WG_TILE;
   const int64_t k0 = (int64_t)blockIdx.z * per_split;
   const int64_t k1 = (k0 + per_split < B) ? k0 + per_split : B;
   const int tid = threadIdx.x, tr = tid / 16, tc = tid % 16;
   float acc[4][4] = {};
   for (int64_t kb = k0; kb < k1; kb += WG_KC) {
-    for (int idx = tid; idx < WG_KC * WG_TILE; idx += 256) {
-      const int kk = idx / WG_TILE, j = idx % WG_TILE;
+    // all of a chunk's loads are issued before its first LDS store
+    constexpr int PER = WG_KC * WG_TILE / 256;
+    const int j = tid % WG_TILE;
+    float va[PER], vb[PER];
+#pragma unroll
+    for (int it = 0; it < PER; ++it) {
+      const int kk = it * (256 / WG_TILE) + tid / WG_TILE;
       const int64_t b = kb + kk;
       const bool kin = b < k1;
-      sa[kk][j] = (kin && r0 + j < rows) ? A[b * lda + r0 + j] : 0.0f;
+      va[it] = (kin && r0 + j < rows) ? A[b * lda + r0 + j] : 0.0f;
       float v = 0.0f;
       if (kin && c0 + j < cols) v = xs ? (float)xs[b * ldb + c0 + j] : (Bm ? Bm[b * ldb + c0 + j] : 1.0f);
-      sb[kk][j] = v;
+      vb[it] = v;
+    }
+#pragma unroll
+    for (int it = 0; it < PER; ++it) {
+      const int kk = it * (256 / WG_TILE) + tid / WG_TILE;
+      sa[kk][j] = va[it];
+      sb[kk][j] = vb[it];
     }
     __syncthreads();
-#pragma unroll
+#pragma unroll 8
     for (int kk = 0; kk < WG_KC; ++kk) {
-      float av[4], bv[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        av[i] = sa[kk][tr + 16 * i];
-        bv[i] = sb[kk][tc + 16 * i];
-      }
+      const f32x4 av = *reinterpret_cast<const f32x4*>(&sa[kk][4 * tr]);
+      const f32x4 bv = *reinterpret_cast<const f32x4*>(&sb[kk][4 * tc]);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -203,7 +210,7 @@ __global__ __launch_bounds__(256) void made_wgrad_kernel(const float* __restrict
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int r = r0 + tr + 16 * i, c = c0 + tc + 16 * j;
+      const int r = r0 + 4 * tr + i, c = c0 + 4 * tc + j;
       if (r < rows && c < cols) o[(int64_t)r * cols + c] = acc[i][j];
     }
 }

@@ -163,8 +163,61 @@ __global__ __launch_bounds__(64) void mh_rbm_kernel(MhArgs a) {
     tr[k] = r;
     ti[k] = i;
   }
+  // Re lncosh(theta) of the current state, kept with theta (recomputed
+  // only where theta changes)
+  float lc[HPL];
+#pragma unroll
+  for (int k = 0; k < HPL; ++k) lc[k] = (lane + 64 * k < M) ? lncosh_re(tr[k], ti[k]) : 0.0f;
   const int64_t total = a.n_burn + a.n_record * (a.unused + 1);
   int64_t acc = 0;
+  // proposal of a step: sites (j2 = -1 for local moves) and uniform, from
+  // the injected arrays or the counter hash; it does not depend on earlier
+  // acceptances, so the next step's is drawn and its weight rows and
+  // visible biases are requested while the current step computes
+  int cj1 = 0, cj2 = -1;
+  double cu = 0.0;
+  float pw1r[HPL], pw1i[HPL], pw2r[HPL], pw2i[HPL];
+  float pa1 = 0.0f, pa2 = 0.0f;
+  auto prefetch = [&](int64_t st) {
+    int j1, j2 = -1;
+    double u;
+    if (a.r_u) {
+      j1 = a.r_site[st * a.chains + ch];
+      u = a.r_u[st * a.chains + ch];
+      if (a.kind == 1) {
+        const int mv = a.r_move[st * a.chains + ch];
+        const int dy = mv / 3 - 1, dx = mv % 3 - 1;
+        const int y = j1 / a.W, x = j1 - (j1 / a.W) * a.W;
+        j2 = ((y + dy + a.H) % a.H) * a.W + ((x + dx + a.W) % a.W);
+      }
+    } else {
+      j1 = (int)(u01(a.seed, ch, st, 0) * N);
+      if (j1 >= N) j1 = N - 1;
+      u = u01(a.seed, ch, st, 1);
+      if (a.kind == 1) {
+        int mv = (int)(u01(a.seed, ch, st, 2) * 9.0);
+        if (mv > 8) mv = 8;
+        const int dy = mv / 3 - 1, dx = mv % 3 - 1;
+        const int y = j1 / a.W, x = j1 - (j1 / a.W) * a.W;
+        j2 = ((y + dy + a.H) % a.H) * a.W + ((x + dx + a.W) % a.W);
+      }
+    }
+    cj1 = j1;
+    cj2 = j2;
+    cu = u;
+    const int jb = j2 >= 0 ? j2 : j1;
+#pragma unroll
+    for (int k = 0; k < HPL; ++k) {
+      const int h = lane + 64 * k;
+      pw1r[k] = h < M ? p.w_re[(int64_t)j1 * M + h] : 0.0f;
+      pw1i[k] = h < M ? p.w_im[(int64_t)j1 * M + h] : 0.0f;
+      pw2r[k] = h < M ? p.w_re[(int64_t)jb * M + h] : 0.0f;
+      pw2i[k] = h < M ? p.w_im[(int64_t)jb * M + h] : 0.0f;
+    }
+    pa1 = p.a_re[j1];
+    pa2 = p.a_re[jb];
+  };
+  if (total > 0) prefetch(0);
   for (int64_t step = 0; step < total; ++step) {
     if (step > 0 && (step & 255) == 0) {
       // refresh theta from the state (bounds the drift of the FP32
@@ -181,33 +234,23 @@ __global__ __launch_bounds__(64) void mh_rbm_kernel(MhArgs a) {
           }
           tr[k] = r;
           ti[k] = i;
+          lc[k] = lncosh_re(r, i);
         }
       }
     }
-    // ---- proposal
-    int j1, j2 = -1;
-    double u;
-    if (a.r_u) {
-      j1 = a.r_site[step * a.chains + ch];
-      u = a.r_u[step * a.chains + ch];
-      if (a.kind == 1) {
-        const int mv = a.r_move[step * a.chains + ch];
-        const int dy = mv / 3 - 1, dx = mv % 3 - 1;
-        const int y = j1 / a.W, x = j1 - (j1 / a.W) * a.W;
-        j2 = ((y + dy + a.H) % a.H) * a.W + ((x + dx + a.W) % a.W);
-      }
-    } else {
-      j1 = (int)(u01(a.seed, ch, step, 0) * N);
-      if (j1 >= N) j1 = N - 1;
-      u = u01(a.seed, ch, step, 1);
-      if (a.kind == 1) {
-        int mv = (int)(u01(a.seed, ch, step, 2) * 9.0);
-        if (mv > 8) mv = 8;
-        const int dy = mv / 3 - 1, dx = mv % 3 - 1;
-        const int y = j1 / a.W, x = j1 - (j1 / a.W) * a.W;
-        j2 = ((y + dy + a.H) % a.H) * a.W + ((x + dx + a.W) % a.W);
-      }
+    // ---- proposal: drawn (and its weight rows requested) one step ahead
+    const int j1 = cj1, j2 = cj2;
+    const double u = cu;
+    float w1r[HPL], w1i[HPL], w2r[HPL], w2i[HPL];
+#pragma unroll
+    for (int k = 0; k < HPL; ++k) {
+      w1r[k] = pw1r[k];
+      w1i[k] = pw1i[k];
+      w2r[k] = pw2r[k];
+      w2i[k] = pw2i[k];
     }
+    const float a1 = pa1, a2 = pa2;
+    if (step + 1 < total) prefetch(step + 1);
     // sites that change: local -> j1; exchange -> j1 and j2 if they differ
     const float s1 = (float)s_state[j1];
     bool flip1 = true, flip2 = false;
@@ -217,7 +260,7 @@ __global__ __launch_bounds__(64) void mh_rbm_kernel(MhArgs a) {
       flip2 = flip1;
     }
     float dre = 0.0f;
-    float nr[HPL], ni[HPL];
+    float nr[HPL], ni[HPL], nl[HPL];
     if (flip1) {
       const float s2 = flip2 ? (float)s_state[j2] : 0.0f;
 #pragma unroll
@@ -226,18 +269,19 @@ __global__ __launch_bounds__(64) void mh_rbm_kernel(MhArgs a) {
         nr[k] = tr[k];
         ni[k] = ti[k];
         if (h < M) {
-          nr[k] -= 2.0f * s1 * p.w_re[(int64_t)j1 * M + h];
-          ni[k] += 2.0f * s1 * p.w_im[(int64_t)j1 * M + h];
+          nr[k] -= 2.0f * s1 * w1r[k];
+          ni[k] += 2.0f * s1 * w1i[k];
           if (flip2) {
-            nr[k] -= 2.0f * s2 * p.w_re[(int64_t)j2 * M + h];
-            ni[k] += 2.0f * s2 * p.w_im[(int64_t)j2 * M + h];
+            nr[k] -= 2.0f * s2 * w2r[k];
+            ni[k] += 2.0f * s2 * w2i[k];
           }
-          dre += lncosh_re(nr[k], ni[k]) - lncosh_re(tr[k], ti[k]);
+          nl[k] = lncosh_re(nr[k], ni[k]);
+          dre += nl[k] - lc[k];
         }
       }
       dre = wave_sum(dre);
-      dre += -2.0f * s1 * p.a_re[j1];
-      if (flip2) dre += -2.0f * (float)s_state[j2] * p.a_re[j2];
+      dre += -2.0f * s1 * a1;
+      if (flip2) dre += -2.0f * (float)s_state[j2] * a2;
     }
     // ---- acceptance (samplers/metropolis_hastings.py:106-110)
     const float lr = fminf(2.0f * dre, 0.0f);
@@ -248,6 +292,7 @@ __global__ __launch_bounds__(64) void mh_rbm_kernel(MhArgs a) {
         for (int k = 0; k < HPL; ++k) {
           tr[k] = nr[k];
           ti[k] = ni[k];
+          if (lane + 64 * k < M) lc[k] = nl[k];
         }
         __syncthreads();
         if (lane == 0) {
