@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void sum_f64_kernel(const double* __restrict__
 // covers ROWS rows so each v_k is read once per ROWS rows; ov_c = Obar.v
 // (re, im) in FP64 is subtracted at the end.  With P a multiple of 4 the rows
 // are streamed as float4.
-constexpr int OV_ROWS = 8;
+constexpr int OV_ROWS = 4;
 __global__ __launch_bounds__(256) void o_times_v_kernel(const float* __restrict__ ore, const float* __restrict__ oim,
                                                         const double* __restrict__ ov_c, const float* __restrict__ v,
                                                         int64_t B, int64_t P, float* __restrict__ u) {
@@ -160,7 +160,7 @@ __global__ void oh_times_u_kernel(const float* __restrict__ ore, const float* __
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     f32x4 part = {0.f, 0.f, 0.f, 0.f};
     int cnt = 0;
-#pragma unroll 4
+#pragma unroll 8
     for (int64_t b = 0; b < B; ++b) {
       const f32x4 x = reinterpret_cast<const f32x4*>(ore + b * P)[t];
       const f32x4 y = reinterpret_cast<const f32x4*>(oim + b * P)[t];
