@@ -198,6 +198,56 @@ def bench_mh(seconds_burn=None):
     return res
 
 
+# FP64 exact-diagonalisation ground energy of the 4x4 periodic Heisenberg
+# model in the Pauli convention (pinned by the oracle tests).
+E0_HEISENBERG_4X4_PBC = -44.9139328337
+
+
+def bench_small(steps=100):
+    """The small plain-method configuration: 4x4 periodic Heisenberg,
+    ConvNetAutoregressive2D depth 5, 16 channels, ELU, FP32, Glorot weights
+    from the seed; 1024 exact autoregressive samples per step, 100 Adam steps
+    at lr 1e-3.  The exact energy of psi over all 65,536 configurations (the
+    exact enumeration path) before and after, against the ED ground energy."""
+    import torch
+    import flowket_amd as fk
+    m = fk.ConvNetAutoregressive2D((4, 4), depth=5, num_of_channels=16, activation='elu',
+                                   weights_normalization=False, precision='fp32', seed=11)
+    op = fk.Heisenberg(hilbert_state_shape=(4, 4), pbc=True)
+    ev = fk.ExactVariational(m, op, batch_size=65536)
+    ev.machine_updated()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        ev.machine_updated()
+    torch.cuda.synchronize()
+    exact_ms = 1e3 * (time.perf_counter() - t0) / reps
+    e_start = ev.energy_observable.current_energy.real
+    vmc = fk.VariationalMonteCarlo(m, op, fk.AutoregressiveSampler(m, 1024, seed=12))
+    opt = fk.Adam(m, lr=1e-3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sampled = []
+    for _ in range(steps):
+        configs, _ = vmc.next_batch()
+        sampled.append(vmc.current_energy.real)
+        opt.step(configs, vmc.current_local_energy, vmc.current_energy)
+    torch.cuda.synchronize()
+    step_ms = 1e3 * (time.perf_counter() - t0) / steps
+    ev.machine_updated()
+    e_end = ev.energy_observable.current_energy.real
+    return {'workload': '4x4 periodic Heisenberg, depth 5, 16 channels, elu, fp32, 1024 samples, Adam lr 1e-3',
+            'steps': steps, 'ms_per_step': round(step_ms, 3),
+            'exact_energy_start': e_start, 'exact_energy_end': e_end,
+            'sampled_energy_last10': float(np.mean(sampled[-10:])),
+            'ed_ground_energy': E0_HEISENBERG_4X4_PBC,
+            'exact_energy_end_per_site_above_ed': (e_end - E0_HEISENBERG_4X4_PBC) / 16,
+            'exact_variance_end': ev.energy_observable.current_local_energy_variance,
+            'exact_pass_ms': round(exact_ms, 3),
+            'exact_states_per_s': round(65536 / (exact_ms * 1e-3), 1)}
+
+
 def bench_large(n_samples=1024, n_eloc=64):
     """The largest configuration's network and lattice: 20x20 periodic
     Heisenberg, depth 24, 64 channels, bf16.  Fast sampling of n_samples
@@ -415,6 +465,7 @@ def main():
         out['mh_rbm_8x8'] = bench_mh()
     if not args.no_large:
         out['large_20x20_d24_c64'] = bench_large()
+    out['small_4x4_exact'] = bench_small()
     # ---- the same ragged step with fp16 MFMA (same speed class as bf16,
     # finer rounding)
     mh = fk.ConvNetAutoregressive2D((H, W), depth=DEPTH, num_of_channels=CHANNELS, activation=ACT,

@@ -15,5 +15,6 @@ from .optimizers import ComplexValuesStochasticReconfiguration, Adam, energy_gra
 from . import autotune  # noqa: F401
 from .graphs import LocalEnergyGraph  # noqa: F401
 from .proposals import MADEProposal, MetropolisHastingsLearnedProposal  # noqa: F401
+from .exact import ExactVariational, ExactObservable, ExactSampler  # noqa: F401
 
 __version__ = '0.1.0'

@@ -74,6 +74,12 @@ SIGNATURES = {
     'fk_made_fit_workspace_floats': (_I64, [_I32, _I32, _I64]),
     'fk_made_fit_step': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P,
                                         ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I64, _P]),
+    'fk_exact_states': (ctypes.c_int, [_I32, _I64, _I64, _P, _P]),
+    'fk_exact_workspace_doubles': (_I64, [_I64]),
+    'fk_exact_normalize': (ctypes.c_int, [_P, _I64, _P, _P, _P, _P]),
+    'fk_exact_local_energy': (ctypes.c_int, [_P, _I64, _I32, _I32, _P, _P, _P, _P, _I32, _P, _P]),
+    'fk_exact_energy': (ctypes.c_int, [_P, _P, _I64, _P, _P, _P, _P]),
+    'fk_exact_sample': (ctypes.c_int, [_P, _I64, _I32, _P, _I64, _P, _P, _P]),
 }
 
 _lib = None

@@ -351,6 +351,52 @@ int fk_made_fit_step(float* master, float* adam_m, float* adam_v, void* w1,
                      int64_t batch, float* workspace, float lr, float beta1,
                      float beta2, float eps, int64_t step, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Exact enumeration over all 2^n_sites configurations (state index k: site i
+ * is +1 when bit i of k is set, else -1 -- exact/utils.py:49-58).
+ *   fk_exact_states:        the configurations first .. first+count-1,
+ *                           int8 (count, n_sites).  Replaces the `states`
+ *                           table of ExactVariational._build_wave_function_arrays
+ *                           (optimization/exact_variational.py:104-117).
+ *   fk_exact_normalize:     probs[k] = |psi_k|^2 / sum |psi|^2 from log psi
+ *                           (n_states, 2) float32; scalars[0] = log sum
+ *                           |psi|^2, scalars[1] = max 2 Re log psi.
+ *                           ExactVariational._update_wave_function_arrays
+ *                           (exact_variational.py:127-133) with
+ *                           complex_norm_log_fsum_exp (exact/utils.py:79-82).
+ *   fk_exact_local_energy:  eloc (n_states, 2) float64 = sum over used slots
+ *                           of mel * psi(k') / psi(k), from fk_find_conn's mel
+ *                           and use over the enumerated states, the connected
+ *                           configurations located by index (no network
+ *                           evaluation).  ExactObservable.update_local_energy
+ *                           (exact_variational.py:80-95, its naive energies).
+ *   fk_exact_energy:        moments[0..1] = E = sum p E_loc, moments[2] =
+ *                           sum p (Re E_loc - Re E)^2; coef (n_states, 2) =
+ *                           p conj(E_loc - E) when coef is not NULL
+ *                           (exact_variational.py:86-100,135-138).
+ *   fk_exact_sample:        batch configurations drawn from probs by the
+ *                           inverse CDF of the uniforms (batch) float64 --
+ *                           ExactSampler.__next__ (samplers/exact_sampler.py:14-20).
+ * Workspace: fk_exact_workspace_doubles(n_states) doubles, shared by
+ * normalize, energy and sample (which uses the n_states-long tail for the CDF).
+ * ------------------------------------------------------------------------- */
+int fk_exact_states(int32_t n_sites, int64_t first, int64_t count,
+                    int8_t* states, void* stream);
+int64_t fk_exact_workspace_doubles(int64_t n_states);
+int fk_exact_normalize(const float* logpsi, int64_t n_states, double* probs,
+                       double* scalars, double* workspace, void* stream);
+int fk_exact_local_energy(const float* logpsi, int64_t n_states,
+                          int32_t n_sites, int32_t kind, const int32_t* conn_a,
+                          const int32_t* conn_b, const double* mel,
+                          const uint8_t* use, int32_t n_conn, double* eloc,
+                          void* stream);
+int fk_exact_energy(const double* probs, const double* eloc, int64_t n_states,
+                    double* coef, double* moments, double* workspace,
+                    void* stream);
+int fk_exact_sample(const double* probs, int64_t n_states, int32_t n_sites,
+                    const double* uniforms, int64_t batch, int8_t* out,
+                    double* workspace, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -128,6 +128,60 @@ def test_exact_local_energy_identity(rng):
     assert abs(e_mc - e_ex) < 1e-9
 
 
+def _random_log_psi(rng, n):
+    """Complex log amplitudes of moderate spread over 2^n states."""
+    return 0.7 * rng.standard_normal(2 ** n) + 1j * rng.uniform(-np.pi, np.pi, 2 ** n)
+
+
+@pytest.mark.parametrize('op', ['heisenberg', 'ising', 'j1j2'])
+def test_exact_variational_energy_is_rayleigh_quotient(rng, op):
+    """ExactObservable's current_energy (optimization/exact_variational.py:
+    80-95) equals <psi|H|psi>/<psi|psi> from the independently assembled
+    sparse H, for a complex psi; the coefficients sum to zero and equal
+    p conj(E_loc - E)."""
+    fc = {'heisenberg': lambda s: oop.heisenberg_find_conn(s, pbc=True),
+          'ising': lambda s: oop.ising_find_conn(s, h=0.7, j=1.3, pbc=True),
+          'j1j2': lambda s: oop.j1j2_find_conn(s, j2=0.5, pbc=False)}[op]
+    shape = (3, 3)
+    lp = _random_log_psi(rng, 9)
+    r = oex.exact_variational(lp, fc, shape)
+    rq = oex.exact_energy(fc, shape, lp)
+    assert abs(np.conj(r['current_energy']) - rq) < 1e-10 * max(1.0, abs(rq))
+    assert abs(r['probs'].sum() - 1.0) < 1e-12
+    assert abs(r['coefficients'].sum()) < 1e-12
+    e = np.conj(r['current_energy'])
+    assert np.allclose(r['coefficients'], r['probs'] * np.conj(r['eloc'] - e), atol=1e-14, rtol=1e-10)
+    # variance = <(Re E_loc - Re E)^2> by its definition
+    assert r['variance'] == pytest.approx(np.sum(r['probs'] * (r['eloc'].real - e.real) ** 2), rel=1e-10)
+
+
+def test_exact_variational_at_the_ground_state():
+    """For psi = the exact ground state every local energy equals E0 and the
+    variance and gradient coefficients vanish (a closed form)."""
+    fc = lambda s: oop.ising_find_conn(s, h=1.0, j=1.0, pbc=True)  # noqa: E731
+    shape = (3, 4)
+    H = oex.sparse_hamiltonian(fc, shape)
+    import scipy.sparse.linalg
+    w, v = scipy.sparse.linalg.eigsh(H, k=1, which='SA', tol=1e-14)
+    psi = v[:, 0] * np.sign(v[np.argmax(np.abs(v[:, 0])), 0])
+    assert psi.min() > 0  # transverse-field Ising: a positive ground state on every configuration
+    r = oex.exact_variational(np.log(psi), fc, shape)
+    assert np.abs(r['eloc'] - w[0]).max() < 1e-8
+    assert r['variance'] < 1e-14
+    assert np.abs(r['coefficients']).max() < 1e-10
+
+
+def test_exact_sample_rule():
+    """samplers/exact_sampler.py:14-20 through numpy's choice: index = first k
+    with cdf_k > u (brute force on a tiny distribution)."""
+    p = np.array([0.1, 0.0, 0.25, 0.4, 0.25])
+    u = np.array([0.0, 0.05, 0.1, 0.1000001, 0.34, 0.35, 0.74, 0.75, 0.999])
+    idx, cdf = oex.exact_sample(p, u)
+    brute = [int(np.nonzero(cdf > x)[0][0]) for x in u]
+    assert list(idx) == brute
+    assert list(idx) == [0, 0, 2, 2, 2, 3, 3, 4, 4]
+
+
 # ---------------------------------------------------------------- network
 def test_convnet_normalised(rng):
     """tests/test_autoregressive.py:18-30."""
