@@ -10,7 +10,8 @@ from .observables import BaseObservable, Observable, energy_moments  # noqa: F40
 from .samplers import (Sampler, AutoregressiveSampler, FastAutoregressiveSampler,  # noqa: F401
                        MetropolisHastingsSampler, MetropolisHastingsLocal, MetropolisHastingsExchange)
 from .optimization import VariationalMonteCarlo, MiniBatchGenerator, loss_for_energy_minimization  # noqa: F401
-from .optimizers import ComplexValuesStochasticReconfiguration, Adam, energy_gradient, conjugate_gradient  # noqa: F401
+from .optimizers import (ComplexValuesStochasticReconfiguration, Adam, SGD, energy_gradient, loss_gradient,  # noqa: F401
+                         conjugate_gradient, AccumulateGradientOptimizer, convert_to_accumulate_gradient_optimizer)
 
 from . import autotune  # noqa: F401
 from .graphs import LocalEnergyGraph  # noqa: F401

@@ -26,16 +26,21 @@ def _complex_eloc_pairs(local_energies, energy):
 
 
 class _OFactor:
-    """O (per-sample gradients) of one batch with its column means."""
+    """O (per-sample gradients) of one batch with its column means (zero means
+    when ``centered`` is False: products with O itself)."""
 
-    def __init__(self, machine, configurations, stream=None):
+    def __init__(self, machine, configurations, stream=None, centered=True):
         self.o_re, self.o_im, self.logpsi = machine.per_sample_gradients(configurations, stream=stream)
         self.B, self.P = self.o_re.shape
         dev = self.o_re.device
-        self.mean_re = torch.empty(self.P, dtype=torch.float32, device=dev)
-        self.mean_im = torch.empty(self.P, dtype=torch.float32, device=dev)
-        _lib.call('fk_o_mean', _lib.ptr(self.o_re), _lib.ptr(self.o_im), self.B, self.P, _lib.ptr(self.mean_re),
-                  _lib.ptr(self.mean_im), _lib.stream_ptr(stream))
+        if centered:
+            self.mean_re = torch.empty(self.P, dtype=torch.float32, device=dev)
+            self.mean_im = torch.empty(self.P, dtype=torch.float32, device=dev)
+            _lib.call('fk_o_mean', _lib.ptr(self.o_re), _lib.ptr(self.o_im), self.B, self.P, _lib.ptr(self.mean_re),
+                      _lib.ptr(self.mean_im), _lib.stream_ptr(stream))
+        else:
+            self.mean_re = torch.zeros(self.P, dtype=torch.float32, device=dev)
+            self.mean_im = torch.zeros(self.P, dtype=torch.float32, device=dev)
         self.scratch = torch.empty(int(_lib.load().fk_sr_scratch_doubles()), dtype=torch.float64, device=dev)
         self.u = torch.empty((self.B, 2), dtype=torch.float32, device=dev)
         self.stream = stream
@@ -70,6 +75,18 @@ def energy_gradient(machine, configurations, local_energies, energy=None):
         energy = local_energies.mean()
     u = _complex_eloc_pairs(local_energies, energy)
     return of.oh(u, 2.0 / of.B)
+
+
+def loss_gradient(machine, configurations, coefficients):
+    """Gradient of the VMC loss sum_b 2 Re(log psi_b y_b) (optimization/
+    loss.py:4-5) over one (mini-)batch of configurations with coefficients y
+    (the second output of VariationalMonteCarlo.next_batch, conj(E_loc - E)/B):
+    2 Re sum_b O_b y_b, uncentered, so that the sums over the mini-batches of
+    a batch add up to the batch's energy gradient."""
+    of = _OFactor(machine, configurations, centered=False)
+    y = torch.as_tensor(coefficients).to(of.o_re.device).to(torch.complex128).reshape(-1)
+    u = torch.view_as_real(torch.conj(y).resolve_conj()).to(torch.float32).contiguous()
+    return of.oh(u, 2.0)
 
 
 def conjugate_gradient(apply, dot, rhs, tol=1e-4, max_iter=20):
@@ -145,10 +162,103 @@ class Adam:
         self.machine = machine
         self.opt = torch.optim.Adam([machine.params], lr=lr, betas=(beta_1, beta_2), eps=epsilon)
 
+    @property
+    def params(self):
+        return self.machine.params
+
     def step(self, configurations, local_energies, energy=None):
         g = energy_gradient(self.machine, configurations, local_energies, energy)
-        self.machine.params.grad = g
+        self.apply_gradient(g)
+        return g
+
+    def apply_gradient(self, g):
+        """One Adam update of the machine's parameters with gradient g."""
+        self.machine.params.grad = g.to(self.machine.params.dtype)
         self.opt.step()
         self.machine.params.grad = None
         self.machine.mark_params_updated()
+        self.iterations = getattr(self, 'iterations', 0) + 1
+
+
+class SGD:
+    """Plain gradient descent on a flat parameter tensor (Keras SGD without
+    momentum): params <- params - lr * g."""
+
+    def __init__(self, params, lr=0.01, on_update=None):
+        self.params = params
+        self.lr = lr
+        self.iterations = 0
+        self.on_update = on_update
+
+    def apply_gradient(self, g):
+        with torch.no_grad():
+            self.params -= self.lr * g.to(self.params.dtype)
+        self.iterations += 1
+        if self.on_update is not None:
+            self.on_update()
+
+
+class AccumulateGradientOptimizer:
+    """optimizers/accumulate_gradient_optimizer.py:13-89 around an optimizer
+    of this package (one with ``params`` and ``apply_gradient``): gradients are
+    summed -- or averaged, dividing each by update_params_frequency -- over
+    update_params_frequency calls, and the wrapped optimizer steps on the
+    accumulated gradient every update_params_frequency-th call, after which
+    the accumulator is cleared.  With ema_decay > 0 an exponential moving
+    average of the parameters is kept at every update and set_weights_ema
+    writes its bias-corrected value (e / (1 - decay^updates)) into them."""
+
+    def __init__(self, orig_optimizer, update_params_frequency, accumulate_sum_or_mean=True, ema_decay=0):
+        if update_params_frequency < 1:
+            raise ValueError('update_params_frequency must be >= 1')
+        self.orig = orig_optimizer
+        self.update_params_frequency = int(update_params_frequency)
+        self.accumulate_sum_or_mean = accumulate_sum_or_mean
+        self.ema_decay = ema_decay
+        self.accumulated_iterations = 0
+        self.total_iterations = 0
+        self._acc = torch.zeros_like(orig_optimizer.params)
+        self._ema = torch.zeros_like(orig_optimizer.params) if ema_decay > 0 else None
+
+    @property
+    def params(self):
+        return self.orig.params
+
+    def set_update_params_frequency(self, update_params_frequency):
+        self.update_params_frequency = int(update_params_frequency)
+
+    def apply_gradient(self, g):
+        self.accumulated_iterations += 1
+        g = g.to(self._acc.dtype)
+        if not self.accumulate_sum_or_mean:
+            g = g / self.update_params_frequency
+        self._acc += g
+        if self.accumulated_iterations % self.update_params_frequency == 0:
+            self.orig.apply_gradient(self._acc.clone())
+            self._acc.zero_()
+            if self._ema is not None:
+                self.total_iterations += 1
+                with torch.no_grad():
+                    self._ema.mul_(self.ema_decay).add_((1.0 - self.ema_decay) * self.orig.params)
+
+    def step(self, machine, configurations, coefficients):
+        """One mini-batch of VariationalMonteCarlo (configurations and its
+        loss coefficients) into the accumulator."""
+        g = loss_gradient(machine, configurations, coefficients)
+        self.apply_gradient(g)
         return g
+
+    def set_weights_ema(self):
+        if self._ema is None or self.total_iterations == 0:
+            return
+        with torch.no_grad():
+            self.orig.params.copy_(self._ema / (1.0 - self.ema_decay ** self.total_iterations))
+        mark = getattr(getattr(self.orig, 'machine', None), 'mark_params_updated', None)
+        if mark is not None:
+            mark()
+
+
+def convert_to_accumulate_gradient_optimizer(orig_optimizer, update_params_frequency, accumulate_sum_or_mean=True,
+                                             ema_decay=0):
+    """optimizers/accumulate_gradient_optimizer.py:13-89 (no Horovod: one GPU)."""
+    return AccumulateGradientOptimizer(orig_optimizer, update_params_frequency, accumulate_sum_or_mean, ema_decay)

@@ -224,3 +224,35 @@ def test_sr_products_c_abi(rng, B, P):
     uu = got_u
     ref = 0.5 * np.real(np.conj(Oc).T @ uu) + 0.25 * v.astype(np.float32)
     assert np.abs(out.double().cpu().numpy() - ref).max() < 1e-4 * np.abs(ref).max()
+
+
+def test_mini_batch_loss_gradients_add_up_to_the_energy_gradient():
+    """VariationalMonteCarlo with mini_batch_size = B/4: the loss gradients of
+    the four mini-batches (optimization/loss.py:4-5) sum to the energy
+    gradient of the whole batch, which is what the accumulating optimizer
+    applies (optimizers/accumulate_gradient_optimizer.py:13-89)."""
+    gpu_banner('mini-batch gradient accumulation 4x4')
+    import flowket_amd as fk
+    m = fk.ConvNetAutoregressive2D((4, 4), depth=3, num_of_channels=32, activation='elu',
+                                   weights_normalization=False, precision='fp32', seed=2)
+    op = fk.Heisenberg(hilbert_state_shape=(4, 4), pbc=True)
+    B = 256
+    vmc = fk.VariationalMonteCarlo(m, op, fk.FastAutoregressiveSampler(m, B, seed=4), mini_batch_size=B // 4)
+    assert vmc.update_params_frequency == 4
+    parts, xs = [], []
+    for _ in range(4):
+        x, y = next(vmc)
+        xs.append(x)
+        parts.append(fk.loss_gradient(m, x, y))
+    acc = torch.stack(parts).sum(0)
+    full = fk.energy_gradient(m, torch.cat(xs), vmc.current_local_energy, vmc.current_energy)
+    assert torch.allclose(acc, full, rtol=1e-4, atol=1e-5 * full.abs().max().item())
+    # the wrapper steps once per batch on that sum
+    sgd = fk.SGD(m.params, lr=1e-3, on_update=m.mark_params_updated)
+    opt = fk.convert_to_accumulate_gradient_optimizer(sgd, vmc.update_params_frequency)
+    before = m.params.clone()
+    for x, g in zip(xs, parts):
+        opt.apply_gradient(g)
+    assert sgd.iterations == 1
+    # (params are FP32: the difference carries their rounding, a few 1e-8)
+    assert torch.allclose(before - m.params, 1e-3 * acc, rtol=1e-4, atol=2e-7)
