@@ -100,10 +100,71 @@ def profile_copy():
     return 0
 
 
-def flops_per_forward():
-    C = CHANNELS
-    nb = 2 * DEPTH - 2
-    return 2 * H * W * ((10 * C * C + 12 * C) + (nb - 1) * 22 * C * C + 4 * C)
+def flops_per_forward(C=CHANNELS, depth=DEPTH, h=H, w=W):
+    nb = 2 * depth - 2
+    return 2 * h * w * ((10 * C * C + 12 * C) + (nb - 1) * 22 * C * C + 4 * C)
+
+
+def bench_j1j2(batch=4096, plain_samples=1024, reps=5):
+    """The J1-J2 configuration: 10x10 periodic, J2/J1 = 0.5, depth 20, 48
+    channels (run on the 64-channel kernel; the padding channels stay zero),
+    bf16, 4096 fast samples.  The ragged set of used connections (1 to 401
+    per sample) goes through the hipGraph-captured local-energy chain, with
+    the forward schedule autotuned on this workload first; the comparison is
+    the plain per-sample-padded FP32 path (all 401 slots of every sample,
+    timed on 1024 of the samples).  FLOPs count the 48 real channels."""
+    import torch
+    import flowket_amd as fk
+    from flowket_amd import _lib, autotune
+    C, depth = 48, 20
+    m16 = fk.ConvNetAutoregressive2D((H, W), depth=depth, num_of_channels=C, activation=ACT,
+                                     weights_normalization=False, precision='bf16', seed=21)
+    m32 = fk.ConvNetAutoregressive2D((H, W), depth=depth, num_of_channels=C, activation=ACT,
+                                     weights_normalization=False, precision='fp32', params=m16.params)
+    op = fk.J1J2(hilbert_state_shape=(H, W), j2=0.5, pbc=True)
+    obs = fk.Observable(op)
+    t0 = time.perf_counter()
+    spins = next(fk.FastAutoregressiveSampler(m16, batch, seed=22)).reshape(batch, H * W).contiguous()
+    torch.cuda.synchronize()
+    sample_s = time.perf_counter() - t0
+    tune = autotune.tune_forward(m16, lambda: obs.local_values(m16, spins), reps=2, persist=False)
+    g = fk.LocalEnergyGraph(m16, op, batch)
+    el16, _ = g(spins)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        el16, mom = g(spins)
+    torch.cuda.synchronize()
+    step_s = (time.perf_counter() - t0) / reps
+    n_conn = int(g.num_connections.item())
+    e16 = el16.real.mean().item() / (H * W)
+    # plain FP32 path on a subset
+    sub = spins[:plain_samples].contiguous()
+    nc = op.num_connections
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = op.find_conn_device(sub, want_conn=True)
+    lp, _ = m32.forward(r['conn'].reshape(nc * plain_samples, H * W))
+    el32 = torch.empty((plain_samples, 2), dtype=torch.float64, device=spins.device)
+    _lib.call('fk_local_energy_dense', _lib.ptr(lp), _lib.ptr(r['mel']), nc, plain_samples, _lib.ptr(el32),
+              _lib.stream_ptr())
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain_rate = plain_samples / plain_s
+    sub16 = el16[:plain_samples].real.cpu().numpy()
+    sub32 = el32[:, 0].cpu().numpy()
+    return {'workload': '10x10 periodic J1-J2 (J2/J1 0.5), depth 20, 48 channels (64-channel kernel), bf16, '
+                        '4096 fast samples, hipGraph-captured ragged local energies',
+            'fast_samples_per_s': round(batch / sample_s, 1),
+            'connected_configurations': n_conn, 'mean_connections_per_sample': round(n_conn / batch, 1),
+            'E_loc_evals_per_s': round(batch / step_s, 1), 'ms_per_step': round(1e3 * step_s, 2),
+            'forward_tflops_48ch': round(n_conn * flops_per_forward(C, depth) / step_s / 1e12, 1),
+            'forward_schedule': tune['schedule'], 'schedule_ms': [round(t, 2) for t in tune['ms']],
+            'fp32_plain_E_loc_evals_per_s': round(plain_rate, 1),
+            'speedup_vs_fp32_plain': round((batch / step_s) / plain_rate, 2),
+            'energy_per_site_bf16': e16,
+            'eloc_rms_rel_err_bf16_vs_fp32_subset': float(np.sqrt(np.mean((sub16 - sub32) ** 2))
+                                                         / np.sqrt(np.mean(sub32 ** 2)))}
 
 
 def bench_sr(machine, op, spins, shift=1e-3, tol=1e-6, max_iter=50):
@@ -271,6 +332,8 @@ def bench_large(n_samples=1024, n_eloc=64):
     sps = n_samples / (time.perf_counter() - t0)
     spins = x.reshape(n_samples, Hl * Wl)[:n_eloc].contiguous()
     obs = fk.Observable(op)
+    from flowket_amd import autotune
+    tune = autotune.tune_forward(m, lambda: obs.local_values(m, spins), reps=1, persist=False)
     obs.local_values(m, spins)
     r = op.compact(spins)
     total = int(r['offsets'][-1])
@@ -289,6 +352,7 @@ def bench_large(n_samples=1024, n_eloc=64):
             'connected_logpsi_per_s': round(total / dt, 1),
             'forward_tflops': round(total * fwd / dt / 1e12, 1),
             'forward_frac_of_bf16_peak': round(total * fwd / dt / PEAK_BF16, 4),
+            'forward_schedule': tune['schedule'], 'schedule_ms': [round(t, 2) for t in tune['ms']],
             'energy_per_site': float(el.real.mean().item()) / (Hl * Wl)}
 
 
@@ -466,6 +530,8 @@ def main():
     if not args.no_large:
         out['large_20x20_d24_c64'] = bench_large()
     out['small_4x4_exact'] = bench_small()
+    if not args.no_large:
+        out['j1j2_10x10_c48'] = bench_j1j2()
     # ---- the same ragged step with fp16 MFMA (same speed class as bf16,
     # finer rounding)
     mh = fk.ConvNetAutoregressive2D((H, W), depth=DEPTH, num_of_channels=CHANNELS, activation=ACT,

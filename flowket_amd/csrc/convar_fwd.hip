@@ -214,6 +214,10 @@ int fk_pack_gather(const float* params, const int32_t* map, int64_t n, int32_t p
 
 namespace fk {
 
+// Waves per workgroup of the HG kernels (tap-outer only): the selected
+// schedule's, or 8 for a tile-outer one.
+static int hg_waves(const FwdSchedule& sc) { return sc.sched == 0 ? sc.nwave : 8; }
+
 // Bytes of global scratch the HG kernels need for a lattice (0 when the five
 // lattices fit in LDS under the schedule currently selected for it).
 static int64_t forward_workspace_bytes(int H, int W, int CP, int prec, FwdPlan* hg_plan) {
@@ -223,7 +227,7 @@ static int64_t forward_workspace_bytes(int H, int W, int CP, int prec, FwdPlan* 
   if (si < 0 || si >= nsched) si = 0;
   FwdPlan pl;
   if (plan_forward(H * W, CP, prec, table[si].nwave, 1 << 30, &pl) == FK_OK) return 0;
-  if (plan_forward(H * W, CP, prec, 8, 1 << 30, &pl, true) != FK_OK) return -1;
+  if (plan_forward(H * W, CP, prec, hg_waves(table[si]), 1 << 30, &pl, true) != FK_OK) return -1;
   if (hg_plan) *hg_plan = pl;
   const int pixb = CP * (prec == FK_PREC_FP32 ? 4 : 2);
   const int per_cu = (int)((160 * 1024) / pl.lds) > 0 ? (int)((160 * 1024) / pl.lds) : 1;
@@ -283,7 +287,7 @@ static int convar_forward(const void* wpacked, const float* aux, int32_t precisi
                 "(fk_convar_forward_workspace_bytes, fk_convar_forward_ws)", H, W, channels, (long long)need);
       return FK_ERR_ARG;
     }
-    if (plan_forward(H * W, CP, precision, 8, m_count, &pl, true) != FK_OK) return FK_ERR_UNSUPPORTED;
+    if (plan_forward(H * W, CP, precision, hg_waves(sc), m_count, &pl, true) != FK_OK) return FK_ERR_UNSUPPORTED;
     a.T = pl.T;
     a.npix = pl.npix;
     a.hscr = reinterpret_cast<uint8_t*>(workspace);

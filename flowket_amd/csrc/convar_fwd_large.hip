@@ -3,13 +3,15 @@
 // configuration): the vertical stacks and the concatenation stay in LDS, the
 // horizontal-stack lattices H0/H1 (read by the 1x3 convolution only, 3 taps)
 // live in a per-workgroup global scratch that stays L2-resident.  Same kernel
-// template and arithmetic as convar_fwd.hip (tap-outer, 8 waves).
+// template and arithmetic as convar_fwd.hip (tap-outer, 8 or 4 waves: the
+// plan's wave count).
 #include "convar_fwd_kernel.h"
 
 namespace fk {
 
 template <int PREC, int C>
 static int launch_hg(const FwdArgs& a, const FwdPlan& pl, int act, hipStream_t s) {
+  if (pl.nwave == 4) return launch_variant<PREC, C, 4, 0, true>(a, pl, act, s);
   return launch_variant<PREC, C, 8, 0, true>(a, pl, act, s);
 }
 

@@ -246,3 +246,24 @@ def test_global_scratch_forward_connected_and_workspace(rng):
                                        _lib.FK_ACT_RELU, _lib.ptr(s), None, None, B, 0, None, None,
                                        _lib.ptr(out), None, _lib.stream_ptr())
     assert rc == 1 and b'workspace' in _lib.load().fk_last_error()
+
+
+def test_global_scratch_forward_both_wave_counts(rng):
+    """The 20x20 / 64-channel forward under its 8-wave and 4-wave schedules:
+    a tile's arithmetic does not depend on the wave that runs it, so the two
+    agree bit for bit (and the 8-wave one is pinned against the oracle by the
+    HG cases above)."""
+    import flowket_amd as fk
+    from flowket_amd import _lib, autotune
+    gpu_banner('global-scratch forward 20x20 C64 bf16, 8 vs 4 waves')
+    m = _machine((20, 20), 3, 64, 'relu', 'bf16', seed=4)
+    s = torch.as_tensor(rng.choice([-1, 1], size=(40, 400)).astype(np.int8)).cuda()
+    try:
+        autotune.set_schedule(m, _lib.FK_PREC_BF16, 0)
+        a, _ = m.forward(s)
+        autotune.set_schedule(m, _lib.FK_PREC_BF16, 1)
+        assert autotune.schedules(64, _lib.FK_PREC_BF16)[1]['waves'] == 4
+        b, _ = m.forward(s)
+    finally:
+        autotune.set_schedule(m, _lib.FK_PREC_BF16, -1)
+    assert torch.equal(a, b)
