@@ -123,8 +123,11 @@ def bench_j1j2(batch=4096, plain_samples=1024, reps=5):
                                      weights_normalization=False, precision='fp32', params=m16.params)
     op = fk.J1J2(hilbert_state_shape=(H, W), j2=0.5, pbc=True)
     obs = fk.Observable(op)
+    fs = fk.FastAutoregressiveSampler(m16, batch, seed=22)
+    next(fs)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    spins = next(fk.FastAutoregressiveSampler(m16, batch, seed=22)).reshape(batch, H * W).contiguous()
+    spins = next(fs).reshape(batch, H * W).contiguous()
     torch.cuda.synchronize()
     sample_s = time.perf_counter() - t0
     tune = autotune.tune_forward(m16, lambda: obs.local_values(m16, spins), reps=2, persist=False)
