@@ -17,5 +17,8 @@ from . import autotune  # noqa: F401
 from .graphs import LocalEnergyGraph  # noqa: F401
 from .proposals import MADEProposal, MetropolisHastingsLearnedProposal  # noqa: F401
 from .exact import ExactVariational, ExactObservable, ExactSampler  # noqa: F401
+from .ensemble import (SymmetrizedMachine, make_2d_obc_invariants, make_up_down_invariant,  # noqa: F401
+                       make_pbc_invariants)
+from .keras_weights import load_keras_weights  # noqa: F401
 
 __version__ = '0.1.0'

@@ -80,6 +80,8 @@ SIGNATURES = {
     'fk_exact_local_energy': (ctypes.c_int, [_P, _I64, _I32, _I32, _P, _P, _P, _P, _I32, _P, _P]),
     'fk_exact_energy': (ctypes.c_int, [_P, _P, _I64, _P, _P, _P, _P]),
     'fk_exact_sample': (ctypes.c_int, [_P, _I64, _I32, _P, _I64, _P, _P, _P]),
+    'fk_transform_spins': (ctypes.c_int, [_P, _I64, _I32, _P, _I32, _P, _P]),
+    'fk_ensemble_combine': (ctypes.c_int, [_P, _I32, _I64, _I64, _P, _P, _P]),
 }
 
 _lib = None

@@ -147,6 +147,18 @@ class ConvNetAutoregressive2D:
     def mark_params_updated(self):
         self._version += 1
 
+    def load_weights(self, filepath):
+        """Keras ``model.load_weights`` of an HDF5 file that the reference's
+        ConvNetAutoregressive2D of the same configuration saved
+        (experiments/run_evaluation.py:13-14): kernels, biases and, with weight
+        normalisation, the g of every wrapped layer."""
+        from .keras_weights import convnet_params_from_keras, load_keras_weights
+        flat, g = convnet_params_from_keras(load_keras_weights(filepath), self.shapes, self.weights_normalization)
+        if self.weights_normalization:
+            self.wn_g = torch.from_numpy(g)
+        self.set_params(torch.from_numpy(flat))
+        return self
+
     def effective_params(self):
         """The flat kernel/bias vector the network actually uses: the raw
         parameters, or with weight normalisation kernel = l2_normalize(v)

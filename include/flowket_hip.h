@@ -397,6 +397,25 @@ int fk_exact_sample(const double* probs, int64_t n_states, int32_t n_sites,
                     const double* uniforms, int64_t batch, int8_t* out,
                     double* workspace, void* stream);
 
+/* Symmetrised wave functions (machines/ensemble.py:14-69).
+ *   fk_transform_spins:  out[b, q] = sign * in[b, src[q]] for a batch of
+ *                        (batch, n_sites) int8 configurations (the lattice
+ *                        symmetries Rot90 / FlipLeftRight of
+ *                        layers/dihedral_4_invariants.py:8-30 and the spin
+ *                        flip x -> -x of ensemble.py:65-69 as a gather).
+ *   fk_ensemble_combine: the probabilistic ensemble of K log psi blocks of
+ *                        `stride` (re, im) float32 rows (ensemble.py:14-20):
+ *                        Re = 0.5 logsumexp(2 Re_k) - 0.5 log K,
+ *                        Im = arg(mean exp(i Im_k)); computed in FP64.  Only
+ *                        the first min(*m_dev, m) rows are written when m_dev
+ *                        is non-NULL. */
+int fk_transform_spins(const int8_t* in, int64_t batch, int32_t n_sites,
+                       const int32_t* src, int32_t sign, int8_t* out,
+                       void* stream);
+int fk_ensemble_combine(const float* logpsi, int32_t k_members, int64_t stride,
+                        int64_t m, const int64_t* m_dev, float* out,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

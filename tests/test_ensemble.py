@@ -1,0 +1,91 @@
+"""Symmetrised wave functions on the CPU: the oracle restatement pinned by
+closed forms and invariances, and the package's index tables (the gathers the
+GPU applies) against image operations."""
+import numpy as np
+import pytest
+
+from oracle import convnet as ocn
+from oracle import ensemble as oen
+
+
+def test_probabilistic_ensemble_closed_forms(rng):
+    z = rng.normal(size=(5,)) + 1j * rng.uniform(-3, 3, size=(5,))
+    # one member, or K equal members, is the member itself (phase mod 2 pi)
+    for k in (1, 4):
+        r = oen.probabilistic_ensemble(np.repeat(z[:, None], k, axis=1))
+        assert np.allclose(r.real, z.real)
+        assert np.allclose(np.exp(1j * r.imag), np.exp(1j * z.imag))
+    # |psi|^2 is the mean of the members' |psi|^2; the phase is arg of the
+    # mean of unit phase factors (brute force)
+    x = rng.normal(size=(7, 3)) + 1j * rng.uniform(-3, 3, size=(7, 3))
+    r = oen.probabilistic_ensemble(x)
+    assert np.allclose(np.exp(2 * r.real), np.mean(np.exp(2 * x.real), axis=1))
+    ph = np.exp(1j * x.imag).mean(axis=1)
+    assert np.allclose(np.exp(1j * r.imag), ph / np.abs(ph))
+    # large magnitudes do not overflow
+    assert np.isfinite(oen.probabilistic_ensemble(np.array([[800.0, 790.0]])).real).all()
+
+
+def test_d4_images_form_the_dihedral_group(rng):
+    x = rng.integers(0, 2, size=(3, 4, 4)) * 2 - 1
+    imgs = oen.d4_images(x)
+    # closure: applying any member to any member stays in the set
+    base = {oen.d4_images(x)[k].tobytes() for k in range(8)}
+    for t in imgs:
+        assert {u.tobytes() for u in oen.d4_images(np.ascontiguousarray(t))} == base
+
+
+def test_evaluation_log_psi_is_invariant(rng):
+    """log psi of the symmetrised 4x4 network is the same on every image of a
+    configuration under the 16 symmetries (and differs from the base)."""
+    params = ocn.glorot_params(rng, 3, 8, bias_scale=0.3)
+
+    def lp(y):
+        return ocn.log_psi(params, y, 3, activation='elu')
+
+    x = (rng.integers(0, 2, size=(6, 4, 4)) * 2 - 1).astype(np.float64)
+    ref = oen.evaluation_log_psi(lp, x)
+    for t in oen.d4_images(x):
+        for s in (1, -1):
+            got = oen.evaluation_log_psi(lp, s * np.ascontiguousarray(t))
+            assert np.allclose(got.real, ref.real, atol=1e-12)
+            assert np.allclose(np.exp(1j * got.imag), np.exp(1j * ref.imag), atol=1e-12)
+    assert not np.allclose(lp(x).real, ref.real)
+
+
+def test_symmetry_index_tables_match_image_operations(rng):
+    """src of every dihedral member gathers the rotated/flipped image, and the
+    relabelling inv maps a flip (or exchange) of x onto the transformed
+    configuration: T(flip_a x) == flip_inv[a] T(x)."""
+    from flowket_amd.ensemble import _roll_src, _rot90_src
+    H = W = 5
+    x = rng.integers(0, 2, size=(1, H, W)) * 2 - 1
+    imgs = oen.d4_images(x)
+    for i, (flip, k) in enumerate([(f, k) for f in (False, True) for k in range(4)]):
+        src = _rot90_src(H, W, k, flip)
+        assert np.array_equal(x.reshape(-1)[src], imgs[i].reshape(-1))
+        inv = np.empty_like(src)
+        inv[src] = np.arange(H * W)
+        for a in range(H * W):
+            xf = x.reshape(-1).copy()
+            xf[a] *= -1
+            y = x.reshape(-1)[src].copy()
+            y[inv[a]] *= -1
+            assert np.array_equal(xf[src], y)
+        b = (3 * 7) % (H * W)
+        xs = x.reshape(-1).copy()
+        xs[[0, b]] = xs[[b, 0]]
+        y = x.reshape(-1)[src].copy()
+        y[[inv[0], inv[b]]] = y[[inv[b], inv[0]]]
+        assert np.array_equal(xs[src], y)
+    src = _roll_src((H, W), [1, 2])
+    assert np.array_equal(x.reshape(-1)[src], np.roll(x[0], (1, 2), axis=(0, 1)).reshape(-1))
+
+
+def test_symmetrized_machine_rejects_non_permutations():
+    import flowket_amd as fk
+    m = fk.RBM((2, 2), alpha=1, device='cpu')
+    with pytest.raises(ValueError):
+        fk.SymmetrizedMachine(m, [(np.array([0, 0, 1, 2]), 1)])
+    with pytest.raises(ValueError):
+        fk.make_2d_obc_invariants(fk.RBM((2, 3), alpha=1, device='cpu'))

@@ -1,0 +1,100 @@
+"""Symmetrised wave functions on the GPU (fk_transform_spins,
+fk_ensemble_combine and the relabelled connected-configuration forward)
+against the oracle restatement, and the trained 12x12 Ising network of the
+reference's experiments against its published evaluation."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import convnet as ocn
+from oracle import ensemble as oen
+from tests.conftest import gpu_banner
+from tests.test_gpu_convnet import _machine, _oracle_params
+
+pytestmark = pytest.mark.gpu
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'ising_12x12_h3_weights.npz')
+
+
+def test_transform_spins(rng):
+    import flowket_amd as fk
+    from flowket_amd.ensemble import _rot90_src
+    gpu_banner('transform spins 7x7')
+    x = (rng.integers(0, 2, size=(300, 7, 7)) * 2 - 1).astype(np.int8)
+    m = fk.make_up_down_invariant(fk.make_2d_obc_invariants(fk.RBM((7, 7), alpha=1)))
+    inner = m.model
+    xs = torch.from_numpy(x.reshape(300, 49)).cuda()
+    imgs = oen.d4_images(x)
+    for k in range(8):
+        assert np.array_equal(inner._transform(xs, k, None).cpu().numpy(), imgs[k].reshape(300, 49))
+        assert np.array_equal(inner.members[k][0], _rot90_src(7, 7, k % 4, k >= 4))
+    assert np.array_equal(m._transform(xs, 1, None).cpu().numpy(), -x.reshape(300, 49))
+
+
+def test_symmetrised_log_psi_vs_oracle(rng):
+    """up/down(obc(ConvNetAutoregressive2D)) on 5x5 (FP32): log psi of every
+    configuration against the oracle's nested ensemble in FP64."""
+    import flowket_amd as fk
+    gpu_banner('symmetrised log psi 5x5')
+    m = _machine((5, 5), 4, 16, 'elu', 'fp32', seed=2, bias_scale=0.3)
+    sym = fk.make_up_down_invariant(fk.make_2d_obc_invariants(m))
+    x = (rng.integers(0, 2, size=(512, 5, 5)) * 2 - 1).astype(np.int8)
+    got = sym.log_psi(torch.from_numpy(x)).cpu().numpy()
+    params = _oracle_params(m)
+    ref = oen.evaluation_log_psi(lambda y: ocn.log_psi(params, y, 4, activation='elu'), x.astype(np.float64))
+    assert np.abs(got.real - ref.real).max() < 2e-5 * max(1.0, np.abs(ref.real).max())
+    assert np.abs(np.angle(np.exp(1j * (got.imag - ref.imag)))).max() < 2e-5
+
+
+@pytest.mark.parametrize('name', ['heisenberg', 'ising', 'j1j2'])
+def test_symmetrised_local_energy_relabelled_vs_materialised(name, rng):
+    """E_loc of the symmetrised network through the relabelled connection
+    tables (no connected configurations stored) equals E_loc through the
+    materialised connected configurations (balanced layout)."""
+    import flowket_amd as fk
+    gpu_banner(f'symmetrised E_loc 4x4 {name}')
+    shape = (4, 4)
+    op = {'heisenberg': lambda: fk.Heisenberg(hilbert_state_shape=shape, pbc=True),
+          'ising': lambda: fk.Ising(h=3.0, hilbert_state_shape=shape, pbc=False),
+          'j1j2': lambda: fk.J1J2(hilbert_state_shape=shape, j2=0.5, pbc=False)}[name]()
+    m = _machine(shape, 3, 16, 'relu', 'fp32', seed=5, bias_scale=0.2)
+    sym = fk.make_up_down_invariant(fk.make_2d_obc_invariants(m))
+    x = next(fk.AutoregressiveSampler(m, 256, seed=7))
+    obs = fk.Observable(op)
+    a = obs.local_values(sym, x).cpu().numpy()
+    b = obs.local_values_balanced(sym, obs._spins(x)).cpu().numpy()
+    assert np.abs(a - b).max() < 1e-5 * max(1.0, np.abs(b).max())
+
+
+def _trained_ising(precision):
+    import flowket_amd as fk
+    w = np.load(FIXTURE)
+    m = fk.ConvNetAutoregressive2D((12, 12), depth=10, num_of_channels=32, precision=precision,
+                                   params=torch.from_numpy(w['params']))
+    m.wn_g = torch.from_numpy(w['g'])
+    m.set_params(torch.from_numpy(w['params']))
+    return m
+
+
+def test_trained_ising_12x12_reproduces_the_published_energy():
+    """experiments/README.md:38-47, h = 3: the symmetrised trained network
+    evaluated on samples of the base network (experiments/run_evaluation.py)
+    gives E = -457.0420317 with local-energy variance 0.000821 (2^15
+    samples).  2048 samples here (FP32): the energy agrees within 5 combined
+    standard errors and the variance within a factor 2."""
+    import flowket_amd as fk
+    from flowket_amd.evaluation import evaluate
+    gpu_banner('trained Ising 12x12 h=3 evaluation')
+    m = _trained_ising('fp32')
+    op = fk.Ising(hilbert_state_shape=(12, 12), pbc=False, h=3.0)
+    sym = fk.make_up_down_invariant(fk.make_2d_obc_invariants(m))
+    B, steps = 1024, 2
+    vmc = fk.VariationalMonteCarlo(sym, op, fk.FastAutoregressiveSampler(m, B, seed=11))
+    r = evaluate(vmc, steps, true_ground_state_energy=-457.039)
+    e, var = r['energy/energy'], r['energy/local_energy_variance']
+    se = np.sqrt(var / (B * steps) + 0.000821 / 2 ** 15)
+    assert abs(e - (-457.0420317)) < 5 * se + 1e-4, (e, var)
+    assert 0.000821 / 2 < var < 0.000821 * 2, var
+    assert abs(r['energy/relative_error']) < 2e-5
