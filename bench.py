@@ -312,6 +312,45 @@ def bench_small(steps=100):
             'exact_states_per_s': round(65536 / (exact_ms * 1e-3), 1)}
 
 
+def bench_trained_ising(n_samples=2 ** 15, mini_batch=2 ** 12, precision='fp16'):
+    """The reference's published evaluation (experiments/README.md:38-47,
+    experiments/run_evaluation.py:12-26): its trained 12x12 open-boundary
+    transverse-field Ising network (h = 3, depth 10, 32 channels, weight
+    normalisation; weights imported from its HDF5 file into
+    tests/golden/ising_12x12_h3_weights.npz), samples from the base network's
+    fast sampler, local energies of the up/down x dihedral symmetrised network
+    (16 members), 2^15 samples as published."""
+    import torch
+    import flowket_amd as fk
+    from flowket_amd.evaluation import evaluate
+    w = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tests', 'golden',
+                             'ising_12x12_h3_weights.npz'))
+    m = fk.ConvNetAutoregressive2D((12, 12), depth=10, num_of_channels=32, precision=precision,
+                                   params=torch.from_numpy(w['params']))
+    m.wn_g = torch.from_numpy(w['g'])
+    m.set_params(torch.from_numpy(w['params']))
+    op = fk.Ising(hilbert_state_shape=(12, 12), pbc=False, h=3.0)
+    sym = fk.make_up_down_invariant(fk.make_2d_obc_invariants(m))
+    vmc = fk.VariationalMonteCarlo(sym, op, fk.FastAutoregressiveSampler(m, mini_batch, seed=21))
+    evaluate(vmc, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    steps = n_samples // mini_batch
+    r = evaluate(vmc, steps, true_ground_state_energy=-457.039)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    e, var = r['energy/energy'], r['energy/local_energy_variance']
+    published_e, published_var = -457.0420317, 0.000821
+    se = float(np.sqrt(var / n_samples + published_var / 2 ** 15))
+    return {'workload': '12x12 obc Ising h=3, trained depth 10, 32 channels, %s, 2^15 samples, '
+                        '16-member symmetrised E_loc' % precision,
+            'seconds': round(dt, 3), 'samples_per_s': round(n_samples / dt, 1),
+            'network_evals_per_s': round(n_samples * 145 * 16 / dt, 1),
+            'energy': e, 'local_energy_variance': var, 'relative_error_vs_dmrg': r['energy/relative_error'],
+            'published_energy': published_e, 'published_variance': published_var,
+            'energy_diff_in_standard_errors': round((e - published_e) / se, 2)}
+
+
 def bench_large(n_samples=1024, n_eloc=64):
     """The largest configuration's network and lattice: 20x20 periodic
     Heisenberg, depth 24, 64 channels, bf16.  Fast sampling of n_samples
@@ -535,6 +574,7 @@ def main():
     out['small_4x4_exact'] = bench_small()
     if not args.no_large:
         out['j1j2_10x10_c48'] = bench_j1j2()
+        out['trained_ising_12x12_eval'] = bench_trained_ising()
     # ---- the same ragged step with fp16 MFMA (same speed class as bf16,
     # finer rounding)
     mh = fk.ConvNetAutoregressive2D((H, W), depth=DEPTH, num_of_channels=CHANNELS, activation=ACT,
