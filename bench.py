@@ -347,6 +347,7 @@ def bench_trained_ising(n_samples=2 ** 15, mini_batch=2 ** 12, precision='fp16')
             'seconds': round(dt, 3), 'samples_per_s': round(n_samples / dt, 1),
             'network_evals_per_s': round(n_samples * 145 * 16 / dt, 1),
             'energy': e, 'local_energy_variance': var, 'relative_error_vs_dmrg': r['energy/relative_error'],
+            'abs_sigma_z': r['observables/abs_sigma_z'], 'published_abs_sigma_z': 0.1622390747,
             'published_energy': published_e, 'published_variance': published_var,
             'energy_diff_in_standard_errors': round((e - published_e) / se, 2)}
 

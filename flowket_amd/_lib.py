@@ -36,6 +36,7 @@ SIGNATURES = {
     'fk_local_energy': (ctypes.c_int, [_P, _P, _P, _P, _I64, _P, _P]),
     'fk_local_energy_dense': (ctypes.c_int, [_P, _P, _I32, _I64, _P, _P]),
     'fk_energy_moments': (ctypes.c_int, [_P, _I64, _P, _P]),
+    'fk_sigma_z': (ctypes.c_int, [_P, _I64, _I32, _I32, _P, _P]),
     'fk_convar_packed_sizes': (ctypes.c_int, [_I32, _I32, _I32, _P, _P]),
     'fk_convar_pack_maps': (ctypes.c_int, [_I32, _I32, _I32, _P, _P]),
     'fk_pack_gather': (ctypes.c_int, [_P, _P, _I64, _I32, _P, _P]),

@@ -291,6 +291,25 @@ __global__ void local_energy_dense_kernel(const float* __restrict__ logpsi, cons
   eloc[2 * b + 1] = acci;
 }
 
+// Magnetisation per sample (observables/monte_carlo/sigma_z.py:7-16): one
+// wave per sample sums its int8 spins exactly in integers; the value (or its
+// absolute value) divided by n_sites is written as a (re, 0) pair.
+__global__ __launch_bounds__(256) void sigma_z_kernel(const int8_t* __restrict__ spins, int64_t batch, int n_sites,
+                                                      int absolute, double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (b >= batch) return;
+  const int8_t* row = spins + b * n_sites;
+  int acc = 0;
+  for (int q = lane; q < n_sites; q += 64) acc += row[q];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) {
+    out[2 * b] = (double)(absolute ? abs(acc) : acc) / (double)n_sites;
+    out[2 * b + 1] = 0.0;
+  }
+}
+
 // Two-pass FP64 mean / population variance of Re (numpy.mean / numpy.var in
 // observables/monte_carlo/observable.py:12-13), fixed reduction order.
 __global__ __launch_bounds__(1024) void energy_moments_kernel(const double* __restrict__ eloc, int64_t batch,
@@ -422,6 +441,16 @@ int fk_local_energy_dense(const float* logpsi, const double* mel, int32_t n_conn
   if (batch <= 0) return FK_OK;
   hipLaunchKernelGGL(local_energy_dense_kernel, dim3(div_up(batch, 256)), dim3(256), 0, (hipStream_t)stream,
                      logpsi, mel, n_conn, batch, eloc_out);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_sigma_z(const int8_t* spins, int64_t batch, int32_t n_sites, int32_t absolute, double* out, void* stream) {
+  FK_REQUIRE(spins && out, "fk_sigma_z: null pointer");
+  FK_REQUIRE(batch >= 0 && n_sites > 0, "fk_sigma_z: bad sizes");
+  if (batch == 0) return FK_OK;
+  hipLaunchKernelGGL(sigma_z_kernel, dim3(div_up(batch, 4)), dim3(256), 0, (hipStream_t)stream, spins, batch,
+                     n_sites, absolute, out);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }

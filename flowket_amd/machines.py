@@ -147,11 +147,20 @@ class ConvNetAutoregressive2D:
     def mark_params_updated(self):
         self._version += 1
 
+    def save_weights(self, filepath):
+        """Parameters (and weight-normalisation g) to a safetensors file."""
+        from .checkpoint import save_weights
+        save_weights(self, filepath)
+
     def load_weights(self, filepath):
-        """Keras ``model.load_weights`` of an HDF5 file that the reference's
+        """Keras ``model.load_weights``: an HDF5 file that the reference's
         ConvNetAutoregressive2D of the same configuration saved
-        (experiments/run_evaluation.py:13-14): kernels, biases and, with weight
-        normalisation, the g of every wrapped layer."""
+        (experiments/run_evaluation.py:13-14) -- kernels, biases and, with weight
+        normalisation, the g of every wrapped layer -- or a file written by
+        ``save_weights``."""
+        from .checkpoint import is_hdf5, load_weights
+        if not is_hdf5(filepath):
+            return load_weights(self, filepath)
         from .keras_weights import convnet_params_from_keras, load_keras_weights
         flat, g = convnet_params_from_keras(load_keras_weights(filepath), self.shapes, self.weights_normalization)
         if self.weights_normalization:
@@ -399,6 +408,14 @@ class RBM:
     def log_psi_connected(self, spins, pairs, m, operator, m_dev=None, precision=None, stream=None):
         out, _ = self.forward(spins, pairs=pairs, m=m, m_dev=m_dev, operator=operator, stream=stream)
         return out
+
+    def save_weights(self, filepath):
+        from .checkpoint import save_weights
+        save_weights(self, filepath)
+
+    def load_weights(self, filepath):
+        from .checkpoint import load_weights
+        return load_weights(self, filepath)
 
     def count_params(self):
         return self.n_params

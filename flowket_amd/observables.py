@@ -86,3 +86,33 @@ class Observable(BaseObservable):
         _lib.call('fk_local_energy_dense', _lib.ptr(lp), _lib.ptr(r['mel']), nc, B, _lib.ptr(eloc),
                   _lib.stream_ptr(stream))
         return torch.complex(eloc[:, 0], eloc[:, 1])
+
+
+class _SigmaZBase(BaseObservable):
+    """observables/monte_carlo/sigma_z.py:7-20 (LambdaObservable of the
+    configurations only; the wave function is not evaluated)."""
+
+    absolute = 0
+
+    def local_values(self, wave_function, configurations, stream=None):
+        _lib.require_gpu()
+        x = configurations
+        if not isinstance(x, torch.Tensor):
+            x = torch.as_tensor(np.asarray(x))
+        dev = torch.device('cuda', torch.cuda.current_device())
+        B = x.shape[0]
+        spins = x.to(dev).to(torch.int8).reshape(B, -1).contiguous()
+        out = torch.empty((B, 2), dtype=torch.float64, device=dev)
+        _lib.call('fk_sigma_z', _lib.ptr(spins), B, spins.shape[1], self.absolute, _lib.ptr(out),
+                  _lib.stream_ptr(stream))
+        return torch.complex(out[:, 0], out[:, 1])
+
+
+class SigmaZ(_SigmaZBase):
+    """Mean spin per site of each configuration."""
+    absolute = 0
+
+
+class AbsSigmaZ(_SigmaZBase):
+    """|mean spin per site| of each configuration."""
+    absolute = 1

@@ -179,6 +179,28 @@ class Adam:
         self.machine.mark_params_updated()
         self.iterations = getattr(self, 'iterations', 0) + 1
 
+    def get_weights(self):
+        """Optimizer state as a list of tensors (Keras ``optimizer.weights``
+        order: iterations, first moments, second moments)."""
+        p = self.machine.params
+        st = self.opt.state.get(p, {})
+        it = torch.tensor([float(getattr(self, 'iterations', 0))], dtype=torch.float64)
+        m = st.get('exp_avg', torch.zeros_like(p)).detach().clone()
+        v = st.get('exp_avg_sq', torch.zeros_like(p)).detach().clone()
+        return [it, m, v]
+
+    def set_weights(self, weights):
+        it, m, v = weights
+        p = self.machine.params
+        n = int(torch.as_tensor(it).reshape(-1)[0].item())
+        self.iterations = n
+        if n == 0:
+            self.opt.state.pop(p, None)
+            return
+        self.opt.state[p] = {'step': torch.tensor(float(n)),
+                             'exp_avg': torch.as_tensor(m).to(p.device, p.dtype).reshape(p.shape).clone(),
+                             'exp_avg_sq': torch.as_tensor(v).to(p.device, p.dtype).reshape(p.shape).clone()}
+
 
 class SGD:
     """Plain gradient descent on a flat parameter tensor (Keras SGD without
@@ -196,6 +218,12 @@ class SGD:
         self.iterations += 1
         if self.on_update is not None:
             self.on_update()
+
+    def get_weights(self):
+        return [torch.tensor([float(self.iterations)], dtype=torch.float64)]
+
+    def set_weights(self, weights):
+        self.iterations = int(torch.as_tensor(weights[0]).reshape(-1)[0].item())
 
 
 class AccumulateGradientOptimizer:
@@ -247,6 +275,29 @@ class AccumulateGradientOptimizer:
         g = loss_gradient(machine, configurations, coefficients)
         self.apply_gradient(g)
         return g
+
+    def get_weights(self):
+        """The wrapped optimizer's state, then the call counters, the
+        accumulator and (with ema_decay > 0) the parameter average."""
+        counters = torch.tensor([float(self.accumulated_iterations), float(self.total_iterations)],
+                                dtype=torch.float64)
+        out = list(self.orig.get_weights()) + [counters, self._acc.detach().clone()]
+        if self._ema is not None:
+            out.append(self._ema.detach().clone())
+        return out
+
+    def set_weights(self, weights):
+        n_orig = len(self.orig.get_weights())
+        self.orig.set_weights(weights[:n_orig])
+        rest = weights[n_orig:]
+        counters = torch.as_tensor(rest[0]).reshape(-1)
+        self.accumulated_iterations = int(counters[0].item())
+        self.total_iterations = int(counters[1].item())
+        self._acc.copy_(torch.as_tensor(rest[1]).to(self._acc.device, self._acc.dtype).reshape(self._acc.shape))
+        if self._ema is not None:
+            if len(rest) < 3:
+                raise ValueError('the saved state has no parameter average (ema_decay was 0)')
+            self._ema.copy_(torch.as_tensor(rest[2]).to(self._ema.device, self._ema.dtype).reshape(self._ema.shape))
 
     def set_weights_ema(self):
         if self._ema is None or self.total_iterations == 0:

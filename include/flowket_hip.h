@@ -104,6 +104,13 @@ int fk_local_energy_dense(const float* logpsi, const double* mel,
 int fk_energy_moments(const double* eloc, int64_t batch, double* out,
                       void* stream);
 
+/* SigmaZ / AbsSigmaZ local values (observables/monte_carlo/sigma_z.py:7-20):
+ * out[2b] = (sum of the n_sites spins of sample b) / n_sites, or its absolute
+ * value when `absolute` is non-zero; out[2b+1] = 0 (complex pairs, as
+ * fk_energy_moments reads them). */
+int fk_sigma_z(const int8_t* spins, int64_t batch, int32_t n_sites,
+               int32_t absolute, double* out, void* stream);
+
 /* ---------------------------------------------------------------------------
  * ConvNetAutoregressive2D (machines/conv_net_autoregressive_2D.py:8-78 with
  * AutoNormalizedAutoregressiveMachine, machines/abstract_machine.py:49-71).

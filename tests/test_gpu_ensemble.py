@@ -98,3 +98,7 @@ def test_trained_ising_12x12_reproduces_the_published_energy():
     assert abs(e - (-457.0420317)) < 5 * se + 1e-4, (e, var)
     assert 0.000821 / 2 < var < 0.000821 * 2, var
     assert abs(r['energy/relative_error']) < 2e-5
+    # |Mz| of the base samples (published 0.1622390747; the per-sample spread
+    # of |Mz| is below 0.15, so 5 standard errors of 2048 samples are < 0.017)
+    assert abs(r['observables/abs_sigma_z'] - 0.1622390747) < 0.017, r
+    assert abs(r['observables/sigma_z']) < 0.05, r
