@@ -40,11 +40,12 @@ def test_load_weights_matches_fixture():
     # Keras saves the last block's hconv before its vconv (topological order)
     names = [n.split('/')[0] for n, _ in w]
     assert names.index('weight_normalization_86') < names.index('weight_normalization_85')
-    m = fk.ConvNetAutoregressive2D((12, 12), depth=10, num_of_channels=32, device='cpu')
-    m.load_weights(H5)
-    fx = np.load(FIXTURE)
-    assert np.array_equal(m.params.numpy(), fx['params'])
-    assert np.array_equal(m.wn_g.numpy(), fx['g'])
+    for h in ['2', '2_5', '3', '3_5', '4']:
+        m = fk.ConvNetAutoregressive2D((12, 12), depth=10, num_of_channels=32, device='cpu')
+        m.load_weights(H5.replace('ising_3', 'ising_' + h))
+        fx = np.load(FIXTURE.replace('_h3_', '_h%s_' % h))
+        assert np.array_equal(m.params.numpy(), fx['params'])
+        assert np.array_equal(m.wn_g.numpy(), fx['g'])
 
 
 @pytest.mark.skipif(not os.path.exists(H5), reason='the reference weight files are not present')
