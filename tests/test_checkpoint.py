@@ -111,3 +111,15 @@ def test_safetensors_files_hold_no_code(tmp_path):
         assert sorted(f.keys()) == ['w0000', 'w0001', 'w0002']
         assert f.metadata()['count'] == '3'
         assert np.array_equal(f.get_tensor('w0000').numpy(), [1.0])
+
+
+def test_rbm_weights_round_trip(tmp_path):
+    import flowket_amd as fk
+    a = fk.RBM((3, 4), alpha=2, stddev=0.3, seed=1, device='cpu')
+    b = fk.RBM((3, 4), alpha=2, stddev=0.3, seed=2, device='cpu')
+    path = str(tmp_path / 'rbm.safetensors')
+    a.save_weights(path)
+    b.load_weights(path)
+    assert torch.equal(a.params, b.params)
+    with pytest.raises(ValueError):
+        fk.RBM((3, 4), alpha=1, device='cpu').load_weights(path)
