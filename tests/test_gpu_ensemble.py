@@ -111,3 +111,28 @@ def test_trained_ising_12x12_reproduces_the_published_evaluation(h):
     assert var_pub / 2.5 < var < var_pub * 2.5, var
     assert abs(r['energy/relative_error']) < 5e-5
     assert abs(r['observables/abs_sigma_z'] - mz_pub) < 0.017, r
+
+
+def test_ensemble_combine_through_the_c_abi(rng):
+    """fk_ensemble_combine on K = 3 blocks with a device row count: the live
+    rows equal the oracle's probabilistic ensemble (FP64 inside, FP32 out),
+    rows past *m_dev are not written, and one member gives itself back."""
+    from flowket_amd import _lib
+    gpu_banner('ensemble combine C-ABI')
+    K, m, live = 3, 1000, 777
+    x = np.stack([rng.normal(size=(K, m)) * 5, rng.uniform(-4, 4, size=(K, m))], axis=-1).astype(np.float32)
+    lp = torch.from_numpy(x).cuda().contiguous()
+    out = torch.full((m, 2), 123.0, dtype=torch.float32, device='cuda')
+    mdev = torch.tensor([live], dtype=torch.int64, device='cuda')
+    _lib.call('fk_ensemble_combine', _lib.ptr(lp), K, m, m, _lib.ptr(mdev), _lib.ptr(out), _lib.stream_ptr())
+    got = out.cpu().numpy()
+    z = x[..., 0].astype(np.float64) + 1j * x[..., 1].astype(np.float64)
+    ref = oen.probabilistic_ensemble(z.T)
+    assert np.abs(got[:live, 0] - ref.real[:live]).max() < 1e-5 * max(1.0, np.abs(ref.real).max())
+    assert np.abs(np.angle(np.exp(1j * (got[:live, 1] - ref.imag[:live])))).max() < 1e-5
+    assert (got[live:] == 123.0).all()
+    one = torch.empty((m, 2), dtype=torch.float32, device='cuda')
+    _lib.call('fk_ensemble_combine', _lib.ptr(lp), 1, m, m, None, _lib.ptr(one), _lib.stream_ptr())
+    o = one.cpu().numpy()
+    assert np.abs(o[:, 0] - x[0, :, 0]).max() < 1e-5 * np.abs(x[0, :, 0]).max()
+    assert np.abs(np.angle(np.exp(1j * (o[:, 1] - x[0, :, 1])))).max() < 1e-5
