@@ -11,7 +11,6 @@ that subset raises ``ValueError``.
 them: the file's ``layer_names`` attribute, then each layer group's
 ``weight_names`` attribute (Keras' own load order).
 """
-import struct
 
 import numpy as np
 
