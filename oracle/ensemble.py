@@ -39,3 +39,13 @@ def evaluation_log_psi(log_psi_fn, x):
     """The model experiments/run_evaluation.py:16-19 evaluates:
     up/down invariance around the obc (dihedral) invariance."""
     return up_down_invariant_log_psi(lambda y: obc_invariant_log_psi(log_psi_fn, y), x)
+
+
+def pbc_invariant_log_psi(log_psi_fn, x, apply_also_obc_invariants=True):
+    """make_pbc_invariants(model) (ensemble.py:55-62): every translation of
+    the lattice (tensorflow.roll of the image by (i, j)) around the
+    obc-invariant model."""
+    inner = (lambda y: obc_invariant_log_psi(log_psi_fn, y)) if apply_also_obc_invariants else log_psi_fn
+    H, W = x.shape[1:]
+    members = [inner(np.ascontiguousarray(np.roll(x, (i, j), axis=(1, 2)))) for i in range(H) for j in range(W)]
+    return probabilistic_ensemble(np.stack(members, axis=-1))

@@ -89,3 +89,19 @@ def test_symmetrized_machine_rejects_non_permutations():
         fk.SymmetrizedMachine(m, [(np.array([0, 0, 1, 2]), 1)])
     with pytest.raises(ValueError):
         fk.make_2d_obc_invariants(fk.RBM((2, 3), alpha=1, device='cpu'))
+
+
+def test_pbc_invariant_log_psi_is_translation_invariant(rng):
+    params = ocn.glorot_params(rng, 2, 8, bias_scale=0.3)
+
+    def lp(y):
+        return ocn.log_psi(params, y, 2, activation='relu')
+
+    x = (rng.integers(0, 2, size=(3, 3, 3)) * 2 - 1).astype(np.float64)
+    ref = oen.pbc_invariant_log_psi(lp, x)
+    for sh in [(1, 0), (0, 2), (2, 1)]:
+        got = oen.pbc_invariant_log_psi(lp, np.roll(x, sh, axis=(1, 2)))
+        assert np.allclose(got.real, ref.real, atol=1e-12)
+        assert np.allclose(np.exp(1j * got.imag), np.exp(1j * ref.imag), atol=1e-12)
+    got = oen.pbc_invariant_log_psi(lp, np.ascontiguousarray(np.rot90(x, 1, axes=(1, 2))))
+    assert np.allclose(got.real, ref.real, atol=1e-12)

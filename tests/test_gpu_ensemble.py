@@ -136,3 +136,26 @@ def test_ensemble_combine_through_the_c_abi(rng):
     o = one.cpu().numpy()
     assert np.abs(o[:, 0] - x[0, :, 0]).max() < 1e-5 * np.abs(x[0, :, 0]).max()
     assert np.abs(np.angle(np.exp(1j * (o[:, 1] - x[0, :, 1])))).max() < 1e-5
+
+
+def test_pbc_invariant_log_psi_vs_oracle(rng):
+    """make_pbc_invariants (16 translations around the 8 dihedral members) of
+    a 4x4 FP32 network against the oracle, and its connected-configuration
+    E_loc through relabelled tables against the materialised layout."""
+    import flowket_amd as fk
+    gpu_banner('pbc-invariant log psi 4x4')
+    m = _machine((4, 4), 3, 16, 'elu', 'fp32', seed=6, bias_scale=0.3)
+    sym = fk.make_pbc_invariants(m)
+    assert len(sym.members) == 16 and len(sym.model.members) == 8
+    x = (rng.integers(0, 2, size=(64, 4, 4)) * 2 - 1).astype(np.int8)
+    got = sym.log_psi(torch.from_numpy(x)).cpu().numpy()
+    params = _oracle_params(m)
+    ref = oen.pbc_invariant_log_psi(lambda y: ocn.log_psi(params, y, 3, activation='elu'), x.astype(np.float64))
+    assert np.abs(got.real - ref.real).max() < 2e-5 * max(1.0, np.abs(ref.real).max())
+    assert np.abs(np.angle(np.exp(1j * (got.imag - ref.imag)))).max() < 2e-5
+    op = fk.Heisenberg(hilbert_state_shape=(4, 4), pbc=True)
+    obs = fk.Observable(op)
+    xs = torch.from_numpy(x[:32])
+    a = obs.local_values(sym, xs).cpu().numpy()
+    b = obs.local_values_balanced(sym, obs._spins(xs)).cpu().numpy()
+    assert np.abs(a - b).max() < 1e-5 * max(1.0, np.abs(b).max())
