@@ -34,3 +34,30 @@ def evaluate(generator, steps, true_ground_state_energy=None, observables=None):
     if not logs_arr:
         return {}
     return {key: float(np.mean([logs[key] for logs in logs_arr])) for key in logs_arr[0]}
+
+
+def exact_evaluate(exact_variational, true_ground_state_energy=None):
+    """evaluation/evaluate.py:31-37 with the exact statistics callbacks
+    (callbacks/exact/local_energy.py:12-17, callbacks/exact/sigma_z.py:20-22):
+    the exact energy and variance of the current machine, and the
+    probability-weighted sigma_z and |sigma_z| over all 2^N configurations
+    (FP64 device reductions)."""
+    import torch
+    from . import _lib
+    ev = exact_variational
+    ev.machine_updated()
+    obs = ev.energy_observable
+    energy = float(np.real(obs.current_energy))
+    logs = {'energy/energy': energy, 'energy/local_energy_variance': float(obs.current_local_energy_variance)}
+    if true_ground_state_energy is not None:
+        logs['energy/relative_error'] = (true_ground_state_energy - energy) / true_ground_state_energy
+    vals = torch.empty((ev.num_of_states, 2), dtype=torch.float64, device=ev.device)
+    coef = torch.empty_like(vals)
+    mom = torch.empty(3, dtype=torch.float64, device=ev.device)
+    for name, absolute in (('abs_sigma_z', 1), ('sigma_z', 0)):
+        _lib.call('fk_sigma_z', _lib.ptr(ev.states), ev.num_of_states, ev.number_of_spins, absolute, _lib.ptr(vals),
+                  _lib.stream_ptr())
+        _lib.call('fk_exact_energy', _lib.ptr(ev.probs), _lib.ptr(vals), ev.num_of_states, _lib.ptr(coef),
+                  _lib.ptr(mom), _lib.ptr(ev._ws), _lib.stream_ptr())
+        logs['observables/%s' % name] = float(mom[0].item())
+    return logs

@@ -165,3 +165,22 @@ def test_sampled_energy_agrees_with_exact():
     assert abs(el.mean() - E) < 5 * np.sqrt(var / B)
     assert el.var() == pytest.approx(var, rel=0.1)
     assert E > -44.9139328337
+
+
+def test_exact_evaluate_logs():
+    """exact_evaluate: the energy logs equal the exact observable's, and
+    sigma_z / |sigma_z| equal sum_k p_k m_k over all configurations."""
+    import flowket_amd as fk
+    from flowket_amd.evaluation import exact_evaluate
+    gpu_banner('exact evaluate 4x4')
+    m = fk.RBM((4, 4), alpha=2, stddev=0.3, seed=9)
+    op = fk.Ising(h=1.0, hilbert_state_shape=(4, 4), pbc=True)
+    ev = fk.ExactVariational(m, op, batch_size=65536)
+    logs = exact_evaluate(ev, true_ground_state_energy=-34.0)
+    e = ev.energy_observable.current_energy.real
+    assert logs['energy/energy'] == e
+    assert logs['energy/relative_error'] == pytest.approx((-34.0 - e) / -34.0, rel=1e-12)
+    p = ev.probs.cpu().numpy()
+    mz = oex.all_states((4, 4)).reshape(65536, 16).sum(axis=1) / 16.0
+    assert logs['observables/sigma_z'] == pytest.approx(float(np.dot(p, mz)), abs=1e-12)
+    assert logs['observables/abs_sigma_z'] == pytest.approx(float(np.dot(p, np.abs(mz))), abs=1e-12)
