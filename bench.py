@@ -431,8 +431,8 @@ def main():
     if args.profile_forward:
         return profile_forward(args.profile_forward, args.precision, args.schedule)
     if args.gpus != 1:
-        print(json.dumps({'error': 'this build runs on one GPU; multi-GPU sharding is a design note only '
-                                   '(see DESIGN.md), --gpus must be 1'}))
+        print(json.dumps({'error': 'the VMC path here runs on one GPU (samples shard across GPUs by design, '
+                                   'but no multi-GPU launch is implemented); --gpus must be 1'}))
         return 2
     import torch
     import flowket_amd as fk
@@ -614,7 +614,7 @@ def main():
             acc[f'eloc_max_rel_err_{name}_vs_fp64_oracle'] = float(np.abs(g - el64).max() / np.abs(el64).max())
             acc[f'energy_per_site_err_{name}_vs_fp64_oracle_subset'] = float(
                 abs(el[:n8].real.mean().item() - el64.real.mean()) / (H * W))
-    # the 1e-4 energy-per-site bar, judged against the FP32 path on all B
+    # the 1e-4 energy-per-site bar, measured against the FP32 path on all B
     # samples (the FP32 path itself is ~2e-6 per site from the FP64 oracle)
     acc['within_1e-4_energy_per_site'] = {name: acc[f'energy_per_site_err_{name}_vs_fp32'] <= 1e-4
                                           for name in ('bf16', 'fp16')}
