@@ -16,9 +16,24 @@ import numpy as np
 from .observables import AbsSigmaZ, SigmaZ
 
 
-def evaluate(generator, steps, true_ground_state_energy=None, observables=None):
+def _mcmc_logs(generator):
+    """callbacks/monte_carlo/mcmc_stats.py:11-17 for a Metropolis-Hastings
+    sampler: acceptance ratio, R-hat, correlation sum and effective sample
+    size of Re E_loc."""
+    sampler = generator.sampler
+    r_hat, _, correlations_sum, ess = sampler.calc_r_hat_value(generator.current_local_energy.real)
+    return {'mcmc/acceptance_ratio': float(sampler.acceptance_ratio), 'mcmc/energy_r_hat': float(r_hat),
+            'mcmc/energy_effective_sample_size': float(ess),
+            'mcmc/energy_correlations_sum': float(correlations_sum)}
+
+
+def evaluate(generator, steps, true_ground_state_energy=None, observables=None, mcmc_stats=False):
+    """``mcmc_stats`` adds the MCMCStats logs (Metropolis-Hastings samplers
+    only)."""
     if observables is None:
         observables = {'sigma_z': SigmaZ(), 'abs_sigma_z': AbsSigmaZ()}
+    if mcmc_stats and not hasattr(generator.sampler, 'calc_r_hat_value'):
+        raise ValueError('mcmc_stats needs a Metropolis-Hastings sampler')
     logs_arr = []
     for _ in range(int(steps)):
         next(generator)
@@ -30,6 +45,8 @@ def evaluate(generator, steps, true_ground_state_energy=None, observables=None):
         for name, obs in observables.items():
             value, _, _ = obs.estimate(generator.wave_function, generator.current_batch)
             logs['observables/%s' % name] = float(np.real(value))
+        if mcmc_stats:
+            logs.update(_mcmc_logs(generator))
         logs_arr.append(logs)
     if not logs_arr:
         return {}
