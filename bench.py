@@ -331,6 +331,10 @@ def bench_trained_ising(n_samples=2 ** 15, mini_batch=2 ** 12, precision='fp16')
     m.set_params(torch.from_numpy(w['params']))
     op = fk.Ising(hilbert_state_shape=(12, 12), pbc=False, h=3.0)
     sym = fk.make_up_down_invariant(fk.make_2d_obc_invariants(m))
+    from flowket_amd import autotune
+    tune_spins = next(fk.FastAutoregressiveSampler(m, mini_batch, seed=20))
+    obs = fk.Observable(op)
+    tune = autotune.tune_forward(m, lambda: obs.local_values(sym, tune_spins), reps=1, persist=False)
     vmc = fk.VariationalMonteCarlo(sym, op, fk.FastAutoregressiveSampler(m, mini_batch, seed=21))
     evaluate(vmc, 1)
     torch.cuda.synchronize()
@@ -349,7 +353,8 @@ def bench_trained_ising(n_samples=2 ** 15, mini_batch=2 ** 12, precision='fp16')
             'energy': e, 'local_energy_variance': var, 'relative_error_vs_dmrg': r['energy/relative_error'],
             'abs_sigma_z': r['observables/abs_sigma_z'], 'published_abs_sigma_z': 0.1622390747,
             'published_energy': published_e, 'published_variance': published_var,
-            'energy_diff_in_standard_errors': round((e - published_e) / se, 2)}
+            'energy_diff_in_standard_errors': round((e - published_e) / se, 2),
+            'forward_schedule': tune['schedule'], 'schedule_ms': [round(t, 2) for t in tune['ms']]}
 
 
 def bench_large(n_samples=1024, n_eloc=64):
