@@ -4,11 +4,13 @@ The compute runs in libflowket_hip.so (hand-written HIP kernels for gfx950,
 C ABI in include/flowket_hip.h); this package is the Python host side that
 mirrors FlowKet's operator / machine / sampler / VMC classes.
 """
+from ._lib import FlowketHipError  # noqa: F401
 from .operators import Operator, OperatorOnGrid, cube_shape, Heisenberg, Ising, J1J2, j1j2_edges  # noqa: F401
 from .machines import ConvNetAutoregressive2D, RBM  # noqa: F401
 from .observables import BaseObservable, Observable, SigmaZ, AbsSigmaZ, energy_moments  # noqa: F401
 from .samplers import (Sampler, AutoregressiveSampler, FastAutoregressiveSampler,  # noqa: F401
-                       MetropolisHastingsSampler, MetropolisHastingsLocal, MetropolisHastingsExchange)
+                       MetropolisHastingsSampler, MetropolisHastingsLocal, MetropolisHastingsExchange,
+                       calc_r_hat_value)
 from .optimization import VariationalMonteCarlo, MiniBatchGenerator, loss_for_energy_minimization  # noqa: F401
 from .optimizers import (ComplexValuesStochasticReconfiguration, Adam, SGD, energy_gradient, loss_gradient,  # noqa: F401
                          conjugate_gradient, AccumulateGradientOptimizer, convert_to_accumulate_gradient_optimizer)

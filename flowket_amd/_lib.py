@@ -16,6 +16,8 @@ FK_OK = 0
 FK_CONN_SWAP = 0
 FK_CONN_SWAP_ZERO_PAD = 1
 FK_CONN_FLIP = 2
+FK_DIAG_SUM_F64 = 0
+FK_DIAG_ISING_F32 = 1
 FK_ACT_RELU = 0
 FK_ACT_ELU = 1
 FK_PREC_FP32 = 0
@@ -30,7 +32,8 @@ _I64 = ctypes.c_int64
 SIGNATURES = {
     'fk_version': (ctypes.c_int, []),
     'fk_last_error': (ctypes.c_char_p, []),
-    'fk_find_conn': (ctypes.c_int, [_P, _I64, _I32, _I32, _P, _P, _P, _I32, _P, _P, _P, _I32, _P, _P, _P, _P, _P]),
+    'fk_find_conn': (ctypes.c_int, [_P, _I64, _I32, _I32, _P, _P, _P, _I32, _P, _P, _P, _I32, _I32, _P, _P, _P, _P,
+                                    _P]),
     'fk_compact_scratch_bytes': (_I64, [_I64]),
     'fk_compact_connections': (ctypes.c_int, [_P, _I64, _I32, _P, _P, _P, _P]),
     'fk_local_energy': (ctypes.c_int, [_P, _P, _P, _P, _I64, _P, _P]),
@@ -44,16 +47,18 @@ SIGNATURES = {
                                          _P, _P, _P, _P, _P]),
     'fk_convar_forward_ws': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _I32,
                                             _P, _P, _P, _P, _P, _I64, _P]),
-    'fk_convar_forward_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32]),
+    'fk_convar_forward_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I32, _I32]),
     'fk_convar_num_schedules': (_I32, [_I32, _I32]),
     'fk_convar_schedule_info': (ctypes.c_int, [_I32, _I32, _I32, _P, _P]),
-    'fk_convar_set_schedule': (ctypes.c_int, [_I32, _I32, _I32, _I32, _I32]),
-    'fk_convar_get_schedule': (_I32, [_I32, _I32, _I32, _I32]),
+    'fk_convar_set_schedule': (ctypes.c_int, [_I32, _I32, _I32, _I32, _I32, _I32, _I32]),
+    'fk_convar_get_schedule': (_I32, [_I32, _I32, _I32, _I32, _I32, _I32]),
     'fk_ar_draw_site': (ctypes.c_int, [_P, _P, _I64, _I32, _I32, _P, _P]),
     'fk_fast_sampler_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I32, _I64]),
     'fk_fast_sample': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I64, _P, _P, _P, _P, _P]),
     'fk_convar_grad_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I64]),
     'fk_convar_grad': (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I64, _P, _P, _P, _P, _P]),
+    'fk_adam_step': (ctypes.c_int, [_P, _P, _P, _P, _I64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, _I64, _P]),
     'fk_o_mean': (ctypes.c_int, [_P, _P, _I64, _I64, _P, _P, _P]),
     'fk_o_times_v': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I64, _P, _P, _P]),
     'fk_oh_times_u': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I64, ctypes.c_double, ctypes.c_double, _P, _P,
@@ -64,10 +69,10 @@ SIGNATURES = {
     'fk_xpby': (ctypes.c_int, [_P, ctypes.c_double, _P, _I64, _P]),
     'fk_rbm_log_psi': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _I64, _I32, _P, _P, _P, _P]),
     'fk_mh_rbm': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I64, _I64, _I64, _I64, _P,
-                                 _P, _P, _P, _P, _P, _P, ctypes.c_uint64, _P]),
+                                 _P, _P, _P, _P, _P, _P, ctypes.c_uint64, _P, _P]),
     'fk_mh_propose': (ctypes.c_int, [_P, _I64, _I32, _I32, _I32, _P, _P, _P, _P]),
-    'fk_mh_accept': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I32, _P, _P]),
-    'fk_mh_accept_independent': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _I64, _I32, _P, _P]),
+    'fk_mh_accept': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I32, _P, _P, _P]),
+    'fk_mh_accept_independent': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _I64, _I32, _P, _P, _P]),
     'fk_made_packed_bytes': (_I64, [_I32, _I32]),
     'fk_made_pack': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
     'fk_made_log_prob': (ctypes.c_int, [_P, _I32, _I32, _P, _I64, _P, _P]),

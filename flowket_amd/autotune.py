@@ -3,10 +3,10 @@
 Every (channels, precision) has a few compiled launch schedules (loop order x
 waves per workgroup, fk_convar_num_schedules).  ``tune_forward`` times each on
 the caller's own workload -- events recorded on the stream the kernels are
-launched on -- selects the fastest for that lattice (fk_convar_set_schedule,
-process wide) and records it in a JSON cache keyed by device, lattice,
-channels, precision, activation and depth, so later processes start from the
-tuned choice (``apply_cached``).  The cache path is $FLOWKET_AMD_TUNE_CACHE or
+launched on -- selects the fastest for that network (fk_convar_set_schedule,
+keyed by lattice, depth, width, activation and precision) and records it in a
+JSON cache keyed by device and the same network description, so later
+processes start from the tuned choice (``apply_cached``).  The cache path is $FLOWKET_AMD_TUNE_CACHE or
 ~/.cache/flowket_amd/forward_schedules.json.  All schedules compute the same
 function; they differ only in FP32 summation order.
 """
@@ -63,12 +63,21 @@ def schedules(channels, prec):
     return out
 
 
+def _act(machine):
+    from .machines import _ACT
+    return _ACT[machine.activation]
+
+
 def set_schedule(machine, prec, index):
-    _lib.call('fk_convar_set_schedule', machine.H, machine.W, machine.num_of_channels, prec, int(index))
+    """Select schedule ``index`` for this network (lattice, depth, width,
+    activation, precision); other networks keep theirs."""
+    _lib.call('fk_convar_set_schedule', machine.H, machine.W, machine.depth, machine.num_of_channels, _act(machine),
+              prec, int(index))
 
 
 def get_schedule(machine, prec):
-    return int(_lib.load().fk_convar_get_schedule(machine.H, machine.W, machine.num_of_channels, prec))
+    return int(_lib.load().fk_convar_get_schedule(machine.H, machine.W, machine.depth, machine.num_of_channels,
+                                                  _act(machine), prec))
 
 
 def apply_cached(machine, prec):

@@ -35,12 +35,28 @@ def _compile(src):
     return obj
 
 
-def build(verbose=False):
+def _stale(src, headers_mtime):
+    obj = os.path.join(OBJ_DIR, os.path.basename(src) + '.o')
+    if not os.path.exists(obj):
+        return True
+    return os.path.getmtime(obj) < max(os.path.getmtime(src), headers_mtime)
+
+
+def build(verbose=False, changed_only=False):
+    """Compiles every source (or, with changed_only, those whose object is
+    older than the source or any header) and links the library."""
     os.makedirs(OBJ_DIR, exist_ok=True)
     srcs = sources()
-    jobs = min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
+    todo = srcs
+    if changed_only:
+        hdrs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith('.h')]
+        hdrs.append(os.path.join(HERE, '..', 'include', 'flowket_hip.h'))
+        hm = max(os.path.getmtime(h) for h in hdrs)
+        todo = [s for s in srcs if _stale(s, hm)]
+    jobs = min(max(len(todo), 1), max(1, min(8, os.cpu_count() or 1)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        list(ex.map(_compile, todo))
+    objs = [os.path.join(OBJ_DIR, os.path.basename(s) + '.o') for s in srcs]
     tmp = LIB + '.tmp'
     cmd = [HIPCC, '-shared', '-fPIC', f'--offload-arch={ARCH}', '-o', tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -53,5 +69,5 @@ def build(verbose=False):
 
 
 if __name__ == '__main__':
-    build(verbose=True)
+    build(verbose=True, changed_only='--changed' in sys.argv)
     sys.exit(0)

@@ -3,6 +3,7 @@
 // resident in LDS.  The kernel template is in convar_fwd_kernel.h.
 // HBM traffic: H*W int8 spins in, 8 bytes out per configuration.
 #include <map>
+#include <utility>
 #include <mutex>
 
 #include "convar_fwd_kernel.h"
@@ -45,17 +46,21 @@ static int fwd_schedules(int C, int prec, const FwdSchedule** out) {
   return 2;
 }
 
+// Selected schedule per network: lattice, depth, kernel channel width,
+// activation and precision (two networks that differ in any of them keep
+// their own choice, so tuning one never changes the other's summation order).
 static std::mutex g_sched_mu;
-static std::map<uint64_t, int> g_sched;
+static std::map<std::pair<uint64_t, uint64_t>, int> g_sched;
 
-static uint64_t sched_key(int H, int W, int C, int prec) {
-  return ((uint64_t)(uint32_t)H << 40) | ((uint64_t)(uint32_t)W << 20) | ((uint64_t)(uint32_t)C << 4) |
-         (uint64_t)(prec & 15);
+static std::pair<uint64_t, uint64_t> sched_key(int H, int W, int depth, int C, int act, int prec) {
+  return {((uint64_t)(uint32_t)H << 32) | (uint64_t)(uint32_t)W,
+          ((uint64_t)(uint32_t)depth << 32) | ((uint64_t)(uint32_t)C << 8) | ((uint64_t)(act & 15) << 4) |
+              (uint64_t)(prec & 15)};
 }
 
-static int current_schedule(int H, int W, int C, int prec) {
+static int current_schedule(int H, int W, int depth, int C, int act, int prec) {
   std::lock_guard<std::mutex> g(g_sched_mu);
-  auto it = g_sched.find(sched_key(H, W, C, prec));
+  auto it = g_sched.find(sched_key(H, W, depth, C, act, prec));
   return it == g_sched.end() ? 0 : it->second;
 }
 
@@ -220,10 +225,10 @@ static int hg_waves(const FwdSchedule& sc) { return sc.sched == 0 ? sc.nwave : 8
 
 // Bytes of global scratch the HG kernels need for a lattice (0 when the five
 // lattices fit in LDS under the schedule currently selected for it).
-static int64_t forward_workspace_bytes(int H, int W, int CP, int prec, FwdPlan* hg_plan) {
+static int64_t forward_workspace_bytes(int H, int W, int depth, int CP, int act, int prec, FwdPlan* hg_plan) {
   const FwdSchedule* table = nullptr;
   const int nsched = fwd_schedules(CP, prec, &table);
-  int si = current_schedule(H, W, CP, prec);
+  int si = current_schedule(H, W, depth, CP, act, prec);
   if (si < 0 || si >= nsched) si = 0;
   FwdPlan pl;
   if (plan_forward(H * W, CP, prec, table[si].nwave, 1 << 30, &pl) == FK_OK) return 0;
@@ -251,7 +256,7 @@ static int convar_forward(const void* wpacked, const float* aux, int32_t precisi
   const int CP = kernel_channels(channels, precision);
   const FwdSchedule* table = nullptr;
   const int nsched = fwd_schedules(CP, precision, &table);
-  int si = current_schedule(H, W, CP, precision);
+  int si = current_schedule(H, W, depth, CP, activation, precision);
   if (si < 0 || si >= nsched) si = 0;
   const FwdSchedule sc = table[si];
   FwdArgs a;
@@ -276,7 +281,7 @@ static int convar_forward(const void* wpacked, const float* aux, int32_t precisi
   if (plan_forward(H * W, CP, precision, sc.nwave, m_count, &pl) != FK_OK) {
     // horizontal stack in global scratch
     FwdPlan hp;
-    const int64_t need = forward_workspace_bytes(H, W, CP, precision, &hp);
+    const int64_t need = forward_workspace_bytes(H, W, depth, CP, activation, precision, &hp);
     if (need < 0) {
       set_error("ConvNetAutoregressive2D kernels: lattice %dx%d with %d channels does not fit the LDS even with the "
                 "horizontal stack in global scratch", H, W, channels);
@@ -329,9 +334,10 @@ int fk_convar_forward_ws(const void* wpacked, const float* aux, int32_t precisio
                         conn_kind, conn_a, conn_b, logpsi, cond_logp, workspace, workspace_bytes, stream);
 }
 
-int64_t fk_convar_forward_workspace_bytes(int32_t H, int32_t W, int32_t channels, int32_t precision) {
-  if (H < 1 || W < 1 || !convar_supported(channels, precision)) return -1;
-  return forward_workspace_bytes(H, W, kernel_channels(channels, precision), precision, nullptr);
+int64_t fk_convar_forward_workspace_bytes(int32_t H, int32_t W, int32_t depth, int32_t channels, int32_t activation,
+                                          int32_t precision) {
+  if (H < 1 || W < 1 || depth < 1 || !convar_supported(channels, precision)) return -1;
+  return forward_workspace_bytes(H, W, depth, kernel_channels(channels, precision), activation, precision, nullptr);
 }
 
 int32_t fk_convar_num_schedules(int32_t channels, int32_t precision) {
@@ -352,19 +358,21 @@ int fk_convar_schedule_info(int32_t channels, int32_t precision, int32_t schedul
   return FK_OK;
 }
 
-int fk_convar_set_schedule(int32_t H, int32_t W, int32_t channels, int32_t precision, int32_t schedule) {
+int fk_convar_set_schedule(int32_t H, int32_t W, int32_t depth, int32_t channels, int32_t activation,
+                           int32_t precision, int32_t schedule) {
   const int n = fk_convar_num_schedules(channels, precision);
   FK_REQUIRE(n > 0, "fk_convar_set_schedule: unsupported channels/precision");
   FK_REQUIRE(schedule >= -1 && schedule < n, "fk_convar_set_schedule: schedule out of range");
   const int CP = kernel_channels(channels, precision);
   std::lock_guard<std::mutex> g(g_sched_mu);
-  if (schedule < 0) g_sched.erase(sched_key(H, W, CP, precision));
-  else g_sched[sched_key(H, W, CP, precision)] = schedule;
+  if (schedule < 0) g_sched.erase(sched_key(H, W, depth, CP, activation, precision));
+  else g_sched[sched_key(H, W, depth, CP, activation, precision)] = schedule;
   return FK_OK;
 }
 
-int32_t fk_convar_get_schedule(int32_t H, int32_t W, int32_t channels, int32_t precision) {
-  return current_schedule(H, W, kernel_channels(channels, precision), precision);
+int32_t fk_convar_get_schedule(int32_t H, int32_t W, int32_t depth, int32_t channels, int32_t activation,
+                               int32_t precision) {
+  return current_schedule(H, W, depth, kernel_channels(channels, precision), activation, precision);
 }
 
 }  // extern "C"

@@ -34,15 +34,76 @@ int device_cu_count() {
 // sample: for each connection slot c a wave writes mel[c][b0..] and
 // use[c][b0..] as one contiguous run (the slot-major layout of the dense
 // contract), the slots shared out over the waves.
-// Diagonal element: sequential double accumulation in table order per sample
-// (exact for the integer/dyadic couplings of the reference models, and the
-// same order as the reference's loop otherwise).
+// Diagonal element: FK_DIAG_SUM_F64 accumulates in double in table order per
+// sample (exact for the integer/dyadic couplings of Heisenberg and J1J2);
+// FK_DIAG_ISING_F32 reproduces Ising.find_conn's float32 arithmetic
+// (operators/ising.py:25,30-41, see IsingDiagCursor / pairwise_sum).
+
+// Per-site values of the Ising diagonal, produced in ascending site order:
+// mel is a float32 array (ising.py:25) from which each bond term -j*s*s' is
+// subtracted in place (:33,37,39), i.e. computed in double and rounded to
+// float32 after every term.  The table lists site a's terms in the
+// reference's order (down, then right) and is grouped by a.
+struct IsingDiagCursor {
+  const int2* ab;
+  const double* jj;
+  const int8_t* row;
+  int n_diag, k, site;
+  __device__ __forceinline__ float next() {
+    float v = 0.0f;
+    while (k < n_diag && ab[k].x == site) {
+      v = (float)((double)v + jj[k] * (double)row[ab[k].x] * (double)row[ab[k].y]);
+      ++k;
+    }
+    ++site;
+    return v;
+  }
+};
+
+// numpy's float32 pairwise summation of n consecutive values (the inner loop
+// of ndarray.sum over contiguous axes, float32 accumulators): fewer than 8
+// values are added one by one; up to 128 go to 8 interleaved partial sums
+// combined as ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)) and then the remainder;
+// longer runs split at n/2 rounded down to a multiple of 8.  The values are
+// consumed in ascending order, so the cursor produces them on demand.
+__device__ __forceinline__ float pairwise_leaf(IsingDiagCursor& c, int n) {
+  if (n < 8) {
+    float res = 0.0f;
+    for (int i = 0; i < n; ++i) res += c.next();
+    return res;
+  }
+  float r[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) r[q] = c.next();
+  int i = 8;
+  for (; i < n - (n % 8); i += 8) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r[q] += c.next();
+  }
+  float res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; ++i) res += c.next();
+  return res;
+}
+template <int LEVEL>
+__device__ __noinline__ float pairwise_sum(IsingDiagCursor& c, int n) {
+  if constexpr (LEVEL == 0) {
+    return pairwise_leaf(c, n);  // n <= 128 here (n_sites <= 65536)
+  } else {
+    if (n <= 128) return pairwise_leaf(c, n);
+    int n2 = n / 2;
+    n2 -= n2 % 8;
+    const float a = pairwise_sum<LEVEL - 1>(c, n2);
+    const float b = pairwise_sum<LEVEL - 1>(c, n - n2);
+    return a + b;
+  }
+}
+
 __global__ __launch_bounds__(1024) void find_conn_kernel(
     const int8_t* __restrict__ spins, int64_t batch, int n_sites, int tile, int kind,
     const int32_t* __restrict__ conn_a, const int32_t* __restrict__ conn_b,
     const double* __restrict__ conn_mel, int n_conn,
     const int32_t* __restrict__ diag_a, const int32_t* __restrict__ diag_b,
-    const double* __restrict__ diag_j, int n_diag,
+    const double* __restrict__ diag_j, int n_diag, int diag_rule,
     int8_t* __restrict__ conn_out, double* __restrict__ mel_out,
     uint8_t* __restrict__ use_out, int32_t* __restrict__ n_used_out) {
   // LDS: the diagonal bond table (site pairs, couplings), then the tile's
@@ -69,10 +130,15 @@ __global__ __launch_bounds__(1024) void find_conn_kernel(
   // workgroup): the off-diagonal slots
   if (wave == 0) {
     double d = 0.0;
+    if (diag_rule == FK_DIAG_ISING_F32) {
+      IsingDiagCursor cur{s_dab, s_dj, row, n_diag, 0, 0};
+      d = (double)pairwise_sum<10>(cur, n_sites);
+    } else {
 #pragma unroll 8
-    for (int k = 0; k < n_diag; ++k) {
-      const int2 ab = s_dab[k];
-      d += s_dj[k] * (double)row[ab.x] * (double)row[ab.y];
+      for (int k = 0; k < n_diag; ++k) {
+        const int2 ab = s_dab[k];
+        d += s_dj[k] * (double)row[ab.x] * (double)row[ab.y];
+      }
     }
     if (ok) {
       mel_out[b] = d;
@@ -376,13 +442,16 @@ const char* fk_last_error(void) { return g_err; }
 int fk_find_conn(const int8_t* spins, int64_t batch, int32_t n_sites, int32_t kind,
                  const int32_t* conn_a, const int32_t* conn_b, const double* conn_mel, int32_t n_conn,
                  const int32_t* diag_a, const int32_t* diag_b, const double* diag_j, int32_t n_diag,
-                 int8_t* conn_out, double* mel_out, uint8_t* use_out, int32_t* n_used_out, void* stream) {
+                 int32_t diag_rule, int8_t* conn_out, double* mel_out, uint8_t* use_out, int32_t* n_used_out,
+                 void* stream) {
   FK_REQUIRE(batch >= 0 && n_sites > 0 && n_sites <= 65536 && n_conn >= 1, "fk_find_conn: bad sizes");
   if (batch == 0) return FK_OK;
   FK_REQUIRE(spins && conn_a && conn_b && conn_mel && mel_out && use_out, "fk_find_conn: null pointer");
   FK_REQUIRE(kind == FK_CONN_SWAP || kind == FK_CONN_SWAP_ZERO_PAD || kind == FK_CONN_FLIP,
              "fk_find_conn: bad kind %d", kind);
   FK_REQUIRE(n_diag == 0 || (diag_a && diag_b && diag_j), "fk_find_conn: null diagonal table");
+  FK_REQUIRE(diag_rule == FK_DIAG_SUM_F64 || diag_rule == FK_DIAG_ISING_F32, "fk_find_conn: bad diag_rule %d",
+             diag_rule);
   if (batch == 0) return FK_OK;
   FK_REQUIRE(batch <= 0x7fffffff, "fk_find_conn: batch too large");
   // tile of samples per workgroup: the diagonal table and their spin rows
@@ -392,7 +461,7 @@ int fk_find_conn(const int8_t* spins, int64_t batch, int32_t n_sites, int32_t ki
   const size_t lds = 16 * (size_t)n_diag + (size_t)tile * n_sites;
   hipLaunchKernelGGL(find_conn_kernel, dim3((unsigned)div_up(batch, tile)), dim3(1024), lds, (hipStream_t)stream,
                      spins, batch, n_sites, tile, kind, conn_a, conn_b, conn_mel, n_conn, diag_a, diag_b, diag_j,
-                     n_diag, conn_out, mel_out, use_out, n_used_out);
+                     n_diag, diag_rule, conn_out, mel_out, use_out, n_used_out);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }

@@ -134,6 +134,7 @@ struct MhArgs {
   const int32_t* r_move;   // exchange: (dy+1)*3 + (dx+1)
   const double* r_u;       // acceptance uniforms
   uint64_t seed;
+  int32_t* nonfinite;      // set to 1 when a proposal's log psi is not finite (or null)
 };
 
 template <int HPL>  // hidden units per lane (M <= 64 * HPL)
@@ -283,9 +284,13 @@ __global__ __launch_bounds__(64) void mh_rbm_kernel(MhArgs a) {
       dre += -2.0f * s1 * a1;
       if (flip2) dre += -2.0f * (float)s_state[j2] * a2;
     }
-    // ---- acceptance (samplers/metropolis_hastings.py:106-110)
+    // ---- acceptance (samplers/metropolis_hastings.py:106-110); a
+    // candidate whose log psi is not finite is reported (the reference
+    // raises, :103-105) and never accepted
+    const bool finite = isfinite(dre);
+    if (!finite && a.nonfinite && lane == 0) a.nonfinite[0] = 1;
     const float lr = fminf(2.0f * dre, 0.0f);
-    const bool accept = (double)expf(lr) > u;
+    const bool accept = finite && (double)expf(lr) > u;
     if (accept) {
       if (flip1) {
 #pragma unroll
@@ -369,11 +374,14 @@ __global__ void mh_propose_kernel(const int8_t* __restrict__ state, int64_t chai
 
 __global__ void mh_accept_kernel(int8_t* __restrict__ state, float* __restrict__ logpsi_cur,
                                  const int8_t* __restrict__ cand, const float* __restrict__ logpsi_cand,
-                                 const double* __restrict__ u, int64_t chains, int N, int32_t* __restrict__ accepted) {
+                                 const double* __restrict__ u, int64_t chains, int N, int32_t* __restrict__ accepted,
+                                 int32_t* __restrict__ nonfinite) {
   const int64_t ch = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (ch >= chains) return;
+  const bool finite = isfinite(logpsi_cand[2 * ch]) && isfinite(logpsi_cand[2 * ch + 1]);
+  if (!finite && nonfinite) nonfinite[0] = 1;
   const float lr = fminf(2.0f * (logpsi_cand[2 * ch] - logpsi_cur[2 * ch]), 0.0f);
-  const bool acc = (double)expf(lr) > u[ch];
+  const bool acc = finite && (double)expf(lr) > u[ch];
   accepted[ch] = acc ? 1 : 0;
   if (acc) {
     for (int j = 0; j < N; ++j) state[ch * N + j] = cand[ch * N + j];
@@ -388,11 +396,13 @@ __global__ void mh_accept_independent_kernel(int8_t* __restrict__ state, float* 
                                              float* __restrict__ logq_cur, const int8_t* __restrict__ cand,
                                              const float* __restrict__ logpsi_cand, const float* __restrict__ logq_cand,
                                              const double* __restrict__ u, int64_t chains, int N,
-                                             int32_t* __restrict__ accepted) {
+                                             int32_t* __restrict__ accepted, int32_t* __restrict__ nonfinite) {
   const int64_t ch = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (ch >= chains) return;
+  const bool finite = isfinite(logpsi_cand[2 * ch]) && isfinite(logpsi_cand[2 * ch + 1]);
+  if (!finite && nonfinite) nonfinite[0] = 1;
   const float lr = fminf(2.0f * (logpsi_cand[2 * ch] - logpsi_cur[2 * ch]) + (logq_cur[ch] - logq_cand[ch]), 0.0f);
-  const bool acc = (double)expf(lr) > u[ch];
+  const bool acc = finite && (double)expf(lr) > u[ch];
   accepted[ch] = acc ? 1 : 0;
   if (acc) {
     for (int j = 0; j < N; ++j) state[ch * N + j] = cand[ch * N + j];
@@ -428,7 +438,7 @@ int fk_mh_rbm(const float* w_re, const float* w_im, const float* b_re, const flo
               const float* a_im, int32_t n_visible, int32_t n_hidden, int32_t H, int32_t W, int32_t kind,
               int64_t chains, int64_t n_burn, int64_t n_record, int64_t unused, int8_t* state, int8_t* out,
               int64_t* accepts, float* logpsi, const int32_t* r_site, const int32_t* r_move, const double* r_u,
-              uint64_t seed, void* stream) {
+              uint64_t seed, int32_t* nonfinite, void* stream) {
   FK_REQUIRE(n_visible == H * W && n_visible > 0 && n_hidden > 0, "fk_mh_rbm: bad sizes");
   FK_REQUIRE(kind == 0 || kind == 1, "fk_mh_rbm: kind must be 0 (local) or 1 (exchange)");
   FK_REQUIRE(chains >= 0 && n_burn >= 0 && n_record >= 0 && unused >= 0, "fk_mh_rbm: bad counts");
@@ -454,6 +464,7 @@ int fk_mh_rbm(const float* w_re, const float* w_im, const float* b_re, const flo
   a.r_move = r_move;
   a.r_u = r_u;
   a.seed = seed;
+  a.nonfinite = nonfinite;
   const size_t lds = (size_t)((n_visible + 15) & ~15);
   hipStream_t s = (hipStream_t)stream;
   const int hpl = (n_hidden + 63) / 64;
@@ -481,24 +492,25 @@ int fk_mh_propose(const int8_t* state, int64_t chains, int32_t H, int32_t W, int
 }
 
 int fk_mh_accept(int8_t* state, float* logpsi_cur, const int8_t* candidates, const float* logpsi_cand,
-                 const double* uniforms, int64_t chains, int32_t n_sites, int32_t* accepted, void* stream) {
+                 const double* uniforms, int64_t chains, int32_t n_sites, int32_t* accepted, int32_t* nonfinite,
+                 void* stream) {
   if (chains <= 0) return FK_OK;
   FK_REQUIRE(state && logpsi_cur && candidates && logpsi_cand && uniforms && accepted, "fk_mh_accept: null pointer");
   hipLaunchKernelGGL(mh_accept_kernel, dim3(div_up(chains, 256)), dim3(256), 0, (hipStream_t)stream, state,
-                     logpsi_cur, candidates, logpsi_cand, uniforms, chains, n_sites, accepted);
+                     logpsi_cur, candidates, logpsi_cand, uniforms, chains, n_sites, accepted, nonfinite);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }
 
 int fk_mh_accept_independent(int8_t* state, float* logpsi_cur, float* logq_cur, const int8_t* candidates,
                              const float* logpsi_cand, const float* logq_cand, const double* uniforms, int64_t chains,
-                             int32_t n_sites, int32_t* accepted, void* stream) {
+                             int32_t n_sites, int32_t* accepted, int32_t* nonfinite, void* stream) {
   if (chains <= 0) return FK_OK;
   FK_REQUIRE(state && logpsi_cur && logq_cur && candidates && logpsi_cand && logq_cand && uniforms && accepted,
              "fk_mh_accept_independent: null pointer");
   hipLaunchKernelGGL(mh_accept_independent_kernel, dim3(div_up(chains, 256)), dim3(256), 0, (hipStream_t)stream,
                      state, logpsi_cur, logq_cur, candidates, logpsi_cand, logq_cand, uniforms, chains, n_sites,
-                     accepted);
+                     accepted, nonfinite);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }

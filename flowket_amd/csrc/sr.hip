@@ -5,6 +5,8 @@
 // and every reduction across a row or a vector is FP64.
 #include "fk_common.h"
 
+#include <cmath>
+
 namespace fk {
 
 // mean over rows of O_re and O_im (column means Obar)
@@ -213,11 +215,42 @@ __global__ void xpby_kernel(const float* __restrict__ r, double beta, float* __r
   if (i < n) p[i] = (float)((double)r[i] + beta * (double)p[i]);
 }
 
+// Keras/TensorFlow Adam (tf.keras Adam, the optimizer of
+// examples/basic_autoregressive_2d.py:24): with lr_t = lr*sqrt(1-b2^t)/(1-b1^t)
+// given by the host, m += (g - m)(1 - b1), v += (g^2 - v)(1 - b2),
+// p -= lr_t * m / (sqrt(v) + eps), all in FP32 as TF's ApplyAdam on float32
+// variables (epsilon added to the uncorrected sqrt(v)).
+__global__ void adam_step_kernel(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                                 const float* __restrict__ g, int64_t n, float lr_t, float one_m_b1, float one_m_b2,
+                                 float eps) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  float mi = m[i], vi = v[i];
+  mi += (gi - mi) * one_m_b1;
+  vi += (gi * gi - vi) * one_m_b2;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr_t * mi / (sqrtf(vi) + eps);
+}
+
 }  // namespace fk
 
 using namespace fk;
 
 extern "C" {
+int fk_adam_step(float* params, float* m, float* v, const float* grad, int64_t n, double lr, double beta_1,
+                 double beta_2, double epsilon, int64_t t, void* stream) {
+  FK_REQUIRE(params && m && v && grad, "fk_adam_step: null pointer");
+  FK_REQUIRE(t >= 1 && beta_1 >= 0 && beta_1 < 1 && beta_2 >= 0 && beta_2 < 1, "fk_adam_step: bad arguments");
+  if (n <= 0) return FK_OK;
+  const float lr_t = (float)(lr * sqrt(1.0 - pow(beta_2, (double)t)) / (1.0 - pow(beta_1, (double)t)));
+  hipLaunchKernelGGL(adam_step_kernel, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, params, m, v, grad,
+                     n, lr_t, (float)(1.0 - beta_1), (float)(1.0 - beta_2), (float)epsilon);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
 
 int fk_o_mean(const float* o_re, const float* o_im, int64_t batch, int64_t n_params, float* mean_re, float* mean_im,
               void* stream) {

@@ -160,7 +160,9 @@ class ExactSampler:
     def __next__(self):
         ev = self.exact_variational
         if self.uniforms is not None:
-            u = torch.as_tensor(self.uniforms, dtype=torch.float64).to(ev.device).contiguous()
+            u = torch.as_tensor(self.uniforms, dtype=torch.float64).reshape(-1).to(ev.device).contiguous()
+            if u.numel() != self.batch_size:
+                raise ValueError(f'ExactSampler: {u.numel()} injected uniforms for a batch of {self.batch_size}')
         else:
             u = torch.rand(self.batch_size, generator=self._gen, device=ev.device, dtype=torch.float64)
         out = torch.empty((self.batch_size, ev.number_of_spins), dtype=torch.int8, device=ev.device)

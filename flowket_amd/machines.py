@@ -262,9 +262,11 @@ class ConvNetAutoregressive2D:
     def _forward_workspace(self, prec):
         """Global scratch of the forward for lattices whose activations exceed
         the LDS (allocated once per precision; None when not needed)."""
-        key = (prec, _lib.load().fk_convar_get_schedule(self.H, self.W, self.num_of_channels, prec))
+        act = _ACT[self.activation]
+        key = (prec, _lib.load().fk_convar_get_schedule(self.H, self.W, self.depth, self.num_of_channels, act, prec))
         if key not in self._fws:
-            n = int(_lib.load().fk_convar_forward_workspace_bytes(self.H, self.W, self.num_of_channels, prec))
+            n = int(_lib.load().fk_convar_forward_workspace_bytes(self.H, self.W, self.depth, self.num_of_channels,
+                                                                   act, prec))
             if n < 0:
                 raise _lib.FlowketHipError(f'ConvNetAutoregressive2D kernels: {self.H}x{self.W} lattice with '
                                            f'{self.num_of_channels} channels is not supported')

@@ -67,6 +67,7 @@ class _BondTableOperator(OperatorOnGrid):
     and a diagonal table (diag_a, diag_b, diag_j) evaluated by fk_find_conn."""
 
     kind = _lib.FK_CONN_SWAP
+    diag_rule = _lib.FK_DIAG_SUM_F64
 
     def _set_tables(self, conn_a, conn_b, conn_mel, diag_a, diag_b, diag_j):
         self._conn_a = np.asarray(conn_a, dtype=np.int32)
@@ -116,7 +117,7 @@ class _BondTableOperator(OperatorOnGrid):
         conn = torch.empty((nc, B, n), dtype=torch.int8, device=dev) if want_conn else None
         _lib.call('fk_find_conn', _lib.ptr(spins), B, n, self.kind, _lib.ptr(tb['conn_a']), _lib.ptr(tb['conn_b']),
                   _lib.ptr(tb['conn_mel']), nc, _lib.ptr(tb['diag_a']), _lib.ptr(tb['diag_b']),
-                  _lib.ptr(tb['diag_j']), tb['n_diag'], _lib.ptr(conn), _lib.ptr(mel), _lib.ptr(use),
+                  _lib.ptr(tb['diag_j']), tb['n_diag'], self.diag_rule, _lib.ptr(conn), _lib.ptr(mel), _lib.ptr(use),
                   _lib.ptr(n_used), _lib.stream_ptr(stream))
         return dict(conn=conn, mel=mel, use=use, n_used=n_used)
 
@@ -205,9 +206,13 @@ class Heisenberg(_BondTableOperator):
 
 
 class Ising(_BondTableOperator):
-    """operators/ising.py:8-46: H = -j sum s s' - h sum sigma_x."""
+    """operators/ising.py:8-46: H = -j sum s s' - h sum sigma_x.  The diagonal
+    follows the reference's float32 arithmetic (ising.py:25-41: per-site
+    float32 values, float32 pairwise sum), bit for bit for any j, as the
+    reference computes it for float64 +-1 samples (the samplers' output)."""
 
     kind = _lib.FK_CONN_FLIP
+    diag_rule = _lib.FK_DIAG_ISING_F32
 
     def __init__(self, h=1.0, j=1.0, **kwargs):
         super().__init__(**kwargs)

@@ -154,13 +154,19 @@ class ComplexValuesStochasticReconfiguration:
 
 
 class Adam:
-    """Keras Adam (beta 0.9 / 0.999) on the machine's flat parameters with the
-    VMC energy gradient (configs 1 and 3 of the reference examples,
-    examples/basic_autoregressive_2d.py:24)."""
+    """Keras Adam (tf.keras.optimizers.Adam, the optimizer of the reference's
+    examples, e.g. examples/basic_autoregressive_2d.py:24) on the machine's
+    flat parameters with the VMC energy gradient.  The update is Keras's, not
+    torch's: lr_t = lr*sqrt(1-beta_2^t)/(1-beta_1^t) and
+    p -= lr_t*m/(sqrt(v)+epsilon), epsilon added to the uncorrected sqrt(v)
+    (fk_adam_step, FP32 on the device)."""
 
     def __init__(self, machine, lr=1e-3, beta_1=0.9, beta_2=0.999, epsilon=1e-7):
         self.machine = machine
-        self.opt = torch.optim.Adam([machine.params], lr=lr, betas=(beta_1, beta_2), eps=epsilon)
+        self.lr, self.beta_1, self.beta_2, self.epsilon = lr, beta_1, beta_2, epsilon
+        self.iterations = 0
+        self._m = torch.zeros_like(machine.params)
+        self._v = torch.zeros_like(machine.params)
 
     @property
     def params(self):
@@ -173,33 +179,32 @@ class Adam:
 
     def apply_gradient(self, g):
         """One Adam update of the machine's parameters with gradient g."""
-        self.machine.params.grad = g.to(self.machine.params.dtype)
-        self.opt.step()
-        self.machine.params.grad = None
+        p = self.machine.params
+        if p.device.type != 'cuda':
+            raise _lib.FlowketHipError('Adam updates run on the GPU (fk_adam_step); the parameters are on '
+                                       f'{p.device}')
+        _lib.require_gpu()
+        g = g.to(device=p.device, dtype=torch.float32).contiguous()
+        if g.numel() != p.numel():
+            raise ValueError(f'gradient has {g.numel()} values, the parameters {p.numel()}')
+        self.iterations += 1
+        _lib.call('fk_adam_step', _lib.ptr(p), _lib.ptr(self._m), _lib.ptr(self._v), _lib.ptr(g), p.numel(),
+                  float(self.lr), float(self.beta_1), float(self.beta_2), float(self.epsilon), self.iterations,
+                  _lib.stream_ptr())
         self.machine.mark_params_updated()
-        self.iterations = getattr(self, 'iterations', 0) + 1
 
     def get_weights(self):
         """Optimizer state as a list of tensors (Keras ``optimizer.weights``
         order: iterations, first moments, second moments)."""
-        p = self.machine.params
-        st = self.opt.state.get(p, {})
-        it = torch.tensor([float(getattr(self, 'iterations', 0))], dtype=torch.float64)
-        m = st.get('exp_avg', torch.zeros_like(p)).detach().clone()
-        v = st.get('exp_avg_sq', torch.zeros_like(p)).detach().clone()
-        return [it, m, v]
+        it = torch.tensor([float(self.iterations)], dtype=torch.float64)
+        return [it, self._m.detach().clone(), self._v.detach().clone()]
 
     def set_weights(self, weights):
         it, m, v = weights
         p = self.machine.params
-        n = int(torch.as_tensor(it).reshape(-1)[0].item())
-        self.iterations = n
-        if n == 0:
-            self.opt.state.pop(p, None)
-            return
-        self.opt.state[p] = {'step': torch.tensor(float(n)),
-                             'exp_avg': torch.as_tensor(m).to(p.device, p.dtype).reshape(p.shape).clone(),
-                             'exp_avg_sq': torch.as_tensor(v).to(p.device, p.dtype).reshape(p.shape).clone()}
+        self.iterations = int(torch.as_tensor(it).reshape(-1)[0].item())
+        self._m.copy_(torch.as_tensor(m).to(p.device, p.dtype).reshape(p.shape))
+        self._v.copy_(torch.as_tensor(v).to(p.device, p.dtype).reshape(p.shape))
 
 
 class SGD:

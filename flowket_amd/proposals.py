@@ -193,11 +193,12 @@ class MetropolisHastingsLearnedProposal(MetropolisHastingsSampler):
             accepted = torch.empty(C, dtype=torch.int32, device=m.device)
         _lib.call('fk_mh_accept_independent', _lib.ptr(self.sample), _lib.ptr(logpsi_cur), _lib.ptr(logq_cur),
                   _lib.ptr(cand), _lib.ptr(lc), _lib.ptr(lq), _lib.ptr(u), C, n, _lib.ptr(accepted),
-                  _lib.stream_ptr())
+                  _lib.ptr(self._nonfinite), _lib.stream_ptr())
         return accepted
 
     def _probe(self):
         m = self.machine
+        self._nonfinite_flag()
         lp, _ = m.forward(self.sample)
         lp = lp.contiguous()
         lq = self.proposal.log_prob(self.sample)
@@ -215,6 +216,7 @@ class MetropolisHastingsLearnedProposal(MetropolisHastingsSampler):
         C, n = self.num_of_chains, self.sample.shape[1]
         m = self.machine
         dev = m.device
+        self._nonfinite_flag()
         out = torch.empty((C, spc, n), dtype=torch.int8, device=dev)
         cand, lqc = self.proposal.sample(C * spc, generator=self._gen)
         lcand, _ = m.forward(cand)
@@ -241,6 +243,7 @@ class MetropolisHastingsLearnedProposal(MetropolisHastingsSampler):
             lp = lp_next
             out[:, i] = sweep_out[:, 0]
         self._calls += 1
+        self._raise_if_nonfinite()
         self.proposal_acceptance = float(acc_prop.sum().item()) / (C * spc)
         self.acceptance_ratio = float(acc_loc.sum().item()) / (C * spc * (self.unused_sampels + 1))
         return out.reshape((self.batch_size,) + self.input_size)

@@ -63,6 +63,9 @@ class AutoregressiveSampler(Sampler):
     def _draw_uniforms(self, device):
         if self.uniforms is not None:
             u = torch.as_tensor(self.uniforms, dtype=torch.float64).to(device)
+            n = int(np.prod(self.input_size))
+            if u.numel() != self.batch_size * n:
+                raise ValueError(f'{u.numel()} injected uniforms for a batch of {self.batch_size} x {n} sites')
             return u.reshape(self.batch_size, -1).contiguous()
         if self.generator is None:
             self.generator = torch.Generator(device=device)
@@ -180,6 +183,45 @@ class MetropolisHastingsSampler(Sampler):
         burn = (spc // self.discard_ratio) * self.unused_sampels if self.discard_ratio > 0 else 0
         return spc, burn
 
+    def _nonfinite_flag(self):
+        """Device flag the acceptance kernels set when a candidate's log psi
+        is not finite (cleared here; read by _raise_if_nonfinite)."""
+        if getattr(self, '_nonfinite', None) is None:
+            self._nonfinite = torch.zeros(1, dtype=torch.int32, device=self.machine.device)
+        else:
+            self._nonfinite.zero_()
+        return self._nonfinite
+
+    def _raise_if_nonfinite(self):
+        if int(self._nonfinite.item()):
+            raise Exception('candidates_machine_values has not finite element')
+
+    def _checked_randomness(self, steps, dev):
+        """The injected (sites, moves, uniforms) arrays as device tensors,
+        after checking that they cover every sweep and chain and index inside
+        the lattice (the kernels read them without bounds checks)."""
+        sites, moves, us = self.randomness
+        C, n = self.num_of_chains, self.sample.shape[1]
+        sites = torch.as_tensor(np.asarray(sites)).to(torch.int32)
+        us = torch.as_tensor(np.asarray(us), dtype=torch.float64)
+        if tuple(sites.shape) != (steps, C) or tuple(us.shape) != (steps, C):
+            raise ValueError(f'injected sites and uniforms must have shape {(steps, C)} (sweeps, chains), got '
+                             f'{tuple(sites.shape)} and {tuple(us.shape)}')
+        if sites.numel() and (int(sites.min()) < 0 or int(sites.max()) >= n):
+            raise ValueError(f'injected sites must lie in [0, {n})')
+        if self.kind == 1:
+            if moves is None:
+                raise ValueError('exchange moves need injected moves')
+            moves = torch.as_tensor(np.asarray(moves)).to(torch.int32)
+            if tuple(moves.shape) != (steps, C):
+                raise ValueError(f'injected moves must have shape {(steps, C)}')
+            if moves.numel() and (int(moves.min()) < 0 or int(moves.max()) > 8):
+                raise ValueError('injected moves must lie in [0, 9)')
+            moves = moves.to(dev).contiguous()
+        else:
+            moves = None
+        return sites.to(dev).contiguous(), moves, us.to(dev).contiguous()
+
     def __next__(self):
         _lib.require_gpu()
         spc, burn = self._counts()
@@ -187,21 +229,23 @@ class MetropolisHastingsSampler(Sampler):
         dev = self.machine.device
         out = torch.empty((self.num_of_chains, spc, n), dtype=torch.int8, device=dev)
         steps = burn + spc * (self.unused_sampels + 1)
+        flag = self._nonfinite_flag()
+        self._injected = None if self.randomness is None else self._checked_randomness(steps, dev)
         from .machines import RBM
         if isinstance(self.machine, RBM):
             accepts = torch.zeros(self.num_of_chains, dtype=torch.int64, device=dev)
             r_site = r_move = r_u = None
-            if self.randomness is not None:
-                r_site, r_move, r_u = [None if t is None else torch.as_tensor(t).to(dev).contiguous()
-                                       for t in self.randomness]
+            if self._injected is not None:
+                r_site, r_move, r_u = self._injected
             w = [_lib.ptr(t) for t in self.machine.parts()]
             _lib.call('fk_mh_rbm', *w, n, self.machine.n_hidden, self.H, self.W, self.kind, self.num_of_chains,
                       burn, spc, self.unused_sampels, _lib.ptr(self.sample), _lib.ptr(out), _lib.ptr(accepts), None,
                       _lib.ptr(r_site), _lib.ptr(r_move), _lib.ptr(r_u),
-                      (self.seed * 1000003 + self._calls) & 0xFFFFFFFFFFFFFFFF, _lib.stream_ptr())
+                      (self.seed * 1000003 + self._calls) & 0xFFFFFFFFFFFFFFFF, _lib.ptr(flag), _lib.stream_ptr())
             acc_total = int(accepts.sum().item())
         else:
             acc_total = self._generic(out, burn, spc, steps)
+        self._raise_if_nonfinite()
         self._calls += 1
         self.acceptance_ratio = acc_total / (self.batch_size * (self.unused_sampels + 1))
         return out.reshape((self.batch_size,) + self.input_size)
@@ -220,7 +264,8 @@ class MetropolisHastingsSampler(Sampler):
             w = [_lib.ptr(t) for t in self.machine.parts()]
             _lib.call('fk_mh_rbm', *w, n, self.machine.n_hidden, self.H, self.W, self.kind, C, 0, 1, n_prop - 1,
                       _lib.ptr(self.sample), _lib.ptr(out), _lib.ptr(accepts), _lib.ptr(logpsi), None, None, None,
-                      (self.seed * 1000003 + salt) & 0xFFFFFFFFFFFFFFFF, _lib.stream_ptr())
+                      (self.seed * 1000003 + salt) & 0xFFFFFFFFFFFFFFFF, _lib.ptr(self._nonfinite),
+                      _lib.stream_ptr())
             acc_total += accepts
             return logpsi is not None
         m = self.machine
@@ -239,18 +284,16 @@ class MetropolisHastingsSampler(Sampler):
                       _lib.ptr(move), _lib.ptr(cand), s)
             lc, _ = m.forward(cand)
             _lib.call('fk_mh_accept', _lib.ptr(self.sample), _lib.ptr(cur), _lib.ptr(cand), _lib.ptr(lc),
-                      _lib.ptr(u), C, n, _lib.ptr(accepted), s)
+                      _lib.ptr(u), C, n, _lib.ptr(accepted), _lib.ptr(self._nonfinite), s)
             acc_total += accepted
         out[:, 0] = self.sample
         return False
 
     def _draw(self, step, gen, dev):
         C, n = self.num_of_chains, self.sample.shape[1]
-        if self.randomness is not None:
-            sites, moves, us = self.randomness
-            return (torch.as_tensor(sites[step]).to(dev).to(torch.int32).contiguous(),
-                    None if moves is None else torch.as_tensor(moves[step]).to(dev).to(torch.int32).contiguous(),
-                    torch.as_tensor(us[step], dtype=torch.float64).to(dev).contiguous())
+        if self._injected is not None:
+            sites, moves, us = self._injected
+            return sites[step], None if moves is None else moves[step], us[step]
         site = torch.randint(0, n, (C,), generator=gen, device=dev, dtype=torch.int32)
         move = torch.randint(0, 9, (C,), generator=gen, device=dev, dtype=torch.int32) if self.kind == 1 else None
         u = torch.rand((C,), generator=gen, device=dev, dtype=torch.float64)
@@ -274,7 +317,7 @@ class MetropolisHastingsSampler(Sampler):
                       _lib.ptr(move), _lib.ptr(cand), s)
             lc, _ = m.forward(cand)
             _lib.call('fk_mh_accept', _lib.ptr(self.sample), _lib.ptr(cur), _lib.ptr(cand), _lib.ptr(lc),
-                      _lib.ptr(u), C, n, _lib.ptr(accepted), s)
+                      _lib.ptr(u), C, n, _lib.ptr(accepted), _lib.ptr(self._nonfinite), s)
             if step >= burn:
                 acc_total += accepted.sum()
                 rs = step - burn
@@ -284,22 +327,31 @@ class MetropolisHastingsSampler(Sampler):
 
     def calc_r_hat_value(self, estimated_values):
         """Gelman-Rubin R-hat, variogram correlations and ESS
-        (samplers/metropolis_hastings.py:67-89)."""
-        v = np.asarray(estimated_values.cpu() if isinstance(estimated_values, torch.Tensor) else estimated_values)
-        spc = self.batch_size // self.num_of_chains
-        per = v.reshape(self.num_of_chains, spc)
-        cm = per.mean(axis=1)
-        mean = cm.mean()
-        between = (spc / (self.num_of_chains - 1.0)) * ((cm - mean) ** 2).sum()
-        if spc == 1:
-            return 1.0, between, 0.0, self.batch_size
-        within = ((per - cm[:, None]) ** 2).sum(axis=1).mean() / (spc - 1)
-        var = ((spc - 1) * within + between) / spc
-        r_hat = np.sqrt(var / within)
-        vt = np.array([((per[:, t + 1:] - per[:, :spc - t - 1]) ** 2).mean(axis=1).mean() for t in range(spc - 1)])
-        corr = 1 - vt / (2 * var)
-        cs = sum_correlations(corr)
-        return r_hat, var, cs, self.batch_size / (1 + 2 * cs)
+        (samplers/metropolis_hastings.py:67-89) of this sampler's batch."""
+        return calc_r_hat_value(estimated_values, self.num_of_chains, self.batch_size)
+
+
+def calc_r_hat_value(estimated_values, num_of_chains, batch_size=None):
+    """Gelman-Rubin R-hat, variogram correlations and ESS of values laid out
+    (chains, samples per chain) (samplers/metropolis_hastings.py:67-89):
+    returns (r_hat, var, correlations_sum, effective_sample_size)."""
+    v = np.asarray(estimated_values.cpu() if isinstance(estimated_values, torch.Tensor) else estimated_values)
+    if batch_size is None:
+        batch_size = v.size
+    spc = batch_size // num_of_chains
+    per = v.reshape(num_of_chains, spc)
+    cm = per.mean(axis=1)
+    mean = cm.mean()
+    between = (spc / (num_of_chains - 1.0)) * ((cm - mean) ** 2).sum()
+    if spc == 1:
+        return 1.0, between, 0.0, batch_size
+    within = ((per - cm[:, None]) ** 2).sum(axis=1).mean() / (spc - 1)
+    var = ((spc - 1) * within + between) / spc
+    r_hat = np.sqrt(var / within)
+    vt = np.array([((per[:, t + 1:] - per[:, :spc - t - 1]) ** 2).mean(axis=1).mean() for t in range(spc - 1)])
+    corr = 1 - vt / (2 * var)
+    cs = sum_correlations(corr)
+    return r_hat, var, cs, batch_size / (1 + 2 * cs)
 
 
 class MetropolisHastingsLocal(MetropolisHastingsSampler):

@@ -33,6 +33,10 @@ extern "C" {
 #define FK_CONN_SWAP_ZERO_PAD 1 /* exchange a,b; unused slots all-zero (J1J2)   */
 #define FK_CONN_FLIP 2          /* flip spin a (transverse-field Ising)         */
 
+/* Arithmetic of the diagonal element in fk_find_conn. */
+#define FK_DIAG_SUM_F64 0   /* sum_k diag_j[k]*s_a*s_b in double, table order   */
+#define FK_DIAG_ISING_F32 1 /* Ising.find_conn's float32 rule (ising.py:25-41) */
+
 /* Activations of ConvNetAutoregressive2D. */
 #define FK_ACT_RELU 0
 #define FK_ACT_ELU 1
@@ -57,8 +61,18 @@ const char* fk_last_error(void);
  *   conn_a[c], conn_b[c]  (c = 0..n_conn-1; entry 0 is the diagonal, ignored)
  *       sites of connection c, -1 for "no bond" (use=0, mel=0)
  *   conn_mel[c]           off-diagonal element of connection c when it applies
- *   diag_a/diag_b/diag_j  the diagonal is sum_k diag_j[k]*s[diag_a[k]]*s[diag_b[k]]
- *                         accumulated in double in k order
+ *   diag_a/diag_b/diag_j  diagonal bond table; with diag_rule
+ *       FK_DIAG_SUM_F64    sum_k diag_j[k]*s[diag_a[k]]*s[diag_b[k]]
+ *                          accumulated in double in k order (Heisenberg,
+ *                          J1J2: exact for their integer/dyadic couplings);
+ *       FK_DIAG_ISING_F32  the reference's numpy float32 arithmetic
+ *                          (operators/ising.py:25,30-41): the entries are
+ *                          grouped by site (diag_a non-decreasing), each
+ *                          site's value starts at 0.0f and takes
+ *                          v = (float)((double)v + diag_j[k]*s_a*s_b) per
+ *                          entry in table order, and the n_sites values are
+ *                          summed in float32 in numpy's pairwise order
+ *                          (the ndarray.sum over the lattice axes, :41).
  * spins: (batch, n_sites) int8 +-1.
  * Outputs (any may be NULL except mel/use):
  *   conn_out (n_conn, batch, n_sites) int8, mel_out (n_conn, batch) double,
@@ -68,7 +82,7 @@ int fk_find_conn(const int8_t* spins, int64_t batch, int32_t n_sites,
                  int32_t kind, const int32_t* conn_a, const int32_t* conn_b,
                  const double* conn_mel, int32_t n_conn,
                  const int32_t* diag_a, const int32_t* diag_b,
-                 const double* diag_j, int32_t n_diag,
+                 const double* diag_j, int32_t n_diag, int32_t diag_rule,
                  int8_t* conn_out, double* mel_out, uint8_t* use_out,
                  int32_t* n_used_out, void* stream);
 
@@ -159,7 +173,8 @@ int fk_convar_forward(const void* wpacked, const float* aux, int32_t precision,
  * five activation lattices of one configuration exceed the LDS (e.g. 20x20
  * with 64 bf16 channels), the horizontal-stack lattices are kept in
  * `workspace`, a caller-owned device buffer of at least
- * fk_convar_forward_workspace_bytes(H, W, channels, precision) bytes (0 when
+ * fk_convar_forward_workspace_bytes(H, W, depth, channels, activation,
+ * precision) bytes (0 when
  * everything fits in LDS; then workspace may be NULL).  Calls that share a
  * workspace must be ordered on one stream.  fk_convar_forward returns
  * FK_ERR_ARG for such lattices.  (No counterpart in the reference: its
@@ -173,24 +188,29 @@ int fk_convar_forward_ws(const void* wpacked, const float* aux, int32_t precisio
                          float* logpsi, float* cond_logp, void* workspace,
                          int64_t workspace_bytes, void* stream);
 /* Host-only (queries the current device's CU count).  Bytes of workspace
- * fk_convar_forward_ws needs, 0 if none, -1 if the lattice is unsupported. */
-int64_t fk_convar_forward_workspace_bytes(int32_t H, int32_t W,
-                                          int32_t channels, int32_t precision);
+ * fk_convar_forward_ws needs for a network (under its selected schedule),
+ * 0 if none, -1 if the lattice is unsupported. */
+int64_t fk_convar_forward_workspace_bytes(int32_t H, int32_t W, int32_t depth,
+                                          int32_t channels, int32_t activation,
+                                          int32_t precision);
 
 /* Launch schedules of fk_convar_forward (no counterpart in the reference:
  * the knobs of this library's autotuner).  Each (channels, precision) has
  * fk_convar_num_schedules() compiled schedules (loop order tile_outer 0/1,
- * waves per workgroup); fk_convar_set_schedule selects one for a lattice
- * H x W (-1 restores the default, index 0) for every later forward launch of
- * the process; results are identical up to FP32 summation order.
+ * waves per workgroup); fk_convar_set_schedule selects one for a network
+ * (lattice H x W, depth, channels, activation, precision; -1 restores the
+ * default, index 0) for every later forward launch of that network in the
+ * process; results are identical up to FP32 summation order.
  * Host-only, thread-safe. */
 int32_t fk_convar_num_schedules(int32_t channels, int32_t precision);
 int fk_convar_schedule_info(int32_t channels, int32_t precision,
                             int32_t schedule, int32_t* tile_outer,
                             int32_t* waves);
-int fk_convar_set_schedule(int32_t H, int32_t W, int32_t channels,
+int fk_convar_set_schedule(int32_t H, int32_t W, int32_t depth,
+                           int32_t channels, int32_t activation,
                            int32_t precision, int32_t schedule);
-int32_t fk_convar_get_schedule(int32_t H, int32_t W, int32_t channels,
+int32_t fk_convar_get_schedule(int32_t H, int32_t W, int32_t depth,
+                               int32_t channels, int32_t activation,
                                int32_t precision);
 
 /* Plain autoregressive draw at one site (deepar/samplers/autoregressive.py:
@@ -266,6 +286,15 @@ int fk_dot_f64(const float* a, const float* b, int64_t n, double* scratch,
 int fk_axpy(double alpha, const float* x, float* y, int64_t n, void* stream);
 int fk_xpby(const float* r, double beta, float* p, int64_t n, void* stream);
 
+/* Adam step t (>= 1) of the Keras/TensorFlow optimizer the reference's
+ * examples train with (tf.keras.optimizers.Adam, examples/
+ * basic_autoregressive_2d.py:24): lr_t = lr*sqrt(1-beta_2^t)/(1-beta_1^t),
+ * m += (g-m)(1-beta_1), v += (g^2-v)(1-beta_2),
+ * params -= lr_t*m/(sqrt(v)+epsilon), elementwise in FP32 on n values. */
+int fk_adam_step(float* params, float* m, float* v, const float* grad,
+                 int64_t n, double lr, double beta_1, double beta_2,
+                 double epsilon, int64_t t, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Complex RBM (machines/rbm.py:35-48): log psi = sum_j a_j s_j +
  * sum_h lncosh(b_h + sum_j W_jh s_j) with each complex parameter stored as
@@ -292,14 +321,18 @@ int fk_rbm_log_psi(const float* w_re, const float* w_im, const float* b_re,
  * logpsi (chains, 2) final values or NULL.  Randomness: injected
  * (steps, chains) arrays r_site (flat site), r_move ((dy+1)*3+(dx+1),
  * exchange only) and r_u (uniforms), or, when r_u is NULL, a counter-based
- * generator keyed by `seed`. */
+ * generator keyed by `seed`.  A proposal whose log psi is not finite is
+ * rejected and sets *nonfinite = 1 (nonfinite may be NULL; the reference
+ * raises, samplers/metropolis_hastings.py:103-105); the same holds for
+ * fk_mh_accept and fk_mh_accept_independent. */
 int fk_mh_rbm(const float* w_re, const float* w_im, const float* b_re,
               const float* b_im, const float* a_re, const float* a_im,
               int32_t n_visible, int32_t n_hidden, int32_t H, int32_t W,
               int32_t kind, int64_t chains, int64_t n_burn, int64_t n_record,
               int64_t unused, int8_t* state, int8_t* out, int64_t* accepts,
               float* logpsi, const int32_t* r_site, const int32_t* r_move,
-              const double* r_u, uint64_t seed, void* stream);
+              const double* r_u, uint64_t seed, int32_t* nonfinite,
+              void* stream);
 
 /* Machine-agnostic MH steps (MetropolisHastingsSymmetricProposal._sweep,
  * samplers/metropolis_hastings.py:99-113): candidates from the chain states
@@ -311,7 +344,7 @@ int fk_mh_propose(const int8_t* state, int64_t chains, int32_t H, int32_t W,
 int fk_mh_accept(int8_t* state, float* logpsi_cur, const int8_t* candidates,
                  const float* logpsi_cand, const double* uniforms,
                  int64_t chains, int32_t n_sites, int32_t* accepted,
-                 void* stream);
+                 int32_t* nonfinite, void* stream);
 
 /* Independent-proposal acceptance with the exact ratio (no reference
  * counterpart: MetropolisHastingsGlobal, samplers/metropolis_hastings.py:
@@ -321,7 +354,7 @@ int fk_mh_accept_independent(int8_t* state, float* logpsi_cur, float* logq_cur,
                              const int8_t* candidates, const float* logpsi_cand,
                              const float* logq_cand, const double* uniforms,
                              int64_t chains, int32_t n_sites, int32_t* accepted,
-                             void* stream);
+                             int32_t* nonfinite, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Learned autoregressive proposal (no reference counterpart): a masked MLP

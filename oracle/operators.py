@@ -83,14 +83,18 @@ def ising_find_conn(sample, h=1.0, j=1.0, pbc=True):
     B = sample.shape[0]
     n = int(np.prod(shape))
     flat = sample.reshape(B, n)
+    # the arithmetic below is the reference's for float64 +-1 samples (what
+    # its samplers return): every -j*s*s' term in float64, subtracted in
+    # place from the float32 per-site array, then numpy's float32 sum
+    s64 = sample.astype(np.float64)
     conn = np.repeat(flat[None], n + 1, axis=0).astype(sample.dtype)
     for s in range(n):
         conn[1 + s, :, s] *= -1
     conn = conn.reshape((n + 1, B) + shape)
     if len(shape) == 1:
-        s2 = sample[:, :, None]
+        s2 = s64[:, :, None]
     else:
-        s2 = sample
+        s2 = s64
     H, W = s2.shape[1:]
     mode = 'wrap' if pbc else 'constant'
     padded = np.pad(s2, ((0, 0), (0, 1), (0, 1)), mode=mode)

@@ -106,14 +106,17 @@ def test_schedule_table(lib):
         seen.add((t.value, w.value))
     assert len(seen) == 4 and (0, 8) in seen
     assert L.fk_convar_schedule_info(32, 1, 4, ctypes.byref(t), ctypes.byref(w)) == 1
-    assert L.fk_convar_set_schedule(10, 10, 32, 1, 4) == 1
+    assert L.fk_convar_set_schedule(10, 10, 20, 32, 0, 1, 4) == 1
     assert b'out of range' in L.fk_last_error()
-    assert L.fk_convar_get_schedule(10, 10, 32, 1) == 0
-    assert L.fk_convar_set_schedule(10, 10, 32, 1, 2) == 0
-    assert L.fk_convar_get_schedule(10, 10, 32, 1) == 2
-    assert L.fk_convar_get_schedule(10, 9, 32, 1) == 0
-    assert L.fk_convar_set_schedule(10, 10, 32, 1, -1) == 0
-    assert L.fk_convar_get_schedule(10, 10, 32, 1) == 0
+    assert L.fk_convar_get_schedule(10, 10, 20, 32, 0, 1) == 0
+    assert L.fk_convar_set_schedule(10, 10, 20, 32, 0, 1, 2) == 0
+    assert L.fk_convar_get_schedule(10, 10, 20, 32, 0, 1) == 2
+    assert L.fk_convar_get_schedule(10, 9, 20, 32, 0, 1) == 0
+    # another network on the same lattice and width keeps its own schedule
+    assert L.fk_convar_get_schedule(10, 10, 19, 32, 0, 1) == 0
+    assert L.fk_convar_get_schedule(10, 10, 20, 32, 1, 1) == 0
+    assert L.fk_convar_set_schedule(10, 10, 20, 32, 0, 1, -1) == 0
+    assert L.fk_convar_get_schedule(10, 10, 20, 32, 0, 1) == 0
 
 
 def test_forward_workspace_query_lds_resident(lib):
@@ -121,6 +124,6 @@ def test_forward_workspace_query_lds_resident(lib):
     unsupported widths report -1."""
     from flowket_amd import _lib
     L = _lib.load()
-    assert L.fk_convar_forward_workspace_bytes(10, 10, 32, 1) == 0
-    assert L.fk_convar_forward_workspace_bytes(4, 4, 16, 0) == 0
-    assert L.fk_convar_forward_workspace_bytes(10, 10, 66, 1) == -1
+    assert L.fk_convar_forward_workspace_bytes(10, 10, 20, 32, 0, 1) == 0
+    assert L.fk_convar_forward_workspace_bytes(4, 4, 5, 16, 1, 0) == 0
+    assert L.fk_convar_forward_workspace_bytes(10, 10, 20, 66, 0, 1) == -1

@@ -35,11 +35,31 @@ def test_weights_round_trip(tmp_path):
         fk.ConvNetAutoregressive2D((4, 4), depth=2, num_of_channels=8, device='cpu').load_weights(path)
 
 
-@pytest.mark.parametrize('wrap', ['adam', 'accumulate_adam_ema', 'accumulate_sgd'])
+def test_adam_state_round_trip(tmp_path):
+    """Adam's state (iterations, moments) survives save + load into a fresh
+    optimizer (the updates themselves run on the GPU: tests/test_gpu_checkpoint.py)."""
+    import flowket_amd as fk
+    m = _machine()
+    opt = fk.Adam(m, lr=1e-2)
+    g = torch.Generator().manual_seed(2)
+    opt.set_weights([torch.tensor([5.0], dtype=torch.float64), torch.randn(m.n_params, generator=g),
+                     torch.rand(m.n_params, generator=g)])
+    path = str(tmp_path / 'ck')
+    fk.save_optimizer_weights(opt, path, epoch=1, batch=2)
+    opt2 = fk.Adam(_machine(seed=5))
+    assert fk.load_optimizer_weights(opt2, path) == (1, 2)
+    assert opt2.iterations == 5
+    for a, b in zip(opt.get_weights(), opt2.get_weights()):
+        assert torch.equal(a, b)
+    with pytest.raises(fk.FlowketHipError):
+        opt2.apply_gradient(torch.zeros(m.n_params))
+
+
+@pytest.mark.parametrize('wrap', ['accumulate_sgd'])
 def test_resumed_updates_equal_uninterrupted(tmp_path, wrap):
     """k updates, save, fresh machine and optimizer, load, the remaining
-    updates: parameters (and the EMA weights) equal those of an
-    uninterrupted run bit for bit."""
+    updates: parameters equal those of an uninterrupted run bit for bit
+    (host-side SGD; the Adam variants run on the GPU)."""
     import flowket_amd as fk
 
     def make(seed):
@@ -104,7 +124,7 @@ def test_safetensors_files_hold_no_code(tmp_path):
     import flowket_amd as fk
     m = _machine()
     opt = fk.Adam(m)
-    opt.apply_gradient(torch.ones(m.n_params))
+    opt.set_weights([torch.tensor([1.0], dtype=torch.float64), torch.ones(m.n_params), torch.ones(m.n_params)])
     fk.save_optimizer_weights(opt, str(tmp_path / 'x'), 0, 0)
     from safetensors import safe_open
     with safe_open(str(tmp_path / 'x_optimizer.safetensors'), framework='pt') as f:

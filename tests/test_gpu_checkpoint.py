@@ -61,3 +61,51 @@ def test_sigma_z_observables(rng):
         assert not s.imag.any()
         mean, var, _ = fk.AbsSigmaZ().estimate(None, torch.from_numpy(x))
         assert mean.real == pytest.approx(np.abs(ref).mean(), abs=1e-14)
+
+
+def test_adam_matches_keras_rule(rng):
+    """fk_adam_step over 20 steps equals the Keras Adam restatement
+    (epsilon on the uncorrected sqrt(v), bias-corrected learning rate), with
+    gradients over nine orders of magnitude so epsilon matters."""
+    import flowket_amd as fk
+    from oracle import optim as oopt
+    gpu_banner('adam vs keras rule')
+    m = fk.ConvNetAutoregressive2D((4, 4), depth=2, num_of_channels=8, weights_normalization=False, seed=1)
+    p0 = m.params.cpu().numpy().copy()
+    grads = [(rng.normal(size=m.n_params) * 10.0 ** rng.integers(-9, 1, size=m.n_params)).astype(np.float32)
+             for _ in range(20)]
+    opt = fk.Adam(m, lr=1e-2, epsilon=1e-7)
+    for g in grads:
+        opt.apply_gradient(torch.from_numpy(g))
+    ref = oopt.keras_adam(p0, grads, lr=1e-2, epsilon=1e-7)
+    got = m.params.cpu().numpy()
+    assert np.allclose(got, ref, rtol=2e-5, atol=1e-7)
+    assert opt.iterations == 20
+
+
+def test_adam_resume_equals_uninterrupted(tmp_path, rng):
+    """Plain Adam: 5 updates, save, load into fresh objects, 5 more: equal
+    to 10 uninterrupted updates bit for bit."""
+    import flowket_amd as fk
+    gpu_banner('adam resume')
+
+    def make(seed):
+        m = fk.ConvNetAutoregressive2D((4, 4), depth=3, num_of_channels=8, weights_normalization=False, seed=seed)
+        return m, fk.Adam(m, lr=1e-2)
+
+    grads = [torch.from_numpy(rng.normal(size=make(0)[0].n_params).astype(np.float32)) for _ in range(10)]
+    m_ref, o_ref = make(7)
+    for g in grads:
+        o_ref.apply_gradient(g)
+    m_a, o_a = make(7)
+    for g in grads[:5]:
+        o_a.apply_gradient(g)
+    path = str(tmp_path / 'ck')
+    fk.save_weights(m_a, path)
+    fk.save_optimizer_weights(o_a, path, epoch=0, batch=5)
+    m_b, o_b = make(99)
+    fk.load_weights(m_b, path)
+    fk.load_optimizer_weights(o_b, path)
+    for g in grads[5:]:
+        o_b.apply_gradient(g)
+    assert torch.equal(m_b.params, m_ref.params)

@@ -226,9 +226,10 @@ def test_global_scratch_forward_connected_and_workspace(rng):
     from flowket_amd import _lib
     gpu_banner('global-scratch forward 20x20 C64 bf16 connected')
     m = _machine((20, 20), 2, 64, 'relu', 'bf16')
-    n = int(_lib.load().fk_convar_forward_workspace_bytes(20, 20, 64, _lib.FK_PREC_BF16))
+    n = int(_lib.load().fk_convar_forward_workspace_bytes(20, 20, 2, 64, _lib.FK_ACT_RELU, _lib.FK_PREC_BF16))
     assert n > 0
-    assert int(_lib.load().fk_convar_forward_workspace_bytes(10, 10, 32, _lib.FK_PREC_BF16)) == 0
+    assert int(_lib.load().fk_convar_forward_workspace_bytes(10, 10, 2, 32, _lib.FK_ACT_RELU,
+                                                             _lib.FK_PREC_BF16)) == 0
     op = fk.Heisenberg(hilbert_state_shape=(20, 20), pbc=True)
     B = 6
     s = torch.as_tensor(rng.choice([-1, 1], size=(B, 400)).astype(np.int8)).cuda()
@@ -267,3 +268,22 @@ def test_global_scratch_forward_both_wave_counts(rng):
     finally:
         autotune.set_schedule(m, _lib.FK_PREC_BF16, -1)
     assert torch.equal(a, b)
+
+
+def test_schedule_is_per_network(rng):
+    """Selecting a schedule for one network leaves another network on the same
+    lattice and width (different depth) bit-identical."""
+    from flowket_amd import _lib, autotune
+    gpu_banner('schedule per network')
+    a = _machine((10, 10), 4, 32, 'relu', 'bf16', seed=1)
+    b = _machine((10, 10), 3, 32, 'relu', 'bf16', seed=2)
+    x = rng.choice([-1, 1], size=(64, 10, 10)).astype(np.int8)
+    before = b.log_psi(x).clone()
+    try:
+        for sc in autotune.schedules(32, _lib.FK_PREC_BF16)[1:]:
+            autotune.set_schedule(a, _lib.FK_PREC_BF16, sc['index'])
+            a.log_psi(x)
+            assert autotune.get_schedule(b, _lib.FK_PREC_BF16) == 0
+            assert torch.equal(b.log_psi(x), before)
+    finally:
+        autotune.set_schedule(a, _lib.FK_PREC_BF16, -1)

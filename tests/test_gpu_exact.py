@@ -184,3 +184,16 @@ def test_exact_evaluate_logs():
     mz = oex.all_states((4, 4)).reshape(65536, 16).sum(axis=1) / 16.0
     assert logs['observables/sigma_z'] == pytest.approx(float(np.dot(p, mz)), abs=1e-12)
     assert logs['observables/abs_sigma_z'] == pytest.approx(float(np.dot(p, np.abs(mz))), abs=1e-12)
+
+
+def test_exact_sampler_checks_injected_uniforms():
+    """A uniform array of the wrong length is refused before any launch."""
+    import flowket_amd as fk
+    gpu_banner('exact sampler uniforms length check')
+    m = fk.ConvNetAutoregressive2D((2, 3), depth=2, num_of_channels=16, weights_normalization=False, seed=0)
+    op = fk.Heisenberg(hilbert_state_shape=(2, 3), pbc=False)
+    ev = fk.ExactVariational(m, op, 64)
+    with pytest.raises(ValueError):
+        next(fk.ExactSampler(ev, 16, uniforms=np.full(15, 0.5)))
+    with pytest.raises(ValueError):
+        next(fk.AutoregressiveSampler(m, 4, uniforms=np.full((4, 5), 0.5)))

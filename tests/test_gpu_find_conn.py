@@ -34,7 +34,10 @@ def test_heisenberg_bit_exact(rng, shape, pbc, rotation):
 
 
 ISING_CASES = [((4, 4), True, 3.0, 1.0), ((4, 4), False, 2.5, 1.0), ((12, 12), False, 3.5, 1.0), ((16,), False, 3.0, 1.0),
-               ((16,), True, 1.0, 2.0), ((3, 1), True, 1.0, 1.0), ((5, 6), True, 0.75, -0.5)]
+               ((16,), True, 1.0, 2.0), ((3, 1), True, 1.0, 1.0), ((5, 6), True, 0.75, -0.5),
+               # non-dyadic couplings: the reference's float32 diagonal rule (ising.py:25-41)
+               ((10, 10), True, 3.0, 0.3), ((20, 20), False, 1.0, 0.3), ((300,), True, 1.0, 0.7),
+               ((12, 12), False, 2.0, 1.1), ((9, 1), False, 1.0, 0.1), ((17, 13), True, 0.5, -0.37)]
 
 
 @pytest.mark.parametrize('shape,pbc,h,j', ISING_CASES)

@@ -456,3 +456,160 @@ def test_made_bf16_round_matches_torch(rng):
     x = np.concatenate([rng.normal(scale=10.0, size=5000), [0.0, -0.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9]])
     ref = torch.as_tensor(x, dtype=torch.float32).bfloat16().double().numpy()
     assert np.array_equal(omd.bf16_round(x), ref)
+
+
+# ------------------------------------------------ connection-slot known answers
+# (slot, a, b) of every bond, transcribed by hand from
+# HeisenbergFindConn.calc_conn_and_mel (operators/heisenberg.py:63-68,86-121):
+# the mel/use views are reshaped (H, W, dim, B), so site (i, j) owns slots
+# 1 + (i*W + j)*dim + d, where d counts only the lattice axes longer than one
+# (down first, then right) and dim is the rank of the operator's shape.  A
+# slot not listed is never written: use False, mel 0, conn = the sample.
+HEIS_SLOT_KAT = {
+    ((2, 3), True): [(1, 0, 3), (2, 0, 1), (3, 1, 4), (4, 1, 2), (5, 2, 5), (6, 2, 0), (7, 3, 0), (8, 3, 4),
+                     (9, 4, 1), (10, 4, 5), (11, 5, 2), (12, 5, 3)],
+    ((2, 3), False): [(1, 0, 3), (2, 0, 1), (3, 1, 4), (4, 1, 2), (5, 2, 5), (8, 3, 4), (10, 4, 5)],
+    ((5,), True): [(1, 0, 1), (2, 1, 2), (3, 2, 3), (4, 3, 4), (5, 4, 0)],
+    ((5,), False): [(1, 0, 1), (2, 1, 2), (3, 2, 3), (4, 3, 4)],
+    ((1, 4), True): [(1, 0, 1), (3, 1, 2), (5, 2, 3), (7, 3, 0)],
+    ((1, 4), False): [(1, 0, 1), (3, 1, 2), (5, 2, 3)],
+    ((4, 1), True): [(1, 0, 1), (3, 1, 2), (5, 2, 3), (7, 3, 0)],
+}
+HEIS_NC = {(2, 3): 13, (5,): 6, (1, 4): 9, (4, 1): 9}
+
+
+@pytest.mark.parametrize('shape,pbc', list(HEIS_SLOT_KAT))
+def test_heisenberg_slot_known_answers(shape, pbc):
+    """Every slot of the oracle's Heisenberg find_conn against the hand
+    transcription: which bond it holds (the swapped pair), its use flag and
+    element, on all 2^N configurations (N <= 6)."""
+    n = int(np.prod(shape))
+    kat = {c: (a, b) for c, a, b in HEIS_SLOT_KAT[(shape, pbc)]}
+    states = np.array([[1 if (k >> q) & 1 else -1 for q in range(n)] for k in range(2 ** n)], np.float64)
+    conn, mel, use = oop.heisenberg_find_conn(states.reshape((-1,) + shape), pbc=pbc)
+    assert conn.shape[0] == HEIS_NC[shape]
+    conn = conn.reshape(conn.shape[0], 2 ** n, n)
+    for c in range(1, conn.shape[0]):
+        if c not in kat:
+            assert not use[c].any() and np.all(mel[c] == 0) and np.array_equal(conn[c], states)
+            continue
+        a, b = kat[c]
+        anti = states[:, a] != states[:, b]
+        swapped = states.copy()
+        swapped[:, a], swapped[:, b] = states[:, b], states[:, a]
+        assert np.array_equal(use[c], anti)
+        assert np.array_equal(mel[c], np.where(anti, -2.0, 0.0))
+        assert np.array_equal(conn[c], swapped)
+    diag = sum(states[:, a] * states[:, b] for a, b in kat.values())
+    assert np.array_equal(mel[0], diag)
+
+
+@pytest.mark.parametrize('shape,pbc', list(HEIS_SLOT_KAT))
+def test_heisenberg_operator_tables_known_answers(shape, pbc):
+    """The GPU operator's connection table (built on the host) holds the same
+    bonds in the same slots."""
+    import flowket_amd as fk
+    op = fk.Heisenberg(hilbert_state_shape=shape, pbc=pbc)
+    kat = {c: (a, b) for c, a, b in HEIS_SLOT_KAT[(shape, pbc)]}
+    assert op.num_connections == HEIS_NC[shape]
+    for c in range(1, op.num_connections):
+        got = (int(op._conn_a[c]), int(op._conn_b[c]))
+        assert got == kat.get(c, (-1, -1)), c
+
+
+def _numpy_pairwise_f32(vals):
+    """numpy's float32 pairwise summation written out (the algorithm the GPU
+    kernel implements): < 8 values one by one; <= 128 values in 8 interleaved
+    float32 partial sums, combined ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then
+    the remainder; longer runs split at n/2 rounded down to a multiple of 8."""
+    f = np.float32
+    n = len(vals)
+    if n < 8:
+        res = f(0.0)
+        for v in vals:
+            res = f(res + v)
+        return res
+    if n <= 128:
+        r = [f(v) for v in vals[:8]]
+        i = 8
+        while i < n - (n % 8):
+            for q in range(8):
+                r[q] = f(r[q] + vals[i + q])
+            i += 8
+        res = f(f(f(r[0] + r[1]) + f(r[2] + r[3])) + f(f(r[4] + r[5]) + f(r[6] + r[7])))
+        for v in vals[i:]:
+            res = f(res + v)
+        return res
+    n2 = n // 2
+    n2 -= n2 % 8
+    return f(_numpy_pairwise_f32(vals[:n2]) + _numpy_pairwise_f32(vals[n2:]))
+
+
+ISING_F32_CASES = [((10, 10), True, 0.3), ((20, 20), False, 0.3), ((300,), True, 0.7), ((12, 12), False, 1.1),
+                   ((1, 9), True, 0.1), ((9, 1), False, 0.1), ((5,), True, 0.3), ((17, 13), True, -0.37)]
+
+
+@pytest.mark.parametrize('shape,pbc,j', ISING_F32_CASES)
+def test_ising_diagonal_float32_rule(rng, shape, pbc, j):
+    """The oracle's Ising diagonal (numpy, as ising.py:25-41 computes it) equals
+    per-site float32 values folded term by term (each term in float64, rounded
+    after every subtraction) and summed by the pairwise rule above, bit for
+    bit, for non-dyadic j; and it differs from a plain float64 sum for some
+    sample, so the rule is observable."""
+    s = rng.choice([-1, 1], size=(40,) + shape).astype(np.float64)
+    _, mel, _ = oop.ising_find_conn(s, h=1.0, j=j, pbc=pbc)
+    s2 = s if len(shape) == 2 else s[:, :, None]
+    H, W = s2.shape[1:]
+    want = np.empty(40)
+    plain = np.empty(40)
+    for b in range(40):
+        vals, tot = [], 0.0
+        for i in range(H):
+            for q in range(W):
+                v = np.float32(0.0)
+                terms = []
+                if len(shape) == 1 or H > 1:
+                    terms.append(s2[b, (i + 1) % H, q] if (i + 1 < H or pbc) else 0.0)
+                if len(shape) == 2 and W > 1:
+                    terms.append(s2[b, i, (q + 1) % W] if (q + 1 < W or pbc) else 0.0)
+                for t in terms:
+                    v = np.float32(float(v) - j * s2[b, i, q] * t)
+                    tot -= j * s2[b, i, q] * t
+                vals.append(v)
+        want[b] = float(_numpy_pairwise_f32(vals))
+        plain[b] = tot
+    assert np.array_equal(mel[0], want)
+    if int(np.prod(shape)) >= 100:
+        assert not np.array_equal(mel[0], plain)
+
+
+@pytest.mark.parametrize('chains,spc', [(16, 64), (8, 5), (32, 1), (4, 2)])
+def test_calc_r_hat_value_matches_oracle(rng, chains, spc):
+    """The package's R-hat / ESS (samplers.calc_r_hat_value) equals the
+    oracle's restatement of metropolis_hastings.py:67-89 on fixed correlated
+    chains, including one sample per chain (R-hat 1, ESS = batch)."""
+    from flowket_amd.samplers import calc_r_hat_value
+    x = np.zeros((chains, spc))
+    x[:, 0] = rng.normal(size=chains)
+    for t in range(1, spc):
+        x[:, t] = 0.8 * x[:, t - 1] + rng.normal(size=chains)
+    x += rng.normal(size=(chains, 1)) * 0.3
+    got = calc_r_hat_value(x.reshape(-1), chains)
+    ref = osa.r_hat_and_ess(x.reshape(-1), chains)
+    assert np.allclose(np.array(got, dtype=float), np.array(ref, dtype=float), rtol=1e-12, atol=0)
+    if spc == 1:
+        assert got[0] == 1.0 and got[2] == 0.0 and got[3] == chains
+
+
+def test_keras_adam_first_step_closed_form(rng):
+    """Keras Adam's first step is p - lr*g/(|g| + eps/sqrt(1-beta_2)); a
+    torch-style update (eps after the bias correction) differs from it."""
+    from oracle import optim as oopt
+    p0 = rng.normal(size=50).astype(np.float32)
+    g = (rng.normal(size=50) * 10.0 ** rng.integers(-9, 1, size=50)).astype(np.float32)
+    lr, b2, eps = 1e-2, 0.999, 1e-7
+    got = oopt.keras_adam(p0, [g], lr=lr, beta_2=b2, epsilon=eps)
+    want = p0.astype(np.float64) - lr * g / (np.abs(g) + eps / np.sqrt(1 - b2))
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-9)
+    torch_style = p0.astype(np.float64) - lr * g / (np.abs(g) + eps)
+    assert not np.allclose(got, torch_style, rtol=1e-6, atol=1e-9)
