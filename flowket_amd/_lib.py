@@ -47,6 +47,11 @@ SIGNATURES = {
                                          _P, _P, _P, _P, _P]),
     'fk_convar_forward_ws': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _I32,
                                             _P, _P, _P, _P, _P, _I64, _P]),
+    'fk_bucket_scratch_bytes': (_I64, [_I64, _I32]),
+    'fk_bucket_connections': (ctypes.c_int, [_P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _P, _P]),
+    'fk_convar_rcache_bytes': (_I64, [_I32, _I32, _I32, _I32, _I32]),
+    'fk_convar_forward_rows': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _I32,
+                                              _P, _P, _I32, _P, _I32, _P, _P, _P, _P, _I64, _P]),
     'fk_convar_forward_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I32, _I32]),
     'fk_convar_num_schedules': (_I32, [_I32, _I32]),
     'fk_convar_schedule_info': (ctypes.c_int, [_I32, _I32, _I32, _P, _P]),
@@ -56,9 +61,20 @@ SIGNATURES = {
     'fk_fast_sampler_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I32, _I64]),
     'fk_fast_sample': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I64, _P, _P, _P, _P, _P]),
     'fk_convar_grad_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I64]),
-    'fk_convar_grad': (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I64, _P, _P, _P, _P, _P]),
+    'fk_convar_grad': (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I64, _P, _P, _P, _I64, _P, _P]),
+    'fk_cg_state_doubles': (_I64, []),
+    'fk_cg_init': (ctypes.c_int, [_P, _P, _P, _P, _I64, ctypes.c_double, _I64, _P, _P, _P]),
+    'fk_cg_step': (ctypes.c_int, [_P, _P, _P, _P, _I64, _P, _P, _P]),
+    'fk_center_pairs': (ctypes.c_int, [_P, _I64, _P, _P]),
     'fk_adam_step': (ctypes.c_int, [_P, _P, _P, _P, _I64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, _I64, _P]),
+    'fk_convar_vjp_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I64]),
+    'fk_convar_vjp': (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _P, _I64, _P, ctypes.c_double, _P, _P, _P,
+                                     _P]),
+    'fk_convar_jvp_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I64]),
+    'fk_convar_jvp': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _I64, _P, _P, _P, _P]),
+    'fk_wn_grad_rows': (ctypes.c_int, [_P, _I64, _I64, _P, _P, _P, _I32, _I32, _I32, _I64, _P]),
+    'fk_wn_tangent': (ctypes.c_int, [_P, _P, _P, _I32, _I32, _I32, _P, _P, _I64, _P]),
     'fk_o_mean': (ctypes.c_int, [_P, _P, _I64, _I64, _P, _P, _P]),
     'fk_o_times_v': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I64, _P, _P, _P]),
     'fk_oh_times_u': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I64, ctypes.c_double, ctypes.c_double, _P, _P,
@@ -70,6 +86,7 @@ SIGNATURES = {
     'fk_rbm_log_psi': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _I64, _I32, _P, _P, _P, _P]),
     'fk_mh_rbm': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I64, _I64, _I64, _I64, _P,
                                  _P, _P, _P, _P, _P, _P, ctypes.c_uint64, _P, _P]),
+    'fk_rbm_grad': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I32, _P, _I64, _P, _P, _I64, _P]),
     'fk_mh_propose': (ctypes.c_int, [_P, _I64, _I32, _I32, _I32, _P, _P, _P, _P]),
     'fk_mh_accept': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I32, _P, _P, _P]),
     'fk_mh_accept_independent': (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _I64, _I32, _P, _P, _P]),

@@ -53,9 +53,9 @@ def load_weights(machine, filepath):
     wn = getattr(machine, 'weights_normalization', False)
     if wn != ('wn_g' in t):
         raise ValueError(f'{filepath}: weight normalisation of the file and of the machine differ')
-    if wn:
-        machine.wn_g = t['wn_g']
     machine.set_params(t['params'])
+    if wn:
+        machine.set_wn_g(t['wn_g'])
     return machine
 
 

@@ -38,6 +38,11 @@ struct BwdArgs {
   float* o_im;
   float* logpsi;        // (batch, 2) or null
   int64_t P;
+  int64_t ld;           // row stride of o_re / o_im (>= P)
+  const float* cot;     // GRAD_VJP: (batch, 2) cotangents of (Re, Im) log psi
+  float* partial;       // GRAD_VJP: (gridDim.x, P) per-workgroup partial gradients
+  const float* tangent; // JVP: direction, P floats
+  float* jvp;           // JVP: (batch, 2) output
 };
 
 // Offsets of layer parameters in the flat vector.
@@ -86,7 +91,7 @@ __device__ __forceinline__ void tap_rc(int kind, int t, int& dr, int& dc) {
 // out[p][co] = b[co] + sum_{tap,ci} K[tap][ci][co] * in[p + off][ci]
 // (cin channels; in/out are [HW][cs] with channel strides cs_in/cs_out).
 __device__ void conv_fwd(float* out, int cs_out, const float* in, int cs_in, int cin, int cout, const float* K,
-                         const float* b, int kind, int ntap, int H, int W) {
+                         const float* b, int kind, int ntap, int H, int W, bool accumulate = false) {
   const int HW = H * W;
   for (int idx = threadIdx.x; idx < HW * cout; idx += blockDim.x) {
     const int p = idx / cout, co = idx - (idx / cout) * cout;
@@ -101,7 +106,8 @@ __device__ void conv_fwd(float* out, int cs_out, const float* in, int cs_in, int
       const float* k = K + (int64_t)t * cin * cout + co;
       for (int ci = 0; ci < cin; ++ci) acc += k[(int64_t)ci * cout] * src[ci];
     }
-    out[p * cs_out + co] = acc;
+    if (accumulate) out[p * cs_out + co] += acc;
+    else out[p * cs_out + co] = acc;
   }
 }
 
@@ -129,7 +135,7 @@ __device__ void conv_bwd_data(float* din, int cs_in, bool accumulate, const floa
 
 // dK[tap][ci][co] = sum_p in[p + off][ci] * dout[p][co]; db[co] = sum_p dout[p][co]
 __device__ void conv_bwd_weight(float* dK, float* db, const float* in, int cs_in, const float* dout, int cs_out,
-                                int cin, int cout, int kind, int ntap, int H, int W) {
+                                int cin, int cout, int kind, int ntap, int H, int W, bool accumulate = false) {
   const int HW = H * W;
   const int nk = ntap * cin * cout;
   for (int idx = threadIdx.x; idx < nk + cout; idx += blockDim.x) {
@@ -137,7 +143,8 @@ __device__ void conv_bwd_weight(float* dK, float* db, const float* in, int cs_in
       const int co = idx - nk;
       float acc = 0.0f;
       for (int p = 0; p < HW; ++p) acc += dout[p * cs_out + co];
-      db[co] = acc;
+      if (accumulate) db[co] += acc;
+      else db[co] = acc;
       continue;
     }
     const int t = idx / (cin * cout);
@@ -155,7 +162,8 @@ __device__ void conv_bwd_weight(float* dK, float* db, const float* in, int cs_in
         acc += in[(r * W + c) * cs_in + ci] * dout[(i * W + j) * cs_out + co];
       }
     }
-    dK[idx] = acc;
+    if (accumulate) dK[idx] += acc;
+    else dK[idx] = acc;
   }
 }
 
@@ -285,7 +293,7 @@ __global__ __launch_bounds__(256) void convar_grad_kernel(BwdArgs a) {
       }
     }
     __syncthreads();
-    float* O = (part == 0 ? a.o_re : a.o_im) + b * a.P;
+    float* O = (part == 0 ? a.o_re : a.o_im) + b * a.ld;
     // head weight / bias gradients and d act(h_F)
     for (int idx = threadIdx.x; idx < 4 * C + 4; idx += blockDim.x) {
       float acc = 0.0f;
@@ -456,7 +464,8 @@ __device__ __forceinline__ void wave_gemm(int mtiles, int ntiles, int kouter, in
 
 // out[p][co_off + co] = bias[co] + sum_{tap, ci} K[tap][ci][co] in[src(tap, p)][ci]
 __device__ void mconv_fwd(float* out, int cs_out, int co_off, const float* in, int cs_in, int cin, int cout,
-                          const float* K, const float* bias, const int16_t* tab, int ntap, int HW) {
+                          const float* K, const float* bias, const int16_t* tab, int ntap, int HW,
+                          bool accumulate = false) {
   const int lane = threadIdx.x & 63, l16 = lane & 15, kk = lane >> 4;
   wave_gemm(
       cout / 16, (HW + 15) / 16, ntap, cin / 4,
@@ -470,8 +479,11 @@ __device__ void mconv_fwd(float* out, int cs_out, int co_off, const float* in, i
         const int p = 16 * nt + l16;
         if (p >= HW) return;
         const int co = 16 * mt + 4 * kk;
-        const f32x4 b = *reinterpret_cast<const f32x4*>(bias + co);
-        *reinterpret_cast<f32x4*>(out + (int64_t)p * cs_out + co_off + co) = c + b;
+        f32x4 r = c;
+        if (bias) r += *reinterpret_cast<const f32x4*>(bias + co);
+        f32x4* dst = reinterpret_cast<f32x4*>(out + (int64_t)p * cs_out + co_off + co);
+        if (accumulate) r += *dst;
+        *dst = r;
       });
 }
 
@@ -497,7 +509,8 @@ __device__ void mconv_bwd_data(float* din, int cs_in, const float* dout, int cs_
 // dK[tap][ci][co] = sum_p in[src(tap, p)][ci] dout[p][co_off + co]  (D[co][ci]),
 // every tap's 2x2 tile blocks shared out over the waves; db[co] = sum_p dout.
 __device__ void mconv_bwd_weight(float* dK, float* db, const float* in, int cs_in, const float* dout, int cs_out,
-                                 int co_off, int cin, int cout, const int16_t* tab, int ntap, int HW) {
+                                 int co_off, int cin, int cout, const int16_t* tab, int ntap, int HW,
+                                 bool accumulate = false) {
   const int lane = threadIdx.x & 63, l16 = lane & 15, kk = lane >> 4;
   const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int mtiles = cout / 16, ntiles = cin / 16;
@@ -527,18 +540,32 @@ __device__ void mconv_bwd_weight(float* dK, float* db, const float* in, int cs_i
     }
     auto st = [&](int mt, int nt, f32x4 c) {
       const int ci = 16 * nt + l16, co = 16 * mt + 4 * kk;
-      *reinterpret_cast<f32x4*>(dK + ((int64_t)t * cin + ci) * cout + co) = c;
+      f32x4* dst = reinterpret_cast<f32x4*>(dK + ((int64_t)t * cin + ci) * cout + co);
+      *dst = accumulate ? *dst + c : c;
     };
     st(mt0, nt0, c00);
     if (n1) st(mt0, nt0 + 1, c01);
     if (m1) st(mt0 + 1, nt0, c10);
     if (m1 && n1) st(mt0 + 1, nt0 + 1, c11);
   }
-  for (int co = threadIdx.x; co < cout; co += blockDim.x) {
-    float acc = 0.0f;
-    for (int p = 0; p < HW; ++p) acc += dout[(int64_t)p * cs_out + co_off + co];
-    db[co] = acc;
+  // bias: db[co] = sum_p dout[p][co], every thread on a strided share of the
+  // pixels, the partial sums combined in a fixed order (deterministic)
+  __shared__ float s_red[256];
+  const int nparts = (int)blockDim.x / cout;
+  const int co = threadIdx.x % cout, part = threadIdx.x / cout;
+  float acc = 0.0f;
+  if (part < nparts)
+    for (int p = part; p < HW; p += nparts) acc += dout[(int64_t)p * cs_out + co_off + co];
+  __syncthreads();
+  s_red[threadIdx.x] = acc;
+  __syncthreads();
+  if ((int)threadIdx.x < cout) {
+    float t = 0.0f;
+    for (int k = 0; k < nparts; ++k) t += s_red[k * cout + threadIdx.x];
+    if (accumulate) db[threadIdx.x] += t;
+    else db[threadIdx.x] = t;
   }
+  __syncthreads();
 }
 
 // Per-sample workspace of the MFMA kernel: the scalar kernel's layout plus six
@@ -547,40 +574,21 @@ __device__ __forceinline__ float* ws_grad(float* ws, int HW, int C, int nb, int 
   return ws + ((HW + 3) & ~3) + ((int64_t)nb * 5 + which) * HW * C;
 }
 
-template <int ACT>
-__global__ __launch_bounds__(256) void convar_grad_mfma_kernel(BwdArgs a) {
-  extern __shared__ int16_t tabs[];  // GT_N forward rows, then GT_N inverse rows, of HW entries
-  __shared__ float s_lp[2];
-  const int H = a.H, W = a.W, HW = a.HW, C = a.C, nb = a.nb;
-  const int F = nb - 1;
-  const int64_t b = blockIdx.x;
-  if (b >= a.batch) return;
-  float* ws = a.work + (int64_t)blockIdx.x * a.work_floats;
-  int16_t* ftab = tabs;
-  int16_t* itab = tabs + GT_N * HW;
-  float* dz = reinterpret_cast<float*>(tabs + 2 * GT_N * HW + (2 * GT_N * HW & 1));
-  float* dV = ws_grad(ws, HW, C, nb, 0);
-  float* dH = ws_grad(ws, HW, C, nb, 1);
-  float* dVr = ws_grad(ws, HW, C, nb, 2);
-  float* dHr = ws_grad(ws, HW, C, nb, 3);
-  float* t0 = ws_grad(ws, HW, C, nb, 4);
-  float* t1 = ws_grad(ws, HW, C, nb, 5);
-  const float* prm = a.params;
-
-  // source tables (and their inverses) of every tap / shift
+// Source tables (and their inverses) of every tap / shift of the lattice.
+__device__ void build_tap_tables(int16_t* ftab, int16_t* itab, int H, int W) {
+  const int HW = H * W;
   for (int idx = threadIdx.x; idx < GT_N * HW; idx += blockDim.x) itab[idx] = -1;
   for (int idx = threadIdx.x; idx < GT_N * HW; idx += blockDim.x) {
     const int row = idx / HW, p = idx - row * HW;
     const int i = p / W, j = p - (p / W) * W;
     int dr = 0, dc = 0;
-    bool ok = true;
     if (row < GT_H) tap_rc(0, row - GT_V, dr, dc);
     else if (row < GT_C) tap_rc(1, row - GT_H, dr, dc);
     else if (row < GT_X) tap_rc(2, row - GT_C, dr, dc);
     else if (row == GT_XF) dc = -1;
     else if (row == GT_Y) dr = -1;
     const int r = i + dr, c = j + dc;
-    ok = r >= 0 && r < H && c >= 0 && c < W;
+    const bool ok = r >= 0 && r < H && c >= 0 && c < W;
     ftab[idx] = ok ? (int16_t)(r * W + c) : (int16_t)-1;
   }
   __syncthreads();
@@ -589,11 +597,19 @@ __global__ __launch_bounds__(256) void convar_grad_mfma_kernel(BwdArgs a) {
     const int src = ftab[idx];
     if (src >= 0) itab[row * HW + src] = (int16_t)p;
   }
-
-  // ------------------------------------------------------------ forward
-  float* sp = ws;
-  for (int q = threadIdx.x; q < HW; q += blockDim.x) sp[q] = (float)a.spins[b * HW + q];
   __syncthreads();
+}
+
+// Forward of one sample (MFMA form), every block's tensors stored in ws.
+// With a tangent direction (tan != null) the forward-mode derivative of every
+// tensor is carried along in wt (same layout as ws): for y = K*x + b,
+// y' = K*x' + Kt*x + bt, and act(y)' = act'(y) y'.  t0/t1 and u0/u1 are
+// scratch tensors of the primal and the tangent.
+template <int ACT>
+__device__ void mfma_forward(const float* prm, const float* tan, float* ws, float* wt, const int16_t* ftab, int H,
+                             int W, int C, int nb, float* t0, float* t1, float* u0, float* u1) {
+  const int HW = H * W, F = nb - 1;
+  const float* sp = ws;
   for (int blk = 0; blk < nb; ++blk) {
     const ParamOffsets o = param_offsets(blk, C);
     const bool first = blk == 0, fin = blk == F;
@@ -608,157 +624,363 @@ __global__ __launch_bounds__(256) void convar_grad_mfma_kernel(BwdArgs a) {
     if (first) {
       conv_fwd(t0, C, Vin, 1, 1, C, prm + o.kv, prm + o.bv, 0, 9, H, W);
       conv_fwd(t1, C, Hin, 1, 1, C, prm + o.kh, prm + o.bh, 1, 3, H, W);
+      if (tan) {  // the spins carry no tangent
+        conv_fwd(u0, C, Vin, 1, 1, C, tan + o.kv, tan + o.bv, 0, 9, H, W);
+        conv_fwd(u1, C, Hin, 1, 1, C, tan + o.kh, tan + o.bh, 1, 3, H, W);
+      }
     } else {
       mconv_fwd(t0, C, 0, Vin, C, C, C, prm + o.kv, prm + o.bv, ftab + GT_V * HW, 9, HW);
       mconv_fwd(t1, C, 0, Hin, C, C, C, prm + o.kh, prm + o.bh, ftab + GT_H * HW, 3, HW);
+      if (tan) {
+        const float* Vd = ws_tensor(wt, HW, C, blk - 1, T_VO);
+        const float* Hd = ws_tensor(wt, HW, C, blk - 1, T_HO);
+        mconv_fwd(u0, C, 0, Vd, C, C, C, prm + o.kv, tan + o.bv, ftab + GT_V * HW, 9, HW);
+        mconv_fwd(u1, C, 0, Hd, C, C, C, prm + o.kh, tan + o.bh, ftab + GT_H * HW, 3, HW);
+        __syncthreads();
+        mconv_fwd(u0, C, 0, Vin, C, C, C, tan + o.kv, nullptr, ftab + GT_V * HW, 9, HW, true);
+        mconv_fwd(u1, C, 0, Hin, C, C, C, tan + o.kh, nullptr, ftab + GT_H * HW, 3, HW, true);
+      }
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
       const float v = t0[idx];
-      VA[idx] = bact<ACT>(v);
-      XA[idx] = bact<ACT>(t1[idx]);
-      if (!fin) VO[idx] = kb ? bact<ACT>(ws_tensor(ws, HW, C, blk - 2, T_VO)[idx] + v) : bact<ACT>(v);
+      const float va = bact<ACT>(v), xa = bact<ACT>(t1[idx]);
+      VA[idx] = va;
+      XA[idx] = xa;
+      float vo = 0.0f;
+      if (!fin) {
+        vo = kb ? bact<ACT>(ws_tensor(ws, HW, C, blk - 2, T_VO)[idx] + v) : va;
+        VO[idx] = vo;
+      }
+      if (tan) {
+        const float dv = u0[idx];
+        ws_tensor(wt, HW, C, blk, T_VA)[idx] = bdact<ACT>(va) * dv;
+        ws_tensor(wt, HW, C, blk, T_XA)[idx] = bdact<ACT>(xa) * u1[idx];
+        if (!fin) {
+          const float din = kb ? ws_tensor(wt, HW, C, blk - 2, T_VO)[idx] + dv : dv;
+          ws_tensor(wt, HW, C, blk, T_VO)[idx] = bdact<ACT>(vo) * din;
+        }
+      }
     }
     __syncthreads();
     mconv_fwd(CAT, C, 0, XA, C, C, C / 2, prm + o.kx, prm + o.bx, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW);
     mconv_fwd(CAT, C, C / 2, VA, C, C, C / 2, prm + o.ky, prm + o.by, ftab + GT_Y * HW, 1, HW);
+    if (tan) {
+      float* CATd = ws_tensor(wt, HW, C, blk, T_CAT);
+      const float* XAd = ws_tensor(wt, HW, C, blk, T_XA);
+      const float* VAd = ws_tensor(wt, HW, C, blk, T_VA);
+      mconv_fwd(CATd, C, 0, XAd, C, C, C / 2, prm + o.kx, tan + o.bx, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW);
+      mconv_fwd(CATd, C, C / 2, VAd, C, C, C / 2, prm + o.ky, tan + o.by, ftab + GT_Y * HW, 1, HW);
+      __syncthreads();
+      mconv_fwd(CATd, C, 0, XA, C, C, C / 2, tan + o.kx, nullptr, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW, true);
+      mconv_fwd(CATd, C, C / 2, VA, C, C, C / 2, tan + o.ky, nullptr, ftab + GT_Y * HW, 1, HW, true);
+    }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) CAT[idx] = bact<ACT>(CAT[idx]);
+    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+      const float c = bact<ACT>(CAT[idx]);
+      CAT[idx] = c;
+      if (tan) ws_tensor(wt, HW, C, blk, T_CAT)[idx] *= bdact<ACT>(c);
+    }
     __syncthreads();
     mconv_fwd(t0, C, 0, CAT, C, C, C, prm + o.kc, prm + o.bc, ftab + GT_C * HW, 9, HW);
+    if (tan) {
+      mconv_fwd(u0, C, 0, ws_tensor(wt, HW, C, blk, T_CAT), C, C, C, prm + o.kc, tan + o.bc, ftab + GT_C * HW, 9,
+                HW);
+      __syncthreads();
+      mconv_fwd(u0, C, 0, CAT, C, C, C, tan + o.kc, nullptr, ftab + GT_C * HW, 9, HW, true);
+    }
     __syncthreads();
     for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
       const float h = t0[idx];
-      HO[idx] = kb ? bact<ACT>(ws_tensor(ws, HW, C, blk - 2, T_HO)[idx] + h) : bact<ACT>(h);
+      const float ho = kb ? bact<ACT>(ws_tensor(ws, HW, C, blk - 2, T_HO)[idx] + h) : bact<ACT>(h);
+      HO[idx] = ho;
+      if (tan) {
+        const float din = kb ? ws_tensor(wt, HW, C, blk - 2, T_HO)[idx] + u0[idx] : u0[idx];
+        ws_tensor(wt, HW, C, blk, T_HO)[idx] = bdact<ACT>(ho) * din;
+      }
     }
     __syncthreads();
   }
+}
 
-  // head and its cotangents (as in the scalar kernel)
+// Head logits z[k] (k = 0, 1: Re for spin +1 / -1; 2, 3: Im) of pixel p.
+__device__ __forceinline__ void head_logits(const float* prm, int64_t ho, const float* HF, int C, int p, float z[4]) {
+  for (int k = 0; k < 4; ++k) {
+    float acc = prm[ho + 4 * C + k];
+    for (int ci = 0; ci < C; ++ci) acc += prm[ho + ci * 4 + k] * HF[p * C + ci];
+    z[k] = acc;
+  }
+}
+
+// Backward of one sample from the head cotangent dz (HW x 4): d/d theta into
+// O (overwritten, or added to when accumulate).
+template <int ACT>
+__device__ void mfma_backward(const float* prm, float* ws, const int16_t* ftab, const int16_t* itab, const float* dz,
+                              int H, int W, int C, int nb, float* O, bool accumulate, float* dV, float* dH, float* dVr,
+                              float* dHr, float* t0, float* t1) {
+  const int HW = H * W, F = nb - 1;
+  const float* sp = ws;
   const int64_t ho = head_offset(nb, C);
   const float* HF = ws_tensor(ws, HW, C, F, T_HO);
-  if (threadIdx.x == 0) s_lp[0] = s_lp[1] = 0.0f;
+  for (int idx = threadIdx.x; idx < 4 * C + 4; idx += blockDim.x) {
+    float acc = 0.0f;
+    if (idx < 4 * C) {
+      const int ci = idx / 4, k = idx - (idx / 4) * 4;
+      for (int p = 0; p < HW; ++p) acc += HF[p * C + ci] * dz[p * 4 + k];
+    } else {
+      for (int p = 0; p < HW; ++p) acc += dz[p * 4 + idx - 4 * C];
+    }
+    if (accumulate) O[ho + idx] += acc;
+    else O[ho + idx] = acc;
+  }
+  for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+    const int p = idx / C, ci = idx - (idx / C) * C;
+    float g = 0.0f;
+    for (int k = 0; k < 4; ++k) g += prm[ho + ci * 4 + k] * dz[p * 4 + k];
+    dH[idx] = g;
+    dV[idx] = 0.0f;
+    dVr[idx] = 0.0f;
+    dHr[idx] = 0.0f;
+  }
   __syncthreads();
-  for (int part = 0; part < 2; ++part) {
+  for (int blk = F; blk >= 0; --blk) {
+    const ParamOffsets o = param_offsets(blk, C);
+    const bool first = blk == 0, fin = blk == F;
+    const bool kb = !first && !fin && ((blk & 1) == 0);
+    const bool ka = !first && !fin && ((blk & 1) == 1);
+    const float* Vin = first ? sp : ws_tensor(ws, HW, C, blk - 1, T_VO);
+    const float* Hin = first ? sp : ws_tensor(ws, HW, C, blk - 1, T_HO);
+    const float* VA = ws_tensor(ws, HW, C, blk, T_VA);
+    const float* XA = ws_tensor(ws, HW, C, blk, T_XA);
+    const float* CAT = ws_tensor(ws, HW, C, blk, T_CAT);
+    const float* VO = ws_tensor(ws, HW, C, blk, T_VO);
+    const float* HO = ws_tensor(ws, HW, C, blk, T_HO);
+    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+      const float gh = dH[idx] * bdact<ACT>(HO[idx]);
+      t1[idx] = gh;
+      const float gv = fin ? 0.0f : dV[idx] * bdact<ACT>(VO[idx]);
+      t0[idx] = gv;
+      if (kb) {
+        dVr[idx] = gv;
+        dHr[idx] = gh;
+      }
+    }
+    __syncthreads();
+    mconv_bwd_weight(O + o.kc, O + o.bc, CAT, C, t1, C, 0, C, C, ftab + GT_C * HW, 9, HW, accumulate);
+    mconv_bwd_data(dV, C, t1, C, 0, C, C, prm + o.kc, itab + GT_C * HW, 9, HW);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) dV[idx] *= bdact<ACT>(CAT[idx]);
+    __syncthreads();
+    const int xr = fin ? GT_XF : GT_X;
+    mconv_bwd_weight(O + o.kx, O + o.bx, XA, C, dV, C, 0, C, C / 2, ftab + xr * HW, 1, HW, accumulate);
+    mconv_bwd_weight(O + o.ky, O + o.by, VA, C, dV, C, C / 2, C, C / 2, ftab + GT_Y * HW, 1, HW, accumulate);
+    mconv_bwd_data(dH, C, dV, C, 0, C, C / 2, prm + o.kx, itab + xr * HW, 1, HW);
+    mconv_bwd_data(t1, C, dV, C, C / 2, C, C / 2, prm + o.ky, itab + GT_Y * HW, 1, HW);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+      dH[idx] *= bdact<ACT>(XA[idx]);
+      t0[idx] += t1[idx] * bdact<ACT>(VA[idx]);
+    }
+    __syncthreads();
+    if (first) {
+      conv_bwd_weight(O + o.kh, O + o.bh, Hin, 1, dH, C, 1, C, 1, 3, H, W, accumulate);
+      conv_bwd_weight(O + o.kv, O + o.bv, Vin, 1, t0, C, 1, C, 0, 9, H, W, accumulate);
+      break;
+    }
+    mconv_bwd_weight(O + o.kh, O + o.bh, Hin, C, dH, C, 0, C, C, ftab + GT_H * HW, 3, HW, accumulate);
+    mconv_bwd_weight(O + o.kv, O + o.bv, Vin, C, t0, C, 0, C, C, ftab + GT_V * HW, 9, HW, accumulate);
+    mconv_bwd_data(t1, C, dH, C, 0, C, C, prm + o.kh, itab + GT_H * HW, 3, HW);
+    mconv_bwd_data(dV, C, t0, C, 0, C, C, prm + o.kv, itab + GT_V * HW, 9, HW);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
+      float gh = t1[idx], gv = dV[idx];
+      if (ka) {
+        gh += dHr[idx];
+        gv += dVr[idx];
+      }
+      dH[idx] = gh;
+      dV[idx] = gv;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+}
+
+enum { GRAD_O = 0, GRAD_VJP = 1 };
+
+// MODE GRAD_O: one workgroup per sample, O_re / O_im rows from two backward
+// passes (cotangents of Re and Im log psi).  MODE GRAD_VJP: a persistent grid,
+// each workgroup walks samples b = blockIdx.x, blockIdx.x + gridDim.x, ...
+// and adds cot_re[b] d Re log psi_b + cot_im[b] d Im log psi_b (one backward
+// pass with the combined head cotangent) into its own partial gradient row.
+template <int ACT, int MODE>
+__global__ __launch_bounds__(256) void convar_grad_mfma_kernel(BwdArgs a) {
+  extern __shared__ int16_t tabs[];  // GT_N forward rows, then GT_N inverse rows, of HW entries
+  __shared__ float s_lp[2];
+  const int H = a.H, W = a.W, HW = a.HW, C = a.C, nb = a.nb;
+  const int F = nb - 1;
+  if (MODE == GRAD_O && (int64_t)blockIdx.x >= a.batch) return;
+  float* ws = a.work + (int64_t)blockIdx.x * a.work_floats;
+  int16_t* ftab = tabs;
+  int16_t* itab = tabs + GT_N * HW;
+  float* dz = reinterpret_cast<float*>(tabs + 2 * GT_N * HW + (2 * GT_N * HW & 1));
+  float* dV = ws_grad(ws, HW, C, nb, 0);
+  float* dH = ws_grad(ws, HW, C, nb, 1);
+  float* dVr = ws_grad(ws, HW, C, nb, 2);
+  float* dHr = ws_grad(ws, HW, C, nb, 3);
+  float* t0 = ws_grad(ws, HW, C, nb, 4);
+  float* t1 = ws_grad(ws, HW, C, nb, 5);
+  const float* prm = a.params;
+  build_tap_tables(ftab, itab, H, W);
+  float* part = nullptr;
+  if (MODE == GRAD_VJP) {
+    part = a.partial + (int64_t)blockIdx.x * a.P;
+    for (int64_t q = threadIdx.x; q < a.P; q += blockDim.x) part[q] = 0.0f;
+    __syncthreads();
+  }
+  const int64_t step = MODE == GRAD_VJP ? (int64_t)gridDim.x : a.batch;
+  for (int64_t b = blockIdx.x; b < a.batch; b += step) {
+    float* sp = ws;
+    for (int q = threadIdx.x; q < HW; q += blockDim.x) sp[q] = (float)a.spins[b * HW + q];
+    if (threadIdx.x == 0) s_lp[0] = s_lp[1] = 0.0f;
+    __syncthreads();
+    mfma_forward<ACT>(prm, nullptr, ws, nullptr, ftab, H, W, C, nb, t0, t1, nullptr, nullptr);
+    const int64_t ho = head_offset(nb, C);
+    const float* HF = ws_tensor(ws, HW, C, F, T_HO);
+    const float cr = MODE == GRAD_VJP ? a.cot[2 * b] : 1.0f;
+    const float ci = MODE == GRAD_VJP ? a.cot[2 * b + 1] : 1.0f;
+    for (int part_i = 0; part_i < (MODE == GRAD_VJP ? 1 : 2); ++part_i) {
+      for (int p = threadIdx.x; p < HW; p += blockDim.x) {
+        float z[4];
+        head_logits(prm, ho, HF, C, p, z);
+        const float mx = fmaxf(z[0], z[1]);
+        const float e0 = expf(2.0f * (z[0] - mx)), e1 = expf(2.0f * (z[1] - mx));
+        const float p0 = e0 / (e0 + e1), p1 = e1 / (e0 + e1);
+        const bool up = sp[p] > 0.0f;
+        const float wr = (MODE == GRAD_VJP || part_i == 0) ? cr : 0.0f;
+        const float wi = (MODE == GRAD_VJP || part_i == 1) ? ci : 0.0f;
+        dz[p * 4 + 0] = wr * ((up ? 1.0f : 0.0f) - p0);
+        dz[p * 4 + 1] = wr * ((up ? 0.0f : 1.0f) - p1);
+        dz[p * 4 + 2] = wi * (up ? 1.0f : 0.0f);
+        dz[p * 4 + 3] = wi * (up ? 0.0f : 1.0f);
+        if (part_i == 0) {
+          const float lse = mx + 0.5f * logf(e0 + e1);
+          atomicAdd(&s_lp[0], (up ? z[0] : z[1]) - lse);
+          atomicAdd(&s_lp[1], up ? z[2] : z[3]);
+        }
+      }
+      __syncthreads();
+      float* O = MODE == GRAD_VJP ? part : (part_i == 0 ? a.o_re : a.o_im) + b * a.ld;
+      mfma_backward<ACT>(prm, ws, ftab, itab, dz, H, W, C, nb, O, MODE == GRAD_VJP, dV, dH, dVr, dHr, t0, t1);
+    }
+    if (threadIdx.x == 0 && a.logpsi) {
+      a.logpsi[2 * b] = s_lp[0];
+      a.logpsi[2 * b + 1] = s_lp[1];
+    }
+    __syncthreads();
+  }
+}
+
+// Forward-mode derivative (JVP) of log psi along a parameter direction:
+// out[b] = (d Re log psi_b, d Im log psi_b) . tan, a persistent grid of
+// workgroups over the samples, primal and tangent tensors in the workspace.
+template <int ACT>
+__global__ __launch_bounds__(256) void convar_jvp_mfma_kernel(BwdArgs a) {
+  extern __shared__ int16_t tabs[];
+  __shared__ float s_acc[4];
+  const int H = a.H, W = a.W, HW = a.HW, C = a.C, nb = a.nb;
+  const int F = nb - 1;
+  float* ws = a.work + (int64_t)blockIdx.x * a.work_floats;
+  float* wt = ws + a.work_floats / 2;  // tangent region, same layout
+  int16_t* ftab = tabs;
+  int16_t* itab = tabs + GT_N * HW;
+  float* t0 = ws_grad(ws, HW, C, nb, 0);
+  float* t1 = ws_grad(ws, HW, C, nb, 1);
+  float* u0 = ws_grad(ws, HW, C, nb, 2);
+  float* u1 = ws_grad(ws, HW, C, nb, 3);
+  const float* prm = a.params;
+  const float* tan = a.tangent;
+  build_tap_tables(ftab, itab, H, W);
+  for (int64_t b = blockIdx.x; b < a.batch; b += gridDim.x) {
+    float* sp = ws;
+    for (int q = threadIdx.x; q < HW; q += blockDim.x) sp[q] = (float)a.spins[b * HW + q];
+    if (threadIdx.x < 4) s_acc[threadIdx.x] = 0.0f;
+    __syncthreads();
+    mfma_forward<ACT>(prm, tan, ws, wt, ftab, H, W, C, nb, t0, t1, u0, u1);
+    const int64_t ho = head_offset(nb, C);
+    const float* HF = ws_tensor(ws, HW, C, F, T_HO);
+    const float* HFd = ws_tensor(wt, HW, C, F, T_HO);
     for (int p = threadIdx.x; p < HW; p += blockDim.x) {
-      float z[4];
+      float z[4], zd[4];
+      head_logits(prm, ho, HF, C, p, z);
       for (int k = 0; k < 4; ++k) {
-        float acc = prm[ho + 4 * C + k];
-        for (int ci = 0; ci < C; ++ci) acc += prm[ho + ci * 4 + k] * HF[p * C + ci];
-        z[k] = acc;
+        float acc = tan[ho + 4 * C + k];
+        for (int ci = 0; ci < C; ++ci)
+          acc += prm[ho + ci * 4 + k] * HFd[p * C + ci] + tan[ho + ci * 4 + k] * HF[p * C + ci];
+        zd[k] = acc;
       }
       const float mx = fmaxf(z[0], z[1]);
       const float e0 = expf(2.0f * (z[0] - mx)), e1 = expf(2.0f * (z[1] - mx));
       const float p0 = e0 / (e0 + e1), p1 = e1 / (e0 + e1);
       const bool up = sp[p] > 0.0f;
-      if (part == 0) {
-        dz[p * 4 + 0] = (up ? 1.0f : 0.0f) - p0;
-        dz[p * 4 + 1] = (up ? 0.0f : 1.0f) - p1;
-        dz[p * 4 + 2] = 0.0f;
-        dz[p * 4 + 3] = 0.0f;
-        const float lse = mx + 0.5f * logf(e0 + e1);
-        atomicAdd(&s_lp[0], (up ? z[0] : z[1]) - lse);
-        atomicAdd(&s_lp[1], up ? z[2] : z[3]);
-      } else {
-        dz[p * 4 + 0] = 0.0f;
-        dz[p * 4 + 1] = 0.0f;
-        dz[p * 4 + 2] = up ? 1.0f : 0.0f;
-        dz[p * 4 + 3] = up ? 0.0f : 1.0f;
+      // d/dz of z_s - 0.5 log(e^{2 z0} + e^{2 z1}) is delta_s - p
+      atomicAdd(&s_acc[0], (up ? zd[0] : zd[1]) - (p0 * zd[0] + p1 * zd[1]));
+      atomicAdd(&s_acc[1], up ? zd[2] : zd[3]);
+      const float lse = mx + 0.5f * logf(e0 + e1);
+      atomicAdd(&s_acc[2], (up ? z[0] : z[1]) - lse);
+      atomicAdd(&s_acc[3], up ? z[2] : z[3]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      a.jvp[2 * b] = s_acc[0];
+      a.jvp[2 * b + 1] = s_acc[1];
+      if (a.logpsi) {
+        a.logpsi[2 * b] = s_acc[2];
+        a.logpsi[2 * b + 1] = s_acc[3];
       }
     }
     __syncthreads();
-    float* O = (part == 0 ? a.o_re : a.o_im) + b * a.P;
-    for (int idx = threadIdx.x; idx < 4 * C + 4; idx += blockDim.x) {
-      float acc = 0.0f;
-      if (idx < 4 * C) {
-        const int ci = idx / 4, k = idx - (idx / 4) * 4;
-        for (int p = 0; p < HW; ++p) acc += HF[p * C + ci] * dz[p * 4 + k];
-      } else {
-        for (int p = 0; p < HW; ++p) acc += dz[p * 4 + idx - 4 * C];
-      }
-      O[ho + idx] = acc;
-    }
-    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
-      const int p = idx / C, ci = idx - (idx / C) * C;
-      float g = 0.0f;
-      for (int k = 0; k < 4; ++k) g += prm[ho + ci * 4 + k] * dz[p * 4 + k];
-      dH[idx] = g;
-      dV[idx] = 0.0f;
-      dVr[idx] = 0.0f;
-      dHr[idx] = 0.0f;
-    }
-    __syncthreads();
+  }
+}
 
-    for (int blk = F; blk >= 0; --blk) {
-      const ParamOffsets o = param_offsets(blk, C);
-      const bool first = blk == 0, fin = blk == F;
-      const bool kb = !first && !fin && ((blk & 1) == 0);
-      const bool ka = !first && !fin && ((blk & 1) == 1);
-      const float* Vin = first ? sp : ws_tensor(ws, HW, C, blk - 1, T_VO);
-      const float* Hin = first ? sp : ws_tensor(ws, HW, C, blk - 1, T_HO);
-      const float* VA = ws_tensor(ws, HW, C, blk, T_VA);
-      const float* XA = ws_tensor(ws, HW, C, blk, T_XA);
-      const float* CAT = ws_tensor(ws, HW, C, blk, T_CAT);
-      const float* VO = ws_tensor(ws, HW, C, blk, T_VO);
-      const float* HO = ws_tensor(ws, HW, C, blk, T_HO);
-      for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
-        const float gh = dH[idx] * bdact<ACT>(HO[idx]);
-        t1[idx] = gh;
-        const float gv = fin ? 0.0f : dV[idx] * bdact<ACT>(VO[idx]);
-        t0[idx] = gv;
-        if (kb) {
-          dVr[idx] = gv;
-          dHr[idx] = gh;
-        }
-      }
-      __syncthreads();
-      mconv_bwd_weight(O + o.kc, O + o.bc, CAT, C, t1, C, 0, C, C, ftab + GT_C * HW, 9, HW);
-      mconv_bwd_data(dV, C, t1, C, 0, C, C, prm + o.kc, itab + GT_C * HW, 9, HW);
-      __syncthreads();
-      for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) dV[idx] *= bdact<ACT>(CAT[idx]);
-      __syncthreads();
-      const int xr = fin ? GT_XF : GT_X;
-      mconv_bwd_weight(O + o.kx, O + o.bx, XA, C, dV, C, 0, C, C / 2, ftab + xr * HW, 1, HW);
-      mconv_bwd_weight(O + o.ky, O + o.by, VA, C, dV, C, C / 2, C, C / 2, ftab + GT_Y * HW, 1, HW);
-      mconv_bwd_data(dH, C, dV, C, 0, C, C / 2, prm + o.kx, itab + xr * HW, 1, HW);
-      mconv_bwd_data(t1, C, dV, C, C / 2, C, C / 2, prm + o.ky, itab + GT_Y * HW, 1, HW);
-      __syncthreads();
-      for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
-        dH[idx] *= bdact<ACT>(XA[idx]);
-        t0[idx] += t1[idx] * bdact<ACT>(VA[idx]);
-      }
-      __syncthreads();
-      if (first) {
-        conv_bwd_weight(O + o.kh, O + o.bh, Hin, 1, dH, C, 1, C, 1, 3, H, W);
-        conv_bwd_weight(O + o.kv, O + o.bv, Vin, 1, t0, C, 1, C, 0, 9, H, W);
-        break;
-      }
-      mconv_bwd_weight(O + o.kh, O + o.bh, Hin, C, dH, C, 0, C, C, ftab + GT_H * HW, 3, HW);
-      mconv_bwd_weight(O + o.kv, O + o.bv, Vin, C, t0, C, 0, C, C, ftab + GT_V * HW, 9, HW);
-      mconv_bwd_data(t1, C, dH, C, 0, C, C, prm + o.kh, itab + GT_H * HW, 3, HW);
-      mconv_bwd_data(dV, C, t0, C, 0, C, C, prm + o.kv, itab + GT_V * HW, 9, HW);
-      __syncthreads();
-      for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
-        float gh = t1[idx], gv = dV[idx];
-        if (ka) {
-          gh += dHr[idx];
-          gv += dVr[idx];
-        }
-        dH[idx] = gh;
-        dV[idx] = gv;
-      }
-      __syncthreads();
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && a.logpsi) {
-    a.logpsi[2 * b] = s_lp[0];
-    a.logpsi[2 * b + 1] = s_lp[1];
-  }
+// Sum of the per-workgroup partial gradients in a fixed order (FP64), scaled.
+__global__ void reduce_partials_kernel(const float* __restrict__ part, int64_t nparts, int64_t P, double scale,
+                                       float* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P) return;
+  double acc = 0.0;
+  for (int64_t g = 0; g < nparts; ++g) acc += (double)part[g * P + q];
+  out[q] = (float)(scale * acc);
 }
 
 int64_t grad_work_floats(int HW, int C, int nb) {
   return (int64_t)((HW + 3) & ~3) + (((int64_t)nb * 5 + 6) * HW * C + 3) / 4 * 4;
+}
+
+int64_t flat_param_count(int C, int nb) {
+  const int64_t c = C;
+  return (9 * c + c + 3 * c + c + c * (c / 2) * 2 + c + 9 * c * c + c) +
+         (int64_t)(nb - 1) * (9 * c * c + c + 3 * c * c + c + c * (c / 2) * 2 + c + 9 * c * c + c) + 4 * c + 4;
+}
+
+// Workgroups of the persistent VJP / JVP grids: two per CU, at most one per
+// sample.
+int64_t implicit_grid(int64_t batch) {
+  const int64_t g = 2LL * device_cu_count();
+  return batch < g ? (batch > 0 ? batch : 1) : g;
+}
+
+size_t grad_lds_bytes(int HW) {
+  return (size_t)2 * GT_N * HW * sizeof(int16_t) + 4 + (size_t)4 * HW * sizeof(float);
+}
+
+template <class K>
+int launch_with_lds(K kern, int64_t grid, size_t lds, hipStream_t s, const BwdArgs& a) {
+  FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), lds, s, a);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
 }
 
 }  // namespace fk
@@ -773,14 +995,14 @@ int64_t fk_convar_grad_workspace_bytes(int32_t H, int32_t W, int32_t depth, int3
 }
 
 int fk_convar_grad(const float* params, int32_t H, int32_t W, int32_t depth, int32_t channels, int32_t activation,
-                   const int8_t* spins, int64_t batch, void* workspace, float* o_re, float* o_im, float* logpsi,
-                   void* stream) {
+                   const int8_t* spins, int64_t batch, void* workspace, float* o_re, float* o_im, int64_t row_stride,
+                   float* logpsi, void* stream) {
   FK_REQUIRE(H >= 1 && W >= 1 && depth >= 1 && channels >= 2 && channels % 2 == 0, "fk_convar_grad: bad shape");
   FK_REQUIRE(activation == FK_ACT_RELU || activation == FK_ACT_ELU, "fk_convar_grad: bad activation");
   if (batch <= 0) return FK_OK;
   FK_REQUIRE(params && spins && workspace && o_re && o_im, "fk_convar_grad: null pointer");
   FK_REQUIRE(batch <= 0x7fffffff, "fk_convar_grad: batch too large");
-  BwdArgs a;
+  BwdArgs a{};
   a.params = params;
   a.H = H;
   a.W = W;
@@ -794,26 +1016,17 @@ int fk_convar_grad(const float* params, int32_t H, int32_t W, int32_t depth, int
   a.o_re = o_re;
   a.o_im = o_im;
   a.logpsi = logpsi;
-  // P = flat parameter count
-  const int64_t C = channels;
-  a.P = (9 * C + C + 3 * C + C + C * (C / 2) * 2 + C + 9 * C * C + C) +
-        (int64_t)(a.nb - 1) * (9 * C * C + C + 3 * C * C + C + C * (C / 2) * 2 + C + 9 * C * C + C) + 4 * C + 4;
+  a.P = flat_param_count(channels, a.nb);
+  FK_REQUIRE(row_stride == 0 || row_stride >= a.P, "fk_convar_grad: row stride shorter than the parameters");
+  a.ld = row_stride > 0 ? row_stride : a.P;
   hipStream_t s = (hipStream_t)stream;
   if (channels % 32 == 0) {
     // MFMA kernel: LDS holds the tap source tables and the head cotangents
     const size_t lds = (size_t)2 * GT_N * a.HW * sizeof(int16_t) + 4 + (size_t)4 * a.HW * sizeof(float);
     FK_REQUIRE(lds <= 160 * 1024 && a.HW <= 32767, "fk_convar_grad: lattice too large");
-    if (activation == FK_ACT_ELU) {
-      FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(convar_grad_mfma_kernel<FK_ACT_ELU>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL(convar_grad_mfma_kernel<FK_ACT_ELU>, dim3((unsigned)batch), dim3(256), lds, s, a);
-    } else {
-      FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(convar_grad_mfma_kernel<FK_ACT_RELU>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL(convar_grad_mfma_kernel<FK_ACT_RELU>, dim3((unsigned)batch), dim3(256), lds, s, a);
-    }
-    FK_CHECK_LAUNCH();
-    return FK_OK;
+    return activation == FK_ACT_ELU
+               ? launch_with_lds(convar_grad_mfma_kernel<FK_ACT_ELU, GRAD_O>, batch, lds, s, a)
+               : launch_with_lds(convar_grad_mfma_kernel<FK_ACT_RELU, GRAD_O>, batch, lds, s, a);
   }
   const size_t lds = ((size_t)6 * a.HW * channels + 4 * a.HW) * sizeof(float);
   FK_REQUIRE(lds <= 160 * 1024, "fk_convar_grad: lattice too large for the LDS gradient buffers");
@@ -828,6 +1041,88 @@ int fk_convar_grad(const float* params, int32_t H, int32_t W, int32_t depth, int
   }
   FK_CHECK_LAUNCH();
   return FK_OK;
+}
+
+/* ---- implicit products: batched VJP and JVP (no per-sample O) ---------- */
+
+int64_t fk_convar_vjp_workspace_bytes(int32_t H, int32_t W, int32_t depth, int32_t channels, int64_t batch) {
+  if (H < 1 || W < 1 || depth < 1 || channels < 32 || (channels % 32) || batch < 0) return -1;
+  const int nb = num_causal_blocks(depth);
+  const int64_t g = implicit_grid(batch);
+  return g * (grad_work_floats(H * W, channels, nb) + flat_param_count(channels, nb)) * (int64_t)sizeof(float);
+}
+
+int fk_convar_vjp(const float* params, int32_t H, int32_t W, int32_t depth, int32_t channels, int32_t activation,
+                  const int8_t* spins, int64_t batch, const float* cot, double scale, void* workspace, float* grad,
+                  float* logpsi, void* stream) {
+  FK_REQUIRE(H >= 1 && W >= 1 && depth >= 1 && channels >= 32 && channels % 32 == 0,
+             "fk_convar_vjp: the batched VJP needs a channel count that is a multiple of 32");
+  FK_REQUIRE(activation == FK_ACT_RELU || activation == FK_ACT_ELU, "fk_convar_vjp: bad activation");
+  FK_REQUIRE(batch >= 1 && batch <= 0x7fffffff, "fk_convar_vjp: bad batch");
+  FK_REQUIRE(params && spins && cot && workspace && grad, "fk_convar_vjp: null pointer");
+  BwdArgs a{};
+  a.params = params;
+  a.H = H;
+  a.W = W;
+  a.HW = H * W;
+  a.C = channels;
+  a.nb = num_causal_blocks(depth);
+  a.batch = batch;
+  a.spins = spins;
+  a.work = reinterpret_cast<float*>(workspace);
+  a.work_floats = grad_work_floats(a.HW, channels, a.nb);
+  a.logpsi = logpsi;
+  a.P = flat_param_count(channels, a.nb);
+  a.cot = cot;
+  const int64_t g = implicit_grid(batch);
+  a.partial = a.work + g * a.work_floats;
+  const size_t lds = grad_lds_bytes(a.HW);
+  FK_REQUIRE(lds <= 160 * 1024 && a.HW <= 32767, "fk_convar_vjp: lattice too large");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = activation == FK_ACT_ELU ? launch_with_lds(convar_grad_mfma_kernel<FK_ACT_ELU, GRAD_VJP>, g, lds, s, a)
+                                    : launch_with_lds(convar_grad_mfma_kernel<FK_ACT_RELU, GRAD_VJP>, g, lds, s, a);
+  if (rc != FK_OK) return rc;
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(div_up(a.P, 256)), dim3(256), 0, s, a.partial, g, a.P, scale,
+                     grad);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int64_t fk_convar_jvp_workspace_bytes(int32_t H, int32_t W, int32_t depth, int32_t channels, int64_t batch) {
+  if (H < 1 || W < 1 || depth < 1 || channels < 32 || (channels % 32) || batch < 0) return -1;
+  return implicit_grid(batch) * 2 * grad_work_floats(H * W, channels, num_causal_blocks(depth)) *
+         (int64_t)sizeof(float);
+}
+
+int fk_convar_jvp(const float* params, const float* tangent, int32_t H, int32_t W, int32_t depth, int32_t channels,
+                  int32_t activation, const int8_t* spins, int64_t batch, void* workspace, float* jvp, float* logpsi,
+                  void* stream) {
+  FK_REQUIRE(H >= 1 && W >= 1 && depth >= 1 && channels >= 32 && channels % 32 == 0,
+             "fk_convar_jvp: the JVP needs a channel count that is a multiple of 32");
+  FK_REQUIRE(activation == FK_ACT_RELU || activation == FK_ACT_ELU, "fk_convar_jvp: bad activation");
+  FK_REQUIRE(batch >= 1 && batch <= 0x7fffffff, "fk_convar_jvp: bad batch");
+  FK_REQUIRE(params && tangent && spins && workspace && jvp, "fk_convar_jvp: null pointer");
+  BwdArgs a{};
+  a.params = params;
+  a.H = H;
+  a.W = W;
+  a.HW = H * W;
+  a.C = channels;
+  a.nb = num_causal_blocks(depth);
+  a.batch = batch;
+  a.spins = spins;
+  a.work = reinterpret_cast<float*>(workspace);
+  a.work_floats = 2 * grad_work_floats(a.HW, channels, a.nb);
+  a.logpsi = logpsi;
+  a.P = flat_param_count(channels, a.nb);
+  a.tangent = tangent;
+  a.jvp = jvp;
+  const size_t lds = grad_lds_bytes(a.HW);
+  FK_REQUIRE(lds <= 160 * 1024 && a.HW <= 32767, "fk_convar_jvp: lattice too large");
+  const int64_t g = implicit_grid(batch);
+  hipStream_t s = (hipStream_t)stream;
+  return activation == FK_ACT_ELU ? launch_with_lds(convar_jvp_mfma_kernel<FK_ACT_ELU>, g, lds, s, a)
+                                  : launch_with_lds(convar_jvp_mfma_kernel<FK_ACT_RELU>, g, lds, s, a);
 }
 
 }  // extern "C"

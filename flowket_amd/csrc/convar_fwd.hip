@@ -25,8 +25,8 @@ __global__ void pack_gather_kernel(const float* __restrict__ src, const int32_t*
 // Activation buffers of P + ZERO_PIX pixels (five in LDS, or three with the
 // horizontal stack in global scratch) and the int8 spins; the per-site
 // log-probabilities live in V1.
-static size_t fwd_lds_bytes(int P, int pixb, bool hg) {
-  return (size_t)(P + ZERO_PIX) * pixb * (hg ? 3 : 5) + (size_t)P;
+static size_t fwd_lds_bytes(int P, int pixb, bool hg, int T) {
+  return (size_t)(P + ZERO_PIX) * pixb * (hg ? 3 : 5) + (size_t)((P + 3) & ~3) + 4 * (size_t)T;
 }
 
 // One launch shape of the forward: loop order and waves per workgroup.
@@ -66,14 +66,16 @@ static int current_schedule(int H, int W, int depth, int C, int act, int prec) {
 
 // Largest tile of configurations whose activation lattices fit in the 160 KB
 // LDS with at most 4 (8 waves) or 8 (4 waves) 16-pixel tiles per wave.
-static int plan_forward(int HW, int C, int prec, int NW, int64_t m, FwdPlan* plan, bool hg = false) {
+// HWL: LDS pixels per configuration, HWC: computed pixels per configuration
+// (they differ for row-reuse launches, whose halo rows are not computed).
+static int plan_forward(int HWL, int HWC, int C, int prec, int NW, int64_t m, FwdPlan* plan, bool hg = false) {
   const int pixb = C * (prec == FK_PREC_FP32 ? 4 : 2);
-  const size_t cap = 160 * 1024;
+  const size_t cap = FWD_DYN_LDS;
   const int maxt_cap = (NW == 4) ? 8 : 4;
   int best = 0;
   for (int T = 1; T <= 256; ++T) {
-    const int npix = ((T * HW + 15) / 16) * 16;
-    if (fwd_lds_bytes(T * HW, pixb, hg) > cap) break;
+    const int npix = ((T * HWC + 15) / 16) * 16;
+    if (fwd_lds_bytes(T * HWL, pixb, hg, T) > cap) break;
     const int per_wave = (npix / 16 + NW - 1) / NW;
     if (per_wave > maxt_cap) break;
     best = T;
@@ -81,8 +83,8 @@ static int plan_forward(int HW, int C, int prec, int NW, int64_t m, FwdPlan* pla
   }
   if (best == 0) return FK_ERR_UNSUPPORTED;
   plan->T = best;
-  plan->npix = ((best * HW + 15) / 16) * 16;
-  plan->lds = fwd_lds_bytes(best * HW, pixb, hg);
+  plan->npix = ((best * HWC + 15) / 16) * 16;
+  plan->lds = fwd_lds_bytes(best * HWL, pixb, hg, best);
   plan->nwave = NW;
   const int per_wave = (plan->npix / 16 + NW - 1) / NW;
   plan->maxt = per_wave <= 2 ? 2 : (per_wave <= 4 ? 4 : 8);
@@ -231,19 +233,40 @@ static int64_t forward_workspace_bytes(int H, int W, int depth, int CP, int act,
   int si = current_schedule(H, W, depth, CP, act, prec);
   if (si < 0 || si >= nsched) si = 0;
   FwdPlan pl;
-  if (plan_forward(H * W, CP, prec, table[si].nwave, 1 << 30, &pl) == FK_OK) return 0;
-  if (plan_forward(H * W, CP, prec, hg_waves(table[si]), 1 << 30, &pl, true) != FK_OK) return -1;
+  if (plan_forward(H * W, H * W, CP, prec, table[si].nwave, 1 << 30, &pl) == FK_OK) return 0;
+  if (plan_forward(H * W, H * W, CP, prec, hg_waves(table[si]), 1 << 30, &pl, true) != FK_OK) return -1;
   if (hg_plan) *hg_plan = pl;
   const int pixb = CP * (prec == FK_PREC_FP32 ? 4 : 2);
-  const int per_cu = (int)((160 * 1024) / pl.lds) > 0 ? (int)((160 * 1024) / pl.lds) : 1;
-  return (int64_t)device_cu_count() * per_cu * 2 * (int64_t)(pl.T * H * W + ZERO_PIX) * pixb;
+  // the whole-lattice launch and every row-reuse launch (rows R0-2 .. H-1)
+  int64_t need = 0;
+  for (int r0 = 0; r0 < H; ++r0) {
+    if (r0 == 1) continue;
+    const int hr = r0 == 0 ? H : H - r0 + 2, hc = r0 == 0 ? H : H - r0;
+    FwdPlan q;
+    if (plan_forward(hr * W, hc * W, CP, prec, hg_waves(table[si]), 1 << 30, &q, true) != FK_OK) return -1;
+    const int per_cu = fwd_per_cu(q);
+    const int64_t b = (int64_t)device_cu_count() * per_cu * 2 * (int64_t)(q.T * hr * W + ZERO_PIX) * pixb;
+    if (b > need) need = b;
+  }
+  return need;
 }
+
+// Row reuse (see FwdArgs): R0 = 0 with rcache and cache_write is the sample
+// pass; R0 >= 2 a restricted pass over rows R0-2 .. H-1.
+struct RowReuse {
+  int R0 = 0;
+  uint8_t* rcache = nullptr;
+  int cache_write = 0;
+  float* prefix = nullptr;
+  const int64_t* out_index = nullptr;
+  const int64_t* m_range = nullptr;
+};
 
 static int convar_forward(const void* wpacked, const float* aux, int32_t precision, int32_t H, int32_t W,
                           int32_t depth, int32_t channels, int32_t activation, const int8_t* spins,
                           const int32_t* pairs, const int64_t* m_dev, int64_t m_count, int32_t conn_kind,
                           const int32_t* conn_a, const int32_t* conn_b, float* logpsi, float* cond_logp,
-                          void* workspace, int64_t workspace_bytes, void* stream) {
+                          void* workspace, int64_t workspace_bytes, void* stream, const RowReuse& rr = RowReuse()) {
   FK_REQUIRE(wpacked && aux && spins && logpsi, "fk_convar_forward: null pointer");
   FK_REQUIRE(H >= 1 && W >= 1 && depth >= 1, "fk_convar_forward: bad shape");
   FK_REQUIRE(activation == FK_ACT_RELU || activation == FK_ACT_ELU, "fk_convar_forward: bad activation");
@@ -276,9 +299,20 @@ static int convar_forward(const void* wpacked, const float* aux, int32_t precisi
   a.logpsi = logpsi;
   a.cond = cond_logp;
   a.hscr = nullptr;
+  a.R0 = rr.R0;
+  a.ROW0 = rr.R0 >= 2 ? 2 : 0;
+  a.HR = rr.R0 >= 2 ? H - rr.R0 + 2 : H;
+  a.HC = rr.R0 >= 2 ? H - rr.R0 : H;
+  a.rcache = rr.rcache;
+  a.rcache_stride = (int64_t)a.nblocks * 2 * H * W * CP * (precision == FK_PREC_FP32 ? 4 : 2);
+  a.prefix = rr.prefix;
+  a.cache_write = rr.cache_write;
+  a.out_index = rr.out_index;
+  a.m_range = rr.m_range;
+  const int HWL = a.HR * W, HWC = a.HC * W;
   hipStream_t s = (hipStream_t)stream;
   FwdPlan pl;
-  if (plan_forward(H * W, CP, precision, sc.nwave, m_count, &pl) != FK_OK) {
+  if (plan_forward(HWL, HWC, CP, precision, sc.nwave, m_count, &pl) != FK_OK) {
     // horizontal stack in global scratch
     FwdPlan hp;
     const int64_t need = forward_workspace_bytes(H, W, depth, CP, activation, precision, &hp);
@@ -292,7 +326,12 @@ static int convar_forward(const void* wpacked, const float* aux, int32_t precisi
                 "(fk_convar_forward_workspace_bytes, fk_convar_forward_ws)", H, W, channels, (long long)need);
       return FK_ERR_ARG;
     }
-    if (plan_forward(H * W, CP, precision, hg_waves(sc), m_count, &pl, true) != FK_OK) return FK_ERR_UNSUPPORTED;
+    if (plan_forward(HWL, HWC, CP, precision, hg_waves(sc), m_count, &pl, true) != FK_OK) return FK_ERR_UNSUPPORTED;
+    if (fwd_grid(a, pl) * 2 * (int64_t)(pl.T * HWL + ZERO_PIX) * (CP * (precision == FK_PREC_FP32 ? 4 : 2)) >
+        workspace_bytes) {
+      set_error("ConvNetAutoregressive2D kernels: workspace too small for this launch");
+      return FK_ERR_ARG;
+    }
     a.T = pl.T;
     a.npix = pl.npix;
     a.hscr = reinterpret_cast<uint8_t*>(workspace);
@@ -332,6 +371,34 @@ int fk_convar_forward_ws(const void* wpacked, const float* aux, int32_t precisio
                          void* workspace, int64_t workspace_bytes, void* stream) {
   return convar_forward(wpacked, aux, precision, H, W, depth, channels, activation, spins, pairs, m_dev, m_count,
                         conn_kind, conn_a, conn_b, logpsi, cond_logp, workspace, workspace_bytes, stream);
+}
+
+int64_t fk_convar_rcache_bytes(int32_t H, int32_t W, int32_t depth, int32_t channels, int32_t precision) {
+  if (H < 1 || W < 1 || depth < 1 || !convar_supported(channels, precision)) return -1;
+  const int CP = kernel_channels(channels, precision);
+  return (int64_t)num_causal_blocks(depth) * 2 * H * W * CP * (precision == FK_PREC_FP32 ? 4 : 2);
+}
+
+int fk_convar_forward_rows(const void* wpacked, const float* aux, int32_t precision, int32_t H, int32_t W,
+                           int32_t depth, int32_t channels, int32_t activation, const int8_t* spins,
+                           const int32_t* pairs, const int64_t* m_range, int64_t m_count, int32_t conn_kind,
+                           const int32_t* conn_a, const int32_t* conn_b, int32_t row0, void* rcache,
+                           int32_t cache_write, float* prefix, const int64_t* out_index, float* logpsi,
+                           void* workspace, int64_t workspace_bytes, void* stream) {
+  FK_REQUIRE(H >= 3 && H <= 64, "fk_convar_forward_rows: 3 <= H <= 64");
+  FK_REQUIRE(row0 == 0 || (row0 >= 2 && row0 < H), "fk_convar_forward_rows: row0 must be 0 or in [2, H)");
+  FK_REQUIRE(!cache_write || (row0 == 0 && !pairs), "fk_convar_forward_rows: the sample pass takes rows of spins");
+  FK_REQUIRE(row0 == 0 || (pairs && rcache && prefix), "fk_convar_forward_rows: a row pass needs pairs and the cache");
+  FK_REQUIRE(!cache_write || (rcache && prefix), "fk_convar_forward_rows: the sample pass needs the cache");
+  RowReuse rr;
+  rr.R0 = row0;
+  rr.rcache = reinterpret_cast<uint8_t*>(rcache);
+  rr.cache_write = cache_write;
+  rr.prefix = prefix;
+  rr.out_index = out_index;
+  rr.m_range = m_range;
+  return convar_forward(wpacked, aux, precision, H, W, depth, channels, activation, spins, pairs, nullptr, m_count,
+                        conn_kind, conn_a, conn_b, logpsi, nullptr, workspace, workspace_bytes, stream, rr);
 }
 
 int64_t fk_convar_forward_workspace_bytes(int32_t H, int32_t W, int32_t depth, int32_t channels, int32_t activation,

@@ -25,6 +25,9 @@ namespace fk {
 // all-zero pixels after each activation buffer (one per bank placement of
 // the 8-pixel swizzle period)
 constexpr int ZERO_PIX = 8;
+// LDS of the forward (all of it dynamic)
+constexpr int FWD_STATIC_LDS = 0;
+constexpr int FWD_DYN_LDS = 160 * 1024;
 
 // LDS addressing.  A pixel holds C channels as 16-byte chunks; chunk indices
 // are XOR-swizzled with the pixel index so that 16 lanes reading the same
@@ -158,7 +161,29 @@ struct FwdArgs {
   float* logpsi;
   float* cond;
   uint8_t* hscr;  // HG kernels: 2 lattices per workgroup of global scratch
+  // Row reuse.  A connected configuration equal to its sample on every row
+  // before R0 has the sample's activations on those rows (every activation
+  // of row r depends on input rows <= r only).  Such a launch keeps rows
+  // R0-2 .. H-1 of each configuration (HR rows): the first ROW0 = 2 are halo
+  // rows whose block inputs (vertical stack, concatenation) are copied from
+  // the sample's cache, the HC = H - R0 rows after them are computed, and
+  // the sample's prefix sum of site log-probabilities over rows < R0 is added.
+  // R0 = 0: the whole lattice (HR = HC = H, ROW0 = 0).
+  int R0, HR, HC, ROW0;
+  uint8_t* rcache;           // per sample: for every block its V input and CAT lattices
+  int64_t rcache_stride;     // bytes per sample
+  float* prefix;             // per sample: (H + 1) x (re, im) prefix sums over rows
+  int cache_write;           // sample pass: write rcache and prefix
+  const int64_t* out_index;  // output row of configuration m (null: m)
+  const int64_t* m_range;    // device (first, end) of the configurations within pairs (null: 0, count)
 };
+
+// Byte offset, within one sample's cache, of lattice `which` (0 = the block's
+// vertical input, 1 = its concatenation) of block blk: pixels in global
+// order, PIXB bytes each, 16-byte chunks in their unswizzled order.
+__device__ __forceinline__ int64_t rcache_off(int blk, int which, int HW, int PIXB) {
+  return ((int64_t)blk * 2 + which) * HW * PIXB;
+}
 
 // Per-tile LDS byte offsets (from the buffer start) of the 12 causal taps
 // k = (dr+2)*4 + (dc+2) of this lane's pixel, two 16-bit offsets per
@@ -314,6 +339,36 @@ __device__ __forceinline__ void conv_tile(f32x4 (&acc)[MTO], const uint8_t* buf,
       for (int ks = 0; ks < KS; ++ks) acc[mt] = P::mfma(cw.w[t][mt][ks], b[t][ks], acc[mt]);
 }
 
+// Row-reuse cache copies between a lattice in LDS (swizzled pixels of T
+// configurations, HWL pixels each) and the samples' caches (pixels in global
+// order, chunks unswizzled): local pixels [q0, q1) of every configuration,
+// 4-channel quads [quad0, C/4); to_cache: LDS -> cache, else cache -> LDS.
+template <int PREC, int C>
+__device__ __forceinline__ void rcache_copy(uint8_t* buf, const FwdArgs& a, int blk, int which, int T, int HWL,
+                                            int q0, int q1, bool to_cache, const int* s_samp, int quad0 = 0) {
+  using L = Lds<PREC, C>;
+  constexpr int QB = Prec<PREC>::H16 ? 8 : 16;
+  const int nq = C / 4 - quad0;
+  const int per = (q1 - q0) * nq;
+  const int gofs = (a.R0 - a.ROW0) * a.W;
+  for (int idx = threadIdx.x; idx < T * per; idx += blockDim.x) {
+    const int t = idx / per, r = idx - t * per;
+    const int b = s_samp[t];
+    if (b < 0) continue;
+    const int ql = q0 + r / nq, qd = quad0 + (r - (r / nq) * nq);
+    uint8_t* lp = buf + L::quad(t * HWL + ql, qd);
+    uint8_t* cp = a.rcache + (int64_t)b * a.rcache_stride + rcache_off(blk, which, a.HW, L::PIXB) +
+                  (int64_t)(ql + gofs) * L::PIXB + L::quad(0, qd);
+    if constexpr (QB == 8) {
+      if (to_cache) *reinterpret_cast<uint2*>(cp) = *reinterpret_cast<const uint2*>(lp);
+      else *reinterpret_cast<uint2*>(lp) = *reinterpret_cast<const uint2*>(cp);
+    } else {
+      if (to_cache) *reinterpret_cast<uint4*>(cp) = *reinterpret_cast<const uint4*>(lp);
+      else *reinterpret_cast<uint4*>(lp) = *reinterpret_cast<const uint4*>(cp);
+    }
+  }
+}
+
 // SCHED 0: taps outer -- each tap's weights are loaded once and serve all the
 // wave's tiles (accumulators of every tile live in registers).
 // SCHED 1: tiles outer -- a 3x3 convolution's weights stay in registers for
@@ -340,7 +395,9 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   // outside reads zero pixel NPX + ((q - NPX) & 7), which has the bank
   // placement q itself would have, so border tiles stay conflict-free.
   const int npix = a.npix;
-  const int NPX = a.T * a.HW;
+  // per configuration: HWL pixels in LDS (halo rows included), HWC computed
+  const int HWL = a.HR * a.W, HWC = a.HC * a.W;
+  const int NPX = a.T * HWL;
   const int BUFB = (NPX + ZERO_PIX) * PIXB;
   uint8_t* V0 = lds;
   uint8_t* V1 = V0 + BUFB;
@@ -357,6 +414,8 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
     H1 = H0 + BUFB;
     sp = reinterpret_cast<int8_t*>(H1 + BUFB);
   }
+  // after the spins: the sample of each configuration of the group (-1: none)
+  int* s_samp = reinterpret_cast<int*>(sp + ((NPX + 3) & ~3));
   // per-site log-probabilities (and, when asked for, the conditionals) of
   // the final block: V1 is neither read nor written by the final block, so
   // they reuse its space
@@ -370,6 +429,8 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   const int NW = blockDim.x >> 6;
   const int n16 = lane & 15, g = lane >> 4;
   const int H = a.H, W = a.W, HW = a.HW, T = a.T;
+  const int HR = a.HR, ROW0 = a.ROW0;
+  const int gofs = (a.R0 - ROW0) * W;  // global index of a configuration's LDS pixel 0
   const int ntiles = npix >> 4;
   const int nt = (ntiles > wave) ? (ntiles - wave + NW - 1) / NW : 0;
 
@@ -379,11 +440,13 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   bool inp[MAXT];
 #pragma unroll
   for (int u = 0; u < MAXT; ++u) {
-    const int p = (wave + u * NW) * 16 + n16;
-    const bool in = (u < nt) && (p < T * HW);
-    const int t = in ? p / HW : 0;
-    const int rem = in ? p - t * HW : 0;
-    const int i = rem / W, j = rem - (rem / W) * W;
+    // tiles run over the computed pixels; pix is the LDS pixel
+    const int pc = (wave + u * NW) * 16 + n16;
+    const bool in = (u < nt) && (pc < T * HWC);
+    const int t = in ? pc / HWC : 0;
+    const int rem = in ? pc - t * HWC : 0;
+    const int i = ROW0 + rem / W, j = rem - (rem / W) * W;
+    const int p = in ? t * HWL + i * W + j : NPX + (n16 & (ZERO_PIX - 1));
     uint32_t m = 0;
     if (in) {
 #pragma unroll
@@ -421,6 +484,11 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   }
 
   int64_t M = a.m_dev ? *a.m_dev : a.m_count;
+  int64_t m0 = 0;
+  if (a.m_range) {
+    m0 = a.m_range[0];
+    M = a.m_range[1] - m0;
+  }
   if (M > a.m_count) M = a.m_count;
   const int64_t ngroups = (M + T - 1) / T;
   const int nb = a.nblocks;
@@ -441,8 +509,8 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
       int64_t b = m;
       int c = 0;
       if (live && a.pairs) {
-        b = a.pairs[2 * m];
-        c = a.pairs[2 * m + 1];
+        b = a.pairs[2 * (m0 + m)];
+        c = a.pairs[2 * (m0 + m) + 1];
       }
       int ca = -1, cb = -1;
       if (live && c > 0) {
@@ -450,7 +518,8 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
         if (a.conn_kind != FK_CONN_FLIP && ca >= 0) cb = a.conn_b[c];
       }
       const int8_t* row = a.spins + (live ? b : 0) * HW;
-      for (int q = lane; q < HW; q += 64) {
+      for (int ql = lane; ql < HWL; ql += 64) {
+        const int q = ql + gofs;
         int8_t v = 0;
         if (live) {
           v = row[q];
@@ -461,8 +530,9 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
             else if (q == cb) v = row[ca];
           }
         }
-        sp[t * HW + q] = v;
+        sp[t * HWL + ql] = v;
       }
+      if (lane == 0) s_samp[t] = live ? b : -1;
     }
     __syncthreads();
 
@@ -470,6 +540,14 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
       const int kind = block_kind(blk, nb);
       const uint8_t* Vin = (kind == BK_B) ? V1 : V0;
       const uint8_t* Hin = (kind == BK_B) ? H1 : H0;
+      // row reuse: the sample pass stores this block's vertical input; a
+      // restricted pass takes the halo rows of it from the sample's cache
+      if (a.rcache && kind != BK_FIRST && (a.cache_write || ROW0 > 0)) {
+        uint8_t* vb = (kind == BK_B) ? V1 : V0;
+        if (a.cache_write) rcache_copy<PREC, C>(vb, a, blk, 0, T, HWL, 0, HWL, true, s_samp);
+        else rcache_copy<PREC, C>(vb, a, blk, 0, T, HWL, 0, ROW0 * W, false, s_samp);
+        __syncthreads();
+      }
       const float* bias = a.aux + lay.aux_bias(blk);
       const float* by = bias + 2 * C + C / 2;
       const float* bx = bias + 2 * C;
@@ -562,8 +640,8 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
           }
           d_to_b<PREC, MT, KS>(av, bf);
         }
-        const bool down = pi[u] + 1 < H;
-        const int tgt = down ? pix[u] + W : pix[u] - (H - 1) * W;
+        const bool down = pi[u] + 1 < HR;
+        const int tgt = down ? pix[u] + W : pix[u] - (HR - 1) * W;
 #pragma unroll
         for (int mh = 0; mh < MTH; ++mh) {
           f32x4 y = by0[mh];
@@ -680,6 +758,18 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
         }
       }
       __syncthreads();
+      // row reuse: the concatenation of the sample pass to its cache; the
+      // halo rows of a restricted pass (and the y half of its first computed
+      // row, which the DownShift takes from the halo) from the cache
+      if (a.rcache && (a.cache_write || ROW0 > 0)) {
+        if (a.cache_write) {
+          rcache_copy<PREC, C>(CAT, a, blk, 1, T, HWL, 0, HWL, true, s_samp);
+        } else {
+          rcache_copy<PREC, C>(CAT, a, blk, 1, T, HWL, 0, ROW0 * W, false, s_samp);
+          rcache_copy<PREC, C>(CAT, a, blk, 1, T, HWL, ROW0 * W, (ROW0 + 1) * W, false, s_samp, C / 8);
+          __syncthreads();
+        }
+      }
 
       // ================= phase H3: kxk on the concatenation ================
       frag whd[KS];
@@ -770,28 +860,49 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
       __syncthreads();
     }
 
-    // ---- per-configuration site sums (deterministic tree per wave)
+    // ---- per-configuration sums of the site log-probabilities: each lane
+    //      sums one row in column order, then the rows are added in order
+    //      (starting from the sample's prefix over the rows before R0); the
+    //      sample pass records the prefix sums over rows
     for (int t = wave; t < T; t += NW) {
-      float sr = 0.0f, si = 0.0f;
-      for (int q = lane; q < HW; q += 64) {
-        const float2 v = part[t * HW + q];
-        sr += v.x;
-        si += v.y;
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        sr += __shfl_xor(sr, off, 64);
-        si += __shfl_xor(si, off, 64);
-      }
       const int64_t m = grp * T + t;
+      float rr = 0.0f, ri = 0.0f;
+      const int nrows = HR - ROW0;
+      if (lane < nrows) {
+        const float2* rowp = part + t * HWL + (ROW0 + lane) * W;
+        for (int j = 0; j < W; ++j) {
+          rr += rowp[j].x;
+          ri += rowp[j].y;
+        }
+      }
+      float sr = 0.0f, si = 0.0f;
+      const int b = s_samp[t];
+      if (ROW0 > 0 && b >= 0) {
+        sr = a.prefix[((int64_t)b * (H + 1) + a.R0) * 2];
+        si = a.prefix[((int64_t)b * (H + 1) + a.R0) * 2 + 1];
+      }
+      float* pref = (a.cache_write && b >= 0) ? a.prefix + (int64_t)b * (H + 1) * 2 : nullptr;
+      if (pref && lane == 0) {
+        pref[0] = 0.0f;
+        pref[1] = 0.0f;
+      }
+      for (int r = 0; r < nrows; ++r) {
+        sr += __shfl(rr, r, 64);
+        si += __shfl(ri, r, 64);
+        if (pref && lane == 0) {
+          pref[2 * (r + 1)] = sr;
+          pref[2 * (r + 1) + 1] = si;
+        }
+      }
       if (lane == 0 && m < M) {
-        a.logpsi[2 * m] = sr;
-        a.logpsi[2 * m + 1] = si;
+        const int64_t mo = a.out_index ? a.out_index[m0 + m] : m;
+        a.logpsi[2 * mo] = sr;
+        a.logpsi[2 * mo + 1] = si;
       }
       if (a.cond && m < M) {
-        for (int q = lane; q < HW; q += 64) {
-          const float2 c = cpart[t * HW + q];
-          float* o = a.cond + (m * HW + q) * 2;
+        for (int q = lane; q < HWL; q += 64) {
+          const float2 c = cpart[t * HWL + q];
+          float* o = a.cond + (m * HWL + q) * 2;
           o[0] = c.x;
           o[1] = c.y;
         }
@@ -806,10 +917,16 @@ struct FwdPlan {
   size_t lds;
 };
 
+// workgroups that fit a CU's 160 KB of LDS (dynamic + static)
+inline int fwd_per_cu(const FwdPlan& pl) {
+  const int n = (int)((160 * 1024) / (pl.lds + FWD_STATIC_LDS));
+  return n > 0 ? n : 1;
+}
+
 // persistent grid: CU count x workgroups per CU, at most one per group
 inline int64_t fwd_grid(const FwdArgs& a, const FwdPlan& pl) {
   const int64_t ngroups = (a.m_count + pl.T - 1) / pl.T;
-  const int per_cu = (int)((160 * 1024) / pl.lds) > 0 ? (int)((160 * 1024) / pl.lds) : 1;
+  const int per_cu = fwd_per_cu(pl);
   int64_t grid = (int64_t)device_cu_count() * per_cu;
   if (grid > ngroups) grid = ngroups;
   if (grid < 1) grid = 1;
@@ -822,7 +939,7 @@ int launch_fwd(const FwdArgs& a, const FwdPlan& pl, hipStream_t s) {
   // raise the dynamic LDS limit once per instantiation (not a stream
   // operation, so launches may later be captured into a graph)
   static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, FWD_DYN_LDS);
   FK_CHECK_HIP(attr);
   hipLaunchKernelGGL(kern, dim3((unsigned)fwd_grid(a, pl)), dim3(64 * NWAVE), pl.lds, s, a);
   FK_CHECK_LAUNCH();

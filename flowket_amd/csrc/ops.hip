@@ -295,6 +295,95 @@ __global__ __launch_bounds__(256) void write_pairs_kernel(const uint8_t* __restr
   }
 }
 
+// Row buckets of the ragged map (the forward's row reuse): bucket_of[c] is
+// the bucket of connection slot c.  Counts per (bucket, sample) with one lane
+// per sample walking the slots in order (counters in LDS).
+constexpr int BK_MAX = 128;
+__global__ __launch_bounds__(64) void bucket_count_kernel(const uint8_t* __restrict__ use, int64_t batch, int n_conn,
+                                                          const int32_t* __restrict__ bucket_of, int nbk,
+                                                          int32_t* __restrict__ counts) {
+  __shared__ int s_cnt[BK_MAX][64];
+  const int lane = threadIdx.x;
+  const int64_t b = (int64_t)blockIdx.x * 64 + lane;
+  for (int k = 0; k < nbk; ++k) s_cnt[k][lane] = 0;
+  if (b < batch)
+    for (int c = 0; c < n_conn; ++c)
+      if (use[(int64_t)c * batch + b]) s_cnt[bucket_of[c]][lane] += 1;
+  if (b < batch)
+    for (int k = 0; k < nbk; ++k) counts[(int64_t)k * batch + b] = s_cnt[k][lane];
+}
+
+// In-place exclusive scan of counts[k][0..batch) per bucket k (a workgroup
+// per bucket); totals[k] = the bucket's size.
+__global__ __launch_bounds__(1024) void bucket_scan_kernel(int32_t* __restrict__ counts, int64_t batch,
+                                                           int64_t* __restrict__ totals) {
+  __shared__ int64_t s_wave[16];
+  __shared__ int64_t s_carry;
+  int32_t* row = counts + (int64_t)blockIdx.x * batch;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < batch; base += blockDim.x) {
+    const int64_t i = base + tid;
+    const int64_t x = i < batch ? row[i] : 0;
+    int64_t v = x;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int64_t o = __shfl_up(v, off, 64);
+      if (lane >= off) v += o;
+    }
+    if (lane == 63) s_wave[wave] = v;
+    __syncthreads();
+    if (tid == 0) {
+      int64_t run = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        const int64_t t = s_wave[w];
+        s_wave[w] = run;
+        run += t;
+      }
+    }
+    __syncthreads();
+    v += s_wave[wave] + s_carry;
+    if (i < batch) row[i] = (int32_t)(v - x);
+    __syncthreads();
+    if (tid == (int)blockDim.x - 1) s_carry = v;
+    __syncthreads();
+  }
+  if (tid == 0) totals[blockIdx.x] = s_carry;
+}
+
+__global__ void bucket_offsets_kernel(const int64_t* __restrict__ totals, int nbk, int64_t* __restrict__ boff) {
+  int64_t run = 0;
+  for (int k = 0; k < nbk; ++k) {
+    boff[k] = run;
+    run += totals[k];
+  }
+  boff[nbk] = run;
+}
+
+// pairs in (bucket, sample, slot) order and, for each, its index in the
+// sample-major order of fk_compact_connections.
+__global__ __launch_bounds__(64) void bucket_write_kernel(const uint8_t* __restrict__ use, int64_t batch, int n_conn,
+                                                          const int32_t* __restrict__ bucket_of, int nbk,
+                                                          const int32_t* __restrict__ starts,
+                                                          const int64_t* __restrict__ boff,
+                                                          const int64_t* __restrict__ offsets,
+                                                          int32_t* __restrict__ pairs, int64_t* __restrict__ out_index) {
+  __shared__ int s_run[BK_MAX][64];
+  const int lane = threadIdx.x;
+  const int64_t b = (int64_t)blockIdx.x * 64 + lane;
+  for (int k = 0; k < nbk; ++k) s_run[k][lane] = 0;
+  if (b >= batch) return;
+  int64_t kk = offsets[b];
+  for (int c = 0; c < n_conn; ++c) {
+    if (!use[(int64_t)c * batch + b]) continue;
+    const int bk = bucket_of[c];
+    const int64_t pos = boff[bk] + starts[(int64_t)bk * batch + b] + s_run[bk][lane]++;
+    pairs[2 * pos] = (int32_t)b;
+    pairs[2 * pos + 1] = c;
+    out_index[pos] = kk++;
+  }
+}
+
 __device__ __forceinline__ void cexp_diff(float dr, float di, double& er, double& ei) {
   // exp of a complex64 difference, evaluated in float like the reference's
   // numpy complex64 arithmetic (observables/monte_carlo/operator.py:31).
@@ -490,6 +579,36 @@ int fk_compact_connections(const uint8_t* use, int64_t batch, int32_t n_conn, in
   FK_CHECK_LAUNCH();
   hipLaunchKernelGGL(write_pairs_kernel, dim3(div_up(batch, 64)), dim3(tb), 0, s, use, batch, n_conn, offsets,
                      pairs);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int64_t fk_bucket_scratch_bytes(int64_t batch, int32_t n_buckets) {
+  if (batch < 0 || n_buckets < 1) return -1;
+  return (int64_t)n_buckets * batch * 4 + ((int64_t)n_buckets * 8 + 15) / 16 * 16;
+}
+
+int fk_bucket_connections(const uint8_t* use, int64_t batch, int32_t n_conn, const int32_t* bucket_of,
+                          int32_t n_buckets, const int64_t* offsets, int32_t* pairs, int64_t* out_index,
+                          int64_t* bucket_offsets, void* scratch, void* stream) {
+  FK_REQUIRE(batch >= 1 && n_conn >= 1 && n_buckets >= 1 && n_buckets <= BK_MAX,
+             "fk_bucket_connections: bad sizes (1 <= n_buckets <= %d)", BK_MAX);
+  FK_REQUIRE(use && bucket_of && offsets && pairs && out_index && bucket_offsets && scratch,
+             "fk_bucket_connections: null pointer");
+  FK_REQUIRE(batch <= 0x7fffffff, "fk_bucket_connections: batch too large");
+  hipStream_t s = (hipStream_t)stream;
+  int32_t* counts = reinterpret_cast<int32_t*>(scratch);
+  int64_t* totals = reinterpret_cast<int64_t*>(reinterpret_cast<uint8_t*>(scratch) +
+                                               ((int64_t)n_buckets * batch * 4 + 15) / 16 * 16);
+  hipLaunchKernelGGL(bucket_count_kernel, dim3(div_up(batch, 64)), dim3(64), 0, s, use, batch, n_conn, bucket_of,
+                     n_buckets, counts);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(bucket_scan_kernel, dim3(n_buckets), dim3(1024), 0, s, counts, batch, totals);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(bucket_offsets_kernel, dim3(1), dim3(1), 0, s, totals, n_buckets, bucket_offsets);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(bucket_write_kernel, dim3(div_up(batch, 64)), dim3(64), 0, s, use, batch, n_conn, bucket_of,
+                     n_buckets, counts, bucket_offsets, offsets, pairs, out_index);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }

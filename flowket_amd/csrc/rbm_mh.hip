@@ -412,6 +412,68 @@ __global__ void mh_accept_independent_kernel(int8_t* __restrict__ state, float* 
   }
 }
 
+// Per-sample O = d log psi / d(parameter) of the RBM (the Jacobian the
+// reference's ComplexValuesOptimizer takes of complex layers,
+// optimizers/complex_values_optimizer.py:60-76, written over the stored real
+// and imaginary parts): with theta_h = b_h + sum_j W_jh s_j and
+// T_h = tanh(theta_h), d/dW_re[j,h] = s_j T_h, d/dW_im[j,h] = -i s_j T_h,
+// d/db_re[h] = T_h, d/db_im[h] = -i T_h, d/da_re[j] = s_j, d/da_im[j] = -i s_j
+// (every complex parameter is used as real - i*imag, base_layer.py:30-35).
+// One workgroup per sample; rows in the flat order W_re, W_im, b_re, b_im,
+// a_re, a_im.
+__global__ __launch_bounds__(256) void rbm_grad_kernel(RbmParams p, const int8_t* __restrict__ spins, int64_t batch,
+                                                       float* __restrict__ o_re, float* __restrict__ o_im,
+                                                       int64_t ld) {
+  extern __shared__ float s_t[];  // T_h (re, im), then the spins
+  const int N = p.N, M = p.M;
+  const int64_t b = blockIdx.x;
+  if (b >= batch) return;
+  float* tre = s_t;
+  float* tim = s_t + M;
+  float* sv = s_t + 2 * M;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) sv[j] = (float)spins[b * N + j];
+  __syncthreads();
+  for (int h = threadIdx.x; h < M; h += blockDim.x) {
+    float x = p.b_re[h], y = -p.b_im[h];
+    for (int j = 0; j < N; ++j) {
+      x += p.w_re[(int64_t)j * M + h] * sv[j];
+      y -= p.w_im[(int64_t)j * M + h] * sv[j];
+    }
+    // tanh(x + iy) = (sinh 2x + i sin 2y) / (cosh 2x + cos 2y), written with
+    // e = exp(-2|x|) so that it cannot overflow
+    const float e = expf(-2.0f * fabsf(x));
+    float sn, cs;
+    sincosf(2.0f * y, &sn, &cs);
+    const float den = 1.0f + e * e + 2.0f * e * cs;
+    tre[h] = copysignf((1.0f - e * e) / den, x);
+    tim[h] = 2.0f * e * sn / den;
+  }
+  __syncthreads();
+  float* ore = o_re + b * ld;
+  float* oim = o_im + b * ld;
+  const int64_t NM = (int64_t)N * M;
+  for (int64_t q = threadIdx.x; q < NM; q += blockDim.x) {
+    const int j = (int)(q / M), h = (int)(q - (q / M) * M);
+    const float sj = sv[j];
+    ore[q] = sj * tre[h];
+    oim[q] = sj * tim[h];
+    ore[NM + q] = sj * tim[h];
+    oim[NM + q] = -sj * tre[h];
+  }
+  for (int h = threadIdx.x; h < M; h += blockDim.x) {
+    ore[2 * NM + h] = tre[h];
+    oim[2 * NM + h] = tim[h];
+    ore[2 * NM + M + h] = tim[h];
+    oim[2 * NM + M + h] = -tre[h];
+  }
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    ore[2 * NM + 2 * M + j] = sv[j];
+    oim[2 * NM + 2 * M + j] = 0.0f;
+    ore[2 * NM + 2 * M + N + j] = 0.0f;
+    oim[2 * NM + 2 * M + N + j] = -sv[j];
+  }
+}
+
 }  // namespace fk
 
 using namespace fk;
@@ -476,6 +538,24 @@ int fk_mh_rbm(const float* w_re, const float* w_im, const float* b_re, const flo
     set_error("fk_mh_rbm: at most 512 hidden units");
     return FK_ERR_UNSUPPORTED;
   }
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_rbm_grad(const float* w_re, const float* w_im, const float* b_re, const float* b_im, const float* a_re,
+                const float* a_im, int32_t n_visible, int32_t n_hidden, const int8_t* spins, int64_t batch,
+                float* o_re, float* o_im, int64_t row_stride, void* stream) {
+  FK_REQUIRE(n_visible > 0 && n_hidden > 0 && batch >= 0, "fk_rbm_grad: bad sizes");
+  if (batch == 0) return FK_OK;
+  FK_REQUIRE(w_re && w_im && b_re && b_im && a_re && a_im && spins && o_re && o_im, "fk_rbm_grad: null pointer");
+  const int64_t P = 2 * (int64_t)n_visible * n_hidden + 2 * (int64_t)n_hidden + 2 * (int64_t)n_visible;
+  FK_REQUIRE(row_stride == 0 || row_stride >= P, "fk_rbm_grad: row stride shorter than the parameters");
+  FK_REQUIRE(batch <= 0x7fffffff, "fk_rbm_grad: batch too large");
+  const size_t lds = (size_t)(2 * n_hidden + n_visible) * sizeof(float);
+  FK_REQUIRE(lds <= 64 * 1024, "fk_rbm_grad: too many units");
+  RbmParams p{w_re, w_im, b_re, b_im, a_re, a_im, n_visible, n_hidden};
+  hipLaunchKernelGGL(rbm_grad_kernel, dim3((unsigned)batch), dim3(256), lds, (hipStream_t)stream, p, spins, batch,
+                     o_re, o_im, row_stride > 0 ? row_stride : P);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }

@@ -149,6 +149,14 @@ __global__ __launch_bounds__(1024) void sum_u_kernel(const float* __restrict__ u
   }
 }
 
+// u_b -= mean_b u (complex pairs; sums from sum_u_kernel in FP64).
+__global__ void center_pairs_kernel(float* __restrict__ u, int64_t B, const double* __restrict__ sums) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  u[2 * b] = (float)((double)u[2 * b] - sums[0] / (double)B);
+  u[2 * b + 1] = (float)((double)u[2 * b + 1] - sums[1] / (double)B);
+}
+
 // out_k = scale * Re[sum_b conj(O_bk - Obar_k) u_b] + shift * v_k; each
 // thread owns 4 consecutive columns (float4 rows) when P is a multiple of 4.
 __global__ void oh_times_u_kernel(const float* __restrict__ ore, const float* __restrict__ oim,
@@ -213,6 +221,61 @@ __global__ void axpy_kernel(double alpha, const float* __restrict__ x, float* __
 __global__ void xpby_kernel(const float* __restrict__ r, double beta, float* __restrict__ p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = (float)((double)r[i] + beta * (double)p[i]);
+}
+
+// Device-side conjugate gradient (linear_equations.py:87-131, no
+// preconditioner) for a host loop with no host round trip.  State in
+// st[CG_N] doubles: gamma = r.r, p.z, gamma_new, tol_abs (tol*||b|| with both
+// factors rounded to float32, as the reference's float norms), done flag,
+// iterations, max_iter.  A step after convergence (or after max_iter) leaves
+// x, r, p unchanged, so a fixed number of launched steps gives the same
+// result as the reference's early exit.
+enum { CG_GAMMA = 0, CG_PZ = 1, CG_GNEW = 2, CG_TOL = 3, CG_DONE = 4, CG_ITER = 5, CG_MAXIT = 6, CG_N = 8 };
+
+__global__ void cg_init_kernel(const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
+                               float* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v = b[i];
+  x[i] = 0.0f;
+  r[i] = v;
+  p[i] = v;
+}
+
+__global__ void cg_init_scalars_kernel(double* st, double tol, double max_iter) {
+  const float nb = (float)sqrt(st[CG_GAMMA]);
+  st[CG_TOL] = (double)((float)tol * nb);
+  st[CG_ITER] = 0.0;
+  st[CG_MAXIT] = max_iter;
+  st[CG_DONE] = (max_iter <= 0.0 || !((double)nb > st[CG_TOL])) ? 1.0 : 0.0;
+}
+
+__global__ void cg_update_xr_kernel(const double* __restrict__ st, const float* __restrict__ p,
+                                    const float* __restrict__ z, float* __restrict__ x, float* __restrict__ r,
+                                    int64_t n) {
+  if (st[CG_DONE] != 0.0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double alpha = st[CG_GAMMA] / st[CG_PZ];
+  x[i] = (float)((double)x[i] + alpha * (double)p[i]);
+  r[i] = (float)((double)r[i] - alpha * (double)z[i]);
+}
+
+__global__ void cg_update_p_kernel(const double* __restrict__ st, const float* __restrict__ r, float* __restrict__ p,
+                                   int64_t n) {
+  if (st[CG_DONE] != 0.0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double beta = st[CG_GNEW] / st[CG_GAMMA];
+  p[i] = (float)((double)r[i] + beta * (double)p[i]);
+}
+
+__global__ void cg_finish_step_kernel(double* st) {
+  if (st[CG_DONE] != 0.0) return;
+  st[CG_GAMMA] = st[CG_GNEW];
+  st[CG_ITER] += 1.0;
+  const float nr = (float)sqrt(st[CG_GAMMA]);
+  if (!((double)nr > st[CG_TOL]) || st[CG_ITER] >= st[CG_MAXIT]) st[CG_DONE] = 1.0;
 }
 
 // Keras/TensorFlow Adam (tf.keras Adam, the optimizer of
@@ -306,6 +369,48 @@ int fk_dot_f64(const float* a, const float* b, int64_t n, double* scratch, doubl
   hipLaunchKernelGGL(dot_f64_kernel, dim3(blocks), dim3(1024), 0, s, a, b, n, scratch);
   FK_CHECK_LAUNCH();
   hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, s, scratch, blocks, out);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int64_t fk_cg_state_doubles(void) { return CG_N; }
+
+int fk_cg_init(const float* rhs, float* x, float* r, float* p, int64_t n, double tol, int64_t max_iter,
+               double* state, double* scratch, void* stream) {
+  FK_REQUIRE(rhs && x && r && p && state && scratch && n > 0, "fk_cg_init: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(cg_init_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, rhs, x, r, p, n);
+  FK_CHECK_LAUNCH();
+  const int rc = fk_dot_f64(rhs, rhs, n, scratch, state + CG_GAMMA, stream);
+  if (rc != FK_OK) return rc;
+  hipLaunchKernelGGL(cg_init_scalars_kernel, dim3(1), dim3(1), 0, s, state, tol, (double)max_iter);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_cg_step(float* x, float* r, float* p, const float* z, int64_t n, double* state, double* scratch,
+               void* stream) {
+  FK_REQUIRE(x && r && p && z && state && scratch && n > 0, "fk_cg_step: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = fk_dot_f64(p, z, n, scratch, state + CG_PZ, stream);
+  if (rc != FK_OK) return rc;
+  hipLaunchKernelGGL(cg_update_xr_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, state, p, z, x, r, n);
+  FK_CHECK_LAUNCH();
+  rc = fk_dot_f64(r, r, n, scratch, state + CG_GNEW, stream);
+  if (rc != FK_OK) return rc;
+  hipLaunchKernelGGL(cg_update_p_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, state, r, p, n);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(cg_finish_step_kernel, dim3(1), dim3(1), 0, s, state);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
+int fk_center_pairs(float* u, int64_t batch, double* scratch, void* stream) {
+  FK_REQUIRE(u && scratch && batch > 0, "fk_center_pairs: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(sum_u_kernel, dim3(1), dim3(1024), 0, s, u, batch, scratch);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(center_pairs_kernel, dim3(div_up(batch, 256)), dim3(256), 0, s, u, batch, scratch);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }

@@ -60,7 +60,10 @@ class LocalEnergyGraph:
         r = op.compact(self.spins)
         offsets, pairs = r['offsets'], r['pairs']
         kw = {} if self.precision is None else {'precision': self.precision}
-        logpsi = m.log_psi_connected(self.spins, pairs, pairs.shape[0], op, m_dev=offsets[B:], **kw)
+        if getattr(m, 'uses_row_reuse', lambda o: False)(op):
+            logpsi = m.log_psi_connected_rows(self.spins, r, op, **kw)
+        else:
+            logpsi = m.log_psi_connected(self.spins, pairs, pairs.shape[0], op, m_dev=offsets[B:], **kw)
         eloc = torch.empty((B, 2), dtype=torch.float64, device=self.spins.device)
         s = _lib.stream_ptr()
         _lib.call('fk_local_energy', _lib.ptr(offsets), _lib.ptr(pairs), _lib.ptr(logpsi), _lib.ptr(r['mel']), B,

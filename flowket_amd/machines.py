@@ -107,8 +107,6 @@ class ConvNetAutoregressive2D:
             out.append((torch.rand(kshape, generator=gen, dtype=torch.float64) * 2 - 1) * lim)
             out.append(torch.zeros(bshape, dtype=torch.float64))
         flat = torch.cat([t.reshape(-1) for t in out]).to(torch.float32)
-        if self.weights_normalization:
-            self.wn_g = self._wn_initial_g(flat)
         return flat
 
     def _wn_layer_views(self, flat):
@@ -129,20 +127,43 @@ class ConvNetAutoregressive2D:
             gs.append(torch.log(norm + 1e-10) if self.exponential_norm else norm)
         return torch.cat(gs).to(torch.float32)
 
+    def _n_wn_g(self):
+        return sum(k[3] for k, _ in self.shapes[:-1]) if self.weights_normalization else 0
+
     def set_params(self, params):
+        """Copies ``params`` (the flat kernels and biases; with weight
+        normalisation the kernels are the v of wrappers.py:123-135) into the
+        machine.  The trainable values live in one device tensor
+        ``trainable`` = [params | wn_g]; ``params`` and ``wn_g`` are views of
+        it, updated in place so that device pointers (and captured graphs)
+        stay valid.  On the first call with weight normalisation the g start
+        at log(|v| + 1e-10) per output channel (CopyNormaInitializer,
+        wrappers.py:26-39)."""
         p = torch.as_tensor(params, dtype=torch.float32).reshape(-1)
         if p.numel() != self.n_params:
             raise ValueError(f'expected {self.n_params} parameters, got {p.numel()}')
-        if getattr(self, 'params', None) is not None:
-            # in place: device pointers stay valid for captured graphs
-            self.params.copy_(p.to(self.device))
-        else:
-            self.params = p.to(self.device).clone().contiguous()
-        if self.weights_normalization and not hasattr(self, 'wn_g'):
-            self.wn_g = self._wn_initial_g(self.params.cpu())
-        if self.weights_normalization:
-            self.wn_g = torch.as_tensor(self.wn_g, dtype=torch.float32).to(self.device)
+        if getattr(self, 'trainable', None) is None:
+            self.trainable = torch.zeros(self.n_params + self._n_wn_g(), dtype=torch.float32, device=self.device)
+            self.params = self.trainable[:self.n_params]
+            if self.weights_normalization:
+                self.wn_g = self.trainable[self.n_params:]
+                self.wn_g.copy_(self._wn_initial_g(p.cpu()).to(self.device))
+        self.params.copy_(p.to(self.device))
         self._version = getattr(self, '_version', 0) + 1
+
+    @property
+    def n_trainable(self):
+        return int(self.trainable.numel())
+
+    def set_wn_g(self, g):
+        """Sets the weight-normalisation g of every wrapped layer (in place)."""
+        if not self.weights_normalization:
+            raise ValueError('the machine has no weight normalisation')
+        g = torch.as_tensor(g, dtype=torch.float32).reshape(-1)
+        if g.numel() != self.wn_g.numel():
+            raise ValueError(f'expected {self.wn_g.numel()} g values, got {g.numel()}')
+        self.wn_g.copy_(g.to(self.device))
+        self._version += 1
 
     def mark_params_updated(self):
         self._version += 1
@@ -163,9 +184,9 @@ class ConvNetAutoregressive2D:
             return load_weights(self, filepath)
         from .keras_weights import convnet_params_from_keras, load_keras_weights
         flat, g = convnet_params_from_keras(load_keras_weights(filepath), self.shapes, self.weights_normalization)
-        if self.weights_normalization:
-            self.wn_g = torch.from_numpy(g)
         self.set_params(torch.from_numpy(flat))
+        if self.weights_normalization:
+            self.set_wn_g(torch.from_numpy(g))
         return self
 
     def effective_params(self):
@@ -185,8 +206,9 @@ class ConvNetAutoregressive2D:
                 g = self.wn_g[gi:gi + cout]
                 gi += cout
                 scale = torch.exp(g) if self.exponential_norm else g
-                norm = torch.sqrt((v * v).sum(dim=(0, 1, 2)))
-                v = v / torch.clamp(norm, min=1e-12) * scale
+                # tf.nn.l2_normalize: v / sqrt(max(sum v^2, 1e-12))
+                norm = torch.sqrt(torch.clamp((v * v).sum(dim=(0, 1, 2)), min=1e-12))
+                v = v / norm * scale
             parts += [v.reshape(-1), p[off + nk:off + nk + nb]]
         return torch.cat(parts)
 
@@ -299,27 +321,200 @@ class ConvNetAutoregressive2D:
     def count_params(self):
         return self.n_params
 
+    # ------------------------------------------------------------- row reuse
+    row_reuse = True
+
+    def uses_row_reuse(self, operator):
+        """Whether local energies of this network run the row-reuse forward
+        (fk_convar_forward_rows): lattices of 3..64 rows and an operator with
+        a connection table."""
+        return bool(self.row_reuse) and 3 <= self.H <= 64 and hasattr(operator, 'row_buckets')
+
+    def log_psi_connected_rows(self, spins, r, operator, precision=None, stream=None):
+        """log psi of every used connected configuration of the ragged map
+        ``r`` (operator.compact: use, offsets, pairs) in its sample-major
+        order, with row reuse: the samples' forward stores every block's
+        vertical input and concatenation; each connection, bucketed by the
+        first lattice row it changes (fk_bucket_connections), is then
+        evaluated on its rows from there on only (fk_convar_forward_rows).
+        No host round trip: bucket sizes stay on the device."""
+        precision = precision or self.precision
+        prec = _PREC[precision]
+        wbuf, abuf = self._pack(precision)
+        B = spins.shape[0]
+        H, W = self.H, self.W
+        dev = self.device
+        nc = operator.num_connections
+        bucket_of, nbk, sizes = operator.row_buckets(W, dev)
+        tb = operator.tables(dev)
+        ws = self._rows_workspace(prec, B, nc, nbk)
+        _lib.call('fk_bucket_connections', _lib.ptr(r['use']), B, nc, _lib.ptr(bucket_of), nbk,
+                  _lib.ptr(r['offsets']), _lib.ptr(ws['pairs']), _lib.ptr(ws['out_index']), _lib.ptr(ws['boff']),
+                  _lib.ptr(ws['scratch']), _lib.stream_ptr(stream))
+        logpsi = torch.empty((nc * B, 2), dtype=torch.float32, device=dev)
+        fws = self._forward_workspace(prec)
+        fwn = 0 if fws is None else fws.numel()
+        act = _ACT[self.activation]
+        common = (_lib.ptr(wbuf), _lib.ptr(abuf), prec, H, W, self.depth, self.num_of_channels, act)
+        # the samples themselves (slot 0): write the cache and the row prefixes
+        _lib.call('fk_convar_forward_rows', *common, _lib.ptr(spins), None, None, B, operator.kind,
+                  _lib.ptr(tb['conn_a']), _lib.ptr(tb['conn_b']), 0, _lib.ptr(ws['rcache']), 1,
+                  _lib.ptr(ws['prefix']), _lib.ptr(r['offsets']), _lib.ptr(logpsi), _lib.ptr(fws), fwn,
+                  _lib.stream_ptr(stream))
+        for k in range(1, nbk):
+            upper = int(sizes[k]) * B
+            if upper == 0:
+                continue
+            row0 = k - 1
+            if row0 == 1:
+                row0 = 0  # two halo rows are needed above the first computed row
+            rng = ws['boff'][k:k + 2]
+            _lib.call('fk_convar_forward_rows', *common, _lib.ptr(spins), _lib.ptr(ws['pairs']), _lib.ptr(rng), upper,
+                      operator.kind, _lib.ptr(tb['conn_a']), _lib.ptr(tb['conn_b']), row0, _lib.ptr(ws['rcache']),
+                      0, _lib.ptr(ws['prefix']), _lib.ptr(ws['out_index']), _lib.ptr(logpsi), _lib.ptr(fws), fwn,
+                      _lib.stream_ptr(stream))
+        return logpsi
+
+    def _rows_workspace(self, prec, B, nc, nbk):
+        key = (prec, B, nc, nbk)
+        cache = self.__dict__.setdefault('_rows_ws', {})
+        if key not in cache:
+            dev = self.device
+            per = int(_lib.load().fk_convar_rcache_bytes(self.H, self.W, self.depth, self.num_of_channels, prec))
+            cache.clear()
+            cache[key] = dict(
+                pairs=torch.empty((nc * B, 2), dtype=torch.int32, device=dev),
+                out_index=torch.empty(nc * B, dtype=torch.int64, device=dev),
+                boff=torch.empty(nbk + 1, dtype=torch.int64, device=dev),
+                scratch=torch.empty(int(_lib.load().fk_bucket_scratch_bytes(B, nbk)), dtype=torch.uint8, device=dev),
+                rcache=torch.empty(per * B, dtype=torch.uint8, device=dev),
+                prefix=torch.empty((B, self.H + 1, 2), dtype=torch.float32, device=dev))
+        return cache[key]
+
+    def supports_implicit_products(self):
+        """vjp / jvp (batched products with O that never form it) run on the
+        MFMA kernels, for channel counts that are a multiple of 32."""
+        return self.num_of_channels % 32 == 0
+
+    def _implicit_workspace(self, kind, B):
+        fn = {'vjp': 'fk_convar_vjp_workspace_bytes', 'jvp': 'fk_convar_jvp_workspace_bytes'}[kind]
+        nbytes = int(getattr(_lib.load(), fn)(self.H, self.W, self.depth, self.num_of_channels, B))
+        if nbytes < 0:
+            raise _lib.FlowketHipError(f'{kind}: {self.num_of_channels} channels are not supported (the implicit '
+                                       'products need a multiple of 32)')
+        cache = self.__dict__.setdefault('_implicit_ws', {})
+        ws = cache.get(kind)
+        if ws is None or ws.numel() < nbytes:
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            cache[kind] = ws
+        return ws
+
+    def vjp_effective(self, x, cot, scale=1.0, stream=None):
+        """scale * sum_b (cot[b,0] dRe log psi_b + cot[b,1] dIm log psi_b) with
+        respect to the effective kernels and biases (P floats): scale *
+        Re[O^H u] for u = cot[:,0] + i cot[:,1], through fk_convar_vjp (one
+        batched backward pass, O never formed)."""
+        _lib.require_gpu()
+        spins = self._spins(x)
+        B = spins.shape[0]
+        cot = torch.as_tensor(cot).to(self.device, torch.float32).reshape(B, 2).contiguous()
+        grad = torch.empty(self.n_params, dtype=torch.float32, device=self.device)
+        ws = self._implicit_workspace('vjp', B)
+        _lib.call('fk_convar_vjp', _lib.ptr(self.effective_params().contiguous()), self.H, self.W, self.depth,
+                  self.num_of_channels, _ACT[self.activation], _lib.ptr(spins), B, _lib.ptr(cot), float(scale),
+                  _lib.ptr(ws), _lib.ptr(grad), None, _lib.stream_ptr(stream))
+        return grad
+
+    def jvp_effective(self, x, tangent, stream=None):
+        """(B, 2) float32: (dRe log psi_b . t, dIm log psi_b . t) for a
+        direction t over the effective kernels and biases (O t), by
+        forward-mode differentiation (fk_convar_jvp)."""
+        _lib.require_gpu()
+        spins = self._spins(x)
+        B = spins.shape[0]
+        t = torch.as_tensor(tangent).to(self.device, torch.float32).reshape(-1).contiguous()
+        if t.numel() != self.n_params:
+            raise ValueError(f'tangent has {t.numel()} values, the network {self.n_params}')
+        out = torch.empty((B, 2), dtype=torch.float32, device=self.device)
+        ws = self._implicit_workspace('jvp', B)
+        _lib.call('fk_convar_jvp', _lib.ptr(self.effective_params().contiguous()), _lib.ptr(t), self.H, self.W,
+                  self.depth, self.num_of_channels, _ACT[self.activation], _lib.ptr(spins), B, _lib.ptr(ws),
+                  _lib.ptr(out), None, _lib.stream_ptr(stream))
+        return out
+
     def per_sample_gradients(self, x, workspace=None, stream=None):
         """O = d log psi / d theta per sample (machines/abstract_machine.py:24-28,
         utils/jacobian.py:114-123) through fk_convar_grad: returns (o_re, o_im,
         logpsi) with o_* (B, P) float32 in the flat parameter order and logpsi
-        (B, 2) float32 (FP32 forward of the gradient kernel)."""
-        if self.weights_normalization:
-            raise NotImplementedError('per-sample gradients are computed w.r.t. the effective kernels; '
-                                      'build the machine with weights_normalization=False')
+        (B, 2) float32 (FP32 forward of the gradient kernel).  The columns are
+        the trainable values: the flat parameters, and with weight
+        normalisation the g of every wrapped layer after them (the kernel
+        columns then hold d/dv), n_trainable columns in all."""
         _lib.require_gpu()
         spins = self._spins(x)
         B = spins.shape[0]
-        o_re = torch.empty((B, self.n_params), dtype=torch.float32, device=self.device)
-        o_im = torch.empty((B, self.n_params), dtype=torch.float32, device=self.device)
+        P, PT = self.n_params, self.n_trainable
+        o_re = torch.empty((B, PT), dtype=torch.float32, device=self.device)
+        o_im = torch.empty((B, PT), dtype=torch.float32, device=self.device)
         lp = torch.empty((B, 2), dtype=torch.float32, device=self.device)
         nbytes = _lib.load().fk_convar_grad_workspace_bytes(self.H, self.W, self.depth, self.num_of_channels, B)
         if workspace is None or workspace.numel() < nbytes:
             workspace = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.device)
-        _lib.call('fk_convar_grad', _lib.ptr(self.params), self.H, self.W, self.depth, self.num_of_channels,
-                  _ACT[self.activation], _lib.ptr(spins), B, _lib.ptr(workspace), _lib.ptr(o_re), _lib.ptr(o_im),
-                  _lib.ptr(lp), _lib.stream_ptr(stream))
+        _lib.call('fk_convar_grad', _lib.ptr(self.effective_params().contiguous()), self.H, self.W, self.depth,
+                  self.num_of_channels, _ACT[self.activation], _lib.ptr(spins), B, _lib.ptr(workspace),
+                  _lib.ptr(o_re), _lib.ptr(o_im), PT, _lib.ptr(lp), _lib.stream_ptr(stream))
+        if self.weights_normalization:
+            for o in (o_re, o_im):
+                self._wn_rows(o, B)
         return o_re, o_im, lp
+
+    # ---------------------------------------------------- weight normalisation
+    def _wn_table(self):
+        """(device int64 table (layers, 4) = kernel offset, kh*kw*cin, cout,
+        g offset; number of layers; largest cout) of the wrapped layers."""
+        if getattr(self, '_wn_tab', None) is None:
+            rows, off, goff = [], 0, 0
+            for idx, (kshape, bshape) in enumerate(self.shapes):
+                nk = int(np.prod(kshape))
+                if idx < len(self.shapes) - 1:
+                    rows.append([off, nk // kshape[3], kshape[3], goff])
+                    goff += kshape[3]
+                off += nk + int(np.prod(bshape))
+            tab = torch.as_tensor(np.asarray(rows, dtype=np.int64)).to(self.device)
+            self._wn_tab = (tab, len(rows), max(r[2] for r in rows))
+        return self._wn_tab
+
+    def _wn_rows(self, rows, nrows):
+        """d/dk rows -> d/d(v, g) rows in place (fk_wn_grad_rows)."""
+        tab, nl, mc = self._wn_table()
+        _lib.call('fk_wn_grad_rows', _lib.ptr(rows), nrows, rows.shape[-1], _lib.ptr(self.params),
+                  _lib.ptr(self.wn_g), _lib.ptr(tab), nl, mc, int(self.exponential_norm), self.n_params,
+                  _lib.stream_ptr())
+
+    def vjp(self, x, cot, scale=1.0, stream=None):
+        """scale * Re[O^H u] (u = cot[:,0] + i cot[:,1]) over the trainable
+        values (n_trainable floats), O never formed: fk_convar_vjp, then the
+        weight-normalisation chain rule."""
+        g = self.vjp_effective(x, cot, scale, stream)
+        if not self.weights_normalization:
+            return g
+        out = torch.zeros(self.n_trainable, dtype=torch.float32, device=self.device)
+        out[:self.n_params] = g
+        self._wn_rows(out, 1)
+        return out
+
+    def jvp(self, x, tangent, stream=None):
+        """O t (B, 2) for a direction t over the trainable values."""
+        t = torch.as_tensor(tangent).to(self.device, torch.float32).reshape(-1).contiguous()
+        if t.numel() != self.n_trainable:
+            raise ValueError(f'tangent has {t.numel()} values, the machine {self.n_trainable} trainable values')
+        if self.weights_normalization:
+            tab, nl, mc = self._wn_table()
+            t_eff = torch.empty(self.n_params, dtype=torch.float32, device=self.device)
+            _lib.call('fk_wn_tangent', _lib.ptr(self.params), _lib.ptr(self.wn_g), _lib.ptr(tab), nl, mc,
+                      int(self.exponential_norm), _lib.ptr(t), _lib.ptr(t_eff), self.n_params, _lib.stream_ptr())
+            t = t_eff
+        return self.jvp_effective(x, t, stream)
 
 
 class RBM:
@@ -366,6 +561,32 @@ class RBM:
 
     def mark_params_updated(self):
         pass
+
+    @property
+    def trainable(self):
+        """The trainable values (the stored real and imaginary parts)."""
+        return self.params
+
+    @property
+    def n_trainable(self):
+        return self.n_params
+
+    def supports_implicit_products(self):
+        return False
+
+    def per_sample_gradients(self, x, stream=None):
+        """O = d log psi / d(parameter) per sample over the stored real and
+        imaginary parts (fk_rbm_grad): (o_re, o_im, logpsi), o_* (B, P)."""
+        _lib.require_gpu()
+        spins = self._spins(x)
+        B = spins.shape[0]
+        o_re = torch.empty((B, self.n_params), dtype=torch.float32, device=self.device)
+        o_im = torch.empty((B, self.n_params), dtype=torch.float32, device=self.device)
+        w = [_lib.ptr(t) for t in self.parts()]
+        _lib.call('fk_rbm_grad', *w, self.n_visible, self.n_hidden, _lib.ptr(spins), B, _lib.ptr(o_re),
+                  _lib.ptr(o_im), self.n_params, _lib.stream_ptr(stream))
+        lp, _ = self.forward(spins, stream=stream)
+        return o_re, o_im, lp
 
     def parts(self):
         """(w_re, w_im, b_re, b_im, a_re, a_im) views of the flat parameters."""

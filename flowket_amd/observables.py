@@ -2,10 +2,11 @@
 observables/monte_carlo/observable.py) on the GPU.
 
 For a machine with a fused connected-configuration path the whole local
-energy is three launches and no host round trip: fk_find_conn (mel/use),
-fk_compact_connections (ragged index map), fk_convar_forward over the ragged
-(sample, connection) list with the count read from device memory, then
-fk_local_energy.  Any other callable wave function goes through the dense
+energy runs without a host round trip: fk_find_conn (mel/use),
+fk_compact_connections (ragged index map), the network over the ragged
+(sample, connection) list with the count read from device memory (for
+ConvNetAutoregressive2D with row reuse: the samples' pass, then each bucket of
+connections evaluated from the first row it changes), then fk_local_energy.  Any other callable wave function goes through the dense
 (balanced) layout.
 """
 import abc
@@ -65,7 +66,10 @@ class Observable(BaseObservable):
             r = op.compact(spins, stream=stream)
             offsets, pairs = r['offsets'], r['pairs']
             upper = pairs.shape[0]
-            logpsi = wave_function.log_psi_connected(spins, pairs, upper, op, m_dev=offsets[B:], stream=stream)
+            if getattr(wave_function, 'uses_row_reuse', lambda o: False)(op):
+                logpsi = wave_function.log_psi_connected_rows(spins, r, op, stream=stream)
+            else:
+                logpsi = wave_function.log_psi_connected(spins, pairs, upper, op, m_dev=offsets[B:], stream=stream)
             eloc = torch.empty((B, 2), dtype=torch.float64, device=spins.device)
             _lib.call('fk_local_energy', _lib.ptr(offsets), _lib.ptr(pairs), _lib.ptr(logpsi), _lib.ptr(r['mel']),
                       B, _lib.ptr(eloc), _lib.stream_ptr(stream))

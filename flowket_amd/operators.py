@@ -101,6 +101,27 @@ class _BondTableOperator(OperatorOnGrid):
                 diag_a=up(da), diag_b=up(db), diag_j=up(dj), n_diag=int(self._diag_a.size))
         return self._device_tables[key]
 
+    def row_buckets(self, W, device):
+        """Per connection slot, the bucket of the row-reuse forward on a
+        lattice W sites wide: 0 for slot 0 (the sample itself), else 1 + the
+        first lattice row the connection changes (its lowest site's row).
+        Returns (device int32 table, number of buckets, slots per bucket)."""
+        key = ('row_buckets', int(W), str(device))
+        if key not in self._device_tables:
+            n = self.n_sites
+            H = n // W
+            a, b = self._conn_a, self._conn_b
+            bk = np.zeros(a.size, np.int32)
+            for c in range(1, a.size):
+                if a[c] < 0:
+                    bk[c] = 1
+                    continue
+                q0 = min(a[c], b[c]) if (self.kind != _lib.FK_CONN_FLIP and b[c] >= 0) else a[c]
+                bk[c] = 1 + q0 // W
+            sizes = np.bincount(bk, minlength=H + 1)
+            self._device_tables[key] = (torch.as_tensor(bk).to(device), H + 1, sizes)
+        return self._device_tables[key]
+
     def find_conn_device(self, spins, want_conn=True, stream=None):
         """spins (B, n_sites) int8 on the GPU -> dict of device tensors
         conn (Nc, B, n_sites) int8 or None, mel (Nc, B) float64, use (Nc, B)

@@ -97,6 +97,22 @@ int fk_compact_connections(const uint8_t* use, int64_t batch, int32_t n_conn,
                            int64_t* offsets, int32_t* pairs, void* scratch,
                            void* stream);
 
+/* The same pairs regrouped by a per-slot bucket (bucket_of[c] in
+ * [0, n_buckets), n_buckets <= 128; the row-reuse forward buckets a
+ * connection by the first lattice row it changes): pairs (total, 2) in
+ * (bucket, sample, slot) order, out_index[k] = the position of pair k in the
+ * sample-major order of fk_compact_connections (offsets from it),
+ * bucket_offsets (n_buckets + 1) the start of every bucket and the total.
+ * `scratch` holds fk_bucket_scratch_bytes(batch, n_buckets) bytes.  No
+ * counterpart in the reference (its E_loc evaluates every connection with a
+ * full forward). */
+int64_t fk_bucket_scratch_bytes(int64_t batch, int32_t n_buckets);
+int fk_bucket_connections(const uint8_t* use, int64_t batch, int32_t n_conn,
+                          const int32_t* bucket_of, int32_t n_buckets,
+                          const int64_t* offsets, int32_t* pairs,
+                          int64_t* out_index, int64_t* bucket_offsets,
+                          void* scratch, void* stream);
+
 /* E_loc (observables/monte_carlo/operator.py:13-33): for sample b,
  *   sum_k mel[c_k, b] * exp(logpsi[k] - logpsi[offsets[b]])
  * over k in [offsets[b], offsets[b+1]), difference and exp in complex64 as the
@@ -187,6 +203,41 @@ int fk_convar_forward_ws(const void* wpacked, const float* aux, int32_t precisio
                          const int32_t* conn_a, const int32_t* conn_b,
                          float* logpsi, float* cond_logp, void* workspace,
                          int64_t workspace_bytes, void* stream);
+/* Row reuse for connected configurations (no counterpart in the reference,
+ * which runs a full forward per connection; the results are the same up to
+ * FP32 summation order).  Every activation of lattice row r depends on input
+ * rows <= r only, so a configuration equal to its sample on the rows before
+ * row0 shares the sample's activations there:
+ *   sample pass (row0 = 0, cache_write = 1, pairs = NULL): the forward of
+ *     m_count rows of `spins`, storing for every sample its blocks' vertical
+ *     inputs and concatenations in rcache (fk_convar_rcache_bytes(...) bytes
+ *     per sample) and its per-row prefix sums of the site log-probabilities
+ *     in prefix ((batch, H + 1, 2) floats);
+ *   row pass (2 <= row0 < H, pairs given): configurations whose rows before
+ *     row0 equal their sample's; only rows row0 .. H-1 are computed, the two
+ *     rows above them are copied from the sample's cache and the prefix sum
+ *     of the rows before row0 is added;
+ *   row0 = 0, cache_write = 0: a plain pass.
+ * m_range (may be NULL) is a device pointer to (first, end): the launch
+ * evaluates pairs[first .. end) (m_count bounds end - first and sizes the
+ * grid), so buckets of fk_bucket_connections need no host round trip.
+ * out_index (may be NULL) places the result of configuration m at row
+ * out_index[first + m] of logpsi.  3 <= H <= 64.  Workspace as fk_convar_forward_ws
+ * (its size covers the row passes). */
+int64_t fk_convar_rcache_bytes(int32_t H, int32_t W, int32_t depth,
+                               int32_t channels, int32_t precision);
+int fk_convar_forward_rows(const void* wpacked, const float* aux,
+                           int32_t precision, int32_t H, int32_t W,
+                           int32_t depth, int32_t channels, int32_t activation,
+                           const int8_t* spins, const int32_t* pairs,
+                           const int64_t* m_range, int64_t m_count,
+                           int32_t conn_kind, const int32_t* conn_a,
+                           const int32_t* conn_b, int32_t row0, void* rcache,
+                           int32_t cache_write, float* prefix,
+                           const int64_t* out_index, float* logpsi,
+                           void* workspace, int64_t workspace_bytes,
+                           void* stream);
+
 /* Host-only (queries the current device's CU count).  Bytes of workspace
  * fk_convar_forward_ws needs for a network (under its selected schedule),
  * 0 if none, -1 if the lattice is unsupported. */
@@ -245,8 +296,9 @@ int fk_fast_sample(const void* wpacked, const float* aux, int32_t precision,
  * complex log psi): Machine.predictions_jacobian (machines/abstract_machine.py
  * :24-28) re-paired as complex O_k (optimizers/complex_values_optimizer.py:
  * 8-9), per example as utils/jacobian.py:114-123.  params: flat FP32
- * parameters (effective kernels); o_re / o_im (batch, P) FP32 in the flat
- * order; logpsi (batch, 2) or NULL.  FP32 throughout.  `workspace` holds
+ * parameters (effective kernels); o_re / o_im (batch, row_stride) FP32, the
+ * first P entries of each row in the flat order (row_stride 0 means P);
+ * logpsi (batch, 2) or NULL.  FP32 throughout.  `workspace` holds
  * fk_convar_grad_workspace_bytes(...) bytes.
  * ------------------------------------------------------------------------- */
 int64_t fk_convar_grad_workspace_bytes(int32_t H, int32_t W, int32_t depth,
@@ -254,7 +306,56 @@ int64_t fk_convar_grad_workspace_bytes(int32_t H, int32_t W, int32_t depth,
 int fk_convar_grad(const float* params, int32_t H, int32_t W, int32_t depth,
                    int32_t channels, int32_t activation, const int8_t* spins,
                    int64_t batch, void* workspace, float* o_re, float* o_im,
-                   float* logpsi, void* stream);
+                   int64_t row_stride, float* logpsi, void* stream);
+
+/* Implicit products with O, never forming it (the
+ * compute_jvp_instead_of_full_jacobian path of optimizers/
+ * stochastic_reconfiguration/optimizer.py:116-124 through
+ * optimizers/utils.py:5-34, and the batch gradient of the VMC loss,
+ * optimization/loss.py:4-5).  Channels a multiple of 32; FP32 MFMA.
+ *   fk_convar_vjp: grad = scale * sum_b (cot[b,0] d Re log psi_b / d theta
+ *                  + cot[b,1] d Im log psi_b / d theta), P floats, i.e.
+ *                  scale * Re[O^H u] for u = cot[:,0] + i cot[:,1].  One
+ *                  backward pass per sample; the samples' contributions are
+ *                  summed in per-workgroup partial rows and reduced in a
+ *                  fixed order in FP64 (deterministic).
+ *   fk_convar_jvp: jvp (batch, 2) = (d Re log psi_b . t, d Im log psi_b . t)
+ *                  for a parameter direction t (P floats), i.e. O t, by
+ *                  forward-mode differentiation of the network.
+ * logpsi (batch, 2) or NULL.  Workspaces hold *_workspace_bytes(...) bytes
+ * (host-only queries: they depend on the device's CU count). */
+int64_t fk_convar_vjp_workspace_bytes(int32_t H, int32_t W, int32_t depth,
+                                      int32_t channels, int64_t batch);
+int fk_convar_vjp(const float* params, int32_t H, int32_t W, int32_t depth,
+                  int32_t channels, int32_t activation, const int8_t* spins,
+                  int64_t batch, const float* cot, double scale,
+                  void* workspace, float* grad, float* logpsi, void* stream);
+int64_t fk_convar_jvp_workspace_bytes(int32_t H, int32_t W, int32_t depth,
+                                      int32_t channels, int64_t batch);
+int fk_convar_jvp(const float* params, const float* tangent, int32_t H,
+                  int32_t W, int32_t depth, int32_t channels,
+                  int32_t activation, const int8_t* spins, int64_t batch,
+                  void* workspace, float* jvp, float* logpsi, void* stream);
+
+/* Weight normalisation chain rule (deepar/layers/wrappers.py:123-135):
+ * a wrapped layer's kernel is k = v/n*s per output channel, n =
+ * sqrt(max(sum v^2, 1e-12)) (tf.nn.l2_normalize), s = exp(g) when exp_norm
+ * else g.  `layers` holds 4 int64 per wrapped layer: kernel offset in the
+ * flat parameters, kh*kw*cin, cout (<= max_cout), offset of its g.  params
+ * are the flat parameters with v in the kernel slots, g the g vector.
+ *   fk_wn_grad_rows: in place on rows (nrows, ld >= P + n_g): each row's
+ *                    d/dk in [0, P) becomes d/dv in the kernel slots, and
+ *                    d/dg is written at P + goff + c (the trainable order
+ *                    [params | g]).
+ *   fk_wn_tangent:   t_eff (P) = dk for a direction t over [params | g]. */
+int fk_wn_grad_rows(float* rows, int64_t nrows, int64_t ld,
+                    const float* params, const float* g, const int64_t* layers,
+                    int32_t n_layers, int32_t max_cout, int32_t exp_norm,
+                    int64_t n_params, void* stream);
+int fk_wn_tangent(const float* params, const float* g, const int64_t* layers,
+                  int32_t n_layers, int32_t max_cout, int32_t exp_norm,
+                  const float* t, float* t_eff, int64_t n_params,
+                  void* stream);
 
 /* ---------------------------------------------------------------------------
  * Stochastic reconfiguration on the implicit S = Re[(O-Obar)^H (O-Obar)]/B
@@ -284,6 +385,27 @@ int64_t fk_sr_scratch_doubles(void);
 int fk_dot_f64(const float* a, const float* b, int64_t n, double* scratch,
                double* out, void* stream);
 int fk_axpy(double alpha, const float* x, float* y, int64_t n, void* stream);
+
+/* Conjugate gradient of linear_equations.py:34-137 (no preconditioner) with
+ * its scalars on the device, so a solve needs no host round trip (and can be
+ * captured in a graph): fk_cg_init sets x = 0, r = p = rhs and the state
+ * (fk_cg_state_doubles() doubles: r.r, p.z, new r.r, tol*||rhs|| with both
+ * factors rounded to float32, done flag, iterations, max_iter); after each
+ * matrix product z = A p the caller launches fk_cg_step, which performs one
+ * iteration unless the reference's loop would already have stopped
+ * (i >= max_iter or ||r|| <= tol*||rhs||), leaving everything unchanged then.
+ * state[5] is the iteration count, sqrt(state[0]) the residual norm.
+ * Dots in FP64 (scratch as fk_dot_f64). */
+int64_t fk_cg_state_doubles(void);
+int fk_cg_init(const float* rhs, float* x, float* r, float* p, int64_t n,
+               double tol, int64_t max_iter, double* state, double* scratch,
+               void* stream);
+int fk_cg_step(float* x, float* r, float* p, const float* z, int64_t n,
+               double* state, double* scratch, void* stream);
+
+/* u (batch, 2) complex pairs: u -= mean(u) (FP64 sums; scratch holds 2
+ * doubles).  Centres O t for the implicit S v = Re[(O-Obar)^H (O-Obar) v]/B. */
+int fk_center_pairs(float* u, int64_t batch, double* scratch, void* stream);
 int fk_xpby(const float* r, double beta, float* p, int64_t n, void* stream);
 
 /* Adam step t (>= 1) of the Keras/TensorFlow optimizer the reference's
@@ -309,6 +431,19 @@ int fk_rbm_log_psi(const float* w_re, const float* w_im, const float* b_re,
                    const int32_t* pairs, const int64_t* m_dev, int64_t m_count,
                    int32_t conn_kind, const int32_t* conn_a,
                    const int32_t* conn_b, float* logpsi, void* stream);
+
+/* Per-sample O = d log psi / d(parameter) of the RBM over its stored real
+ * and imaginary parts (the Jacobian of complex layers that
+ * optimizers/complex_values_optimizer.py:60-76 takes): with T = tanh(theta),
+ * d/dW_re[j,h] = s_j T_h, d/dW_im = -i s_j T_h, d/db_re = T_h, d/db_im =
+ * -i T_h, d/da_re = s_j, d/da_im = -i s_j.  o_re / o_im (batch, row_stride)
+ * FP32 in the flat order W_re, W_im, b_re, b_im, a_re, a_im (row_stride 0 =
+ * the parameter count). */
+int fk_rbm_grad(const float* w_re, const float* w_im, const float* b_re,
+                const float* b_im, const float* a_re, const float* a_im,
+                int32_t n_visible, int32_t n_hidden, const int8_t* spins,
+                int64_t batch, float* o_re, float* o_im, int64_t row_stride,
+                void* stream);
 
 /* Metropolis-Hastings chains of an RBM (samplers/metropolis_hastings.py:
  * 15-145): kind 0 = MetropolisHastingsLocal (flip one site), 1 =

@@ -48,3 +48,4 @@ unless named):
   optim.keras_adam           the closed form of the first step.
   ensemble.*                 the probabilistic ensemble's closed forms
                              (tests/test_ensemble.py).
+"""

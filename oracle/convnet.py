@@ -51,8 +51,9 @@ def weight_norm(v, g, exponential_norm=True):
     """deepar/layers/wrappers.py:123-135: l2-normalise v over all axes but the
     last, scale per output channel by g (exp(g) when exponential_norm)."""
     gg = np.exp(g) if exponential_norm else g
-    norm = np.sqrt((v * v).sum(axis=(0, 1, 2), keepdims=True))
-    return v / np.maximum(norm, 1e-12) * gg
+    # tf.nn.l2_normalize(v, axes): v / sqrt(max(sum v^2, 1e-12))
+    norm = np.sqrt(np.maximum((v * v).sum(axis=(0, 1, 2), keepdims=True), 1e-12))
+    return v / norm * gg
 
 
 def _act(x, activation):

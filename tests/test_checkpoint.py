@@ -42,8 +42,8 @@ def test_adam_state_round_trip(tmp_path):
     m = _machine()
     opt = fk.Adam(m, lr=1e-2)
     g = torch.Generator().manual_seed(2)
-    opt.set_weights([torch.tensor([5.0], dtype=torch.float64), torch.randn(m.n_params, generator=g),
-                     torch.rand(m.n_params, generator=g)])
+    opt.set_weights([torch.tensor([5.0], dtype=torch.float64), torch.randn(m.n_trainable, generator=g),
+                     torch.rand(m.n_trainable, generator=g)])
     path = str(tmp_path / 'ck')
     fk.save_optimizer_weights(opt, path, epoch=1, batch=2)
     opt2 = fk.Adam(_machine(seed=5))
@@ -52,7 +52,7 @@ def test_adam_state_round_trip(tmp_path):
     for a, b in zip(opt.get_weights(), opt2.get_weights()):
         assert torch.equal(a, b)
     with pytest.raises(fk.FlowketHipError):
-        opt2.apply_gradient(torch.zeros(m.n_params))
+        opt2.apply_gradient(torch.zeros(m.n_trainable))
 
 
 @pytest.mark.parametrize('wrap', ['accumulate_sgd'])
@@ -124,7 +124,8 @@ def test_safetensors_files_hold_no_code(tmp_path):
     import flowket_amd as fk
     m = _machine()
     opt = fk.Adam(m)
-    opt.set_weights([torch.tensor([1.0], dtype=torch.float64), torch.ones(m.n_params), torch.ones(m.n_params)])
+    opt.set_weights([torch.tensor([1.0], dtype=torch.float64), torch.ones(m.n_trainable),
+                     torch.ones(m.n_trainable)])
     fk.save_optimizer_weights(opt, str(tmp_path / 'x'), 0, 0)
     from safetensors import safe_open
     with safe_open(str(tmp_path / 'x_optimizer.safetensors'), framework='pt') as f:

@@ -82,8 +82,7 @@ def _trained_ising(precision, h='3'):
     w = np.load(FIXTURE.replace('_h3_', '_h%s_' % h))
     m = fk.ConvNetAutoregressive2D((12, 12), depth=10, num_of_channels=32, precision=precision,
                                    params=torch.from_numpy(w['params']))
-    m.wn_g = torch.from_numpy(w['g'])
-    m.set_params(torch.from_numpy(w['params']))
+    m.set_wn_g(torch.from_numpy(w['g']))
     return m
 
 

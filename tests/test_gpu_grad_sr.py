@@ -86,7 +86,9 @@ def test_sr_matvec_solve_and_energy_gradient(rng):
     x_ref, _, F = osr.real_parameter_sr(O, el.cpu().numpy(), shift)
     sr = fk.ComplexValuesStochasticReconfiguration(m, lr=0.1, diag_shift=shift, conjugate_gradient_tol=1e-6,
                                                    iterative_solver_max_iterations=500)
-    x, it, res = sr.solve(s, el)
+    x, met = sr.solve(s, el)
+    it = met['stochastic_reconfiguration/conjugate_gradient_iterations']
+    res = met['stochastic_reconfiguration/conjugate_gradient_residual_norm']
     x = x.double().cpu().numpy()
     assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-3
     assert it > 0 and res <= 1e-6 * np.linalg.norm(F) * 1.01

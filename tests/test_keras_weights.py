@@ -75,8 +75,7 @@ def test_effective_kernel_of_the_loaded_weights():
     fx = np.load(FIXTURE)
     m = fk.ConvNetAutoregressive2D((12, 12), depth=10, num_of_channels=32, device='cpu',
                                    params=torch.from_numpy(fx['params']))
-    m.wn_g = torch.from_numpy(fx['g'])
-    m.set_params(torch.from_numpy(fx['params']))
+    m.set_wn_g(torch.from_numpy(fx['g']))
     eff = m.effective_params().double().numpy()
     kshape = m.shapes[0][0]
     k0 = eff[:int(np.prod(kshape))].reshape(kshape)

@@ -204,54 +204,22 @@ def test_convnet_causal(rng):
             assert np.abs(c[k + 1:] - base[k + 1:]).max() > 0
 
 
-def test_convnet_matches_torch_conv2d(rng):
-    """Every convolution restated with torch.nn.functional.conv2d (NCHW,
-    explicit padding) -- an independent implementation of Keras 'valid'
-    convolution after ZeroPadding2D."""
-    torch = pytest.importorskip('torch')
-    import torch.nn.functional as F
+@pytest.mark.parametrize('act', ['relu', 'elu'])
+def test_convnet_matches_torch_conv2d(rng, act):
+    """The network restated with torch.nn.functional.conv2d (oracle/
+    convnet_torch.py: NCHW, explicit padding) -- an independent implementation
+    of Keras 'valid' convolution after ZeroPadding2D -- equals the numpy
+    oracle's logits and log psi in float64."""
+    from oracle import convnet_torch as oct_
     depth, C = 3, 16
     params = ocn.glorot_params(rng, depth, C, bias_scale=0.3)
     s = rng.choice([-1, 1], size=(5, 4, 5)).astype(np.float64)
-
-    def conv(x, kb, pad):
-        k, b = kb
-        (pt, pb), (pl, pr) = pad
-        xt = F.pad(x, (pl, pr, pt, pb))
-        w = torch.as_tensor(k).permute(3, 2, 0, 1)
-        return F.conv2d(xt, w, torch.as_tensor(b))
-
-    act = F.relu
-
-    def block(v, h, layers, mask=False):
-        v_out = conv(v, layers[0], ((2, 0), (1, 1)))
-        x = act(conv(h, layers[1], ((0, 0), (2, 0))))
-        if mask:
-            x = F.pad(x, (1, 0, 0, 0))[..., :-1]
-        x = conv(x, layers[2], ((0, 0), (0, 0)))
-        y = F.pad(act(v_out), (0, 0, 1, 0))[..., :-1, :]
-        y = conv(y, layers[3], ((0, 0), (0, 0)))
-        cat = torch.cat([act(x), act(y)], dim=1)
-        return v_out, conv(cat, layers[4], ((2, 0), (2, 0)))
-
-    x = torch.as_tensor(s)[:, None]
-    it = iter(range(0, 5 * ocn.num_causal_blocks(depth), 5))
-    i0 = next(it)
-    v, h = block(x, x, params[i0:i0 + 5])
-    v, h = act(v), act(h)
-    for _ in range(depth - 2):
-        vi, hi = v, h
-        i0 = next(it)
-        v, h = block(v, h, params[i0:i0 + 5])
-        v, h = act(v), act(h)
-        i0 = next(it)
-        v, h = block(v, h, params[i0:i0 + 5])
-        v, h = act(vi + v), act(hi + h)
-    i0 = next(it)
-    _, h = block(v, h, params[i0:i0 + 5], mask=True)
-    logits = conv(act(h), params[-1], ((0, 0), (0, 0))).permute(0, 2, 3, 1).numpy()
-    ref = ocn.unnormalized_logits(params, s, depth)
+    tp = oct_.torch_params(params)
+    logits = oct_.unnormalized_logits(tp, s, depth, act).permute(0, 2, 3, 1).numpy()
+    ref = ocn.unnormalized_logits(params, s, depth, activation=act)
     assert np.abs(logits - ref).max() < 1e-12
+    lp = oct_.log_psi(tp, s, depth, act, chunk=2)
+    assert np.abs(lp - ocn.log_psi(params, s, depth, activation=act)).max() < 1e-12
 
 
 def test_convnet_closed_form_first_site():
