@@ -52,6 +52,9 @@ SIGNATURES = {
     'fk_convar_rcache_bytes': (_I64, [_I32, _I32, _I32, _I32, _I32]),
     'fk_convar_forward_rows': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _I32,
                                               _P, _P, _I32, _P, _I32, _P, _P, _P, _P, _I64, _P]),
+    'fk_convar_forward_noalloc': (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64,
+                                                 _I32, _P, _P, _P, _P, _P]),
+    'fk_convar_forward_noalloc_scratch_bytes': (_I64, []),
     'fk_convar_forward_workspace_bytes': (_I64, [_I32, _I32, _I32, _I32, _I32, _I32]),
     'fk_convar_num_schedules': (_I32, [_I32, _I32]),
     'fk_convar_schedule_info': (ctypes.c_int, [_I32, _I32, _I32, _P, _P]),

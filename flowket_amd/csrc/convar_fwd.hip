@@ -401,6 +401,62 @@ int fk_convar_forward_rows(const void* wpacked, const float* aux, int32_t precis
                         conn_kind, conn_a, conn_b, logpsi, nullptr, workspace, workspace_bytes, stream, rr);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free forward: the global-scratch workspace of the large-lattice
+// kernels is a file-scope device array (one copy per GPU, held until the
+// process ends).  Host code never passes the array's name to a kernel: the
+// per-device address comes from hipGetSymbolAddress, taken once per device
+// with that device current.
+namespace {
+constexpr int64_t NOALLOC_SCRATCH_BYTES = 32ll << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;
+
+int scratch_address(void** out) {
+  int dev = 0;
+  FK_CHECK_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return FK_OK;
+  }
+  void* ptr = nullptr;
+  FK_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[dev] = ptr;
+  *out = ptr;
+  return FK_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int64_t fk_convar_forward_noalloc_scratch_bytes(void) { return NOALLOC_SCRATCH_BYTES; }
+
+int fk_convar_forward_noalloc(const void* wpacked, const float* aux, int32_t precision, int32_t H, int32_t W,
+                              int32_t depth, int32_t channels, int32_t activation, const int8_t* spins,
+                              const int32_t* pairs, const int64_t* m_dev, int64_t m_count, int32_t conn_kind,
+                              const int32_t* conn_a, const int32_t* conn_b, float* logpsi, float* cond_logp,
+                              void* stream) {
+  FK_REQUIRE(H >= 1 && W >= 1 && depth >= 1, "fk_convar_forward_noalloc: bad shape");
+  if (convar_supported(channels, precision)) {
+    const int64_t need = forward_workspace_bytes(H, W, depth, kernel_channels(channels, precision), activation,
+                                                 precision, nullptr);
+    FK_REQUIRE(need <= NOALLOC_SCRATCH_BYTES,
+               "fk_convar_forward_noalloc: a %dx%d lattice with %d channels needs %lld bytes of scratch, more "
+               "than the %lld-byte static buffer (use fk_convar_forward_ws)",
+               H, W, channels, (long long)need, (long long)NOALLOC_SCRATCH_BYTES);
+  }
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc != FK_OK) return rc;
+  return convar_forward(wpacked, aux, precision, H, W, depth, channels, activation, spins, pairs, m_dev, m_count,
+                        conn_kind, conn_a, conn_b, logpsi, cond_logp, ws, NOALLOC_SCRATCH_BYTES, stream);
+}
+
 int64_t fk_convar_forward_workspace_bytes(int32_t H, int32_t W, int32_t depth, int32_t channels, int32_t activation,
                                           int32_t precision) {
   if (H < 1 || W < 1 || depth < 1 || !convar_supported(channels, precision)) return -1;

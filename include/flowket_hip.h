@@ -238,6 +238,27 @@ int fk_convar_forward_rows(const void* wpacked, const float* aux,
                            void* workspace, int64_t workspace_bytes,
                            void* stream);
 
+/* Same as fk_convar_forward_ws with no workspace argument and no allocation
+ * of any kind.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The large-lattice scratch is a
+ * static device buffer of fk_convar_forward_noalloc_scratch_bytes() bytes
+ * (32 MiB) per GPU, held until the process ends; a network whose
+ * fk_convar_forward_workspace_bytes exceeds it is refused with FK_ERR_ARG.
+ * Every call on one GPU shares that buffer: calls that overlap on one GPU
+ * (different streams, or host threads) race and must be ordered by the
+ * caller.  The kernels initialise what they read from it, so no clearing is
+ * needed between calls. */
+int fk_convar_forward_noalloc(const void* wpacked, const float* aux,
+                              int32_t precision, int32_t H, int32_t W,
+                              int32_t depth, int32_t channels,
+                              int32_t activation, const int8_t* spins,
+                              const int32_t* pairs, const int64_t* m_dev,
+                              int64_t m_count, int32_t conn_kind,
+                              const int32_t* conn_a, const int32_t* conn_b,
+                              float* logpsi, float* cond_logp, void* stream);
+int64_t fk_convar_forward_noalloc_scratch_bytes(void);
+
 /* Host-only (queries the current device's CU count).  Bytes of workspace
  * fk_convar_forward_ws needs for a network (under its selected schedule),
  * 0 if none, -1 if the lattice is unsupported. */
