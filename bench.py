@@ -62,6 +62,11 @@ def parse():
     ap.add_argument('--profile-forward', type=int, default=0, metavar='N',
                     help='only run two fused-forward launches over the connected configurations of N random '
                          'samples (a short program for rocprofv3 counter passes) and exit')
+    ap.add_argument('--plain-forward', action='store_true',
+                    help='evaluate every connected configuration with the whole-lattice forward (no row reuse)')
+    ap.add_argument('--profile-grad', type=int, default=0, metavar='N',
+                    help='only run the per-sample gradient kernel twice, then one batched VJP and one JVP, over '
+                         'N fast samples of the headline network (a short program for rocprofv3) and exit')
     ap.add_argument('--profile-copy', action='store_true',
                     help='only run two 256 MiB device-to-device copies (the calibration kernel for the '
                          'FETCH_SIZE / WRITE_SIZE counters) and exit')
@@ -71,7 +76,7 @@ def parse():
     return ap.parse_args()
 
 
-def profile_forward(n, precision, schedule):
+def profile_forward(n, precision, schedule, plain=False):
     import torch
     import flowket_amd as fk
     from flowket_amd import _lib, autotune
@@ -85,11 +90,32 @@ def profile_forward(n, precision, schedule):
     spins = (torch.randint(0, 2, (n, H * W), generator=gen) * 2 - 1).to(torch.int8).cuda()
     r = op.compact(spins)
     total = int(r['offsets'][-1])
+    rows = not plain and m.uses_row_reuse(op)
     for _ in range(2):
-        m.log_psi_connected(spins, r['pairs'], total, op)
+        if rows:
+            m.log_psi_connected_rows(spins, r, op)
+        else:
+            m.log_psi_connected(spins, r['pairs'], total, op)
     torch.cuda.synchronize()
     print(json.dumps({'profile_forward': {'samples': n, 'configurations': total, 'precision': precision,
-                                          'schedule': schedule}}))
+                                          'schedule': schedule, 'row_reuse': rows}}))
+    return 0
+
+
+def profile_grad(n):
+    import torch
+    import flowket_amd as fk
+    m = fk.ConvNetAutoregressive2D((H, W), depth=DEPTH, num_of_channels=CHANNELS, activation=ACT,
+                                   weights_normalization=False, precision='fp16', seed=1234)
+    spins = next(fk.FastAutoregressiveSampler(m, n, seed=3)).reshape(n, H * W).contiguous()
+    for _ in range(2):
+        o_re, o_im, _ = m.per_sample_gradients(spins)
+        del o_re, o_im
+    cot = torch.randn(n, 2, device=spins.device)
+    m.vjp(spins, cot)
+    m.jvp(spins, torch.randn(m.n_trainable, device=spins.device))
+    torch.cuda.synchronize()
+    print(json.dumps({'profile_grad': {'samples': n, 'params': m.n_params}}))
     return 0
 
 
@@ -205,17 +231,65 @@ def bench_sr(machine, op, spins, shift=1e-3, tol=1e-6, max_iter=50):
     cg_ms = ev[1].elapsed_time(ev[2])
     mv_ms = ev[3].elapsed_time(ev[4])
     o_bytes = 2 * 4 * B * P
+    fnorm = float(torch.linalg.vector_norm(force.double()).item())
     out = {'batch': B, 'params': P, 'grad_ms': round(grad_ms, 2),
            'grad_tflops': round(B * 2 * flops_per_forward() / (grad_ms * 1e-3) / 1e12, 2),
            'grad_o_write_gbs': round(o_bytes / (grad_ms * 1e-3) / 1e9, 1),
-           'cg_ms': round(cg_ms, 2), 'cg_iterations': it, 'cg_residual': res,
-           'cg_force_norm': float(torch.linalg.vector_norm(force.double()).item()),
+           'cg_ms': round(cg_ms, 2), 'cg_iterations': it, 'cg_residual': res, 'cg_force_norm': fnorm,
+           'cg_relative_residual': res / fnorm if fnorm > 0 else 0.0, 'cg_tol': tol,
+           'cg_converged': bool(res <= tol * fnorm),
            'matvec_ms': round(mv_ms, 3), 'matvec_gbs': round(2 * o_bytes / (mv_ms * 1e-3) / 1e9, 1),
            'matvec_hbm_frac': round(2 * o_bytes / (mv_ms * 1e-3) / PEAK_HBM, 4),
            'step_ms': round(grad_ms + cg_ms, 2), 'o_gib': round(o_bytes / 2 ** 30, 2)}
+    # the same S.v without O: forward-mode JVP, then one batched VJP
+    if machine.supports_implicit_products():
+        from flowket_amd.optimizers import _ImplicitS
+        im = _ImplicitS(machine, spins)
+        zi = im.s_matvec(force, shift)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        zi = im.s_matvec(force, shift)
+        e1.record()
+        torch.cuda.synchronize()
+        out['implicit_matvec_ms'] = round(e0.elapsed_time(e1), 2)
+        out['implicit_vs_materialised_rel_diff'] = float(
+            (torch.linalg.vector_norm((zi - z).double()) / torch.linalg.vector_norm(z.double())).item())
     del of
     torch.cuda.empty_cache()
     return out
+
+
+def bench_sr_training(steps=10, batch=BATCH, lr=0.05, shift=1e-3):
+    """A short SR optimisation of the headline network and Hamiltonian
+    (fast samples, FP16 local energies, FP32 O_k, CG on the materialised
+    S = Re[(O-Obar)^H (O-Obar)]/B + shift with tol 1e-6 and at most 50
+    iterations): the sampled energy per step and the time of a step."""
+    import torch
+    import flowket_amd as fk
+    m = fk.ConvNetAutoregressive2D((H, W), depth=DEPTH, num_of_channels=CHANNELS, activation=ACT,
+                                   weights_normalization=False, precision='fp16', seed=1234)
+    op = fk.Heisenberg(hilbert_state_shape=(H, W), pbc=True)
+    vmc = fk.VariationalMonteCarlo(m, op, fk.FastAutoregressiveSampler(m, batch, seed=41))
+    sr = fk.ComplexValuesStochasticReconfiguration(m, lr=lr, diag_shift=shift, conjugate_gradient_tol=1e-6,
+                                                   iterative_solver_max_iterations=50)
+    energies, cg_iters = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        x, _ = vmc.next_batch()
+        met = sr.step(x, vmc.current_local_energy, vmc.current_energy)
+        energies.append(float(torch.as_tensor(vmc.current_energy).real) / (H * W))
+        cg_iters.append(int(met['stochastic_reconfiguration/conjugate_gradient_iterations']))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    del sr, vmc
+    torch.cuda.empty_cache()
+    return {'workload': f'10x10 periodic Heisenberg, depth {DEPTH}, {CHANNELS} channels, batch {batch} fast samples, '
+                        f'SR lr {lr} shift {shift}, CG tol 1e-6 max 50 (materialised O)',
+            'steps': steps, 'ms_per_step': round(1e3 * dt / steps, 1),
+            'energy_per_site': [round(e, 5) for e in energies], 'cg_iterations': cg_iters,
+            'energy_decreased': bool(np.mean(energies[-3:]) < np.mean(energies[:3]))}
 
 
 def bench_mh(seconds_burn=None):
@@ -359,7 +433,7 @@ def bench_trained_ising(n_samples=2 ** 15, mini_batch=2 ** 12, precision='fp16')
             'forward_schedule': tune['schedule'], 'schedule_ms': [round(t, 2) for t in tune['ms']]}
 
 
-def bench_large(n_samples=1024, n_eloc=64):
+def bench_large(n_samples=1024, n_eloc=256, n_sv=512):
     """The largest configuration's network and lattice: 20x20 periodic
     Heisenberg, depth 24, 64 channels, bf16.  Fast sampling of n_samples
     configurations and the ragged local energies of the first n_eloc (a
@@ -396,14 +470,35 @@ def bench_large(n_samples=1024, n_eloc=64):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     fwd = 2 * Hl * Wl * ((10 * C * C + 12 * C) + (2 * depth - 3) * 22 * C * C + 4 * C)
-    return {'workload': '20x20 periodic Heisenberg, depth 24, 64 channels, bf16 (horizontal stack in global scratch)',
-            'params': m.n_params, 'fast_samples_per_s': round(sps, 1), 'eloc_samples': n_eloc,
-            'connected_configurations': total, 'E_loc_evals_per_s': round(n_eloc / dt, 2),
-            'connected_logpsi_per_s': round(total / dt, 1),
-            'forward_tflops': round(total * fwd / dt / 1e12, 1),
-            'forward_frac_of_bf16_peak': round(total * fwd / dt / PEAK_BF16, 4),
-            'forward_schedule': tune['schedule'], 'schedule_ms': [round(t, 2) for t in tune['ms']],
-            'energy_per_site': float(el.real.mean().item()) / (Hl * Wl)}
+    out = {'workload': '20x20 periodic Heisenberg, depth 24, 64 channels, bf16 (horizontal stack in global scratch)',
+           'params': m.n_params, 'fast_samples_per_s': round(sps, 1), 'eloc_samples': n_eloc,
+           'connected_configurations': total, 'E_loc_evals_per_s': round(n_eloc / dt, 2),
+           'connected_logpsi_per_s': round(total / dt, 1),
+           'forward_tflops': round(total * fwd / dt / 1e12, 1),
+           'forward_frac_of_bf16_peak': round(total * fwd / dt / PEAK_BF16, 4),
+           'forward_schedule': tune['schedule'], 'schedule_ms': [round(t, 2) for t in tune['ms']],
+           'row_reuse_first_row': tune.get('row_reuse'),
+           'energy_per_site': float(el.real.mean().item()) / (Hl * Wl)}
+    # SR at this size never forms O (16384 x 4.1M complex would be 538 GB):
+    # one implicit S.v (JVP, then a batched VJP) over n_sv samples, and the
+    # CG step it implies at 16384 samples and 50 iterations
+    from flowket_amd.optimizers import _ImplicitS
+    xs = x.reshape(n_samples, Hl * Wl)[:n_sv].contiguous()
+    im = _ImplicitS(m, xs)
+    v = torch.randn(m.n_trainable, device=xs.device)
+    im.s_matvec(v, 1e-3)
+    torch.cuda.synchronize()
+    e0.record()
+    im.s_matvec(v, 1e-3)
+    e1.record()
+    torch.cuda.synchronize()
+    sv_ms = e0.elapsed_time(e1)
+    out['implicit_sr'] = {'samples': n_sv, 'matvec_ms': round(sv_ms, 1),
+                          'matvec_us_per_sample': round(1e3 * sv_ms / n_sv, 1),
+                          'projected_cg_step_s_16384x50': round(50 * sv_ms * 16384 / n_sv / 1e3, 1)}
+    del im
+    torch.cuda.empty_cache()
+    return out
 
 
 def _oracle_params(machine):
@@ -461,7 +556,9 @@ def main():
     if args.profile_copy:
         return profile_copy()
     if args.profile_forward:
-        return profile_forward(args.profile_forward, args.precision, args.schedule)
+        return profile_forward(args.profile_forward, args.precision, args.schedule, args.plain_forward)
+    if args.profile_grad:
+        return profile_grad(args.profile_grad)
     if args.gpus != 1:
         print(json.dumps({'error': 'the VMC path here runs on one GPU (samples shard across GPUs by design, '
                                    'but no multi-GPU launch is implemented); --gpus must be 1'}))
@@ -539,13 +636,16 @@ def main():
 
     # ---- launch schedule of the fused forward: forced, or autotuned on this step
     from flowket_amd import autotune
+    if args.plain_forward:
+        m16.set_row_reuse_policy(0)
     if args.schedule is not None:
         autotune.set_schedule(m16, _lib.FK_PREC_FP16, args.schedule)
         tune = {'schedule': args.schedule, 'forced': True}
     else:
-        tune = autotune.tune_forward(m16, lambda: step(False), reps=2, persist=False)
+        tune = autotune.tune_forward(m16, lambda: step(False), reps=2, persist=False, row_reuse=not args.plain_forward)
         tune = {'schedule': tune['schedule'], 'ms_per_schedule': [round(t, 2) for t in tune['ms']],
-                'schedules': tune['schedules']}
+                'schedules': tune['schedules'], 'row_reuse_first_row': tune.get('row_reuse'),
+                'ms_per_row_reuse_policy': {str(k): round(v, 2) for k, v in tune.get('row_reuse_ms', {}).items()}}
 
     for _ in range(args.warmup):
         step(False)
@@ -624,6 +724,7 @@ def main():
 
     if not args.no_sr:
         out['vmc_sr_step'] = bench_sr(m16, op, spins)
+        out['vmc_sr_training'] = bench_sr_training()
     if not args.no_mh:
         out['mh_rbm_8x8'] = bench_mh()
     if not args.no_large:

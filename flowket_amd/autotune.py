@@ -8,7 +8,9 @@ keyed by lattice, depth, width, activation and precision) and records it in a
 JSON cache keyed by device and the same network description, so later
 processes start from the tuned choice (``apply_cached``).  The cache path is $FLOWKET_AMD_TUNE_CACHE or
 ~/.cache/flowket_amd/forward_schedules.json.  All schedules compute the same
-function; they differ only in FP32 summation order.
+function; they differ only in FP32 summation order.  The row-reuse
+policies give the same log psi up to FP32 summation order too (rows above a
+connection's first changed row are the sample's own, taken from its cache).
 """
 import json
 import os
@@ -86,15 +88,33 @@ def apply_cached(machine, prec):
         entry = _read_cache().get(_key(machine, prec))
     if entry is not None and 0 <= int(entry['schedule']) < len(schedules(machine.num_of_channels, prec)):
         set_schedule(machine, prec, int(entry['schedule']))
+        if 'row_reuse' in entry and hasattr(machine, 'set_row_reuse_policy'):
+            machine.set_row_reuse_policy(int(entry['row_reuse']))
         return int(entry['schedule'])
     return None
 
 
-def tune_forward(machine, run, precision=None, reps=3, warmup=1, persist=True):
+def _time(run, stream, reps, warmup):
+    for _ in range(warmup):
+        run()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        run()
+    end.record(stream)
+    end.synchronize()
+    return start.elapsed_time(end) / reps
+
+
+def tune_forward(machine, run, precision=None, reps=3, warmup=1, persist=True, row_reuse=True):
     """Times ``run()`` (a closure that launches the machine's forward on the
     current stream, e.g. one ragged local-energy evaluation) under every
-    schedule and keeps the fastest.  Returns {'schedule': i, 'ms': [...],
-    'schedules': [...]}."""
+    schedule and keeps the fastest; then, with ``row_reuse`` and a machine
+    that has row-reuse policies (off, or the first row that takes restricted
+    passes), times those under the chosen schedule and keeps the fastest.
+    Returns {'schedule': i, 'ms': [...], 'schedules': [...]} plus
+    'row_reuse' and 'row_reuse_ms' when policies were compared."""
     from .machines import _PREC
     prec = _PREC[precision or machine.precision]
     table = schedules(machine.num_of_channels, prec)
@@ -102,21 +122,26 @@ def tune_forward(machine, run, precision=None, reps=3, warmup=1, persist=True):
     times = []
     for s in table:
         set_schedule(machine, prec, s['index'])
-        for _ in range(warmup):
-            run()
-        start = torch.cuda.Event(enable_timing=True)
-        end = torch.cuda.Event(enable_timing=True)
-        start.record(stream)
-        for _ in range(reps):
-            run()
-        end.record(stream)
-        end.synchronize()
-        times.append(start.elapsed_time(end) / reps)
+        times.append(_time(run, stream, reps, warmup))
     best = min(range(len(table)), key=lambda i: times[i])
     set_schedule(machine, prec, table[best]['index'])
+    out = {'schedule': table[best]['index'], 'ms': times, 'schedules': table}
+    policies = machine.row_reuse_policies() if row_reuse and hasattr(machine, 'row_reuse_policies') else []
+    if len(policies) > 1:
+        rtimes = []
+        for pol in policies:
+            machine.set_row_reuse_policy(pol)
+            rtimes.append(_time(run, stream, reps, warmup))
+        rbest = policies[min(range(len(policies)), key=lambda i: rtimes[i])]
+        machine.set_row_reuse_policy(rbest)
+        out['row_reuse'] = rbest
+        out['row_reuse_ms'] = dict(zip(policies, rtimes))
     if persist:
         with _lock:
             entries = _read_cache()
-            entries[_key(machine, prec)] = {'schedule': table[best]['index'], 'ms': times}
+            entry = {'schedule': table[best]['index'], 'ms': times}
+            if 'row_reuse' in out:
+                entry['row_reuse'] = out['row_reuse']
+            entries[_key(machine, prec)] = entry
             _write_cache(entries)
-    return {'schedule': table[best]['index'], 'ms': times, 'schedules': table}
+    return out

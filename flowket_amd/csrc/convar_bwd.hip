@@ -548,24 +548,32 @@ __device__ void mconv_bwd_weight(float* dK, float* db, const float* in, int cs_i
     if (m1) st(mt0 + 1, nt0, c10);
     if (m1 && n1) st(mt0 + 1, nt0 + 1, c11);
   }
-  // bias: db[co] = sum_p dout[p][co], every thread on a strided share of the
-  // pixels, the partial sums combined in a fixed order (deterministic)
-  __shared__ float s_red[256];
-  const int nparts = (int)blockDim.x / cout;
-  const int co = threadIdx.x % cout, part = threadIdx.x / cout;
-  float acc = 0.0f;
-  if (part < nparts)
-    for (int p = part; p < HW; p += nparts) acc += dout[(int64_t)p * cs_out + co_off + co];
-  __syncthreads();
-  s_red[threadIdx.x] = acc;
-  __syncthreads();
-  if ((int)threadIdx.x < cout) {
-    float t = 0.0f;
-    for (int k = 0; k < nparts; ++k) t += s_red[k * cout + threadIdx.x];
-    if (accumulate) db[threadIdx.x] += t;
-    else db[threadIdx.x] = t;
+  // bias: db[co] = sum_p dout[p][co] on the last wave (it has the fewest
+  // tile blocks when they do not divide evenly): lane l sums the pixels
+  // p = l / cout (mod 64 / cout) of channel l % cout, and the partial sums
+  // are added across the lanes in a fixed order (deterministic, no barrier)
+  if (wave == nw - 1 && cout <= 64) {
+    const int nparts = 64 / cout;
+    const int co = lane % cout, part = lane / cout;
+    float acc = 0.0f;
+    if (part < nparts)
+      for (int p = part; p < HW; p += nparts) acc += dout[(int64_t)p * cs_out + co_off + co];
+    for (int k = 1; k < nparts; ++k) {
+      const float o = __shfl(acc, lane + k * cout, 64);
+      if (part == 0) acc += o;
+    }
+    if (part == 0) {
+      if (accumulate) db[co] += acc;
+      else db[co] = acc;
+    }
+  } else if (cout > 64) {
+    for (int co = threadIdx.x; co < cout; co += blockDim.x) {
+      float acc = 0.0f;
+      for (int p = 0; p < HW; ++p) acc += dout[(int64_t)p * cs_out + co_off + co];
+      if (accumulate) db[co] += acc;
+      else db[co] = acc;
+    }
   }
-  __syncthreads();
 }
 
 // Per-sample workspace of the MFMA kernel: the scalar kernel's layout plus six
@@ -606,7 +614,7 @@ __device__ void build_tap_tables(int16_t* ftab, int16_t* itab, int H, int W) {
 // y' = K*x' + Kt*x + bt, and act(y)' = act'(y) y'.  t0/t1 and u0/u1 are
 // scratch tensors of the primal and the tangent.
 template <int ACT>
-__device__ void mfma_forward(const float* prm, const float* tan, float* ws, float* wt, const int16_t* ftab, int H,
+__device__ __forceinline__ void mfma_forward(const float* prm, const float* tan, float* ws, float* wt, const int16_t* ftab, int H,
                              int W, int C, int nb, float* t0, float* t1, float* u0, float* u1) {
   const int HW = H * W, F = nb - 1;
   const float* sp = ws;
@@ -714,10 +722,11 @@ __device__ __forceinline__ void head_logits(const float* prm, int64_t ho, const 
 
 // Backward of one sample from the head cotangent dz (HW x 4): d/d theta into
 // O (overwritten, or added to when accumulate).
-template <int ACT>
-__device__ void mfma_backward(const float* prm, float* ws, const int16_t* ftab, const int16_t* itab, const float* dz,
-                              int H, int W, int C, int nb, float* O, bool accumulate, float* dV, float* dH, float* dVr,
-                              float* dHr, float* t0, float* t1) {
+template <int ACT, bool ACC>
+__device__ __forceinline__ void mfma_backward(const float* prm, float* ws, const int16_t* ftab, const int16_t* itab,
+                                              const float* dz, int H, int W, int C, int nb, float* O, float* dV,
+                                              float* dH, float* dVr, float* dHr, float* t0, float* t1) {
+  constexpr bool accumulate = ACC;
   const int HW = H * W, F = nb - 1;
   const float* sp = ws;
   const int64_t ho = head_offset(nb, C);
@@ -814,7 +823,7 @@ enum { GRAD_O = 0, GRAD_VJP = 1 };
 // and adds cot_re[b] d Re log psi_b + cot_im[b] d Im log psi_b (one backward
 // pass with the combined head cotangent) into its own partial gradient row.
 template <int ACT, int MODE>
-__global__ __launch_bounds__(256) void convar_grad_mfma_kernel(BwdArgs a) {
+__global__ __launch_bounds__(256, 4) void convar_grad_mfma_kernel(BwdArgs a) {
   extern __shared__ int16_t tabs[];  // GT_N forward rows, then GT_N inverse rows, of HW entries
   __shared__ float s_lp[2];
   const int H = a.H, W = a.W, HW = a.HW, C = a.C, nb = a.nb;
@@ -871,7 +880,7 @@ __global__ __launch_bounds__(256) void convar_grad_mfma_kernel(BwdArgs a) {
       }
       __syncthreads();
       float* O = MODE == GRAD_VJP ? part : (part_i == 0 ? a.o_re : a.o_im) + b * a.ld;
-      mfma_backward<ACT>(prm, ws, ftab, itab, dz, H, W, C, nb, O, MODE == GRAD_VJP, dV, dH, dVr, dHr, t0, t1);
+      mfma_backward<ACT, MODE == GRAD_VJP>(prm, ws, ftab, itab, dz, H, W, C, nb, O, dV, dH, dVr, dHr, t0, t1);
     }
     if (threadIdx.x == 0 && a.logpsi) {
       a.logpsi[2 * b] = s_lp[0];
@@ -885,7 +894,7 @@ __global__ __launch_bounds__(256) void convar_grad_mfma_kernel(BwdArgs a) {
 // out[b] = (d Re log psi_b, d Im log psi_b) . tan, a persistent grid of
 // workgroups over the samples, primal and tangent tensors in the workspace.
 template <int ACT>
-__global__ __launch_bounds__(256) void convar_jvp_mfma_kernel(BwdArgs a) {
+__global__ __launch_bounds__(256, 4) void convar_jvp_mfma_kernel(BwdArgs a) {
   extern __shared__ int16_t tabs[];
   __shared__ float s_acc[4];
   const int H = a.H, W = a.W, HW = a.HW, C = a.C, nb = a.nb;
@@ -963,10 +972,10 @@ int64_t flat_param_count(int C, int nb) {
          (int64_t)(nb - 1) * (9 * c * c + c + 3 * c * c + c + c * (c / 2) * 2 + c + 9 * c * c + c) + 4 * c + 4;
 }
 
-// Workgroups of the persistent VJP / JVP grids: two per CU, at most one per
-// sample.
+// Workgroups of the persistent VJP / JVP grids: four per CU (the kernels
+// fit four 4-wave workgroups per CU), at most one per sample.
 int64_t implicit_grid(int64_t batch) {
-  const int64_t g = 2LL * device_cu_count();
+  const int64_t g = 4LL * device_cu_count();
   return batch < g ? (batch > 0 ? batch : 1) : g;
 }
 

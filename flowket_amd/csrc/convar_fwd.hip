@@ -91,17 +91,6 @@ static int plan_forward(int HWL, int HWC, int C, int prec, int NW, int64_t m, Fw
   return FK_OK;
 }
 
-template <int PREC, int C>
-static int dispatch_fwd_c(const FwdArgs& a, const FwdPlan& pl, const FwdSchedule& sc, int act, hipStream_t s) {
-  if constexpr ((PREC == FK_PREC_BF16 && C == 32) || (PREC == FK_PREC_FP32 && C == 16)) {
-    if (sc.sched == 1)
-      return sc.nwave == 4 ? launch_variant<PREC, C, 4, 1, false>(a, pl, act, s)
-                           : launch_variant<PREC, C, 8, 1, false>(a, pl, act, s);
-  }
-  return sc.nwave == 4 ? launch_variant<PREC, C, 4, 0, false>(a, pl, act, s)
-                       : launch_variant<PREC, C, 8, 0, false>(a, pl, act, s);
-}
-
 bool convar_supported(int channels, int precision) { return kernel_channels(channels, precision) > 0; }
 
 }  // namespace fk
@@ -254,6 +243,7 @@ static int64_t forward_workspace_bytes(int H, int W, int depth, int CP, int act,
 // Row reuse (see FwdArgs): R0 = 0 with rcache and cache_write is the sample
 // pass; R0 >= 2 a restricted pass over rows R0-2 .. H-1.
 struct RowReuse {
+  bool rows = false;  // the sample pass or a restricted pass (kernel variant with ROWS)
   int R0 = 0;
   uint8_t* rcache = nullptr;
   int cache_write = 0;
@@ -335,21 +325,15 @@ static int convar_forward(const void* wpacked, const float* aux, int32_t precisi
     a.T = pl.T;
     a.npix = pl.npix;
     a.hscr = reinterpret_cast<uint8_t*>(workspace);
-    return dispatch_fwd_hg(a, pl, precision, CP, activation, s);
+    if (precision == FK_PREC_BF16) return dispatch_fwd<FK_PREC_BF16, true>(a, pl, CP, 0, pl.nwave, activation, rr.rows, s);
+    if (precision == FK_PREC_FP16) return dispatch_fwd<FK_PREC_FP16, true>(a, pl, CP, 0, pl.nwave, activation, rr.rows, s);
+    return dispatch_fwd<FK_PREC_FP32, true>(a, pl, CP, 0, pl.nwave, activation, rr.rows, s);
   }
   a.T = pl.T;
   a.npix = pl.npix;
-  if (precision == FK_PREC_BF16) {
-    if (CP == 32) return dispatch_fwd_c<FK_PREC_BF16, 32>(a, pl, sc, activation, s);
-    return dispatch_fwd_c<FK_PREC_BF16, 64>(a, pl, sc, activation, s);
-  }
-  if (precision == FK_PREC_FP16) {
-    if (CP == 32) return dispatch_fwd_c<FK_PREC_FP16, 32>(a, pl, sc, activation, s);
-    return dispatch_fwd_c<FK_PREC_FP16, 64>(a, pl, sc, activation, s);
-  }
-  if (CP == 16) return dispatch_fwd_c<FK_PREC_FP32, 16>(a, pl, sc, activation, s);
-  if (CP == 32) return dispatch_fwd_c<FK_PREC_FP32, 32>(a, pl, sc, activation, s);
-  return dispatch_fwd_c<FK_PREC_FP32, 64>(a, pl, sc, activation, s);
+  if (precision == FK_PREC_BF16) return dispatch_fwd<FK_PREC_BF16, false>(a, pl, CP, sc.sched, sc.nwave, activation, rr.rows, s);
+  if (precision == FK_PREC_FP16) return dispatch_fwd<FK_PREC_FP16, false>(a, pl, CP, sc.sched, sc.nwave, activation, rr.rows, s);
+  return dispatch_fwd<FK_PREC_FP32, false>(a, pl, CP, sc.sched, sc.nwave, activation, rr.rows, s);
 }
 
 }  // namespace fk
@@ -391,6 +375,7 @@ int fk_convar_forward_rows(const void* wpacked, const float* aux, int32_t precis
   FK_REQUIRE(row0 == 0 || (pairs && rcache && prefix), "fk_convar_forward_rows: a row pass needs pairs and the cache");
   FK_REQUIRE(!cache_write || (rcache && prefix), "fk_convar_forward_rows: the sample pass needs the cache");
   RowReuse rr;
+  rr.rows = row0 > 0 || cache_write;
   rr.R0 = row0;
   rr.rcache = reinterpret_cast<uint8_t*>(rcache);
   rr.cache_write = cache_write;

@@ -1,8 +1,9 @@
 // Fused ConvNetAutoregressive2D forward kernel template (machines/
 // conv_net_autoregressive_2D.py:33-74 + machines/abstract_machine.py:31-71),
-// shared by convar_fwd.hip (lattices resident in LDS) and convar_fwd_large.hip
-// (the horizontal-stack lattices in a per-workgroup global scratch, for
-// lattices whose five buffers exceed the LDS).
+// instantiated by convar_fwd_<precision>_lds.hip (lattices resident in LDS)
+// and convar_fwd_<precision>_hg.hip (the horizontal-stack lattices in a
+// per-workgroup global scratch, for lattices whose five buffers exceed the
+// LDS); convar_fwd.hip holds the host side.
 //
 // One workgroup evaluates a tile of T configurations through the whole network
 // (buffers V0, V1, CAT, H0, H1 of T*H*W pixels x C channels).  Each
@@ -339,32 +340,84 @@ __device__ __forceinline__ void conv_tile(f32x4 (&acc)[MTO], const uint8_t* buf,
       for (int ks = 0; ks < KS; ++ks) acc[mt] = P::mfma(cw.w[t][mt][ks], b[t][ks], acc[mt]);
 }
 
-// Row-reuse cache copies between a lattice in LDS (swizzled pixels of T
-// configurations, HWL pixels each) and the samples' caches (pixels in global
-// order, chunks unswizzled): local pixels [q0, q1) of every configuration,
-// 4-channel quads [quad0, C/4); to_cache: LDS -> cache, else cache -> LDS.
+// Row-reuse cache copies.  In LDS a configuration's pixels are swizzled
+// 16-byte chunks (Lds::chunk); in a sample's cache the pixels are in global
+// order with their chunks unswizzled (the layout of pixel 0).
+// rcache_out: every local pixel of every configuration, LDS -> cache (the
+// sample pass, whose local pixels are the global ones).
 template <int PREC, int C>
-__device__ __forceinline__ void rcache_copy(uint8_t* buf, const FwdArgs& a, int blk, int which, int T, int HWL,
-                                            int q0, int q1, bool to_cache, const int* s_samp, int quad0 = 0) {
+__device__ __forceinline__ void rcache_out(const uint8_t* buf, const FwdArgs& a, int blk, int which, int T, int HWL,
+                                           const int* s_samp) {
   using L = Lds<PREC, C>;
-  constexpr int QB = Prec<PREC>::H16 ? 8 : 16;
-  const int nq = C / 4 - quad0;
-  const int per = (q1 - q0) * nq;
-  const int gofs = (a.R0 - a.ROW0) * a.W;
+  constexpr int CPP = L::CPP;
+  const int per = HWL * CPP;
   for (int idx = threadIdx.x; idx < T * per; idx += blockDim.x) {
     const int t = idx / per, r = idx - t * per;
     const int b = s_samp[t];
     if (b < 0) continue;
-    const int ql = q0 + r / nq, qd = quad0 + (r - (r / nq) * nq);
-    uint8_t* lp = buf + L::quad(t * HWL + ql, qd);
+    const int ql = r / CPP, c = r % CPP;
     uint8_t* cp = a.rcache + (int64_t)b * a.rcache_stride + rcache_off(blk, which, a.HW, L::PIXB) +
-                  (int64_t)(ql + gofs) * L::PIXB + L::quad(0, qd);
-    if constexpr (QB == 8) {
-      if (to_cache) *reinterpret_cast<uint2*>(cp) = *reinterpret_cast<const uint2*>(lp);
-      else *reinterpret_cast<uint2*>(lp) = *reinterpret_cast<const uint2*>(cp);
-    } else {
-      if (to_cache) *reinterpret_cast<uint4*>(cp) = *reinterpret_cast<const uint4*>(lp);
-      else *reinterpret_cast<uint4*>(lp) = *reinterpret_cast<const uint4*>(cp);
+                  (int64_t)ql * L::PIXB + 16 * c;
+    *reinterpret_cast<uint4*>(cp) = *reinterpret_cast<const uint4*>(buf + L::chunk(t * HWL + ql, c));
+  }
+}
+
+// rcache_in (restricted pass): the ROW0 halo rows of every configuration,
+// cache -> LDS, and with yrow also the y half (quads C/8 .. C/4-1) of the
+// first computed row.  Each thread issues up to RC_BATCH reads before it
+// writes any, so one memory latency covers the copy.
+constexpr int RC_BATCH = 4;
+template <int PREC, int C>
+__device__ __forceinline__ void rcache_in(uint8_t* buf, const FwdArgs& a, int blk, int which, int T, int HWL,
+                                          bool yrow, const int* s_samp) {
+  using L = Lds<PREC, C>;
+  constexpr int CPP = L::CPP;
+  constexpr bool Q8 = Prec<PREC>::H16;  // 16-bit quads are 8 bytes, fp32 quads 16
+  constexpr int NQ = C / 8;             // quads of the y half of a pixel
+  constexpr int HALF8 = 1 << 30;        // marks an 8-byte piece
+  const int W = a.W;
+  const int nfull = a.ROW0 * W * CPP;
+  const int per = nfull + (yrow ? W * NQ : 0);
+  const int total = T * per;
+  const int gofs = (a.R0 - a.ROW0) * W;
+  for (int base = threadIdx.x; base < total; base += RC_BATCH * (int)blockDim.x) {
+    uint4 v[RC_BATCH];
+    int lo[RC_BATCH];
+#pragma unroll
+    for (int r = 0; r < RC_BATCH; ++r) {
+      const int idx = base + r * (int)blockDim.x;
+      lo[r] = -1;
+      v[r] = make_uint4(0u, 0u, 0u, 0u);
+      if (idx < total) {
+        const int t = idx / per, k = idx - t * per;
+        const int b = s_samp[t];
+        if (b >= 0) {
+          const uint8_t* src = a.rcache + (int64_t)b * a.rcache_stride + rcache_off(blk, which, a.HW, L::PIXB);
+          if (k < nfull) {
+            const int ql = k / CPP, c = k % CPP;
+            v[r] = *reinterpret_cast<const uint4*>(src + (int64_t)(ql + gofs) * L::PIXB + 16 * c);
+            lo[r] = L::chunk(t * HWL + ql, c);
+          } else {
+            const int kk = k - nfull;
+            const int ql = a.ROW0 * W + kk / NQ, qd = NQ + kk % NQ;
+            const uint8_t* g = src + (int64_t)(ql + gofs) * L::PIXB + L::quad(0, qd);
+            if constexpr (Q8) {
+              const uint2 x = *reinterpret_cast<const uint2*>(g);
+              v[r] = make_uint4(x.x, x.y, 0u, 0u);
+              lo[r] = L::quad(t * HWL + ql, qd) | HALF8;
+            } else {
+              v[r] = *reinterpret_cast<const uint4*>(g);
+              lo[r] = L::quad(t * HWL + ql, qd);
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RC_BATCH; ++r) {
+      if (lo[r] < 0) continue;
+      if (Q8 && (lo[r] & HALF8)) *reinterpret_cast<uint2*>(buf + (lo[r] & ~HALF8)) = make_uint2(v[r].x, v[r].y);
+      else *reinterpret_cast<uint4*>(buf + lo[r]) = v[r];
     }
   }
 }
@@ -376,7 +429,10 @@ __device__ __forceinline__ void rcache_copy(uint8_t* buf, const FwdArgs& a, int 
 // fp32 C=16).
 // HG: the horizontal-stack lattices H0, H1 live in global scratch (a.hscr,
 // two lattices per workgroup, L2-resident) instead of LDS.
-template <int PREC, int C, int MAXT, int ACT, int NWAVE, int SCHED, bool HG>
+// ROWS: the row-reuse sample pass (cache and prefix writes) and restricted
+// passes (halo rows, prefix add); whole-lattice launches compile without
+// them (device ranges and output scatter, m_range / out_index, work in both).
+template <int PREC, int C, int MAXT, int ACT, int NWAVE, int SCHED, bool HG, bool ROWS>
 __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   using P = Prec<PREC>;
   using frag = typename P::frag;
@@ -396,7 +452,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   // placement q itself would have, so border tiles stay conflict-free.
   const int npix = a.npix;
   // per configuration: HWL pixels in LDS (halo rows included), HWC computed
-  const int HWL = a.HR * a.W, HWC = a.HC * a.W;
+  const int HWL = ROWS ? a.HR * a.W : a.HW, HWC = ROWS ? a.HC * a.W : a.HW;
   const int NPX = a.T * HWL;
   const int BUFB = (NPX + ZERO_PIX) * PIXB;
   uint8_t* V0 = lds;
@@ -429,8 +485,8 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
   const int NW = blockDim.x >> 6;
   const int n16 = lane & 15, g = lane >> 4;
   const int H = a.H, W = a.W, HW = a.HW, T = a.T;
-  const int HR = a.HR, ROW0 = a.ROW0;
-  const int gofs = (a.R0 - ROW0) * W;  // global index of a configuration's LDS pixel 0
+  const int HR = ROWS ? a.HR : H, ROW0 = ROWS ? a.ROW0 : 0;
+  const int gofs = ROWS ? (a.R0 - ROW0) * W : 0;  // global index of a configuration's LDS pixel 0
   const int ntiles = npix >> 4;
   const int nt = (ntiles > wave) ? (ntiles - wave + NW - 1) / NW : 0;
 
@@ -532,7 +588,7 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
         }
         sp[t * HWL + ql] = v;
       }
-      if (lane == 0) s_samp[t] = live ? b : -1;
+      if (ROWS && lane == 0) s_samp[t] = live ? b : -1;
     }
     __syncthreads();
 
@@ -540,14 +596,6 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
       const int kind = block_kind(blk, nb);
       const uint8_t* Vin = (kind == BK_B) ? V1 : V0;
       const uint8_t* Hin = (kind == BK_B) ? H1 : H0;
-      // row reuse: the sample pass stores this block's vertical input; a
-      // restricted pass takes the halo rows of it from the sample's cache
-      if (a.rcache && kind != BK_FIRST && (a.cache_write || ROW0 > 0)) {
-        uint8_t* vb = (kind == BK_B) ? V1 : V0;
-        if (a.cache_write) rcache_copy<PREC, C>(vb, a, blk, 0, T, HWL, 0, HWL, true, s_samp);
-        else rcache_copy<PREC, C>(vb, a, blk, 0, T, HWL, 0, ROW0 * W, false, s_samp);
-        __syncthreads();
-      }
       const float* bias = a.aux + lay.aux_bias(blk);
       const float* by = bias + 2 * C + C / 2;
       const float* bx = bias + 2 * C;
@@ -640,14 +688,18 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
           }
           d_to_b<PREC, MT, KS>(av, bf);
         }
+        // the last row's output wraps to row 0 as the DownShift's zero row
+        // (bias only); in a restricted pass row 0 is a halo row, which holds
+        // the sample's values instead
         const bool down = pi[u] + 1 < HR;
         const int tgt = down ? pix[u] + W : pix[u] - (HR - 1) * W;
+        const bool ystore = inp[u] && (down || ROW0 == 0);
 #pragma unroll
         for (int mh = 0; mh < MTH; ++mh) {
           f32x4 y = by0[mh];
 #pragma unroll
           for (int ks = 0; ks < KS; ++ks) y = P::mfma(wy[mh][ks], bf[ks], y);
-          if (inp[u] && 16 * mh + 4 * g < C / 2) {
+          if (ystore && 16 * mh + 4 * g < C / 2) {
             if constexpr (P::H16) L::store_quad_raw(CAT, tgt, C / 8 + 4 * mh + g, down ? act_quad<PREC, ACT>(y) : aby[mh]);
             else L::store_quad(CAT, tgt, C / 8 + 4 * mh + g, act4<ACT>(down ? y : by0[mh]));
           }
@@ -757,18 +809,19 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
             if (u < nt) epi_h(u, acc[u]);
         }
       }
+      // row reuse, restricted pass: the halo rows of the concatenation (and
+      // the y half of its first computed row, which the DownShift takes from
+      // the halo) from the sample's cache.  This phase neither reads nor
+      // writes those rows, so the copy shares the phase's closing barrier.
+      if (ROWS && a.rcache && !a.cache_write && ROW0 > 0)
+        rcache_in<PREC, C>(CAT, a, blk, 1, T, HWL, true, s_samp);
       __syncthreads();
-      // row reuse: the concatenation of the sample pass to its cache; the
-      // halo rows of a restricted pass (and the y half of its first computed
-      // row, which the DownShift takes from the halo) from the cache
-      if (a.rcache && (a.cache_write || ROW0 > 0)) {
-        if (a.cache_write) {
-          rcache_copy<PREC, C>(CAT, a, blk, 1, T, HWL, 0, HWL, true, s_samp);
-        } else {
-          rcache_copy<PREC, C>(CAT, a, blk, 1, T, HWL, 0, ROW0 * W, false, s_samp);
-          rcache_copy<PREC, C>(CAT, a, blk, 1, T, HWL, ROW0 * W, (ROW0 + 1) * W, false, s_samp, C / 8);
-          __syncthreads();
-        }
+      // row reuse, sample pass: this block's concatenation and the next
+      // block's vertical input (this phase's vertical output, complete now)
+      // to the cache; phase H3 changes neither
+      if (ROWS && a.rcache && a.cache_write) {
+        rcache_out<PREC, C>(CAT, a, blk, 1, T, HWL, s_samp);
+        if (blk + 1 < nb) rcache_out<PREC, C>(block_kind(blk + 1, nb) == BK_B ? V1 : V0, a, blk + 1, 0, T, HWL, s_samp);
       }
 
       // ================= phase H3: kxk on the concatenation ================
@@ -857,6 +910,12 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
         for (int u = 0; u < MAXT; ++u)
           if (u < nt) epi_c(u, acc[u]);
       }
+      // row reuse, restricted pass: the halo rows of the next block's
+      // vertical input from the sample's cache.  That lattice is the other
+      // V buffer from this block's input, and this block writes it only on
+      // computed rows, so the copy shares the block's closing barrier.
+      if (ROWS && a.rcache && !a.cache_write && ROW0 > 0 && blk + 1 < nb)
+        rcache_in<PREC, C>(block_kind(blk + 1, nb) == BK_B ? V1 : V0, a, blk + 1, 0, T, HWL, false, s_samp);
       __syncthreads();
     }
 
@@ -876,12 +935,12 @@ __global__ __launch_bounds__(64 * NWAVE) void convar_fwd_kernel(FwdArgs a) {
         }
       }
       float sr = 0.0f, si = 0.0f;
-      const int b = s_samp[t];
-      if (ROW0 > 0 && b >= 0) {
+      const int b = ROWS ? s_samp[t] : -1;
+      if (ROWS && ROW0 > 0 && b >= 0) {
         sr = a.prefix[((int64_t)b * (H + 1) + a.R0) * 2];
         si = a.prefix[((int64_t)b * (H + 1) + a.R0) * 2 + 1];
       }
-      float* pref = (a.cache_write && b >= 0) ? a.prefix + (int64_t)b * (H + 1) * 2 : nullptr;
+      float* pref = (ROWS && a.cache_write && b >= 0) ? a.prefix + (int64_t)b * (H + 1) * 2 : nullptr;
       if (pref && lane == 0) {
         pref[0] = 0.0f;
         pref[1] = 0.0f;
@@ -933,9 +992,9 @@ inline int64_t fwd_grid(const FwdArgs& a, const FwdPlan& pl) {
   return grid;
 }
 
-template <int PREC, int C, int MAXT, int ACT, int NWAVE, int SCHED, bool HG>
+template <int PREC, int C, int MAXT, int ACT, int NWAVE, int SCHED, bool HG, bool ROWS>
 int launch_fwd(const FwdArgs& a, const FwdPlan& pl, hipStream_t s) {
-  auto kern = convar_fwd_kernel<PREC, C, MAXT, ACT, NWAVE, SCHED, HG>;
+  auto kern = convar_fwd_kernel<PREC, C, MAXT, ACT, NWAVE, SCHED, HG, ROWS>;
   // raise the dynamic LDS limit once per instantiation (not a stream
   // operation, so launches may later be captured into a graph)
   static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -946,17 +1005,61 @@ int launch_fwd(const FwdArgs& a, const FwdPlan& pl, hipStream_t s) {
   return FK_OK;
 }
 
-template <int PREC, int C, int NW, int SCHED, bool HG>
+template <int PREC, int C, int NW, int SCHED, bool HG, bool ROWS>
 int launch_variant(const FwdArgs& a, const FwdPlan& pl, int act, hipStream_t s) {
   constexpr int LO = NW == 4 ? 4 : 2, HI = NW == 4 ? 8 : 4;
   if (pl.maxt <= LO)
-    return act == FK_ACT_ELU ? launch_fwd<PREC, C, LO, FK_ACT_ELU, NW, SCHED, HG>(a, pl, s)
-                             : launch_fwd<PREC, C, LO, FK_ACT_RELU, NW, SCHED, HG>(a, pl, s);
-  return act == FK_ACT_ELU ? launch_fwd<PREC, C, HI, FK_ACT_ELU, NW, SCHED, HG>(a, pl, s)
-                           : launch_fwd<PREC, C, HI, FK_ACT_RELU, NW, SCHED, HG>(a, pl, s);
+    return act == FK_ACT_ELU ? launch_fwd<PREC, C, LO, FK_ACT_ELU, NW, SCHED, HG, ROWS>(a, pl, s)
+                             : launch_fwd<PREC, C, LO, FK_ACT_RELU, NW, SCHED, HG, ROWS>(a, pl, s);
+  return act == FK_ACT_ELU ? launch_fwd<PREC, C, HI, FK_ACT_ELU, NW, SCHED, HG, ROWS>(a, pl, s)
+                           : launch_fwd<PREC, C, HI, FK_ACT_RELU, NW, SCHED, HG, ROWS>(a, pl, s);
 }
 
-// the HG dispatcher (convar_fwd_large.hip)
-int dispatch_fwd_hg(const FwdArgs& a, const FwdPlan& pl, int prec, int cp, int act, hipStream_t s);
+// Launch of one forward: the precision's kernel variant for channel width
+// cp, loop order sched (0 taps outer, 1 tiles outer), nwave waves, the
+// horizontal stack in LDS or global scratch (hg), whole-lattice or row-reuse
+// (rows; those always run taps outer).  Each precision's kernels are compiled
+// in a file of their own (convar_fwd_<precision>_{lds,hg}.hip).
+template <int PREC, bool HG>
+int dispatch_fwd(const FwdArgs& a, const FwdPlan& pl, int cp, int sched, int nwave, int act, bool rows,
+                 hipStream_t s);
+#define FK_DECLARE_DISPATCH_FWD(PREC, HG)                                                                      \
+  template <>                                                                                                  \
+  int dispatch_fwd<PREC, HG>(const FwdArgs& a, const FwdPlan& pl, int cp, int sched, int nwave, int act,      \
+                             bool rows, hipStream_t s);
+FK_DECLARE_DISPATCH_FWD(FK_PREC_BF16, false)
+FK_DECLARE_DISPATCH_FWD(FK_PREC_BF16, true)
+FK_DECLARE_DISPATCH_FWD(FK_PREC_FP16, false)
+FK_DECLARE_DISPATCH_FWD(FK_PREC_FP16, true)
+FK_DECLARE_DISPATCH_FWD(FK_PREC_FP32, false)
+FK_DECLARE_DISPATCH_FWD(FK_PREC_FP32, true)
+
+// The body those files instantiate: channel widths 32/64 (16-bit) or
+// 16/32/64 (fp32); tiles-outer only where it is compiled (bf16 C=32, fp32
+// C=16, LDS-resident, whole lattice).
+template <int PREC, int C, bool HG>
+int dispatch_fwd_c(const FwdArgs& a, const FwdPlan& pl, int sched, int nwave, int act, bool rows, hipStream_t s) {
+  if (rows)
+    return nwave == 4 ? launch_variant<PREC, C, 4, 0, HG, true>(a, pl, act, s)
+                      : launch_variant<PREC, C, 8, 0, HG, true>(a, pl, act, s);
+  if constexpr (!HG && ((PREC == FK_PREC_BF16 && C == 32) || (PREC == FK_PREC_FP32 && C == 16))) {
+    if (sched == 1)
+      return nwave == 4 ? launch_variant<PREC, C, 4, 1, false, false>(a, pl, act, s)
+                        : launch_variant<PREC, C, 8, 1, false, false>(a, pl, act, s);
+  }
+  return nwave == 4 ? launch_variant<PREC, C, 4, 0, HG, false>(a, pl, act, s)
+                    : launch_variant<PREC, C, 8, 0, HG, false>(a, pl, act, s);
+}
+
+#define FK_DEFINE_DISPATCH_FWD(PREC, HG)                                                                       \
+  template <>                                                                                                  \
+  int dispatch_fwd<PREC, HG>(const FwdArgs& a, const FwdPlan& pl, int cp, int sched, int nwave, int act,      \
+                             bool rows, hipStream_t s) {                                                       \
+    if constexpr (PREC == FK_PREC_FP32) {                                                                      \
+      if (cp == 16) return dispatch_fwd_c<PREC, 16, HG>(a, pl, sched, nwave, act, rows, s);                   \
+    }                                                                                                          \
+    if (cp == 32) return dispatch_fwd_c<PREC, 32, HG>(a, pl, sched, nwave, act, rows, s);                     \
+    return dispatch_fwd_c<PREC, 64, HG>(a, pl, sched, nwave, act, rows, s);                                   \
+  }
 
 }  // namespace fk

@@ -323,12 +323,34 @@ class ConvNetAutoregressive2D:
 
     # ------------------------------------------------------------- row reuse
     row_reuse = True
+    # connections first changing a row below this one run a restricted pass
+    # (rows from there on); the others, the whole lattice
+    row_reuse_first = 2
 
     def uses_row_reuse(self, operator):
         """Whether local energies of this network run the row-reuse forward
         (fk_convar_forward_rows): lattices of 3..64 rows and an operator with
         a connection table."""
         return bool(self.row_reuse) and 3 <= self.H <= 64 and hasattr(operator, 'row_buckets')
+
+    def row_reuse_policies(self):
+        """The row-reuse settings the autotuner compares: 0 (off), or the
+        first row that takes restricted passes."""
+        if not 3 <= self.H <= 64:
+            return [0]
+        firsts = sorted({max(2, int(round(f * self.H))) for f in (0.2, 0.3, 0.4, 0.5, 0.6)})
+        return [0] + [f for f in firsts if f <= self.H - 1]
+
+    def set_row_reuse_policy(self, policy):
+        """0: no row reuse; k >= 2: row reuse with restricted passes for
+        connections first changing row k or below."""
+        policy = int(policy)
+        self.row_reuse = policy > 0
+        if policy > 0:
+            self.row_reuse_first = max(2, policy)
+
+    def row_reuse_policy(self):
+        return int(self.row_reuse_first) if self.row_reuse else 0
 
     def log_psi_connected_rows(self, spins, r, operator, precision=None, stream=None):
         """log psi of every used connected configuration of the ragged map
@@ -345,7 +367,7 @@ class ConvNetAutoregressive2D:
         H, W = self.H, self.W
         dev = self.device
         nc = operator.num_connections
-        bucket_of, nbk, sizes = operator.row_buckets(W, dev)
+        bucket_of, nbk, sizes = operator.row_buckets(W, dev, first=self.row_reuse_first)
         tb = operator.tables(dev)
         ws = self._rows_workspace(prec, B, nc, nbk)
         _lib.call('fk_bucket_connections', _lib.ptr(r['use']), B, nc, _lib.ptr(bucket_of), nbk,

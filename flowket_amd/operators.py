@@ -101,15 +101,20 @@ class _BondTableOperator(OperatorOnGrid):
                 diag_a=up(da), diag_b=up(db), diag_j=up(dj), n_diag=int(self._diag_a.size))
         return self._device_tables[key]
 
-    def row_buckets(self, W, device):
+    def row_buckets(self, W, device, first=2, last=None):
         """Per connection slot, the bucket of the row-reuse forward on a
         lattice W sites wide: 0 for slot 0 (the sample itself), else 1 + the
-        first lattice row the connection changes (its lowest site's row).
-        Returns (device int32 table, number of buckets, slots per bucket)."""
-        key = ('row_buckets', int(W), str(device))
+        first lattice row the pass computes.  A connection whose lowest
+        changed site lies on row r is computed from row min(r, last) when r >=
+        first (a restricted pass needs two rows above it), else in full (row
+        0).  last defaults to H - 2.  Returns (device int32 table, number of
+        buckets, slots per bucket)."""
+        n = self.n_sites
+        H = n // W
+        first = max(2, int(first))
+        last = H - 2 if last is None else min(int(last), H - 1)
+        key = ('row_buckets', int(W), first, last, str(device))
         if key not in self._device_tables:
-            n = self.n_sites
-            H = n // W
             a, b = self._conn_a, self._conn_b
             bk = np.zeros(a.size, np.int32)
             for c in range(1, a.size):
@@ -117,7 +122,8 @@ class _BondTableOperator(OperatorOnGrid):
                     bk[c] = 1
                     continue
                 q0 = min(a[c], b[c]) if (self.kind != _lib.FK_CONN_FLIP and b[c] >= 0) else a[c]
-                bk[c] = 1 + q0 // W
+                r = q0 // W
+                bk[c] = 1 + (0 if r < first else min(r, last))
             sizes = np.bincount(bk, minlength=H + 1)
             self._device_tables[key] = (torch.as_tensor(bk).to(device), H + 1, sizes)
         return self._device_tables[key]
