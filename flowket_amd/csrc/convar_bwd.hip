@@ -43,6 +43,8 @@ struct BwdArgs {
   float* partial;       // GRAD_VJP: (gridDim.x, P) per-workgroup partial gradients
   const float* tangent; // JVP: direction, P floats
   float* jvp;           // JVP: (batch, 2) output
+  const float* kt;      // the conv kernels transposed (transpose_kernels_kernel), P floats
+  const float* tkt;     // JVP: the tangent's conv kernels transposed
 };
 
 // Offsets of layer parameters in the flat vector.
@@ -462,18 +464,86 @@ __device__ __forceinline__ void wave_gemm(int mtiles, int ntiles, int kouter, in
   }
 }
 
-// out[p][co_off + co] = bias[co] + sum_{tap, ci} K[tap][ci][co] in[src(tap, p)][ci]
+// The same tiling over the contraction in quads: quad j of lane group kk
+// holds the contraction indices 16 j + 4 kk + r, r = 0..3, which feed four
+// consecutive MFMAs.  Operands contiguous along the contraction then come
+// in one 16-byte load per lane per four MFMAs (instead of four 4-byte
+// loads touching a cache line each).
+template <class FA, class FB, class FS>
+__device__ __forceinline__ void wave_gemm4(int mtiles, int ntiles, int kouter, int kquads, FA fa, FB fb, FS fs) {
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int mb = (mtiles + 1) >> 1, nbk = (ntiles + 1) >> 1;
+  for (int blk = wave; blk < mb * nbk; blk += nw) {
+    const int mt0 = 2 * (blk / nbk), nt0 = 2 * (blk % nbk);
+    const bool m1 = mt0 + 1 < mtiles, n1 = nt0 + 1 < ntiles;
+    const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+    f32x4 c00 = z, c01 = z, c10 = z, c11 = z;
+    for (int t = 0; t < kouter; ++t) {
+#pragma unroll 2
+      for (int j = 0; j < kquads; ++j) {
+        const f32x4 a0 = fa(mt0, t, j), a1 = m1 ? fa(mt0 + 1, t, j) : z;
+        const f32x4 b0 = fb(nt0, t, j), b1 = n1 ? fb(nt0 + 1, t, j) : z;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          c00 = mfma4(a0[r], b0[r], c00);
+          c01 = mfma4(a0[r], b1[r], c01);
+          c10 = mfma4(a1[r], b0[r], c10);
+          c11 = mfma4(a1[r], b1[r], c11);
+        }
+      }
+    }
+    fs(mt0, nt0, c00);
+    if (n1) fs(mt0, nt0 + 1, c01);
+    if (m1) fs(mt0 + 1, nt0, c10);
+    if (m1 && n1) fs(mt0 + 1, nt0 + 1, c11);
+  }
+}
+
+// Conv kernels with the contraction index innermost: KT[tap][co][ci] =
+// K[tap][ci][co] for every kernel with C input channels, at the
+// kernel's own offset in the flat vector (other entries untouched).  One
+// grid row per (block, kernel), threads over the kernel's entries.
+__global__ void transpose_kernels_kernel(const float* __restrict__ prm, float* __restrict__ kt, int C, int nb) {
+  const int row = blockIdx.y, blk = row / 5, which = row % 5;
+  // block 0's vertical and horizontal kernels read the 1-channel input
+  // (scalar path, untransposed)
+  if (blk >= nb || (blk == 0 && which < 2)) return;
+  const ParamOffsets o = param_offsets(blk, C);
+  int64_t off;
+  int ntap, cout;
+  switch (which) {
+    case 0: off = o.kv; ntap = 9; cout = C; break;
+    case 1: off = o.kh; ntap = 3; cout = C; break;
+    case 2: off = o.kx; ntap = 1; cout = C / 2; break;
+    case 3: off = o.ky; ntap = 1; cout = C / 2; break;
+    default: off = o.kc; ntap = 9; cout = C; break;
+  }
+  const int cin = C;
+  const int64_t n = (int64_t)ntap * cin * cout;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int t = (int)(e / ((int64_t)cin * cout));
+    const int rem = (int)(e - (int64_t)t * cin * cout);
+    const int ci = rem / cout, co = rem - (rem / cout) * cout;
+    kt[off + ((int64_t)t * cout + co) * cin + ci] = prm[off + e];
+  }
+}
+
+// out[p][co_off + co] = bias[co] + sum_{tap, ci} K[tap][ci][co] in[src(tap, p)][ci],
+// the kernel given transposed (KT[tap][co][ci], transpose_kernels_kernel)
 __device__ void mconv_fwd(float* out, int cs_out, int co_off, const float* in, int cs_in, int cin, int cout,
-                          const float* K, const float* bias, const int16_t* tab, int ntap, int HW,
+                          const float* KT, const float* bias, const int16_t* tab, int ntap, int HW,
                           bool accumulate = false) {
   const int lane = threadIdx.x & 63, l16 = lane & 15, kk = lane >> 4;
-  wave_gemm(
-      cout / 16, (HW + 15) / 16, ntap, cin / 4,
-      [&](int mt, int t, int s) { return K[(int64_t)(t * cin + 4 * s + kk) * cout + 16 * mt + l16]; },
-      [&](int nt, int t, int s) {
+  const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+  wave_gemm4(
+      cout / 16, (HW + 15) / 16, ntap, cin / 16,
+      [&](int mt, int t, int j) {
+        return *reinterpret_cast<const f32x4*>(KT + ((int64_t)t * cout + 16 * mt + l16) * cin + 16 * j + 4 * kk);
+      },
+      [&](int nt, int t, int j) {
         const int p = 16 * nt + l16;
         const int src = p < HW ? tab[t * HW + p] : -1;
-        return src >= 0 ? in[(int64_t)src * cs_in + 4 * s + kk] : 0.0f;
+        return src >= 0 ? *reinterpret_cast<const f32x4*>(in + (int64_t)src * cs_in + 16 * j + 4 * kk) : z;
       },
       [&](int mt, int nt, f32x4 c) {
         const int p = 16 * nt + l16;
@@ -491,13 +561,17 @@ __device__ void mconv_fwd(float* out, int cs_out, int co_off, const float* in, i
 __device__ void mconv_bwd_data(float* din, int cs_in, const float* dout, int cs_out, int co_off, int cin, int cout,
                                const float* K, const int16_t* inv, int ntap, int HW) {
   const int lane = threadIdx.x & 63, l16 = lane & 15, kk = lane >> 4;
-  wave_gemm(
-      cin / 16, (HW + 15) / 16, ntap, cout / 4,
-      [&](int mt, int t, int s) { return K[(int64_t)(t * cin + 16 * mt + l16) * cout + 4 * s + kk]; },
-      [&](int nt, int t, int s) {
+  const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+  wave_gemm4(
+      cin / 16, (HW + 15) / 16, ntap, cout / 16,
+      [&](int mt, int t, int j) {
+        return *reinterpret_cast<const f32x4*>(K + (int64_t)(t * cin + 16 * mt + l16) * cout + 16 * j + 4 * kk);
+      },
+      [&](int nt, int t, int j) {
         const int q = 16 * nt + l16;
         const int src = q < HW ? inv[t * HW + q] : -1;
-        return src >= 0 ? dout[(int64_t)src * cs_out + co_off + 4 * s + kk] : 0.0f;
+        return src >= 0 ? *reinterpret_cast<const f32x4*>(dout + (int64_t)src * cs_out + co_off + 16 * j + 4 * kk)
+                        : z;
       },
       [&](int mt, int nt, f32x4 c) {
         const int q = 16 * nt + l16;
@@ -614,7 +688,8 @@ __device__ void build_tap_tables(int16_t* ftab, int16_t* itab, int H, int W) {
 // y' = K*x' + Kt*x + bt, and act(y)' = act'(y) y'.  t0/t1 and u0/u1 are
 // scratch tensors of the primal and the tangent.
 template <int ACT>
-__device__ __forceinline__ void mfma_forward(const float* prm, const float* tan, float* ws, float* wt, const int16_t* ftab, int H,
+__device__ __forceinline__ void mfma_forward(const float* prm, const float* tan, const float* kt, const float* tkt,
+                                             float* ws, float* wt, const int16_t* ftab, int H,
                              int W, int C, int nb, float* t0, float* t1, float* u0, float* u1) {
   const int HW = H * W, F = nb - 1;
   const float* sp = ws;
@@ -637,16 +712,16 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
         conv_fwd(u1, C, Hin, 1, 1, C, tan + o.kh, tan + o.bh, 1, 3, H, W);
       }
     } else {
-      mconv_fwd(t0, C, 0, Vin, C, C, C, prm + o.kv, prm + o.bv, ftab + GT_V * HW, 9, HW);
-      mconv_fwd(t1, C, 0, Hin, C, C, C, prm + o.kh, prm + o.bh, ftab + GT_H * HW, 3, HW);
+      mconv_fwd(t0, C, 0, Vin, C, C, C, kt + o.kv, prm + o.bv, ftab + GT_V * HW, 9, HW);
+      mconv_fwd(t1, C, 0, Hin, C, C, C, kt + o.kh, prm + o.bh, ftab + GT_H * HW, 3, HW);
       if (tan) {
         const float* Vd = ws_tensor(wt, HW, C, blk - 1, T_VO);
         const float* Hd = ws_tensor(wt, HW, C, blk - 1, T_HO);
-        mconv_fwd(u0, C, 0, Vd, C, C, C, prm + o.kv, tan + o.bv, ftab + GT_V * HW, 9, HW);
-        mconv_fwd(u1, C, 0, Hd, C, C, C, prm + o.kh, tan + o.bh, ftab + GT_H * HW, 3, HW);
+        mconv_fwd(u0, C, 0, Vd, C, C, C, kt + o.kv, tan + o.bv, ftab + GT_V * HW, 9, HW);
+        mconv_fwd(u1, C, 0, Hd, C, C, C, kt + o.kh, tan + o.bh, ftab + GT_H * HW, 3, HW);
         __syncthreads();
-        mconv_fwd(u0, C, 0, Vin, C, C, C, tan + o.kv, nullptr, ftab + GT_V * HW, 9, HW, true);
-        mconv_fwd(u1, C, 0, Hin, C, C, C, tan + o.kh, nullptr, ftab + GT_H * HW, 3, HW, true);
+        mconv_fwd(u0, C, 0, Vin, C, C, C, tkt + o.kv, nullptr, ftab + GT_V * HW, 9, HW, true);
+        mconv_fwd(u1, C, 0, Hin, C, C, C, tkt + o.kh, nullptr, ftab + GT_H * HW, 3, HW, true);
       }
     }
     __syncthreads();
@@ -671,17 +746,17 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
       }
     }
     __syncthreads();
-    mconv_fwd(CAT, C, 0, XA, C, C, C / 2, prm + o.kx, prm + o.bx, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW);
-    mconv_fwd(CAT, C, C / 2, VA, C, C, C / 2, prm + o.ky, prm + o.by, ftab + GT_Y * HW, 1, HW);
+    mconv_fwd(CAT, C, 0, XA, C, C, C / 2, kt + o.kx, prm + o.bx, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW);
+    mconv_fwd(CAT, C, C / 2, VA, C, C, C / 2, kt + o.ky, prm + o.by, ftab + GT_Y * HW, 1, HW);
     if (tan) {
       float* CATd = ws_tensor(wt, HW, C, blk, T_CAT);
       const float* XAd = ws_tensor(wt, HW, C, blk, T_XA);
       const float* VAd = ws_tensor(wt, HW, C, blk, T_VA);
-      mconv_fwd(CATd, C, 0, XAd, C, C, C / 2, prm + o.kx, tan + o.bx, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW);
-      mconv_fwd(CATd, C, C / 2, VAd, C, C, C / 2, prm + o.ky, tan + o.by, ftab + GT_Y * HW, 1, HW);
+      mconv_fwd(CATd, C, 0, XAd, C, C, C / 2, kt + o.kx, tan + o.bx, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW);
+      mconv_fwd(CATd, C, C / 2, VAd, C, C, C / 2, kt + o.ky, tan + o.by, ftab + GT_Y * HW, 1, HW);
       __syncthreads();
-      mconv_fwd(CATd, C, 0, XA, C, C, C / 2, tan + o.kx, nullptr, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW, true);
-      mconv_fwd(CATd, C, C / 2, VA, C, C, C / 2, tan + o.ky, nullptr, ftab + GT_Y * HW, 1, HW, true);
+      mconv_fwd(CATd, C, 0, XA, C, C, C / 2, tkt + o.kx, nullptr, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW, true);
+      mconv_fwd(CATd, C, C / 2, VA, C, C, C / 2, tkt + o.ky, nullptr, ftab + GT_Y * HW, 1, HW, true);
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
@@ -690,12 +765,12 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
       if (tan) ws_tensor(wt, HW, C, blk, T_CAT)[idx] *= bdact<ACT>(c);
     }
     __syncthreads();
-    mconv_fwd(t0, C, 0, CAT, C, C, C, prm + o.kc, prm + o.bc, ftab + GT_C * HW, 9, HW);
+    mconv_fwd(t0, C, 0, CAT, C, C, C, kt + o.kc, prm + o.bc, ftab + GT_C * HW, 9, HW);
     if (tan) {
-      mconv_fwd(u0, C, 0, ws_tensor(wt, HW, C, blk, T_CAT), C, C, C, prm + o.kc, tan + o.bc, ftab + GT_C * HW, 9,
+      mconv_fwd(u0, C, 0, ws_tensor(wt, HW, C, blk, T_CAT), C, C, C, kt + o.kc, tan + o.bc, ftab + GT_C * HW, 9,
                 HW);
       __syncthreads();
-      mconv_fwd(u0, C, 0, CAT, C, C, C, tan + o.kc, nullptr, ftab + GT_C * HW, 9, HW, true);
+      mconv_fwd(u0, C, 0, CAT, C, C, C, tkt + o.kc, nullptr, ftab + GT_C * HW, 9, HW, true);
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
@@ -823,7 +898,7 @@ enum { GRAD_O = 0, GRAD_VJP = 1 };
 // and adds cot_re[b] d Re log psi_b + cot_im[b] d Im log psi_b (one backward
 // pass with the combined head cotangent) into its own partial gradient row.
 template <int ACT, int MODE>
-__global__ __launch_bounds__(256, 4) void convar_grad_mfma_kernel(BwdArgs a) {
+__global__ __launch_bounds__(256, MODE == GRAD_VJP ? 3 : 4) void convar_grad_mfma_kernel(BwdArgs a) {
   extern __shared__ int16_t tabs[];  // GT_N forward rows, then GT_N inverse rows, of HW entries
   __shared__ float s_lp[2];
   const int H = a.H, W = a.W, HW = a.HW, C = a.C, nb = a.nb;
@@ -853,7 +928,7 @@ __global__ __launch_bounds__(256, 4) void convar_grad_mfma_kernel(BwdArgs a) {
     for (int q = threadIdx.x; q < HW; q += blockDim.x) sp[q] = (float)a.spins[b * HW + q];
     if (threadIdx.x == 0) s_lp[0] = s_lp[1] = 0.0f;
     __syncthreads();
-    mfma_forward<ACT>(prm, nullptr, ws, nullptr, ftab, H, W, C, nb, t0, t1, nullptr, nullptr);
+    mfma_forward<ACT>(prm, nullptr, a.kt, nullptr, ws, nullptr, ftab, H, W, C, nb, t0, t1, nullptr, nullptr);
     const int64_t ho = head_offset(nb, C);
     const float* HF = ws_tensor(ws, HW, C, F, T_HO);
     const float cr = MODE == GRAD_VJP ? a.cot[2 * b] : 1.0f;
@@ -915,7 +990,7 @@ __global__ __launch_bounds__(256, 4) void convar_jvp_mfma_kernel(BwdArgs a) {
     for (int q = threadIdx.x; q < HW; q += blockDim.x) sp[q] = (float)a.spins[b * HW + q];
     if (threadIdx.x < 4) s_acc[threadIdx.x] = 0.0f;
     __syncthreads();
-    mfma_forward<ACT>(prm, tan, ws, wt, ftab, H, W, C, nb, t0, t1, u0, u1);
+    mfma_forward<ACT>(prm, tan, a.kt, a.tkt, ws, wt, ftab, H, W, C, nb, t0, t1, u0, u1);
     const int64_t ho = head_offset(nb, C);
     const float* HF = ws_tensor(ws, HW, C, F, T_HO);
     const float* HFd = ws_tensor(wt, HW, C, F, T_HO);
@@ -972,12 +1047,14 @@ int64_t flat_param_count(int C, int nb) {
          (int64_t)(nb - 1) * (9 * c * c + c + 3 * c * c + c + c * (c / 2) * 2 + c + 9 * c * c + c) + 4 * c + 4;
 }
 
-// Workgroups of the persistent VJP / JVP grids: four per CU (the kernels
-// fit four 4-wave workgroups per CU), at most one per sample.
-int64_t implicit_grid(int64_t batch) {
-  const int64_t g = 4LL * device_cu_count();
+// Workgroups of the persistent VJP / JVP grids: as many 4-wave workgroups
+// per CU as the kernel's registers allow (VJP 3, JVP 4), at most one per
+// sample.
+int64_t implicit_grid(int64_t batch, int per_cu) {
+  const int64_t g = (int64_t)per_cu * device_cu_count();
   return batch < g ? (batch > 0 ? batch : 1) : g;
 }
+constexpr int VJP_PER_CU = 3, JVP_PER_CU = 4;
 
 size_t grad_lds_bytes(int HW) {
   return (size_t)2 * GT_N * HW * sizeof(int16_t) + 4 + (size_t)4 * HW * sizeof(float);
@@ -992,6 +1069,18 @@ int launch_with_lds(K kern, int64_t grid, size_t lds, hipStream_t s, const BwdAr
   return FK_OK;
 }
 
+// Floats reserved at the start of a workspace for transposed kernels (P
+// rounded up to 64 floats, so what follows stays 256-byte aligned).
+int64_t kt_floats(int C, int nb) { return (flat_param_count(C, nb) + 63) / 64 * 64; }
+
+int launch_transpose(const float* prm, float* kt, int C, int nb, hipStream_t s) {
+  const int64_t n = 9LL * C * C;
+  const unsigned gx = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(transpose_kernels_kernel, dim3(gx, (unsigned)(nb * 5)), dim3(256), 0, s, prm, kt, C, nb);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
+
 }  // namespace fk
 
 using namespace fk;
@@ -1000,7 +1089,8 @@ extern "C" {
 
 int64_t fk_convar_grad_workspace_bytes(int32_t H, int32_t W, int32_t depth, int32_t channels, int64_t batch) {
   if (H < 1 || W < 1 || depth < 1 || channels < 2 || (channels % 2) || batch < 0) return -1;
-  return batch * grad_work_floats(H * W, channels, num_causal_blocks(depth)) * (int64_t)sizeof(float);
+  const int nb = num_causal_blocks(depth);
+  return (kt_floats(channels, nb) + batch * grad_work_floats(H * W, channels, nb)) * (int64_t)sizeof(float);
 }
 
 int fk_convar_grad(const float* params, int32_t H, int32_t W, int32_t depth, int32_t channels, int32_t activation,
@@ -1020,7 +1110,8 @@ int fk_convar_grad(const float* params, int32_t H, int32_t W, int32_t depth, int
   a.nb = num_causal_blocks(depth);
   a.batch = batch;
   a.spins = spins;
-  a.work = reinterpret_cast<float*>(workspace);
+  a.kt = reinterpret_cast<float*>(workspace);
+  a.work = reinterpret_cast<float*>(workspace) + kt_floats(channels, a.nb);
   a.work_floats = grad_work_floats(a.HW, channels, a.nb);
   a.o_re = o_re;
   a.o_im = o_im;
@@ -1033,6 +1124,8 @@ int fk_convar_grad(const float* params, int32_t H, int32_t W, int32_t depth, int
     // MFMA kernel: LDS holds the tap source tables and the head cotangents
     const size_t lds = (size_t)2 * GT_N * a.HW * sizeof(int16_t) + 4 + (size_t)4 * a.HW * sizeof(float);
     FK_REQUIRE(lds <= 160 * 1024 && a.HW <= 32767, "fk_convar_grad: lattice too large");
+    const int rc = launch_transpose(params, const_cast<float*>(a.kt), channels, a.nb, s);
+    if (rc != FK_OK) return rc;
     return activation == FK_ACT_ELU
                ? launch_with_lds(convar_grad_mfma_kernel<FK_ACT_ELU, GRAD_O>, batch, lds, s, a)
                : launch_with_lds(convar_grad_mfma_kernel<FK_ACT_RELU, GRAD_O>, batch, lds, s, a);
@@ -1057,8 +1150,9 @@ int fk_convar_grad(const float* params, int32_t H, int32_t W, int32_t depth, int
 int64_t fk_convar_vjp_workspace_bytes(int32_t H, int32_t W, int32_t depth, int32_t channels, int64_t batch) {
   if (H < 1 || W < 1 || depth < 1 || channels < 32 || (channels % 32) || batch < 0) return -1;
   const int nb = num_causal_blocks(depth);
-  const int64_t g = implicit_grid(batch);
-  return g * (grad_work_floats(H * W, channels, nb) + flat_param_count(channels, nb)) * (int64_t)sizeof(float);
+  const int64_t g = implicit_grid(batch, VJP_PER_CU);
+  return (kt_floats(channels, nb) + g * (grad_work_floats(H * W, channels, nb) + flat_param_count(channels, nb))) *
+         (int64_t)sizeof(float);
 }
 
 int fk_convar_vjp(const float* params, int32_t H, int32_t W, int32_t depth, int32_t channels, int32_t activation,
@@ -1078,18 +1172,21 @@ int fk_convar_vjp(const float* params, int32_t H, int32_t W, int32_t depth, int3
   a.nb = num_causal_blocks(depth);
   a.batch = batch;
   a.spins = spins;
-  a.work = reinterpret_cast<float*>(workspace);
+  a.kt = reinterpret_cast<float*>(workspace);
+  a.work = reinterpret_cast<float*>(workspace) + kt_floats(channels, a.nb);
   a.work_floats = grad_work_floats(a.HW, channels, a.nb);
   a.logpsi = logpsi;
   a.P = flat_param_count(channels, a.nb);
   a.cot = cot;
-  const int64_t g = implicit_grid(batch);
+  const int64_t g = implicit_grid(batch, VJP_PER_CU);
   a.partial = a.work + g * a.work_floats;
   const size_t lds = grad_lds_bytes(a.HW);
   FK_REQUIRE(lds <= 160 * 1024 && a.HW <= 32767, "fk_convar_vjp: lattice too large");
   hipStream_t s = (hipStream_t)stream;
-  int rc = activation == FK_ACT_ELU ? launch_with_lds(convar_grad_mfma_kernel<FK_ACT_ELU, GRAD_VJP>, g, lds, s, a)
-                                    : launch_with_lds(convar_grad_mfma_kernel<FK_ACT_RELU, GRAD_VJP>, g, lds, s, a);
+  int rc = launch_transpose(params, const_cast<float*>(a.kt), channels, a.nb, s);
+  if (rc != FK_OK) return rc;
+  rc = activation == FK_ACT_ELU ? launch_with_lds(convar_grad_mfma_kernel<FK_ACT_ELU, GRAD_VJP>, g, lds, s, a)
+                                : launch_with_lds(convar_grad_mfma_kernel<FK_ACT_RELU, GRAD_VJP>, g, lds, s, a);
   if (rc != FK_OK) return rc;
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(div_up(a.P, 256)), dim3(256), 0, s, a.partial, g, a.P, scale,
                      grad);
@@ -1099,7 +1196,8 @@ int fk_convar_vjp(const float* params, int32_t H, int32_t W, int32_t depth, int3
 
 int64_t fk_convar_jvp_workspace_bytes(int32_t H, int32_t W, int32_t depth, int32_t channels, int64_t batch) {
   if (H < 1 || W < 1 || depth < 1 || channels < 32 || (channels % 32) || batch < 0) return -1;
-  return implicit_grid(batch) * 2 * grad_work_floats(H * W, channels, num_causal_blocks(depth)) *
+  const int nb = num_causal_blocks(depth);
+  return (2 * kt_floats(channels, nb) + implicit_grid(batch, JVP_PER_CU) * 2 * grad_work_floats(H * W, channels, nb)) *
          (int64_t)sizeof(float);
 }
 
@@ -1120,7 +1218,9 @@ int fk_convar_jvp(const float* params, const float* tangent, int32_t H, int32_t 
   a.nb = num_causal_blocks(depth);
   a.batch = batch;
   a.spins = spins;
-  a.work = reinterpret_cast<float*>(workspace);
+  a.kt = reinterpret_cast<float*>(workspace);
+  a.tkt = a.kt + kt_floats(channels, a.nb);
+  a.work = reinterpret_cast<float*>(workspace) + 2 * kt_floats(channels, a.nb);
   a.work_floats = 2 * grad_work_floats(a.HW, channels, a.nb);
   a.logpsi = logpsi;
   a.P = flat_param_count(channels, a.nb);
@@ -1128,8 +1228,11 @@ int fk_convar_jvp(const float* params, const float* tangent, int32_t H, int32_t 
   a.jvp = jvp;
   const size_t lds = grad_lds_bytes(a.HW);
   FK_REQUIRE(lds <= 160 * 1024 && a.HW <= 32767, "fk_convar_jvp: lattice too large");
-  const int64_t g = implicit_grid(batch);
+  const int64_t g = implicit_grid(batch, JVP_PER_CU);
   hipStream_t s = (hipStream_t)stream;
+  int rc = launch_transpose(params, const_cast<float*>(a.kt), channels, a.nb, s);
+  if (rc == FK_OK) rc = launch_transpose(tangent, const_cast<float*>(a.tkt), channels, a.nb, s);
+  if (rc != FK_OK) return rc;
   return activation == FK_ACT_ELU ? launch_with_lds(convar_jvp_mfma_kernel<FK_ACT_ELU>, g, lds, s, a)
                                   : launch_with_lds(convar_jvp_mfma_kernel<FK_ACT_RELU>, g, lds, s, a);
 }
