@@ -39,13 +39,17 @@ ACT = 'relu'
 PEAK_BF16 = 2.5e15      # MI355X dense BF16 MFMA FLOP/s
 PEAK_FP32 = 157.3e12    # MI355X FP32 FLOP/s
 PEAK_HBM = 8.0e12       # MI355X HBM3E bytes/s
-# HBM bytes per connected configuration moved by the fused forward, measured
-# with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over `bench.py --profile-forward
-# 1024` (103,432 configurations), each scaled by the reading of a 256 MiB
-# device copy (`bench.py --profile-copy`: FETCH x 2.0, WRITE x 1.0):
-# (10,000.5 KiB x 2 + 5,900.9 KiB) / 103,432 = 256 B (of which about 45 B is
-# a per-launch constant: the launch's one-time register spill of 4.6 MB).
-FWD_HBM_BYTES_PER_CONFIG = 256.4
+# HBM bytes per connected configuration moved by the forward launches of the
+# step, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over
+# `bench.py --profile-forward 1024 --schedule 0` (103,432 configurations),
+# each scaled by the reading of a 256 MiB device copy (`bench.py
+# --profile-copy`: FETCH x 2.0, WRITE x 1.0):
+#   whole-lattice forward (`--plain-forward`):
+#     (13,941.4 KiB x 2 + 5,900.9 KiB) / 103,432 = 334 B
+#   row reuse from row 5 (`--row-first 5`; six launches, the sample pass
+#   writing 486 kB of cache per sample and the restricted passes reading
+#   their halo rows): (682,823 KiB x 2 + 511,385 KiB) / 103,432 = 18,584 B
+FWD_HBM_BYTES_PER_CONFIG = {0: 334.5, 5: 18584.0}
 
 
 def parse():
@@ -62,6 +66,9 @@ def parse():
     ap.add_argument('--profile-forward', type=int, default=0, metavar='N',
                     help='only run two fused-forward launches over the connected configurations of N random '
                          'samples (a short program for rocprofv3 counter passes) and exit')
+    ap.add_argument('--row-first', type=int, default=None, metavar='R',
+                    help='with --profile-forward: the first lattice row whose connections take restricted '
+                         'row-reuse passes (default: the machine default)')
     ap.add_argument('--plain-forward', action='store_true',
                     help='evaluate every connected configuration with the whole-lattice forward (no row reuse)')
     ap.add_argument('--profile-grad', type=int, default=0, metavar='N',
@@ -76,7 +83,7 @@ def parse():
     return ap.parse_args()
 
 
-def profile_forward(n, precision, schedule, plain=False):
+def profile_forward(n, precision, schedule, plain=False, row_first=None):
     import torch
     import flowket_amd as fk
     from flowket_amd import _lib, autotune
@@ -85,6 +92,8 @@ def profile_forward(n, precision, schedule, plain=False):
     if schedule is not None:
         from flowket_amd.machines import _PREC
         autotune.set_schedule(m, _PREC[precision], schedule)
+    if row_first is not None:
+        m.set_row_reuse_policy(row_first)
     op = fk.Heisenberg(hilbert_state_shape=(H, W), pbc=True)
     gen = torch.Generator().manual_seed(0)
     spins = (torch.randint(0, 2, (n, H * W), generator=gen) * 2 - 1).to(torch.int8).cuda()
@@ -98,7 +107,8 @@ def profile_forward(n, precision, schedule, plain=False):
             m.log_psi_connected(spins, r['pairs'], total, op)
     torch.cuda.synchronize()
     print(json.dumps({'profile_forward': {'samples': n, 'configurations': total, 'precision': precision,
-                                          'schedule': schedule, 'row_reuse': rows}}))
+                                          'schedule': schedule, 'row_reuse': rows,
+                                          'row_first': m.row_reuse_policy()}}))
     return 0
 
 
@@ -556,7 +566,8 @@ def main():
     if args.profile_copy:
         return profile_copy()
     if args.profile_forward:
-        return profile_forward(args.profile_forward, args.precision, args.schedule, args.plain_forward)
+        return profile_forward(args.profile_forward, args.precision, args.schedule, args.plain_forward,
+                               args.row_first)
     if args.profile_grad:
         return profile_grad(args.profile_grad)
     if args.gpus != 1:
@@ -670,6 +681,7 @@ def main():
         rows = np.array([H] + [H if k - 1 < 2 else H - (k - 1) for k in range(1, H + 1)], dtype=np.float64)
         executed = float((counts * rows / H).sum()) * flops_per_forward()
     achieved = flop_launch / (fwd_ms * 1e-3)
+    policy = m16.row_reuse_policy() if m16.uses_row_reuse(op) else 0
     mom = mom.cpu().numpy()
 
     out = {
@@ -691,7 +703,9 @@ def main():
         'energy_per_site': float(mom[0]) / (H * W),
         'roofline': {'bound': 'mfma', 'achieved': round(achieved / 1e12, 2), 'peak': PEAK_BF16 / 1e12,
                      'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_BF16, 4),
-                     'traffic': round(FWD_HBM_BYTES_PER_CONFIG * n_conn),
+                     'traffic': (round(FWD_HBM_BYTES_PER_CONFIG[policy] * n_conn)
+                                 if policy in FWD_HBM_BYTES_PER_CONFIG else None),
+                     'row_reuse_first_row': policy,
                      'kernel': 'convar_fwd_kernel<fp16,32>', 'kernel_ms': round(fwd_ms, 3),
                      'flop_per_launch': flop_launch,
                      'note': 'achieved = reference-algorithm FLOPs (a full forward per used connection) per '
