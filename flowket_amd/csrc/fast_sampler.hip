@@ -102,6 +102,25 @@ struct LaneData<FK_PREC_FP32, C> {
   }
 };
 
+// Per-lane activation loads of one horizontal-stack block step of the fast
+// sampler (from the per-tile cache in HBM): the two earlier horizontal-stack
+// pixels, the eight earlier concatenation pixels, and the y half of the
+// current one (the batched form; otherwise only the current pixel).
+template <int PREC, int C, bool BATCH>
+struct HLoads {
+  LaneData<PREC, C> hd[BATCH ? 2 : 1];
+  LaneData<PREC, C> cd[BATCH ? 8 : 1];
+  LaneData<PREC, C> cur2;
+};
+
+// Per-lane activation loads of one pixel of the vertical-stack pass: the
+// nine vertical inputs and the residual (batched form only).
+template <int PREC, int C, bool BATCH>
+struct VLoads {
+  LaneData<PREC, C> vd[BATCH ? 9 : 1];
+  LaneData<PREC, C> vr;
+};
+
 struct SampArgs {
   const void* w;
   const float* aux;
@@ -253,38 +272,80 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         f32x4 hcur[MT], hres[MT];
 #pragma unroll
         for (int m = 0; m < MT; ++m) hcur[m] = hres[m] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int l = 0; l < F; ++l) {
+        // The activation loads of block step l (earlier sites and rows, from
+        // HBM) do not depend on this site's chain.  With batched loads the
+        // steps go in pairs over two register sets: step l+1's activations
+        // are requested before step l computes, so their latency hides behind
+        // step l's dependent MFMA chain; weights (L2) are requested at the
+        // start of their own step.
+        auto issue = [&](HLoads<PREC, C, BATCH>& L, int l) {
+          if constexpr (BATCH) {
+            if (l > 0) {
+#pragma unroll
+              for (int t = 0; t < 2; ++t)
+                if (j + t - 2 >= 0) L.hd[t].load(HAp(l, i * W + j + t - 2));
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+              const int r = i + t / 3 - 2, c = j + t % 3 - 2;
+              if (r >= 0 && c >= 0) L.cd[t].load(CATp(l, r * W + c));
+            }
+          }
+          L.cur2.load_quads(CATp(l, q), MTH, MT);
+        };
+        // One block step: its own loads (biases, weights, the first block's
+        // spins) are requested first, then the next step's activations
+        // (Ln, ln; ln < 0: none), then the step computes.  Waiting for a
+        // load waits for every load issued before it, so this order lets
+        // the step start on its own operands while the prefetch is in flight.
+        auto step = [&](HLoads<PREC, C, BATCH>& L, int l, HLoads<PREC, C, BATCH>& Ln, int ln) {
           const int kind = block_kind(l, nb);
           const float* bl = a.aux + lay.aux_bias(l);
-          f32x4 hacc[MT];
+          f32x4 hacc[MT], cacc[MT], xb[MTH];
 #pragma unroll
           for (int m = 0; m < MT; ++m) hacc[m] = load_bias4(bl + C, 16 * m + 4 * g, C);
+#pragma unroll
+          for (int m = 0; m < MT; ++m) cacc[m] = load_bias4(bl + 3 * C, 16 * m + 4 * g, C);
+#pragma unroll
+          for (int m = 0; m < MTH; ++m) xb[m] = load_bias4(bl + 2 * C, 16 * m + 4 * g, C / 2);
+          frag wh[BATCH ? 3 : 1][MT][KS], wc[BT][MT][KS], wx[MTH][KS];
+#pragma unroll
+          for (int m = 0; m < MTH; ++m)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) wx[m][ks] = wfrag(lay.off_x(l) + m * KS + ks);
+          float sv[3] = {0.0f, 0.0f, 0.0f};
+          if (l == 0) {
+            for (int t = 0; t < 2; ++t) {
+              const int c = j + t - 2;
+              if (c >= 0) sv[t] = spins[(i * W + c) * 16 + n];
+            }
+            sv[2] = s;
+          } else if constexpr (BATCH) {
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+#pragma unroll
+              for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) wh[t][m][ks] = wfrag(lay.off_h(l) + (t * MT + m) * KS + ks);
+          }
+          if constexpr (BATCH) {
+#pragma unroll
+            for (int t = 0; t < 9; ++t)
+#pragma unroll
+              for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) wc[t][m][ks] = wfrag(lay.off_c(l) + (t * MT + m) * KS + ks);
+          }
+          if (ln >= 0) issue(Ln, ln);
           if (l == 0) {
             const float* kh = a.aux + lay.aux_kh0(nb);
             for (int t = 0; t < 3; ++t) {
-              const int c = j + t - 2;
-              const float sv = (c < 0) ? 0.0f : (c == j ? s : spins[(i * W + c) * 16 + n]);
+              if (j + t - 2 < 0) continue;
 #pragma unroll
               for (int m = 0; m < MT; ++m)
-                hacc[m] += *reinterpret_cast<const f32x4*>(kh + t * C + 16 * m + 4 * g) * sv;
+                hacc[m] += *reinterpret_cast<const f32x4*>(kh + t * C + 16 * m + 4 * g) * sv[t];
             }
           } else {
-            // every load of this step is issued before the first MFMA: the
-            // step is a chain of dependent MFMAs, so serialised L2 round trips
-            // would dominate it
-            frag wh[BATCH ? 3 : 1][MT][KS];
-            LD hd[BATCH ? 2 : 1];
-            if constexpr (BATCH) {
-#pragma unroll
-              for (int t = 0; t < 3; ++t)
-#pragma unroll
-                for (int m = 0; m < MT; ++m)
-#pragma unroll
-                  for (int ks = 0; ks < KS; ++ks) wh[t][m][ks] = wfrag(lay.off_h(l) + (t * MT + m) * KS + ks);
-#pragma unroll
-              for (int t = 0; t < 2; ++t)
-                if (j + t - 2 >= 0) hd[t].load(HAp(l, i * W + j + t - 2));
-            }
 #pragma unroll UNR
             for (int t = 0; t < 3; ++t) {
               const int c = j + t - 2;
@@ -293,10 +354,11 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
               if (t == 2) {
                 d_to_b<PREC, MT, KS>(hcur, b);
               } else if constexpr (BATCH) {
-                hd[t].template frags<KS>(b);
+                L.hd[t].template frags<KS>(b);
               } else {
-                hd[0].load(HAp(l, i * W + c));
-                hd[0].template frags<KS>(b);
+                LD hd;
+                hd.load(HAp(l, i * W + c));
+                hd.template frags<KS>(b);
               }
 #pragma unroll
               for (int m = 0; m < MT; ++m)
@@ -314,46 +376,27 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           for (int m = 0; m < MT; ++m) ax[m] = act4<ACT>(hacc[m]);
           frag bxf[KS];
           d_to_b<PREC, MT, KS>(ax, bxf);
-          LD cur2;
-          cur2.load_quads(CATp(l, q), MTH, MT);
-          frag wc[BT][MT][KS];
-          LD cd[BATCH ? 8 : 1];
-          if constexpr (BATCH) {
-#pragma unroll
-            for (int t = 0; t < 9; ++t)
-#pragma unroll
-              for (int m = 0; m < MT; ++m)
-#pragma unroll
-                for (int ks = 0; ks < KS; ++ks) wc[t][m][ks] = wfrag(lay.off_c(l) + (t * MT + m) * KS + ks);
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-              const int r = i + t / 3 - 2, c = j + t % 3 - 2;
-              if (r >= 0 && c >= 0) cd[t].load(CATp(l, r * W + c));
-            }
-          }
 #pragma unroll
           for (int m = 0; m < MTH; ++m) {
-            f32x4 x = load_bias4(bl + 2 * C, 16 * m + 4 * g, C / 2);
+            f32x4 x = xb[m];
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks) x = P::mfma(wfrag(lay.off_x(l) + m * KS + ks), bxf[ks], x);
-            cur2.set(m, act4<ACT>(x));
+            for (int ks = 0; ks < KS; ++ks) x = P::mfma(wx[m][ks], bxf[ks], x);
+            L.cur2.set(m, act4<ACT>(x));
           }
-          cur2.store_quads(CATp(l, q), 0, MTH);
-          f32x4 cacc[MT];
-#pragma unroll
-          for (int m = 0; m < MT; ++m) cacc[m] = load_bias4(bl + 3 * C, 16 * m + 4 * g, C);
+          L.cur2.store_quads(CATp(l, q), 0, MTH);
 #pragma unroll UNR
           for (int t = 0; t < 9; ++t) {
             const int r = i + t / 3 - 2, c = j + t % 3 - 2;
             if (r < 0 || c < 0) continue;
             frag b[KS];
             if (t == 8) {
-              cur2.template frags<KS>(b);
+              L.cur2.template frags<KS>(b);
             } else if constexpr (BATCH) {
-              cd[t].template frags<KS>(b);
+              L.cd[t].template frags<KS>(b);
             } else {
-              cd[0].load(CATp(l, r * W + c));
-              cd[0].template frags<KS>(b);
+              LD cd;
+              cd.load(CATp(l, r * W + c));
+              cd.template frags<KS>(b);
             }
 #pragma unroll
             for (int m = 0; m < MT; ++m)
@@ -379,6 +422,19 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           // the value the next block sees is the stored (rounded) one
 #pragma unroll
           for (int m = 0; m < MT; ++m) hcur[m] = hn.get(m);
+        };
+        HLoads<PREC, C, BATCH> LA, LB;
+        if constexpr (BATCH) {
+          if (F > 0) issue(LA, 0);
+          for (int l = 0; l < F; l += 2) {
+            step(LA, l, LB, l + 1 < F ? l + 1 : -1);
+            if (l + 1 < F) step(LB, l + 1, LA, l + 2 < F ? l + 2 : -1);
+          }
+        } else {
+          for (int l = 0; l < F; ++l) {
+            issue(LA, l);
+            step(LA, l, LB, -1);
+          }
         }
       }
     }
@@ -387,8 +443,17 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
     for (int l = 0; l <= F; ++l) {
       const int kind = block_kind(l, nb);
       const float* bl = a.aux + lay.aux_bias(l);
-      // this block's vertical and y weights serve the whole row
-      frag wv[BT][MT][KS];
+      // this block's weights and biases serve the whole row
+      frag wv[BT][MT][KS], wy[MTH][KS];
+      f32x4 bv0[MT], by0[MTH];
+#pragma unroll
+      for (int m = 0; m < MT; ++m) bv0[m] = load_bias4(bl, 16 * m + 4 * g, C);
+#pragma unroll
+      for (int m = 0; m < MTH; ++m) {
+        by0[m] = load_bias4(bl + 2 * C + C / 2, 16 * m + 4 * g, C / 2);
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) wy[m][ks] = wfrag(lay.off_y(l) + m * KS + ks);
+      }
       if constexpr (BATCH) {
         if (l > 0) {
 #pragma unroll
@@ -399,39 +464,53 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
               for (int ks = 0; ks < KS; ++ks) wv[t][m][ks] = wfrag(lay.off_v(l) + (t * MT + m) * KS + ks);
         }
       }
-      for (int j = 0; j < W; ++j) {
-        const int q = i * W + j;
-        f32x4 acc[MT];
-#pragma unroll
-        for (int m = 0; m < MT; ++m) acc[m] = load_bias4(bl, 16 * m + 4 * g, C);
-        if (l == 0) {
-          const float* kv = a.aux + lay.aux_kv0(nb);
-          for (int t = 0; t < 9; ++t) {
-            const int r = i + t / 3 - 2, c = j + t % 3 - 1;
-            const bool valid = r >= 0 && c >= 0 && c < W;
-            const float sv = valid ? spins[(r * W + c) * 16 + n] : 0.0f;
-#pragma unroll
-            for (int m = 0; m < MT; ++m) acc[m] += *reinterpret_cast<const f32x4*>(kv + t * C + 16 * m + 4 * g) * sv;
-          }
-        } else {
-          LD vd[BT];
-          if constexpr (BATCH) {
+      // the activations pixel j reads (the vertical inputs of rows i-2..i
+      // and the residual) were written by earlier blocks' passes, so with
+      // batched loads pixel j+1's are requested before pixel j computes
+      auto vissue = [&](VLoads<PREC, C, BATCH>& L, int j) {
+        if constexpr (BATCH) {
+          if (l > 0) {
 #pragma unroll
             for (int t = 0; t < 9; ++t) {
               const int r = i + t / 3 - 2, c = j + t % 3 - 1;
-              if (r >= 0 && c >= 0 && c < W) vd[t].load(VAp(l, r * W + c));
+              if (r >= 0 && c >= 0 && c < W) L.vd[t].load(VAp(l, r * W + c));
             }
           }
+          if (l < F && kind == BK_B) L.vr.load(VAp(l - 1, i * W + j));
+        }
+      };
+      auto vstep = [&](VLoads<PREC, C, BATCH>& L, int j, VLoads<PREC, C, BATCH>& Ln, int jn) {
+        const int q = i * W + j;
+        f32x4 acc[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) acc[m] = bv0[m];
+        float sv[9];
+        if (l == 0) {
+          for (int t = 0; t < 9; ++t) {
+            const int r = i + t / 3 - 2, c = j + t % 3 - 1;
+            sv[t] = (r >= 0 && c >= 0 && c < W) ? spins[(r * W + c) * 16 + n] : 0.0f;
+          }
+        }
+        if (jn >= 0) vissue(Ln, jn);
+        if (l == 0) {
+          const float* kv = a.aux + lay.aux_kv0(nb);
+          for (int t = 0; t < 9; ++t) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+              acc[m] += *reinterpret_cast<const f32x4*>(kv + t * C + 16 * m + 4 * g) * sv[t];
+          }
+        } else {
 #pragma unroll UNR
           for (int t = 0; t < 9; ++t) {
             const int r = i + t / 3 - 2, c = j + t % 3 - 1;
             if (r < 0 || c < 0 || c >= W) continue;
             frag b[KS];
             if constexpr (BATCH) {
-              vd[t].template frags<KS>(b);
+              L.vd[t].template frags<KS>(b);
             } else {
-              vd[0].load(VAp(l, r * W + c));
-              vd[0].template frags<KS>(b);
+              LD vd;
+              vd.load(VAp(l, r * W + c));
+              vd.template frags<KS>(b);
             }
 #pragma unroll
             for (int m = 0; m < MT; ++m)
@@ -451,7 +530,8 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           LD vn;
           if (kind == BK_B) {
             LD vr;
-            vr.load(VAp(l - 1, q));
+            if constexpr (BATCH) vr = L.vr;
+            else vr.load(VAp(l - 1, q));
 #pragma unroll
             for (int m = 0; m < MT; ++m) vn.set(m, act4<ACT>(vr.get(m) + acc[m]));
           } else {
@@ -463,17 +543,26 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         if (i + 1 < H) {
           frag b[KS];
           d_to_b<PREC, MT, KS>(av, b);
-          const float* by = bl + 2 * C + C / 2;
           LD yd;
 #pragma unroll
           for (int m = 0; m < MTH; ++m) {
-            f32x4 y = load_bias4(by, 16 * m + 4 * g, C / 2);
+            f32x4 y = by0[m];
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks) y = P::mfma(wfrag(lay.off_y(l) + m * KS + ks), b[ks], y);
+            for (int ks = 0; ks < KS; ++ks) y = P::mfma(wy[m][ks], b[ks], y);
             yd.set(MTH + m, act4<ACT>(y));
           }
           yd.store_quads(CATp(l, q + W), MTH, MT);
         }
+      };
+      VLoads<PREC, C, BATCH> VA, VB;
+      if constexpr (BATCH) {
+        vissue(VA, 0);
+        for (int j = 0; j < W; j += 2) {
+          vstep(VA, j, VB, j + 1 < W ? j + 1 : -1);
+          if (j + 1 < W) vstep(VB, j + 1, VA, j + 2 < W ? j + 2 : -1);
+        }
+      } else {
+        for (int j = 0; j < W; ++j) vstep(VA, j, VB, -1);
       }
     }
   }
