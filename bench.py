@@ -74,6 +74,9 @@ def parse():
     ap.add_argument('--profile-grad', type=int, default=0, metavar='N',
                     help='only run the per-sample gradient kernel twice, then one batched VJP and one JVP, over '
                          'N fast samples of the headline network (a short program for rocprofv3) and exit')
+    ap.add_argument('--sr-large', action='store_true',
+                    help='only run one SR step of the largest configuration (20x20, depth 24, 64 channels, '
+                         '16384 fast samples, implicit S products, 50 CG iterations; minutes) and exit')
     ap.add_argument('--profile-copy', action='store_true',
                     help='only run two 256 MiB device-to-device copies (the calibration kernel for the '
                          'FETCH_SIZE / WRITE_SIZE counters) and exit')
@@ -189,7 +192,7 @@ def bench_j1j2(batch=4096, plain_samples=1024, reps=5):
     r = op.find_conn_device(sub, want_conn=True)
     lp, _ = m32.forward(r['conn'].reshape(nc * plain_samples, H * W))
     el32 = torch.empty((plain_samples, 2), dtype=torch.float64, device=spins.device)
-    _lib.call('fk_local_energy_dense', _lib.ptr(lp), _lib.ptr(r['mel']), nc, plain_samples, _lib.ptr(el32),
+    _lib.call('fk_local_energy_dense', _lib.ptr(lp), _lib.ptr(r['mel']), None, nc, plain_samples, _lib.ptr(el32),
               _lib.stream_ptr())
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t0
@@ -443,7 +446,7 @@ def bench_trained_ising(n_samples=2 ** 15, mini_batch=2 ** 12, precision='fp16')
             'forward_schedule': tune['schedule'], 'schedule_ms': [round(t, 2) for t in tune['ms']]}
 
 
-def bench_large(n_samples=1024, n_eloc=256, n_sv=512):
+def bench_large(n_samples=16384, n_eloc=256, n_sv=512):
     """The largest configuration's network and lattice: 20x20 periodic
     Heisenberg, depth 24, 64 channels, bf16.  Fast sampling of n_samples
     configurations and the ragged local energies of the first n_eloc (a
@@ -464,7 +467,10 @@ def bench_large(n_samples=1024, n_eloc=256, n_sv=512):
     x = next(fs)
     torch.cuda.synchronize()
     sps = n_samples / (time.perf_counter() - t0)
-    spins = x.reshape(n_samples, Hl * Wl)[:n_eloc].contiguous()
+    x = x.reshape(n_samples, Hl * Wl)[:max(n_eloc, n_sv)].contiguous()
+    del fs
+    torch.cuda.empty_cache()
+    spins = x[:n_eloc].contiguous()
     obs = fk.Observable(op)
     from flowket_amd import autotune
     tune = autotune.tune_forward(m, lambda: obs.local_values(m, spins), reps=1, persist=False)
@@ -493,7 +499,7 @@ def bench_large(n_samples=1024, n_eloc=256, n_sv=512):
     # one implicit S.v (JVP, then a batched VJP) over n_sv samples, and the
     # CG step it implies at 16384 samples and 50 iterations
     from flowket_amd.optimizers import _ImplicitS
-    xs = x.reshape(n_samples, Hl * Wl)[:n_sv].contiguous()
+    xs = x[:n_sv].contiguous()
     im = _ImplicitS(m, xs)
     v = torch.randn(m.n_trainable, device=xs.device)
     im.s_matvec(v, 1e-3)
@@ -508,6 +514,78 @@ def bench_large(n_samples=1024, n_eloc=256, n_sv=512):
                           'projected_cg_step_s_16384x50': round(50 * sv_ms * 16384 / n_sv / 1e3, 1)}
     del im
     torch.cuda.empty_cache()
+    return out
+
+
+def bench_sr_large(n_samples=16384, chunk=4096, shift=1e-3, tol=1e-6, max_iter=50, lr=0.01):
+    """One SR step of the largest configuration: 20x20 periodic Heisenberg,
+    depth 24, 64 channels (4.1M parameters), bf16 network, 16384 fast
+    samples, local energies (row reuse, in chunks of 4096 samples), then CG
+    on S = Re[(O-Obar)^H (O-Obar)]/B + shift without forming O (each S.v one
+    forward-mode JVP and one batched VJP over all samples, FP32 MFMA),
+    tol 1e-6, at most 50 iterations; the update is applied.  Prints a
+    progress line per CG iteration."""
+    import torch
+    import flowket_amd as fk
+    from flowket_amd.optimizers import _ImplicitS, _complex_eloc_pairs, conjugate_gradient
+    Hl = Wl = 20
+    m = fk.ConvNetAutoregressive2D((Hl, Wl), depth=24, num_of_channels=64, activation=ACT,
+                                   weights_normalization=False, precision='bf16', seed=99)
+    op = fk.Heisenberg(hilbert_state_shape=(Hl, Wl), pbc=True)
+    obs = fk.Observable(op)
+    out = {'workload': '20x20 periodic Heisenberg, depth 24, 64 channels, bf16 network, %d fast samples, SR shift %g, '
+                       'CG tol %g max %d, implicit S products (FP32 MFMA)' % (n_samples, shift, tol, max_iter),
+           'params': m.n_params}
+
+    def tick(msg):
+        print(json.dumps({'progress': msg, 't': round(time.perf_counter() - t_start, 1)}), flush=True)
+
+    t_start = time.perf_counter()
+    fs = fk.FastAutoregressiveSampler(m, n_samples, seed=5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x = next(fs).reshape(n_samples, Hl * Wl).contiguous()
+    torch.cuda.synchronize()
+    out['sample_s'] = round(time.perf_counter() - t0, 2)
+    tick('sampled')
+    t0 = time.perf_counter()
+    els = []
+    for c0 in range(0, n_samples, chunk):
+        els.append(obs.local_values(m, x[c0:c0 + chunk].contiguous()))
+        torch.cuda.synchronize()
+        tick('local energies %d' % (c0 + chunk))
+    el = torch.cat(els)
+    out['local_energy_s'] = round(time.perf_counter() - t0, 2)
+    E = el.mean()
+    out['energy_per_site'] = float(E.real) / (Hl * Wl)
+    m._rows_ws = {}
+    torch.cuda.empty_cache()
+    t0 = time.perf_counter()
+    im = _ImplicitS(m, x)
+    u = _complex_eloc_pairs(el, E)
+    force = im.force(u.to(x.device))
+    torch.cuda.synchronize()
+    out['force_s'] = round(time.perf_counter() - t0, 2)
+    tick('force')
+    calls = [0]
+
+    def apply(v, z):
+        im.s_matvec(v, shift, z)
+        calls[0] += 1
+        torch.cuda.synchronize()
+        tick('S.v %d' % calls[0])
+
+    t0 = time.perf_counter()
+    sol, it, res = conjugate_gradient(apply, force, tol=tol, max_iter=max_iter, check_every=1)
+    torch.cuda.synchronize()
+    cg_s = time.perf_counter() - t0
+    fnorm = float(torch.linalg.vector_norm(force.double()).item())
+    m.trainable.add_(sol, alpha=-lr)
+    m.mark_params_updated()
+    out.update({'cg_s': round(cg_s, 2), 'cg_iterations': it, 'cg_relative_residual': res / fnorm if fnorm else 0.0,
+                'matvec_s': round(cg_s / max(calls[0], 1), 2),
+                'sr_step_s': round(out['force_s'] + cg_s, 2),
+                'total_step_s': round(out['sample_s'] + out['local_energy_s'] + out['force_s'] + cg_s, 2)})
     return out
 
 
@@ -570,6 +648,9 @@ def main():
                                args.row_first)
     if args.profile_grad:
         return profile_grad(args.profile_grad)
+    if args.sr_large:
+        print(json.dumps({'sr_large': bench_sr_large()}))
+        return 0
     if args.gpus != 1:
         print(json.dumps({'error': 'the VMC path here runs on one GPU (samples shard across GPUs by design, '
                                    'but no multi-GPU launch is implemented); --gpus must be 1'}))
@@ -637,7 +718,7 @@ def main():
             logpsi = m16.log_psi_connected(spins, pairs, pairs.shape[0], op, m_dev=offsets[B:])
         e1.record()
         eloc = torch.empty((B, 2), dtype=torch.float64, device=dev)
-        _lib.call('fk_local_energy', _lib.ptr(offsets), _lib.ptr(pairs), _lib.ptr(logpsi), _lib.ptr(r['mel']),
+        _lib.call('fk_local_energy', _lib.ptr(offsets), _lib.ptr(pairs), _lib.ptr(logpsi), _lib.ptr(r['mel']), None,
                   B, _lib.ptr(eloc), _lib.stream_ptr())
         mom = torch.empty(4, dtype=torch.float64, device=dev)
         _lib.call('fk_energy_moments', _lib.ptr(eloc), B, _lib.ptr(mom), _lib.stream_ptr())
@@ -727,7 +808,7 @@ def main():
         flat = r['conn'].reshape(nc * B, H * W)
         lp, _ = m32.forward(flat)
         eloc32 = torch.empty((B, 2), dtype=torch.float64, device=dev)
-        _lib.call('fk_local_energy_dense', _lib.ptr(lp), _lib.ptr(r['mel']), nc, B, _lib.ptr(eloc32),
+        _lib.call('fk_local_energy_dense', _lib.ptr(lp), _lib.ptr(r['mel']), None, nc, B, _lib.ptr(eloc32),
                   _lib.stream_ptr())
         torch.cuda.synchronize()
         dt32 = time.perf_counter() - t0

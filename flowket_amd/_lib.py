@@ -36,8 +36,8 @@ SIGNATURES = {
                                     _P]),
     'fk_compact_scratch_bytes': (_I64, [_I64]),
     'fk_compact_connections': (ctypes.c_int, [_P, _I64, _I32, _P, _P, _P, _P]),
-    'fk_local_energy': (ctypes.c_int, [_P, _P, _P, _P, _I64, _P, _P]),
-    'fk_local_energy_dense': (ctypes.c_int, [_P, _P, _I32, _I64, _P, _P]),
+    'fk_local_energy': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _P, _P]),
+    'fk_local_energy_dense': (ctypes.c_int, [_P, _P, _P, _I32, _I64, _P, _P]),
     'fk_energy_moments': (ctypes.c_int, [_P, _I64, _P, _P]),
     'fk_sigma_z': (ctypes.c_int, [_P, _I64, _I32, _I32, _P, _P]),
     'fk_convar_packed_sizes': (ctypes.c_int, [_I32, _I32, _I32, _P, _P]),

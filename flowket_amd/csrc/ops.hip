@@ -400,7 +400,8 @@ __device__ __forceinline__ void cexp_diff(float dr, float di, double& er, double
 __global__ __launch_bounds__(256) void local_energy_kernel(const int64_t* __restrict__ offsets,
                                                            const int32_t* __restrict__ pairs,
                                                            const float* __restrict__ logpsi,
-                                                           const double* __restrict__ mel, int64_t batch,
+                                                           const double* __restrict__ mel,
+                                                           const double* __restrict__ mel_im, int64_t batch,
                                                            double* __restrict__ eloc) {
   const int lane = threadIdx.x & 63;
   const int64_t b = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -414,8 +415,9 @@ __global__ __launch_bounds__(256) void local_energy_kernel(const int64_t* __rest
       double er, ei;
       cexp_diff(logpsi[2 * k] - r0, logpsi[2 * k + 1] - i0, er, ei);
       const double m = mel[(int64_t)c * batch + b];
-      accr += m * er;
-      acci += m * ei;
+      const double mi = mel_im ? mel_im[(int64_t)c * batch + b] : 0.0;
+      accr += m * er - mi * ei;
+      acci += m * ei + mi * er;
     }
   }
 #pragma unroll
@@ -430,7 +432,8 @@ __global__ __launch_bounds__(256) void local_energy_kernel(const int64_t* __rest
 }
 
 __global__ void local_energy_dense_kernel(const float* __restrict__ logpsi, const double* __restrict__ mel,
-                                          int n_conn, int64_t batch, double* __restrict__ eloc) {
+                                          const double* __restrict__ mel_im, int n_conn, int64_t batch,
+                                          double* __restrict__ eloc) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= batch) return;
   const float r0 = logpsi[2 * b], i0 = logpsi[2 * b + 1];
@@ -439,8 +442,9 @@ __global__ void local_energy_dense_kernel(const float* __restrict__ logpsi, cons
     const int64_t k = (int64_t)c * batch + b;
     double er, ei;
     cexp_diff(logpsi[2 * k] - r0, logpsi[2 * k + 1] - i0, er, ei);
-    accr += mel[k] * er;
-    acci += mel[k] * ei;
+    const double mi = mel_im ? mel_im[k] : 0.0;
+    accr += mel[k] * er - mi * ei;
+    acci += mel[k] * ei + mi * er;
   }
   eloc[2 * b] = accr;
   eloc[2 * b + 1] = acci;
@@ -614,21 +618,21 @@ int fk_bucket_connections(const uint8_t* use, int64_t batch, int32_t n_conn, con
 }
 
 int fk_local_energy(const int64_t* offsets, const int32_t* pairs, const float* logpsi, const double* mel,
-                    int64_t batch, double* eloc_out, void* stream) {
+                    const double* mel_im, int64_t batch, double* eloc_out, void* stream) {
   FK_REQUIRE(offsets && pairs && logpsi && mel && eloc_out, "fk_local_energy: null pointer");
   if (batch <= 0) return FK_OK;
   hipLaunchKernelGGL(local_energy_kernel, dim3(div_up(batch, 4)), dim3(256), 0, (hipStream_t)stream, offsets,
-                     pairs, logpsi, mel, batch, eloc_out);
+                     pairs, logpsi, mel, mel_im, batch, eloc_out);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }
 
-int fk_local_energy_dense(const float* logpsi, const double* mel, int32_t n_conn, int64_t batch, double* eloc_out,
-                          void* stream) {
+int fk_local_energy_dense(const float* logpsi, const double* mel, const double* mel_im, int32_t n_conn,
+                          int64_t batch, double* eloc_out, void* stream) {
   FK_REQUIRE(logpsi && mel && eloc_out, "fk_local_energy_dense: null pointer");
   if (batch <= 0) return FK_OK;
   hipLaunchKernelGGL(local_energy_dense_kernel, dim3(div_up(batch, 256)), dim3(256), 0, (hipStream_t)stream,
-                     logpsi, mel, n_conn, batch, eloc_out);
+                     logpsi, mel, mel_im, n_conn, batch, eloc_out);
   FK_CHECK_LAUNCH();
   return FK_OK;
 }

@@ -66,8 +66,8 @@ class LocalEnergyGraph:
             logpsi = m.log_psi_connected(self.spins, pairs, pairs.shape[0], op, m_dev=offsets[B:], **kw)
         eloc = torch.empty((B, 2), dtype=torch.float64, device=self.spins.device)
         s = _lib.stream_ptr()
-        _lib.call('fk_local_energy', _lib.ptr(offsets), _lib.ptr(pairs), _lib.ptr(logpsi), _lib.ptr(r['mel']), B,
-                  _lib.ptr(eloc), s)
+        _lib.call('fk_local_energy', _lib.ptr(offsets), _lib.ptr(pairs), _lib.ptr(logpsi), _lib.ptr(r['mel']), None,
+                  B, _lib.ptr(eloc), s)
         mom = torch.empty(4, dtype=torch.float64, device=self.spins.device)
         _lib.call('fk_energy_moments', _lib.ptr(eloc), B, _lib.ptr(mom), s)
         self.offsets = offsets

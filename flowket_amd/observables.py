@@ -62,7 +62,7 @@ class Observable(BaseObservable):
         spins = self._spins(configurations)
         B = spins.shape[0]
         op = self.operator
-        if hasattr(wave_function, 'log_psi_connected'):
+        if hasattr(wave_function, 'log_psi_connected') and hasattr(op, 'compact'):
             r = op.compact(spins, stream=stream)
             offsets, pairs = r['offsets'], r['pairs']
             upper = pairs.shape[0]
@@ -72,13 +72,16 @@ class Observable(BaseObservable):
                 logpsi = wave_function.log_psi_connected(spins, pairs, upper, op, m_dev=offsets[B:], stream=stream)
             eloc = torch.empty((B, 2), dtype=torch.float64, device=spins.device)
             _lib.call('fk_local_energy', _lib.ptr(offsets), _lib.ptr(pairs), _lib.ptr(logpsi), _lib.ptr(r['mel']),
-                      B, _lib.ptr(eloc), _lib.stream_ptr(stream))
+                      _lib.ptr(r.get('mel_im')), B, _lib.ptr(eloc), _lib.stream_ptr(stream))
             self.last_num_connections = offsets[B:]
             return torch.complex(eloc[:, 0], eloc[:, 1])
         return self.local_values_balanced(wave_function, spins, stream=stream)
 
     def local_values_balanced(self, wave_function, spins, stream=None):
-        """operator.py:35-42: every slot evaluated (mel = 0 for unused)."""
+        """operator.py:35-42: every slot evaluated (mel = 0 for unused).
+        Operators without a connection table (a subclass of Operator with a
+        numpy find_conn) come here too; complex matrix elements keep their
+        imaginary parts."""
         op = self.operator
         r = op.find_conn_device(spins, want_conn=True, stream=stream)
         nc, B = r['mel'].shape
@@ -87,8 +90,8 @@ class Observable(BaseObservable):
         lp = lp.reshape(-1)
         lp = torch.view_as_real(lp.to(torch.complex64)).contiguous()
         eloc = torch.empty((B, 2), dtype=torch.float64, device=spins.device)
-        _lib.call('fk_local_energy_dense', _lib.ptr(lp), _lib.ptr(r['mel']), nc, B, _lib.ptr(eloc),
-                  _lib.stream_ptr(stream))
+        _lib.call('fk_local_energy_dense', _lib.ptr(lp), _lib.ptr(r['mel']), _lib.ptr(r.get('mel_im')), nc, B,
+                  _lib.ptr(eloc), _lib.stream_ptr(stream))
         return torch.complex(eloc[:, 0], eloc[:, 1])
 
 

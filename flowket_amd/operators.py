@@ -38,6 +38,37 @@ class Operator(abc.ABC):
     def random_states(self, num_of_states):
         return np.random.choice([-1, 1], size=(num_of_states,) + self.hilbert_state_shape)
 
+    @property
+    def n_sites(self):
+        return int(np.prod(self.hilbert_state_shape))
+
+    def find_conn_device(self, spins, want_conn=True, stream=None):
+        """For an operator defined only by a numpy ``find_conn`` (a subclass
+        of this class, as in operators/operator.py:14-21, or a wrapper of a
+        netket operator, operators/netket_operator.py:44-62): the subclass's
+        find_conn on the host, its (conn, mel, use) uploaded as the device
+        layout of fk_find_conn -- conn (Nc, B, N) int8, mel (Nc, B) float64
+        real parts with unused slots zeroed, use (Nc, B) uint8 -- and, when
+        any element is complex, mel_im (Nc, B) float64 imaginary parts."""
+        B = spins.shape[0]
+        x = spins.detach().to('cpu').numpy().reshape((B,) + self.hilbert_state_shape)
+        conn, mel, use = self.find_conn(x)
+        mel = np.asarray(mel)
+        use = np.asarray(use, dtype=bool)
+        nc = mel.shape[0]
+        if mel.shape != (nc, B) or use.shape != (nc, B):
+            raise ValueError(f'find_conn returned mel {mel.shape} / use {use.shape}, expected ({nc}, {B})')
+        m = np.where(use, mel, 0)
+        dev = spins.device
+        out = {'mel': torch.as_tensor(np.ascontiguousarray(np.real(m), dtype=np.float64)).to(dev),
+               'use': torch.as_tensor(use.astype(np.uint8)).to(dev)}
+        if np.iscomplexobj(m) and np.any(np.imag(m) != 0):
+            out['mel_im'] = torch.as_tensor(np.ascontiguousarray(np.imag(m), dtype=np.float64)).to(dev)
+        if want_conn:
+            c = np.asarray(conn).reshape(nc, B, -1)
+            out['conn'] = torch.as_tensor(np.ascontiguousarray(c).astype(np.int8)).to(dev)
+        return out
+
 
 class OperatorOnGrid(Operator, abc.ABC):
     """operators/operator.py:40-46."""

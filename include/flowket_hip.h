@@ -117,16 +117,19 @@ int fk_bucket_connections(const uint8_t* use, int64_t batch, int32_t n_conn,
  *   sum_k mel[c_k, b] * exp(logpsi[k] - logpsi[offsets[b]])
  * over k in [offsets[b], offsets[b+1]), difference and exp in complex64 as the
  * reference, product and sum in complex128.  logpsi (total, 2) float (re, im);
- * eloc_out (batch, 2) double. */
+ * eloc_out (batch, 2) double.  mel (n_conn, batch) holds the real parts of
+ * the matrix elements; mel_im, when not NULL, their imaginary parts (the
+ * complex128 elements of operators/netket_operator.py:56-60). */
 int fk_local_energy(const int64_t* offsets, const int32_t* pairs,
-                    const float* logpsi, const double* mel, int64_t batch,
-                    double* eloc_out, void* stream);
+                    const float* logpsi, const double* mel,
+                    const double* mel_im, int64_t batch, double* eloc_out,
+                    void* stream);
 
 /* Balanced path (operator.py:35-42): logpsi (n_conn, batch, 2) for every
- * slot, eloc_out (batch, 2). */
+ * slot, eloc_out (batch, 2); mel / mel_im as for fk_local_energy. */
 int fk_local_energy_dense(const float* logpsi, const double* mel,
-                          int32_t n_conn, int64_t batch, double* eloc_out,
-                          void* stream);
+                          const double* mel_im, int32_t n_conn, int64_t batch,
+                          double* eloc_out, void* stream);
 
 /* BaseObservable.estimate (observables/monte_carlo/observable.py:10-14):
  * out[0..3] = (Re mean, Im mean, population variance of Re, batch) in double,

@@ -86,27 +86,37 @@ def test_autoregressive_sampler_matches_oracle_rule(rng):
 def test_reduction_kernels_on_reference_fixture():
     """The E_loc reductions (ragged fk_local_energy over the compacted use
     mask, and fk_local_energy_dense over every slot) on the reference's
-    unbalanced/balanced fixture (tests/test_variational.py:18-46) with the
-    log psi of a linear complex wave function."""
+    unbalanced/balanced fixture (tests/test_variational.py:18-46), whose
+    matrix elements are complex, with the log psi of a linear complex wave
+    function: real and imaginary parts of mel both enter (mel / mel_im)."""
     from flowket_amd import _lib
     from tests.test_oracle import reference_fixture
-    gpu_banner('E_loc reduction kernels on the reference fixture')
+    gpu_banner('E_loc reduction kernels on the reference fixture (complex mel)')
     conn, mel, use, w = reference_fixture(np.random.default_rng(3))
+    assert np.iscomplexobj(mel) and np.any(mel.imag != 0)
     nc, B, n = conn.shape
     lp = (conn.reshape(nc * B, n) @ w).astype(np.complex64)          # (nc*B,)
-    ref = oob.local_values_unbalanced(lambda x: (x.reshape(x.shape[0], -1) @ w).astype(np.complex64),
-                                      conn, mel, use)
+
+    def psi(x):
+        return (x.reshape(x.shape[0], -1) @ w).astype(np.complex64)
+
+    ref = oob.local_values_unbalanced(psi, conn, mel, use)
+    mel_re = torch.as_tensor(np.ascontiguousarray(mel.real)).cuda()
+    mel_im = torch.as_tensor(np.ascontiguousarray(mel.imag)).cuda()
     # dense: every slot
     lpd = torch.view_as_real(torch.as_tensor(lp)).contiguous().cuda()
-    # the operators of the path have real matrix elements: the kernels carry
-    # float64 mel, so the fixture's real parts are used
-    mel_re = torch.as_tensor(np.ascontiguousarray(mel.real)).cuda()
     out = torch.empty((B, 2), dtype=torch.float64, device='cuda')
-    _lib.call('fk_local_energy_dense', _lib.ptr(lpd), _lib.ptr(mel_re), nc, B, _lib.ptr(out), _lib.stream_ptr())
+    _lib.call('fk_local_energy_dense', _lib.ptr(lpd), _lib.ptr(mel_re), _lib.ptr(mel_im), nc, B, _lib.ptr(out),
+              _lib.stream_ptr())
     got_dense = out[:, 0].cpu().numpy() + 1j * out[:, 1].cpu().numpy()
-    ref_re = oob.local_values_balanced(lambda x: (x.reshape(x.shape[0], -1) @ w).astype(np.complex64), conn,
-                                       mel.real)
-    assert np.allclose(got_dense, ref_re, rtol=1e-6)
+    assert np.allclose(got_dense, oob.local_values_balanced(psi, conn, mel), rtol=1e-6)
+    # without the imaginary parts the result differs (the test would catch a
+    # dropped mel_im)
+    _lib.call('fk_local_energy_dense', _lib.ptr(lpd), _lib.ptr(mel_re), None, nc, B, _lib.ptr(out),
+              _lib.stream_ptr())
+    got_re = out[:, 0].cpu().numpy() + 1j * out[:, 1].cpu().numpy()
+    assert np.allclose(got_re, oob.local_values_balanced(psi, conn, mel.real), rtol=1e-6)
+    assert not np.allclose(got_re, got_dense, rtol=1e-3)
     # ragged: compact the use mask, evaluate only used slots
     use_d = torch.as_tensor(use.astype(np.uint8)).contiguous().cuda()
     offsets = torch.empty(B + 1, dtype=torch.int64, device='cuda')
@@ -117,12 +127,75 @@ def test_reduction_kernels_on_reference_fixture():
     assert M == int(use.sum())
     pr = pairs[:M].cpu().numpy()
     lpr = torch.view_as_real(torch.as_tensor(lp.reshape(nc, B)[pr[:, 1], pr[:, 0]])).contiguous().cuda()
-    _lib.call('fk_local_energy', _lib.ptr(offsets), _lib.ptr(pairs), _lib.ptr(lpr), _lib.ptr(mel_re), B,
-              _lib.ptr(out), _lib.stream_ptr())
+    _lib.call('fk_local_energy', _lib.ptr(offsets), _lib.ptr(pairs), _lib.ptr(lpr), _lib.ptr(mel_re),
+              _lib.ptr(mel_im), B, _lib.ptr(out), _lib.stream_ptr())
     got = out[:, 0].cpu().numpy() + 1j * out[:, 1].cpu().numpy()
-    ref_u = oob.local_values_unbalanced(lambda x: (x.reshape(x.shape[0], -1) @ w).astype(np.complex64), conn,
-                                        mel.real, use)
-    assert np.allclose(got, ref_u, rtol=1e-6)
+    assert np.allclose(got, ref, rtol=1e-6)
     # the fixture's point: balanced and unbalanced agree on the mean
     assert np.allclose(np.mean(ref), np.mean(oob.local_values_balanced(
         lambda x: (x.reshape(x.shape[0], -1) @ w).astype(np.complex64), conn, mel)))
+
+
+class _SigmaYField:
+    """H = sum_i sigma^y_i on a small lattice, an operator given only by a
+    numpy find_conn (as a subclass of Operator or a netket wrapper would be):
+    sample s connects to s with site i flipped, with matrix element
+    <s|sigma^y_i|s'> = -i s_i (sigma^y = [[0, -i], [i, 0]] on (up, down))."""
+
+    @staticmethod
+    def make(shape):
+        import flowket_amd as fk
+
+        class SigmaYField(fk.Operator):
+            def find_conn(self, sample):
+                B = sample.shape[0]
+                x = sample.reshape(B, -1)
+                n = x.shape[1]
+                conn = np.repeat(x[None], n, axis=0).astype(np.float64)
+                for i in range(n):
+                    conn[i, :, i] *= -1
+                mel = (-1j * x.T).astype(np.complex128)
+                use = np.ones((n, B), dtype=bool)
+                return conn.reshape((n, B) + sample.shape[1:]), mel, use
+
+        return SigmaYField(shape)
+
+
+def test_complex_matrix_elements_rayleigh_quotient():
+    """Observable.local_values with an operator whose matrix elements are
+    purely imaginary: sum_s |psi(s)|^2 E_loc(s) over all 2^9 states equals
+    psi^H H psi / psi^H psi with H assembled independently from Kronecker
+    products of the Pauli matrix (the real parts alone give 0)."""
+    import flowket_amd as fk
+    gpu_banner('complex mel: sigma^y field, Rayleigh quotient')
+    shape = (3, 3)
+    n = 9
+    m = fk.ConvNetAutoregressive2D(shape, depth=2, num_of_channels=16, activation='elu',
+                                   weights_normalization=False, precision='fp32', seed=5)
+    op = _SigmaYField.make(shape)
+    states = np.array([[1 if (k >> i) & 1 else -1 for i in range(n)] for k in range(2 ** n)], dtype=np.int8)
+    el = fk.Observable(op).local_values(m, states.reshape((-1,) + shape)).cpu().numpy()
+    lp, _ = m.forward(torch.as_tensor(states).cuda())
+    lp = lp.double().cpu().numpy()
+    psi = np.exp(lp[:, 0] + 1j * lp[:, 1])
+    p = np.abs(psi) ** 2
+    # H from Kronecker products; state k has site i = +1 (up, basis index 0)
+    # when bit i of k is set
+    sy = np.array([[0, -1j], [1j, 0]])
+    eye = np.eye(2)
+    Hm = np.zeros((2 ** n, 2 ** n), dtype=np.complex128)
+    for i in range(n):
+        ops = [sy if q == i else eye for q in range(n)]
+        term = ops[0]
+        for o in ops[1:]:
+            term = np.kron(term, o)
+        Hm += term
+    # Kronecker index: site 0 is the most significant factor, basis 0 = up
+    kron_index = np.array([sum((0 if states[k, q] == 1 else 1) << (n - 1 - q) for q in range(n))
+                           for k in range(2 ** n)])
+    v = np.zeros(2 ** n, dtype=np.complex128)
+    v[kron_index] = psi
+    rq = np.vdot(v, Hm @ v) / np.vdot(v, v)
+    e = np.sum(p * el) / np.sum(p)
+    assert abs(rq) > 1e-3
+    assert abs(e - rq) < 1e-5 * max(1.0, abs(rq)), (e, rq)
