@@ -49,7 +49,11 @@ class Operator(abc.ABC):
         find_conn on the host, its (conn, mel, use) uploaded as the device
         layout of fk_find_conn -- conn (Nc, B, N) int8, mel (Nc, B) float64
         real parts with unused slots zeroed, use (Nc, B) uint8 -- and, when
-        any element is complex, mel_im (Nc, B) float64 imaginary parts."""
+        any element is complex, mel_im (Nc, B) float64 imaginary parts.
+        As the reference's estimators require
+        (observables/monte_carlo/operator.py:30-42), connection 0 of every
+        sample must be the sample itself and used: its log psi is the
+        denominator of every ratio.  A find_conn that breaks this raises."""
         B = spins.shape[0]
         x = spins.detach().to('cpu').numpy().reshape((B,) + self.hilbert_state_shape)
         conn, mel, use = self.find_conn(x)
@@ -58,6 +62,10 @@ class Operator(abc.ABC):
         nc = mel.shape[0]
         if mel.shape != (nc, B) or use.shape != (nc, B):
             raise ValueError(f'find_conn returned mel {mel.shape} / use {use.shape}, expected ({nc}, {B})')
+        c0 = np.asarray(conn)[0].reshape(B, -1)
+        if not (use[0].all() and np.array_equal(c0, x.reshape(B, -1))):
+            raise ValueError('find_conn: connection 0 of every sample must be the sample itself, marked used '
+                             '(observables/monte_carlo/operator.py:30-42)')
         m = np.where(use, mel, 0)
         dev = spins.device
         out = {'mel': torch.as_tensor(np.ascontiguousarray(np.real(m), dtype=np.float64)).to(dev),

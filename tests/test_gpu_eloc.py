@@ -139,8 +139,10 @@ def test_reduction_kernels_on_reference_fixture():
 class _SigmaYField:
     """H = sum_i sigma^y_i on a small lattice, an operator given only by a
     numpy find_conn (as a subclass of Operator or a netket wrapper would be):
-    sample s connects to s with site i flipped, with matrix element
-    <s|sigma^y_i|s'> = -i s_i (sigma^y = [[0, -i], [i, 0]] on (up, down))."""
+    connection 0 is the sample itself (diagonal element 0, as the
+    reference's estimators require), connection 1 + i the sample with site i
+    flipped, with matrix element <s|sigma^y_i|s'> = -i s_i (sigma^y =
+    [[0, -i], [i, 0]] on (up, down))."""
 
     @staticmethod
     def make(shape):
@@ -151,12 +153,13 @@ class _SigmaYField:
                 B = sample.shape[0]
                 x = sample.reshape(B, -1)
                 n = x.shape[1]
-                conn = np.repeat(x[None], n, axis=0).astype(np.float64)
+                conn = np.repeat(x[None], n + 1, axis=0).astype(np.float64)
                 for i in range(n):
-                    conn[i, :, i] *= -1
-                mel = (-1j * x.T).astype(np.complex128)
-                use = np.ones((n, B), dtype=bool)
-                return conn.reshape((n, B) + sample.shape[1:]), mel, use
+                    conn[1 + i, :, i] *= -1
+                mel = np.zeros((n + 1, B), dtype=np.complex128)
+                mel[1:] = -1j * x.T
+                use = np.ones((n + 1, B), dtype=bool)
+                return conn.reshape((n + 1, B) + sample.shape[1:]), mel, use
 
         return SigmaYField(shape)
 
