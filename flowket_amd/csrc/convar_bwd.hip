@@ -45,6 +45,7 @@ struct BwdArgs {
   float* jvp;           // JVP: (batch, 2) output
   const float* kt;      // the conv kernels transposed (transpose_kernels_kernel), P floats
   const float* tkt;     // JVP: the tangent's conv kernels transposed
+  int stage;            // LDS row stride (floats) of the two staging tiles, 0: no staging
 };
 
 // Offsets of layer parameters in the flat vector.
@@ -528,6 +529,20 @@ __global__ void transpose_kernels_kernel(const float* __restrict__ prm, float* _
   }
 }
 
+// Copies n channels (from channel off, row stride cs) of an HW-pixel tensor
+// into an LDS tile of row stride SC (C + 4 floats: consecutive pixels start
+// 36 banks apart, so 16 lanes reading 16 bytes of 16 pixels do not
+// conflict).  The 3x3 convolutions, their data adjoints and weight gradients
+// then read their tap-shifted operands from LDS instead of L2.
+__device__ __forceinline__ void stage_in(float* dst, int SC, const float* src, int cs, int off, int n, int HW) {
+  const int q4 = n / 4;
+  for (int idx = threadIdx.x; idx < HW * q4; idx += blockDim.x) {
+    const int p = idx / q4, c = idx - p * q4;
+    *reinterpret_cast<f32x4*>(dst + p * SC + 4 * c) =
+        *reinterpret_cast<const f32x4*>(src + (int64_t)p * cs + off + 4 * c);
+  }
+}
+
 // out[p][co_off + co] = bias[co] + sum_{tap, ci} K[tap][ci][co] in[src(tap, p)][ci],
 // the kernel given transposed (KT[tap][co][ci], transpose_kernels_kernel)
 __device__ void mconv_fwd(float* out, int cs_out, int co_off, const float* in, int cs_in, int cin, int cout,
@@ -687,10 +702,12 @@ __device__ void build_tap_tables(int16_t* ftab, int16_t* itab, int H, int W) {
 // tensor is carried along in wt (same layout as ws): for y = K*x + b,
 // y' = K*x' + Kt*x + bt, and act(y)' = act'(y) y'.  t0/t1 and u0/u1 are
 // scratch tensors of the primal and the tangent.
-template <int ACT>
+// STG: the 3x3 convolutions read their inputs from the LDS staging tile S0
+// (row stride SC); otherwise from L2.
+template <int ACT, bool STG>
 __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan, const float* kt, const float* tkt,
-                                             float* ws, float* wt, const int16_t* ftab, int H,
-                             int W, int C, int nb, float* t0, float* t1, float* u0, float* u1) {
+                                              float* ws, float* wt, const int16_t* ftab, int H, int W, int C, int nb,
+                                              float* t0, float* t1, float* u0, float* u1, float* S0, int SC) {
   const int HW = H * W, F = nb - 1;
   const float* sp = ws;
   for (int blk = 0; blk < nb; ++blk) {
@@ -712,7 +729,13 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
         conv_fwd(u1, C, Hin, 1, 1, C, tan + o.kh, tan + o.bh, 1, 3, H, W);
       }
     } else {
-      mconv_fwd(t0, C, 0, Vin, C, C, C, kt + o.kv, prm + o.bv, ftab + GT_V * HW, 9, HW);
+      const float* Vb = STG ? S0 : Vin;
+      const int vs = STG ? SC : C;
+      if constexpr (STG) {
+        stage_in(S0, SC, Vin, C, 0, C, HW);
+        __syncthreads();
+      }
+      mconv_fwd(t0, C, 0, Vb, vs, C, C, kt + o.kv, prm + o.bv, ftab + GT_V * HW, 9, HW);
       mconv_fwd(t1, C, 0, Hin, C, C, C, kt + o.kh, prm + o.bh, ftab + GT_H * HW, 3, HW);
       if (tan) {
         const float* Vd = ws_tensor(wt, HW, C, blk - 1, T_VO);
@@ -720,7 +743,7 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
         mconv_fwd(u0, C, 0, Vd, C, C, C, kt + o.kv, tan + o.bv, ftab + GT_V * HW, 9, HW);
         mconv_fwd(u1, C, 0, Hd, C, C, C, kt + o.kh, tan + o.bh, ftab + GT_H * HW, 3, HW);
         __syncthreads();
-        mconv_fwd(u0, C, 0, Vin, C, C, C, tkt + o.kv, nullptr, ftab + GT_V * HW, 9, HW, true);
+        mconv_fwd(u0, C, 0, Vb, vs, C, C, tkt + o.kv, nullptr, ftab + GT_V * HW, 9, HW, true);
         mconv_fwd(u1, C, 0, Hin, C, C, C, tkt + o.kh, nullptr, ftab + GT_H * HW, 3, HW, true);
       }
     }
@@ -765,12 +788,18 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
       if (tan) ws_tensor(wt, HW, C, blk, T_CAT)[idx] *= bdact<ACT>(c);
     }
     __syncthreads();
-    mconv_fwd(t0, C, 0, CAT, C, C, C, kt + o.kc, prm + o.bc, ftab + GT_C * HW, 9, HW);
+    const float* Cb = STG ? S0 : CAT;
+    const int cs = STG ? SC : C;
+    if constexpr (STG) {
+      stage_in(S0, SC, CAT, C, 0, C, HW);
+      __syncthreads();
+    }
+    mconv_fwd(t0, C, 0, Cb, cs, C, C, kt + o.kc, prm + o.bc, ftab + GT_C * HW, 9, HW);
     if (tan) {
       mconv_fwd(u0, C, 0, ws_tensor(wt, HW, C, blk, T_CAT), C, C, C, kt + o.kc, tan + o.bc, ftab + GT_C * HW, 9,
                 HW);
       __syncthreads();
-      mconv_fwd(u0, C, 0, CAT, C, C, C, tkt + o.kc, nullptr, ftab + GT_C * HW, 9, HW, true);
+      mconv_fwd(u0, C, 0, Cb, cs, C, C, tkt + o.kc, nullptr, ftab + GT_C * HW, 9, HW, true);
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
@@ -797,10 +826,11 @@ __device__ __forceinline__ void head_logits(const float* prm, int64_t ho, const 
 
 // Backward of one sample from the head cotangent dz (HW x 4): d/d theta into
 // O (overwritten, or added to when accumulate).
-template <int ACT, bool ACC>
+template <int ACT, bool ACC, bool STG>
 __device__ __forceinline__ void mfma_backward(const float* prm, float* ws, const int16_t* ftab, const int16_t* itab,
                                               const float* dz, int H, int W, int C, int nb, float* O, float* dV,
-                                              float* dH, float* dVr, float* dHr, float* t0, float* t1) {
+                                              float* dH, float* dVr, float* dHr, float* t0, float* t1, float* S0,
+                                              float* S1, int SC) {
   constexpr bool accumulate = ACC;
   const int HW = H * W, F = nb - 1;
   const float* sp = ws;
@@ -850,8 +880,18 @@ __device__ __forceinline__ void mfma_backward(const float* prm, float* ws, const
       }
     }
     __syncthreads();
-    mconv_bwd_weight(O + o.kc, O + o.bc, CAT, C, t1, C, 0, C, C, ftab + GT_C * HW, 9, HW, accumulate);
-    mconv_bwd_data(dV, C, t1, C, 0, C, C, prm + o.kc, itab + GT_C * HW, 9, HW);
+    {
+      const float* ib = STG ? S1 : CAT;
+      const float* db = STG ? S0 : t1;
+      const int is = STG ? SC : C, ds = STG ? SC : C;
+      if constexpr (STG) {
+        stage_in(S1, SC, CAT, C, 0, C, HW);
+        stage_in(S0, SC, t1, C, 0, C, HW);
+        __syncthreads();
+      }
+      mconv_bwd_weight(O + o.kc, O + o.bc, ib, is, db, ds, 0, C, C, ftab + GT_C * HW, 9, HW, accumulate);
+      mconv_bwd_data(dV, C, db, ds, 0, C, C, prm + o.kc, itab + GT_C * HW, 9, HW);
+    }
     __syncthreads();
     for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) dV[idx] *= bdact<ACT>(CAT[idx]);
     __syncthreads();
@@ -871,10 +911,20 @@ __device__ __forceinline__ void mfma_backward(const float* prm, float* ws, const
       conv_bwd_weight(O + o.kv, O + o.bv, Vin, 1, t0, C, 1, C, 0, 9, H, W, accumulate);
       break;
     }
-    mconv_bwd_weight(O + o.kh, O + o.bh, Hin, C, dH, C, 0, C, C, ftab + GT_H * HW, 3, HW, accumulate);
-    mconv_bwd_weight(O + o.kv, O + o.bv, Vin, C, t0, C, 0, C, C, ftab + GT_V * HW, 9, HW, accumulate);
-    mconv_bwd_data(t1, C, dH, C, 0, C, C, prm + o.kh, itab + GT_H * HW, 3, HW);
-    mconv_bwd_data(dV, C, t0, C, 0, C, C, prm + o.kv, itab + GT_V * HW, 9, HW);
+    {
+      const float* ib = STG ? S1 : Vin;
+      const float* db = STG ? S0 : t0;
+      const int is = STG ? SC : C, ds = STG ? SC : C;
+      if constexpr (STG) {
+        stage_in(S1, SC, Vin, C, 0, C, HW);
+        stage_in(S0, SC, t0, C, 0, C, HW);
+        __syncthreads();
+      }
+      mconv_bwd_weight(O + o.kh, O + o.bh, Hin, C, dH, C, 0, C, C, ftab + GT_H * HW, 3, HW, accumulate);
+      mconv_bwd_weight(O + o.kv, O + o.bv, ib, is, db, ds, 0, C, C, ftab + GT_V * HW, 9, HW, accumulate);
+      mconv_bwd_data(t1, C, dH, C, 0, C, C, prm + o.kh, itab + GT_H * HW, 3, HW);
+      mconv_bwd_data(dV, C, db, ds, 0, C, C, prm + o.kv, itab + GT_V * HW, 9, HW);
+    }
     __syncthreads();
     for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
       float gh = t1[idx], gv = dV[idx];
@@ -897,7 +947,8 @@ enum { GRAD_O = 0, GRAD_VJP = 1 };
 // each workgroup walks samples b = blockIdx.x, blockIdx.x + gridDim.x, ...
 // and adds cot_re[b] d Re log psi_b + cot_im[b] d Im log psi_b (one backward
 // pass with the combined head cotangent) into its own partial gradient row.
-template <int ACT, int MODE>
+// STG: staging tiles in LDS (a.stage is their row stride).
+template <int ACT, int MODE, bool STG>
 __global__ __launch_bounds__(256, MODE == GRAD_VJP ? 3 : 4) void convar_grad_mfma_kernel(BwdArgs a) {
   extern __shared__ int16_t tabs[];  // GT_N forward rows, then GT_N inverse rows, of HW entries
   __shared__ float s_lp[2];
@@ -908,6 +959,10 @@ __global__ __launch_bounds__(256, MODE == GRAD_VJP ? 3 : 4) void convar_grad_mfm
   int16_t* ftab = tabs;
   int16_t* itab = tabs + GT_N * HW;
   float* dz = reinterpret_cast<float*>(tabs + 2 * GT_N * HW + (2 * GT_N * HW & 1));
+  // staging tiles after dz (4 HW floats; 16-byte aligned: 2 GT_N HW int16
+  // is a multiple of 16 bytes)
+  float* S0 = dz + 4 * HW;
+  float* S1 = S0 + HW * a.stage;
   float* dV = ws_grad(ws, HW, C, nb, 0);
   float* dH = ws_grad(ws, HW, C, nb, 1);
   float* dVr = ws_grad(ws, HW, C, nb, 2);
@@ -928,7 +983,8 @@ __global__ __launch_bounds__(256, MODE == GRAD_VJP ? 3 : 4) void convar_grad_mfm
     for (int q = threadIdx.x; q < HW; q += blockDim.x) sp[q] = (float)a.spins[b * HW + q];
     if (threadIdx.x == 0) s_lp[0] = s_lp[1] = 0.0f;
     __syncthreads();
-    mfma_forward<ACT>(prm, nullptr, a.kt, nullptr, ws, nullptr, ftab, H, W, C, nb, t0, t1, nullptr, nullptr);
+    mfma_forward<ACT, STG>(prm, nullptr, a.kt, nullptr, ws, nullptr, ftab, H, W, C, nb, t0, t1, nullptr, nullptr,
+                           S0, a.stage);
     const int64_t ho = head_offset(nb, C);
     const float* HF = ws_tensor(ws, HW, C, F, T_HO);
     const float cr = MODE == GRAD_VJP ? a.cot[2 * b] : 1.0f;
@@ -955,7 +1011,8 @@ __global__ __launch_bounds__(256, MODE == GRAD_VJP ? 3 : 4) void convar_grad_mfm
       }
       __syncthreads();
       float* O = MODE == GRAD_VJP ? part : (part_i == 0 ? a.o_re : a.o_im) + b * a.ld;
-      mfma_backward<ACT, MODE == GRAD_VJP>(prm, ws, ftab, itab, dz, H, W, C, nb, O, dV, dH, dVr, dHr, t0, t1);
+      mfma_backward<ACT, MODE == GRAD_VJP, STG>(prm, ws, ftab, itab, dz, H, W, C, nb, O, dV, dH, dVr, dHr, t0, t1,
+                                                S0, S1, a.stage);
     }
     if (threadIdx.x == 0 && a.logpsi) {
       a.logpsi[2 * b] = s_lp[0];
@@ -968,7 +1025,7 @@ __global__ __launch_bounds__(256, MODE == GRAD_VJP ? 3 : 4) void convar_grad_mfm
 // Forward-mode derivative (JVP) of log psi along a parameter direction:
 // out[b] = (d Re log psi_b, d Im log psi_b) . tan, a persistent grid of
 // workgroups over the samples, primal and tangent tensors in the workspace.
-template <int ACT>
+template <int ACT, bool STG>
 __global__ __launch_bounds__(256, 4) void convar_jvp_mfma_kernel(BwdArgs a) {
   extern __shared__ int16_t tabs[];
   __shared__ float s_acc[4];
@@ -978,6 +1035,7 @@ __global__ __launch_bounds__(256, 4) void convar_jvp_mfma_kernel(BwdArgs a) {
   float* wt = ws + a.work_floats / 2;  // tangent region, same layout
   int16_t* ftab = tabs;
   int16_t* itab = tabs + GT_N * HW;
+  float* S0 = reinterpret_cast<float*>(tabs + 2 * GT_N * HW) + 4 * HW;
   float* t0 = ws_grad(ws, HW, C, nb, 0);
   float* t1 = ws_grad(ws, HW, C, nb, 1);
   float* u0 = ws_grad(ws, HW, C, nb, 2);
@@ -990,7 +1048,7 @@ __global__ __launch_bounds__(256, 4) void convar_jvp_mfma_kernel(BwdArgs a) {
     for (int q = threadIdx.x; q < HW; q += blockDim.x) sp[q] = (float)a.spins[b * HW + q];
     if (threadIdx.x < 4) s_acc[threadIdx.x] = 0.0f;
     __syncthreads();
-    mfma_forward<ACT>(prm, tan, a.kt, a.tkt, ws, wt, ftab, H, W, C, nb, t0, t1, u0, u1);
+    mfma_forward<ACT, STG>(prm, tan, a.kt, a.tkt, ws, wt, ftab, H, W, C, nb, t0, t1, u0, u1, S0, a.stage);
     const int64_t ho = head_offset(nb, C);
     const float* HF = ws_tensor(ws, HW, C, F, T_HO);
     const float* HFd = ws_tensor(wt, HW, C, F, T_HO);
@@ -1060,6 +1118,18 @@ size_t grad_lds_bytes(int HW) {
   return (size_t)2 * GT_N * HW * sizeof(int16_t) + 4 + (size_t)4 * HW * sizeof(float);
 }
 
+// Staging tiles (two of HW x (C + 4) floats) when they keep a workgroup
+// within a quarter of the CU's LDS (four workgroups per CU): the stride, or 0.
+constexpr size_t GRAD_LDS_PER_WG = 40 * 1024;
+int stage_stride(int HW, int C) {
+  const int SC = C + 4;
+  const size_t b = grad_lds_bytes(HW) + (size_t)2 * HW * SC * sizeof(float);
+  return (C % 4 == 0 && b <= GRAD_LDS_PER_WG) ? SC : 0;
+}
+size_t grad_lds_total(int HW, int stage) {
+  return grad_lds_bytes(HW) + (stage ? (size_t)2 * HW * stage * sizeof(float) : 0);
+}
+
 template <class K>
 int launch_with_lds(K kern, int64_t grid, size_t lds, hipStream_t s, const BwdArgs& a) {
   FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1072,6 +1142,12 @@ int launch_with_lds(K kern, int64_t grid, size_t lds, hipStream_t s, const BwdAr
 // Floats reserved at the start of a workspace for transposed kernels (P
 // rounded up to 64 floats, so what follows stays 256-byte aligned).
 int64_t kt_floats(int C, int nb) { return (flat_param_count(C, nb) + 63) / 64 * 64; }
+
+template <int ACT, int MODE>
+int launch_grad(int64_t grid, size_t lds, hipStream_t s, const BwdArgs& a) {
+  if (a.stage) return launch_with_lds(convar_grad_mfma_kernel<ACT, MODE, true>, grid, lds, s, a);
+  return launch_with_lds(convar_grad_mfma_kernel<ACT, MODE, false>, grid, lds, s, a);
+}
 
 int launch_transpose(const float* prm, float* kt, int C, int nb, hipStream_t s) {
   const int64_t n = 9LL * C * C;
@@ -1122,13 +1198,14 @@ int fk_convar_grad(const float* params, int32_t H, int32_t W, int32_t depth, int
   hipStream_t s = (hipStream_t)stream;
   if (channels % 32 == 0) {
     // MFMA kernel: LDS holds the tap source tables and the head cotangents
-    const size_t lds = (size_t)2 * GT_N * a.HW * sizeof(int16_t) + 4 + (size_t)4 * a.HW * sizeof(float);
+    a.stage = stage_stride(a.HW, channels);
+    const size_t lds = grad_lds_total(a.HW, a.stage);
     FK_REQUIRE(lds <= 160 * 1024 && a.HW <= 32767, "fk_convar_grad: lattice too large");
     const int rc = launch_transpose(params, const_cast<float*>(a.kt), channels, a.nb, s);
     if (rc != FK_OK) return rc;
     return activation == FK_ACT_ELU
-               ? launch_with_lds(convar_grad_mfma_kernel<FK_ACT_ELU, GRAD_O>, batch, lds, s, a)
-               : launch_with_lds(convar_grad_mfma_kernel<FK_ACT_RELU, GRAD_O>, batch, lds, s, a);
+               ? launch_grad<FK_ACT_ELU, GRAD_O>(batch, lds, s, a)
+               : launch_grad<FK_ACT_RELU, GRAD_O>(batch, lds, s, a);
   }
   const size_t lds = ((size_t)6 * a.HW * channels + 4 * a.HW) * sizeof(float);
   FK_REQUIRE(lds <= 160 * 1024, "fk_convar_grad: lattice too large for the LDS gradient buffers");
@@ -1180,13 +1257,14 @@ int fk_convar_vjp(const float* params, int32_t H, int32_t W, int32_t depth, int3
   a.cot = cot;
   const int64_t g = implicit_grid(batch, VJP_PER_CU);
   a.partial = a.work + g * a.work_floats;
-  const size_t lds = grad_lds_bytes(a.HW);
+  a.stage = stage_stride(a.HW, channels);
+  const size_t lds = grad_lds_total(a.HW, a.stage);
   FK_REQUIRE(lds <= 160 * 1024 && a.HW <= 32767, "fk_convar_vjp: lattice too large");
   hipStream_t s = (hipStream_t)stream;
   int rc = launch_transpose(params, const_cast<float*>(a.kt), channels, a.nb, s);
   if (rc != FK_OK) return rc;
-  rc = activation == FK_ACT_ELU ? launch_with_lds(convar_grad_mfma_kernel<FK_ACT_ELU, GRAD_VJP>, g, lds, s, a)
-                                : launch_with_lds(convar_grad_mfma_kernel<FK_ACT_RELU, GRAD_VJP>, g, lds, s, a);
+  rc = activation == FK_ACT_ELU ? launch_grad<FK_ACT_ELU, GRAD_VJP>(g, lds, s, a)
+                                : launch_grad<FK_ACT_RELU, GRAD_VJP>(g, lds, s, a);
   if (rc != FK_OK) return rc;
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(div_up(a.P, 256)), dim3(256), 0, s, a.partial, g, a.P, scale,
                      grad);
@@ -1226,15 +1304,19 @@ int fk_convar_jvp(const float* params, const float* tangent, int32_t H, int32_t 
   a.P = flat_param_count(channels, a.nb);
   a.tangent = tangent;
   a.jvp = jvp;
-  const size_t lds = grad_lds_bytes(a.HW);
+  a.stage = stage_stride(a.HW, channels);
+  const size_t lds = grad_lds_total(a.HW, a.stage);
   FK_REQUIRE(lds <= 160 * 1024 && a.HW <= 32767, "fk_convar_jvp: lattice too large");
   const int64_t g = implicit_grid(batch, JVP_PER_CU);
   hipStream_t s = (hipStream_t)stream;
   int rc = launch_transpose(params, const_cast<float*>(a.kt), channels, a.nb, s);
   if (rc == FK_OK) rc = launch_transpose(tangent, const_cast<float*>(a.tkt), channels, a.nb, s);
   if (rc != FK_OK) return rc;
-  return activation == FK_ACT_ELU ? launch_with_lds(convar_jvp_mfma_kernel<FK_ACT_ELU>, g, lds, s, a)
-                                  : launch_with_lds(convar_jvp_mfma_kernel<FK_ACT_RELU>, g, lds, s, a);
+  if (a.stage)
+    return activation == FK_ACT_ELU ? launch_with_lds(convar_jvp_mfma_kernel<FK_ACT_ELU, true>, g, lds, s, a)
+                                    : launch_with_lds(convar_jvp_mfma_kernel<FK_ACT_RELU, true>, g, lds, s, a);
+  return activation == FK_ACT_ELU ? launch_with_lds(convar_jvp_mfma_kernel<FK_ACT_ELU, false>, g, lds, s, a)
+                                  : launch_with_lds(convar_jvp_mfma_kernel<FK_ACT_RELU, false>, g, lds, s, a);
 }
 
 }  // extern "C"
