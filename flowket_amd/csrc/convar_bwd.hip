@@ -529,6 +529,20 @@ __global__ void transpose_kernels_kernel(const float* __restrict__ prm, float* _
   }
 }
 
+// Elementwise passes over HW x C tensors (C a multiple of 4) in 16-byte
+// vectors: element i of a tensor as a quad, the activation and its
+// derivative per component.
+__device__ __forceinline__ f32x4 L4(const float* p, int i) { return *reinterpret_cast<const f32x4*>(p + 4 * i); }
+__device__ __forceinline__ f32x4& S4(float* p, int i) { return *reinterpret_cast<f32x4*>(p + 4 * i); }
+template <int ACT>
+__device__ __forceinline__ f32x4 bact4(f32x4 x) {
+  return f32x4{bact<ACT>(x[0]), bact<ACT>(x[1]), bact<ACT>(x[2]), bact<ACT>(x[3])};
+}
+template <int ACT>
+__device__ __forceinline__ f32x4 bdact4(f32x4 y) {
+  return f32x4{bdact<ACT>(y[0]), bdact<ACT>(y[1]), bdact<ACT>(y[2]), bdact<ACT>(y[3])};
+}
+
 // Copies n channels (from channel off, row stride cs) of an HW-pixel tensor
 // into an LDS tile of row stride SC (C + 4 floats: consecutive pixels start
 // 36 banks apart, so 16 lanes reading 16 bytes of 16 pixels do not
@@ -748,23 +762,23 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
       }
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
-      const float v = t0[idx];
-      const float va = bact<ACT>(v), xa = bact<ACT>(t1[idx]);
-      VA[idx] = va;
-      XA[idx] = xa;
-      float vo = 0.0f;
+    for (int i = threadIdx.x; i < HW * C / 4; i += blockDim.x) {
+      const f32x4 v = L4(t0, i);
+      const f32x4 va = bact4<ACT>(v), xa = bact4<ACT>(L4(t1, i));
+      S4(VA, i) = va;
+      S4(XA, i) = xa;
+      f32x4 vo = {0.f, 0.f, 0.f, 0.f};
       if (!fin) {
-        vo = kb ? bact<ACT>(ws_tensor(ws, HW, C, blk - 2, T_VO)[idx] + v) : va;
-        VO[idx] = vo;
+        vo = kb ? bact4<ACT>(L4(ws_tensor(ws, HW, C, blk - 2, T_VO), i) + v) : va;
+        S4(VO, i) = vo;
       }
       if (tan) {
-        const float dv = u0[idx];
-        ws_tensor(wt, HW, C, blk, T_VA)[idx] = bdact<ACT>(va) * dv;
-        ws_tensor(wt, HW, C, blk, T_XA)[idx] = bdact<ACT>(xa) * u1[idx];
+        const f32x4 dv = L4(u0, i);
+        S4(ws_tensor(wt, HW, C, blk, T_VA), i) = bdact4<ACT>(va) * dv;
+        S4(ws_tensor(wt, HW, C, blk, T_XA), i) = bdact4<ACT>(xa) * L4(u1, i);
         if (!fin) {
-          const float din = kb ? ws_tensor(wt, HW, C, blk - 2, T_VO)[idx] + dv : dv;
-          ws_tensor(wt, HW, C, blk, T_VO)[idx] = bdact<ACT>(vo) * din;
+          const f32x4 din = kb ? L4(ws_tensor(wt, HW, C, blk - 2, T_VO), i) + dv : dv;
+          S4(ws_tensor(wt, HW, C, blk, T_VO), i) = bdact4<ACT>(vo) * din;
         }
       }
     }
@@ -782,10 +796,10 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
       mconv_fwd(CATd, C, C / 2, VA, C, C, C / 2, tkt + o.ky, nullptr, ftab + GT_Y * HW, 1, HW, true);
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
-      const float c = bact<ACT>(CAT[idx]);
-      CAT[idx] = c;
-      if (tan) ws_tensor(wt, HW, C, blk, T_CAT)[idx] *= bdact<ACT>(c);
+    for (int i = threadIdx.x; i < HW * C / 4; i += blockDim.x) {
+      const f32x4 c = bact4<ACT>(L4(CAT, i));
+      S4(CAT, i) = c;
+      if (tan) S4(ws_tensor(wt, HW, C, blk, T_CAT), i) *= bdact4<ACT>(c);
     }
     __syncthreads();
     const float* Cb = STG ? S0 : CAT;
@@ -802,13 +816,13 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
       mconv_fwd(u0, C, 0, Cb, cs, C, C, tkt + o.kc, nullptr, ftab + GT_C * HW, 9, HW, true);
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
-      const float h = t0[idx];
-      const float ho = kb ? bact<ACT>(ws_tensor(ws, HW, C, blk - 2, T_HO)[idx] + h) : bact<ACT>(h);
-      HO[idx] = ho;
+    for (int i = threadIdx.x; i < HW * C / 4; i += blockDim.x) {
+      const f32x4 h = L4(t0, i);
+      const f32x4 ho = kb ? bact4<ACT>(L4(ws_tensor(ws, HW, C, blk - 2, T_HO), i) + h) : bact4<ACT>(h);
+      S4(HO, i) = ho;
       if (tan) {
-        const float din = kb ? ws_tensor(wt, HW, C, blk - 2, T_HO)[idx] + u0[idx] : u0[idx];
-        ws_tensor(wt, HW, C, blk, T_HO)[idx] = bdact<ACT>(ho) * din;
+        const f32x4 din = kb ? L4(ws_tensor(wt, HW, C, blk - 2, T_HO), i) + L4(u0, i) : L4(u0, i);
+        S4(ws_tensor(wt, HW, C, blk, T_HO), i) = bdact4<ACT>(ho) * din;
       }
     }
     __syncthreads();
@@ -869,14 +883,14 @@ __device__ __forceinline__ void mfma_backward(const float* prm, float* ws, const
     const float* CAT = ws_tensor(ws, HW, C, blk, T_CAT);
     const float* VO = ws_tensor(ws, HW, C, blk, T_VO);
     const float* HO = ws_tensor(ws, HW, C, blk, T_HO);
-    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
-      const float gh = dH[idx] * bdact<ACT>(HO[idx]);
-      t1[idx] = gh;
-      const float gv = fin ? 0.0f : dV[idx] * bdact<ACT>(VO[idx]);
-      t0[idx] = gv;
+    for (int i = threadIdx.x; i < HW * C / 4; i += blockDim.x) {
+      const f32x4 gh = L4(dH, i) * bdact4<ACT>(L4(HO, i));
+      S4(t1, i) = gh;
+      const f32x4 gv = fin ? f32x4{0.f, 0.f, 0.f, 0.f} : L4(dV, i) * bdact4<ACT>(L4(VO, i));
+      S4(t0, i) = gv;
       if (kb) {
-        dVr[idx] = gv;
-        dHr[idx] = gh;
+        S4(dVr, i) = gv;
+        S4(dHr, i) = gh;
       }
     }
     __syncthreads();
@@ -893,7 +907,7 @@ __device__ __forceinline__ void mfma_backward(const float* prm, float* ws, const
       mconv_bwd_data(dV, C, db, ds, 0, C, C, prm + o.kc, itab + GT_C * HW, 9, HW);
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) dV[idx] *= bdact<ACT>(CAT[idx]);
+    for (int i = threadIdx.x; i < HW * C / 4; i += blockDim.x) S4(dV, i) *= bdact4<ACT>(L4(CAT, i));
     __syncthreads();
     const int xr = fin ? GT_XF : GT_X;
     mconv_bwd_weight(O + o.kx, O + o.bx, XA, C, dV, C, 0, C, C / 2, ftab + xr * HW, 1, HW, accumulate);
@@ -901,9 +915,9 @@ __device__ __forceinline__ void mfma_backward(const float* prm, float* ws, const
     mconv_bwd_data(dH, C, dV, C, 0, C, C / 2, prm + o.kx, itab + xr * HW, 1, HW);
     mconv_bwd_data(t1, C, dV, C, C / 2, C, C / 2, prm + o.ky, itab + GT_Y * HW, 1, HW);
     __syncthreads();
-    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
-      dH[idx] *= bdact<ACT>(XA[idx]);
-      t0[idx] += t1[idx] * bdact<ACT>(VA[idx]);
+    for (int i = threadIdx.x; i < HW * C / 4; i += blockDim.x) {
+      S4(dH, i) *= bdact4<ACT>(L4(XA, i));
+      S4(t0, i) += L4(t1, i) * bdact4<ACT>(L4(VA, i));
     }
     __syncthreads();
     if (first) {
@@ -926,14 +940,14 @@ __device__ __forceinline__ void mfma_backward(const float* prm, float* ws, const
       mconv_bwd_data(dV, C, db, ds, 0, C, C, prm + o.kv, itab + GT_V * HW, 9, HW);
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < HW * C; idx += blockDim.x) {
-      float gh = t1[idx], gv = dV[idx];
+    for (int i = threadIdx.x; i < HW * C / 4; i += blockDim.x) {
+      f32x4 gh = L4(t1, i), gv = L4(dV, i);
       if (ka) {
-        gh += dHr[idx];
-        gv += dVr[idx];
+        gh += L4(dHr, i);
+        gv += L4(dVr, i);
       }
-      dH[idx] = gh;
-      dV[idx] = gv;
+      S4(dH, i) = gh;
+      S4(dV, i) = gv;
     }
     __syncthreads();
   }
