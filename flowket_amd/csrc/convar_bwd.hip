@@ -716,12 +716,13 @@ __device__ void build_tap_tables(int16_t* ftab, int16_t* itab, int H, int W) {
 // tensor is carried along in wt (same layout as ws): for y = K*x + b,
 // y' = K*x' + Kt*x + bt, and act(y)' = act'(y) y'.  t0/t1 and u0/u1 are
 // scratch tensors of the primal and the tangent.
-// STG: the 3x3 convolutions read their inputs from the LDS staging tile S0
-// (row stride SC); otherwise from L2.
+// STG: the kxk and 1xk convolutions read their inputs from the LDS staging
+// tiles S0, S1 (row stride SC); otherwise from L2.
 template <int ACT, bool STG>
 __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan, const float* kt, const float* tkt,
-                                              float* ws, float* wt, const int16_t* ftab, int H, int W, int C, int nb,
-                                              float* t0, float* t1, float* u0, float* u1, float* S0, int SC) {
+                                             float* ws, float* wt, const int16_t* ftab, int H, int W, int C, int nb,
+                                             float* t0, float* t1, float* u0, float* u1, float* S0, float* S1,
+                                             int SC) {
   const int HW = H * W, F = nb - 1;
   const float* sp = ws;
   for (int blk = 0; blk < nb; ++blk) {
@@ -744,13 +745,15 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
       }
     } else {
       const float* Vb = STG ? S0 : Vin;
+      const float* Hb = STG ? S1 : Hin;
       const int vs = STG ? SC : C;
       if constexpr (STG) {
         stage_in(S0, SC, Vin, C, 0, C, HW);
+        stage_in(S1, SC, Hin, C, 0, C, HW);
         __syncthreads();
       }
       mconv_fwd(t0, C, 0, Vb, vs, C, C, kt + o.kv, prm + o.bv, ftab + GT_V * HW, 9, HW);
-      mconv_fwd(t1, C, 0, Hin, C, C, C, kt + o.kh, prm + o.bh, ftab + GT_H * HW, 3, HW);
+      mconv_fwd(t1, C, 0, Hb, vs, C, C, kt + o.kh, prm + o.bh, ftab + GT_H * HW, 3, HW);
       if (tan) {
         const float* Vd = ws_tensor(wt, HW, C, blk - 1, T_VO);
         const float* Hd = ws_tensor(wt, HW, C, blk - 1, T_HO);
@@ -758,7 +761,7 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
         mconv_fwd(u1, C, 0, Hd, C, C, C, kt + o.kh, tan + o.bh, ftab + GT_H * HW, 3, HW);
         __syncthreads();
         mconv_fwd(u0, C, 0, Vb, vs, C, C, tkt + o.kv, nullptr, ftab + GT_V * HW, 9, HW, true);
-        mconv_fwd(u1, C, 0, Hin, C, C, C, tkt + o.kh, nullptr, ftab + GT_H * HW, 3, HW, true);
+        mconv_fwd(u1, C, 0, Hb, vs, C, C, tkt + o.kh, nullptr, ftab + GT_H * HW, 3, HW, true);
       }
     }
     __syncthreads();
@@ -998,7 +1001,7 @@ __global__ __launch_bounds__(256, MODE == GRAD_VJP ? 3 : 4) void convar_grad_mfm
     if (threadIdx.x == 0) s_lp[0] = s_lp[1] = 0.0f;
     __syncthreads();
     mfma_forward<ACT, STG>(prm, nullptr, a.kt, nullptr, ws, nullptr, ftab, H, W, C, nb, t0, t1, nullptr, nullptr,
-                           S0, a.stage);
+                           S0, S1, a.stage);
     const int64_t ho = head_offset(nb, C);
     const float* HF = ws_tensor(ws, HW, C, F, T_HO);
     const float cr = MODE == GRAD_VJP ? a.cot[2 * b] : 1.0f;
@@ -1050,6 +1053,7 @@ __global__ __launch_bounds__(256, 4) void convar_jvp_mfma_kernel(BwdArgs a) {
   int16_t* ftab = tabs;
   int16_t* itab = tabs + GT_N * HW;
   float* S0 = reinterpret_cast<float*>(tabs + 2 * GT_N * HW) + 4 * HW;
+  float* S1 = S0 + HW * a.stage;
   float* t0 = ws_grad(ws, HW, C, nb, 0);
   float* t1 = ws_grad(ws, HW, C, nb, 1);
   float* u0 = ws_grad(ws, HW, C, nb, 2);
@@ -1062,7 +1066,7 @@ __global__ __launch_bounds__(256, 4) void convar_jvp_mfma_kernel(BwdArgs a) {
     for (int q = threadIdx.x; q < HW; q += blockDim.x) sp[q] = (float)a.spins[b * HW + q];
     if (threadIdx.x < 4) s_acc[threadIdx.x] = 0.0f;
     __syncthreads();
-    mfma_forward<ACT, STG>(prm, tan, a.kt, a.tkt, ws, wt, ftab, H, W, C, nb, t0, t1, u0, u1, S0, a.stage);
+    mfma_forward<ACT, STG>(prm, tan, a.kt, a.tkt, ws, wt, ftab, H, W, C, nb, t0, t1, u0, u1, S0, S1, a.stage);
     const int64_t ho = head_offset(nb, C);
     const float* HF = ws_tensor(ws, HW, C, F, T_HO);
     const float* HFd = ws_tensor(wt, HW, C, F, T_HO);
