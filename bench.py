@@ -233,7 +233,7 @@ def bench_sr(machine, op, spins, shift=1e-3, tol=1e-6, max_iter=50):
     force = of.oh(u, 1.0 / of.B)
     x, it, res = conjugate_gradient(lambda v, out: of.s_matvec(v, shift, out), force, tol=tol, max_iter=max_iter)
     ev[2].record()
-    # one S.v product on its own (two passes over O)
+    # one S.v product on its own (one pass over O)
     z = torch.empty_like(force)
     ev[3].record()
     of.s_matvec(force, shift, z)
@@ -244,6 +244,22 @@ def bench_sr(machine, op, spins, shift=1e-3, tol=1e-6, max_iter=50):
     cg_ms = ev[1].elapsed_time(ev[2])
     mv_ms = ev[3].elapsed_time(ev[4])
     o_bytes = 2 * 4 * B * P
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        r = fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), r
+
+    two_ms, _ = timed(lambda: of.s_matvec_two_pass(force, shift))
+    rows_ms = {}
+    for r in (16, 32):
+        of.block_rows = r
+        rows_ms[r], z1 = timed(lambda: of.s_matvec(force, shift))
+        rows_ms[r] = round(rows_ms[r], 3)
+    of.block_rows = 0
     fnorm = float(torch.linalg.vector_norm(force.double()).item())
     out = {'batch': B, 'params': P, 'grad_ms': round(grad_ms, 2),
            'grad_tflops': round(B * 2 * flops_per_forward() / (grad_ms * 1e-3) / 1e12, 2),
@@ -251,8 +267,15 @@ def bench_sr(machine, op, spins, shift=1e-3, tol=1e-6, max_iter=50):
            'cg_ms': round(cg_ms, 2), 'cg_iterations': it, 'cg_residual': res, 'cg_force_norm': fnorm,
            'cg_relative_residual': res / fnorm if fnorm > 0 else 0.0, 'cg_tol': tol,
            'cg_converged': bool(res <= tol * fnorm),
-           'matvec_ms': round(mv_ms, 3), 'matvec_gbs': round(2 * o_bytes / (mv_ms * 1e-3) / 1e9, 1),
-           'matvec_hbm_frac': round(2 * o_bytes / (mv_ms * 1e-3) / PEAK_HBM, 4),
+           'matvec_ms': round(mv_ms, 3), 'matvec_o_gbs': round(o_bytes / (mv_ms * 1e-3) / 1e9, 1),
+           'matvec_o_hbm_frac': round(o_bytes / (mv_ms * 1e-3) / PEAK_HBM, 4),
+           'matvec_note': 'the default S.v (two passes over O); matvec_o_gbs = bytes of O / time; '
+                          'matvec_one_pass_ms: one pass over O in row blocks of the given rows, re-read from '
+                          'the last-level cache',
+           'matvec_one_pass_ms': rows_ms,
+           'matvec_two_calls_ms': round(two_ms, 3),
+           'one_pass_vs_two_pass_rel_diff': float(
+               (torch.linalg.vector_norm((z1 - z).double()) / torch.linalg.vector_norm(z.double())).item()),
            'step_ms': round(grad_ms + cg_ms, 2), 'o_gib': round(o_bytes / 2 ** 30, 2)}
     # the same S.v without O: forward-mode JVP, then one batched VJP
     if machine.supports_implicit_products():

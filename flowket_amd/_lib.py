@@ -83,6 +83,9 @@ SIGNATURES = {
     'fk_oh_times_u': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I64, ctypes.c_double, ctypes.c_double, _P, _P,
                                      _P, _P]),
     'fk_sr_scratch_doubles': (_I64, []),
+    'fk_sr_matvec_workspace_bytes': (_I64, [_I64, _I64]),
+    'fk_sr_matvec': (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I64, ctypes.c_double, ctypes.c_double, _P, _P, _I32,
+                                    _P, _I64, _P, _P]),
     'fk_dot_f64': (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
     'fk_axpy': (ctypes.c_int, [ctypes.c_double, _P, _P, _I64, _P]),
     'fk_xpby': (ctypes.c_int, [_P, ctypes.c_double, _P, _I64, _P]),

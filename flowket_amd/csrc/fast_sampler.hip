@@ -20,6 +20,8 @@
 #include "convar_layout.h"
 #include "convar_dev.h"
 
+#include <type_traits>
+
 namespace fk {
 
 // Per-lane data of one pixel of a C-channel tensor for 16 samples: MT quads
@@ -133,7 +135,16 @@ struct SampArgs {
   int64_t tile_bytes;
 };
 
-template <int PREC, int C, int ACT>
+// WR: the whole-row vertical pass (16-bit 32-channel nets, rows of at most
+// VROW_MAX pixels), with its rows in dynamic LDS (fast_vrow_lds bytes).
+constexpr int VROW_MAX = 10;
+template <int PREC, int C>
+constexpr int fast_lb() {
+  return PREC == FK_PREC_FP32 ? (C / 16) * 16 : (C / 16) * 8;
+}
+inline int fast_vrow_lds(int W, int lb) { return (6 * W + 1) * 64 * lb + 3 * W * 16 * 4; }
+
+template <int PREC, int C, int ACT, bool WR>
 __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
   using P = Prec<PREC>;
   using frag = typename P::frag;
@@ -165,6 +176,9 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
   constexpr bool BATCH = MT * KS * (int)(sizeof(frag) / 4) <= 8;
   constexpr int BT = BATCH ? 9 : 1;
   constexpr int UNR = BATCH ? 9 : 1;  // tap loops unrolled only with batched loads
+  static_assert(!WR || BATCH, "the whole-row vertical pass needs the batched loads");
+  constexpr int VROW = VROW_MAX;
+  extern __shared__ __align__(16) uint8_t s_dyn[];
 
   // row 0 of every block's concatenation: y = act(1x1(0) + b) (DownShift zeros)
   for (int l = 0; l <= F; ++l) {
@@ -180,6 +194,106 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
       const int q = i * W + j;
       // ============ conditional of site (i, j): masked final block =========
       {
+        f32x4 z;
+        if constexpr (BATCH) {
+          // Every load of the conditional (weights, biases, the earlier
+          // pixels it reads) is requested first and waited for once.
+          const float* bias = a.aux + lay.aux_bias(F);
+          f32x4 bh[MT], bc[MT], bxv[MTH];
+#pragma unroll
+          for (int m = 0; m < MT; ++m) bh[m] = load_bias4(bias + C, 16 * m + 4 * g, C);
+#pragma unroll
+          for (int m = 0; m < MT; ++m) bc[m] = load_bias4(bias + 3 * C, 16 * m + 4 * g, C);
+#pragma unroll
+          for (int m = 0; m < MTH; ++m) bxv[m] = load_bias4(bias + 2 * C, 16 * m + 4 * g, C / 2);
+          const f32x4 zb = load_bias4(a.aux + lay.aux_head_bias(nb), 4 * g, 4);
+          frag wh[3][MT][KS], wx[MTH][KS], wc[9][MT][KS], wz[KS];
+#pragma unroll
+          for (int t = 0; t < 3; ++t)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+              for (int ks = 0; ks < KS; ++ks) wh[t][m][ks] = wfrag(lay.off_h(F) + (t * MT + m) * KS + ks);
+#pragma unroll
+          for (int m = 0; m < MTH; ++m)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) wx[m][ks] = wfrag(lay.off_x(F) + m * KS + ks);
+#pragma unroll
+          for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+              for (int ks = 0; ks < KS; ++ks) wc[t][m][ks] = wfrag(lay.off_c(F) + (t * MT + m) * KS + ks);
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) wz[ks] = wfrag(lay.head_base(nb) + ks);
+          LD hd[3], cd[8], cur;
+#pragma unroll
+          for (int t = 0; t < 3; ++t)
+            if (j - 3 + t >= 0) hd[t].load(HAp(F, i * W + j - 3 + t));
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            const int r = i + t / 3 - 2, c = j + t % 3 - 2;
+            if (r >= 0 && c >= 0) cd[t].load(CATp(F, r * W + c));
+          }
+          cur.load_quads(CATp(F, q), MTH, MT);
+          f32x4 x2[MTH];
+          if (j >= 1) {
+            f32x4 acc[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) acc[m] = bh[m];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+              if (j - 3 + t < 0) continue;
+              frag b[KS];
+              hd[t].template frags<KS>(b);
+#pragma unroll
+              for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) acc[m] = P::mfma(wh[t][m][ks], b[ks], acc[m]);
+            }
+            f32x4 ax[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) ax[m] = act4<ACT>(acc[m]);
+            frag b[KS];
+            d_to_b<PREC, MT, KS>(ax, b);
+#pragma unroll
+            for (int m = 0; m < MTH; ++m) {
+              f32x4 x = bxv[m];
+#pragma unroll
+              for (int ks = 0; ks < KS; ++ks) x = P::mfma(wx[m][ks], b[ks], x);
+              x2[m] = act4<ACT>(x);
+            }
+          } else {
+#pragma unroll
+            for (int m = 0; m < MTH; ++m) x2[m] = act4<ACT>(bxv[m]);
+          }
+#pragma unroll
+          for (int m = 0; m < MTH; ++m) cur.set(m, x2[m]);
+          f32x4 acc[MT];
+#pragma unroll
+          for (int m = 0; m < MT; ++m) acc[m] = bc[m];
+#pragma unroll
+          for (int t = 0; t < 9; ++t) {
+            const int r = i + t / 3 - 2, c = j + t % 3 - 2;
+            if (r < 0 || c < 0) continue;
+            frag b[KS];
+            if (t == 8) cur.template frags<KS>(b);
+            else cd[t < 8 ? t : 0].template frags<KS>(b);
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+              for (int ks = 0; ks < KS; ++ks) acc[m] = P::mfma(wc[t][m][ks], b[ks], acc[m]);
+          }
+          f32x4 ah[MT];
+#pragma unroll
+          for (int m = 0; m < MT; ++m) ah[m] = act4<ACT>(acc[m]);
+          frag b[KS];
+          d_to_b<PREC, MT, KS>(ah, b);
+          z = zb;
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) z = P::mfma(wz[ks], b[ks], z);
+          cur.store_quads(CATp(F, q), 0, MTH);
+        } else {
         const float* bias = a.aux + lay.aux_bias(F);
         f32x4 x2[MTH];
         const float* bx = bias + 2 * C;
@@ -243,9 +357,10 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         for (int m = 0; m < MT; ++m) ah[m] = act4<ACT>(acc[m]);
         frag b[KS];
         d_to_b<PREC, MT, KS>(ah, b);
-        f32x4 z = load_bias4(a.aux + lay.aux_head_bias(nb), 4 * g, 4);
+        z = load_bias4(a.aux + lay.aux_head_bias(nb), 4 * g, 4);
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) z = P::mfma(wfrag(lay.head_base(nb) + ks), b[ks], z);
+        }
         float s = 0.0f;
         if (g == 0) {
           const float mx = fmaxf(z.x, z.y);
@@ -298,6 +413,19 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         // (Ln, ln; ln < 0: none), then the step computes.  Waiting for a
         // load waits for every load issued before it, so this order lets
         // the step start on its own operands while the prefetch is in flight.
+        // A step's two stores (its concatenation x half and its output) are
+        // issued by the next step after that step's loads: vector memory
+        // operations complete in issue order, so a store issued before a load
+        // would make the wait for the load wait for the store as well.
+        LD pcx, phn;
+        uint8_t *ppc = nullptr, *pph = nullptr;
+        auto flush = [&]() {
+          if (ppc) {
+            pcx.store_quads(ppc, 0, MTH);
+            phn.store(pph);
+            ppc = nullptr;
+          }
+        };
         auto step = [&](HLoads<PREC, C, BATCH>& L, int l, HLoads<PREC, C, BATCH>& Ln, int ln) {
           const int kind = block_kind(l, nb);
           const float* bl = a.aux + lay.aux_bias(l);
@@ -337,6 +465,7 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
                 for (int ks = 0; ks < KS; ++ks) wc[t][m][ks] = wfrag(lay.off_c(l) + (t * MT + m) * KS + ks);
           }
           if (ln >= 0) issue(Ln, ln);
+          flush();
           if (l == 0) {
             const float* kh = a.aux + lay.aux_kh0(nb);
             for (int t = 0; t < 3; ++t) {
@@ -383,7 +512,8 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
             for (int ks = 0; ks < KS; ++ks) x = P::mfma(wx[m][ks], bxf[ks], x);
             L.cur2.set(m, act4<ACT>(x));
           }
-          L.cur2.store_quads(CATp(l, q), 0, MTH);
+          pcx = L.cur2;
+          ppc = CATp(l, q);
 #pragma unroll UNR
           for (int t = 0; t < 9; ++t) {
             const int r = i + t / 3 - 2, c = j + t % 3 - 2;
@@ -418,7 +548,8 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
             const f32x4 v = (kind == BK_B) ? act4<ACT>(hres[m] + cacc[m]) : act4<ACT>(cacc[m]);
             hn.set(m, v);
           }
-          hn.store(HAp(l + 1, q));
+          phn = hn;
+          pph = HAp(l + 1, q);
           // the value the next block sees is the stored (rounded) one
 #pragma unroll
           for (int m = 0; m < MT; ++m) hcur[m] = hn.get(m);
@@ -436,10 +567,187 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
             step(LA, l, LB, -1);
           }
         }
+        flush();
       }
     }
 
     // ============ vertical stack of row i (all blocks) ======================
+    if constexpr (WR) {
+      {
+        // Whole-row form.  Per block, rows i-2 and i-1 of its input (written
+        // by earlier rows' passes) are requested together with its weights,
+        // so the block waits on memory once, and are staged in LDS; row i of
+        // its input is the previous block's output, kept in LDS (the stored,
+        // rounded values), and so is the residual (the input of the block
+        // before).  LDS rows: 0, 1 (4, 5 for odd blocks) = input rows i-2,
+        // i-1; 2 + cur = row i of this block's input, 2 + (cur ^ 1) = the
+        // previous block's, which pixel j replaces with its output once it
+        // has read its residual; then one zero pixel and the spins of rows
+        // i-2..i.
+        // Every lane reads back only the bytes it wrote, except the spins
+        // (a barrier after they are staged).
+        int cur = 0;
+        uint8_t* s_vrow = s_dyn;
+        uint8_t* vzero = s_dyn + 6 * W * PB + lane * LB;  // the zero pixel
+        float* s_vspin = reinterpret_cast<float*>(s_dyn + 6 * W * PB + PB);
+        auto vrow = [&](int which, int c) { return s_vrow + ((which * W + c) * 64 + lane) * LB; };
+        // pixel src (16 bytes per lane) -> LDS row `which`, column c
+        auto vdma = [&](const uint8_t* src, int which, int c) {
+          static_assert(LB == 16, "direct LDS loads move 16 bytes per lane");
+          __builtin_amdgcn_global_load_lds(
+              (const void __attribute__((address_space(1)))*)(src),
+              (void __attribute__((address_space(3)))*)(s_vrow + (which * W + c) * PB), 16, 0, 0);
+        };
+        LD{}.store(vzero);
+        for (int idx = lane; idx < 3 * W * 16; idx += 64) {
+          const int r = idx / (W * 16), c = (idx / 16) % W, nn = idx % 16;
+          const int rr = i + r - 2;
+          s_vspin[idx] = rr >= 0 ? spins[(rr * W + c) * 16 + nn] : 0.0f;
+        }
+        __syncthreads();
+        // Block l's operands -- weights, biases and rows i-2, i-1 of its
+        // input -- are requested before block l-1's pixel loop (which issues
+        // no loads), in one of two register sets, so their latency hides
+        // behind that loop.
+        struct VPre {
+          frag wv[9][MT][KS], wy[MTH][KS];
+          f32x4 bv0[MT], by0[MTH];
+        };
+        // LDS rows of block l's input rows i-2, i-1: 0, 1 for even l and
+        // 4, 5 for odd l (filled by direct global-to-LDS loads, whose lane
+        // layout -- 16 bytes per lane, lane-consecutive -- is the rows')
+        auto vbase = [&](int l) { return (l & 1) ? 4 : 0; };
+        auto vrows = [&](int l) {
+          for (int c = 0; c < W; ++c) {
+            if (i >= 2) vdma(VAp(l, (i - 2) * W + c), vbase(l), c);
+            if (i >= 1) vdma(VAp(l, (i - 1) * W + c), vbase(l) + 1, c);
+          }
+        };
+        auto vfetch = [&](VPre& V, int l) {
+          const float* bl = a.aux + lay.aux_bias(l);
+#pragma unroll
+          for (int m = 0; m < MT; ++m) V.bv0[m] = load_bias4(bl, 16 * m + 4 * g, C);
+#pragma unroll
+          for (int m = 0; m < MTH; ++m) {
+            V.by0[m] = load_bias4(bl + 2 * C + C / 2, 16 * m + 4 * g, C / 2);
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) V.wy[m][ks] = wfrag(lay.off_y(l) + m * KS + ks);
+          }
+#pragma unroll
+          for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+              for (int ks = 0; ks < KS; ++ks) V.wv[t][m][ks] = wfrag(lay.off_v(l) + (t * MT + m) * KS + ks);
+        };
+        // the pixel loop of block l (FIRST: block 0, whose input is the
+        // spins); V holds its operands, all landed
+        auto vpixels = [&](auto first, const VPre& V, const f32x4 (&kv0)[9][MT], int l) {
+          constexpr bool FIRST = decltype(first)::value;
+          const int kind = block_kind(l, nb);
+          for (int j = 0; j < W; ++j) {
+            const int q = i * W + j;
+            f32x4 acc[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) acc[m] = V.bv0[m];
+            if constexpr (FIRST) {
+#pragma unroll
+              for (int t = 0; t < 9; ++t) {
+                const int c = j + t % 3 - 1;
+                const float sv = (c >= 0 && c < W) ? s_vspin[((t / 3) * W + c) * 16 + n] : 0.0f;
+#pragma unroll
+                for (int m = 0; m < MT; ++m) acc[m] += kv0[t][m] * sv;
+              }
+            } else {
+              // every tap's fragment first (padding taps read the zero pixel:
+              // their MFMAs add exact zeros, as the fused forward's do), then
+              // the MFMAs
+              LD x[9];
+#pragma unroll
+              for (int t = 0; t < 9; ++t) {
+                const int r = i + t / 3 - 2, c = j + t % 3 - 1;
+                const bool pad = r < 0 || c < 0 || c >= W;
+                x[t].load(pad ? vzero : vrow(t / 3 == 2 ? 2 + cur : vbase(l) + t / 3, c));
+              }
+#pragma unroll
+              for (int t = 0; t < 9; ++t) {
+                frag b[KS];
+                x[t].template frags<KS>(b);
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+#pragma unroll
+                  for (int ks = 0; ks < KS; ++ks) acc[m] = P::mfma(V.wv[t][m][ks], b[ks], acc[m]);
+              }
+            }
+            f32x4 av[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) av[m] = act4<ACT>(acc[m]);
+            if (l < F) {
+              LD vn;
+              if (kind == BK_B) {
+                LD vr;
+                vr.load(vrow(2 + (cur ^ 1), j));
+#pragma unroll
+                for (int m = 0; m < MT; ++m) vn.set(m, act4<ACT>(vr.get(m) + acc[m]));
+              } else {
+#pragma unroll
+                for (int m = 0; m < MT; ++m) vn.set(m, av[m]);
+              }
+              vn.store(VAp(l + 1, q));
+              vn.store(vrow(2 + (cur ^ 1), j));
+            }
+            if (i + 1 < H) {
+              frag b[KS];
+              d_to_b<PREC, MT, KS>(av, b);
+              LD yd;
+#pragma unroll
+              for (int m = 0; m < MTH; ++m) {
+                f32x4 y = V.by0[m];
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) y = P::mfma(V.wy[m][ks], b[ks], y);
+                yd.set(MTH + m, act4<ACT>(y));
+              }
+              yd.store_quads(CATp(l, q + W), MTH, MT);
+            }
+          }
+          cur ^= 1;
+        };
+        // block l >= 1 with its operands in V and its rows in LDS (requested
+        // earlier): wait for them (and for every earlier store), request
+        // block ln's operands into Vn, then run the pixel loop
+        f32x4 kv0[9][MT];
+        VPre V;
+        auto vblock = [&](int l) {
+          vfetch(V, l);
+          __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
+          if (l + 1 <= F) vrows(l + 1);
+          vpixels(std::false_type{}, V, kv0, l);
+        };
+        VPre& VB = V;
+        {
+          // block 0: its own operands, then block 1's into VA
+          const float* kv = a.aux + lay.aux_kv0(nb);
+#pragma unroll
+          for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int m = 0; m < MT; ++m) kv0[t][m] = *reinterpret_cast<const f32x4*>(kv + t * C + 16 * m + 4 * g);
+          const float* bl = a.aux + lay.aux_bias(0);
+#pragma unroll
+          for (int m = 0; m < MT; ++m) VB.bv0[m] = load_bias4(bl, 16 * m + 4 * g, C);
+#pragma unroll
+          for (int m = 0; m < MTH; ++m) {
+            VB.by0[m] = load_bias4(bl + 2 * C + C / 2, 16 * m + 4 * g, C / 2);
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) VB.wy[m][ks] = wfrag(lay.off_y(0) + m * KS + ks);
+          }
+          __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
+          if (F >= 1) vrows(1);
+          vpixels(std::true_type{}, VB, kv0, 0);
+        }
+        for (int l = 1; l <= F; ++l) vblock(l);
+        __syncthreads();  // the spins staging is rewritten by the next row
+      }
+    } else {
     for (int l = 0; l <= F; ++l) {
       const int kind = block_kind(l, nb);
       const float* bl = a.aux + lay.aux_bias(l);
@@ -565,6 +873,7 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         for (int j = 0; j < W; ++j) vstep(VA, j, VB, -1);
       }
     }
+    }
   }
 }
 
@@ -583,14 +892,29 @@ int64_t fast_tile_bytes(int HW, int nb, int channels, int precision) {
   return (int64_t)(3 * nb - 2) * HW * pb + (int64_t)HW * 16 * 4;
 }
 
-template <int PREC, int C>
-static int launch_fast(const SampArgs& a, int act, int64_t tiles, hipStream_t s) {
+template <int PREC, int C, bool WR>
+static int launch_fast_wr(const SampArgs& a, int act, int64_t tiles, hipStream_t s) {
+  const int lds = WR ? fast_vrow_lds(a.W, fast_lb<PREC, C>()) : 0;
+  if (WR && lds > 64 * 1024) {
+    auto k = act == FK_ACT_ELU ? fast_sample_kernel<PREC, C, FK_ACT_ELU, WR> : fast_sample_kernel<PREC, C, FK_ACT_RELU, WR>;
+    FK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  }
   if (act == FK_ACT_ELU)
-    hipLaunchKernelGGL((fast_sample_kernel<PREC, C, FK_ACT_ELU>), dim3((unsigned)tiles), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((fast_sample_kernel<PREC, C, FK_ACT_ELU, WR>), dim3((unsigned)tiles), dim3(64), lds, s, a);
   else
-    hipLaunchKernelGGL((fast_sample_kernel<PREC, C, FK_ACT_RELU>), dim3((unsigned)tiles), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((fast_sample_kernel<PREC, C, FK_ACT_RELU, WR>), dim3((unsigned)tiles), dim3(64), lds, s, a);
   FK_CHECK_LAUNCH();
   return FK_OK;
+}
+
+// 16-bit 32-channel nets with rows of at most VROW_MAX pixels take the
+// whole-row vertical pass
+template <int PREC, int C>
+static int launch_fast(const SampArgs& a, int act, int64_t tiles, hipStream_t s) {
+  if constexpr (PREC != FK_PREC_FP32 && C == 32) {
+    if (a.W <= VROW_MAX) return launch_fast_wr<PREC, C, true>(a, act, tiles, s);
+  }
+  return launch_fast_wr<PREC, C, false>(a, act, tiles, s);
 }
 
 }  // namespace fk

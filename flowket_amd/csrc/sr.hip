@@ -212,6 +212,166 @@ __global__ void oh_times_u_kernel(const float* __restrict__ ore, const float* __
   out[k] = (float)r;
 }
 
+// S v with O read from HBM once per product.  The rows go in blocks of R
+// (<= SV_MAXR); launch k runs two independent jobs side by side:
+//   job A, block k: partial dots O_b . v of the block's rows over SV_NA column
+//     chunks (one workgroup per chunk, FP64 partials by block parity);
+//   job B, block k-1: u_b = sum of its partials - Obar.v, then
+//     acc += sum_b Re[conj(O_b) u_b] over every column (FP32 over the block's
+//     rows, FP64 across blocks).
+// Job B re-reads rows that job A of the previous launch streamed in, so with
+// two blocks in flight (2 R rows x 2 parts x 4 P bytes) inside the 256 MB
+// last-level cache the second read does not reach HBM.  (Measured at B = 8192,
+// P = 849,156: 20.0 ms at R = 16 against 18.9 ms for the two HBM passes --
+// reads served by the last-level cache come no faster than HBM streams, so
+// the two-pass form stays the default.)  The kernel boundary
+// between the launches orders job A's partials before job B's reads; every
+// job-B workgroup reduces the partials in the same fixed order, so all of
+// them (and the u they store) agree bit for bit.
+constexpr int SV_MAXR = 32;
+constexpr int SV_NA = 256;
+struct SvArgs {
+  const float* ore;
+  const float* oim;
+  const float* v;
+  const double* ov_c;  // Obar.v (re, im)
+  double* part;        // [2][SV_MAXR][SV_NA][2] partial dots, by block parity
+  float* u;            // (B, 2): u of every row (written by job-B workgroup 0)
+  double* acc;         // (P): running sum_b Re[conj(O_b) u_b]
+  int64_t B, P4;
+  int R, nB;        // rows per block, workgroups of job B
+  int64_t ka, kb;  // row blocks of job A and job B (-1: none)
+};
+
+__device__ __forceinline__ double wave_sum_f64(double x) {
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) x += __shfl_xor(x, s, 64);
+  return __shfl(x, 0, 64);  // lane 0's order, the same in every wave
+}
+
+constexpr int SV_T = 512;  // threads per workgroup (8 waves)
+constexpr int SV_RW = SV_MAXR / (SV_T / 64);  // rows per wave in job A
+__global__ __launch_bounds__(SV_T, 4) void sv_rows_kernel(SvArgs a) {
+  __shared__ float s_u[SV_MAXR][2];
+  constexpr int NW = SV_T / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nA = a.ka >= 0 ? SV_NA : 0;
+  const f32x4* ore4 = reinterpret_cast<const f32x4*>(a.ore);
+  const f32x4* oim4 = reinterpret_cast<const f32x4*>(a.oim);
+  if ((int)blockIdx.x < nA) {
+    // job A: rows wave, wave + NW, ... of block ka over this chunk's
+    // columns, two columns per lane in flight
+    const int ch = blockIdx.x;
+    const int64_t per = (a.P4 + SV_NA - 1) / SV_NA;
+    const int64_t c0 = ch * per, c1 = c0 + per < a.P4 ? c0 + per : a.P4;
+    const int64_t b0 = a.ka * a.R;
+    int nr = 0;  // rows of this wave
+    for (int rr = wave; rr < a.R && b0 + rr < a.B; rr += NW) ++nr;
+    float pr[SV_RW], pi[SV_RW];
+#pragma unroll
+    for (int j = 0; j < SV_RW; ++j) pr[j] = pi[j] = 0.0f;
+    const f32x4* v4 = reinterpret_cast<const f32x4*>(a.v);
+    constexpr int U = 2;
+    for (int64_t cb = c0; cb < c1; cb += 64 * U) {
+      f32x4 vk[U], x[U][SV_RW], y[U][SV_RW];
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const int64_t c = cb + 64 * q + lane;
+        const bool in = c < c1;
+        vk[q] = in ? v4[c] : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < SV_RW; ++j) {
+          const int64_t row = (b0 + wave + NW * j) * a.P4 + c;
+          x[q][j] = (in && j < nr) ? ore4[row] : f32x4{0.f, 0.f, 0.f, 0.f};
+          y[q][j] = (in && j < nr) ? oim4[row] : f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < U; ++q)
+#pragma unroll
+        for (int j = 0; j < SV_RW; ++j) {
+          pr[j] += x[q][j].x * vk[q].x + x[q][j].y * vk[q].y + x[q][j].z * vk[q].z + x[q][j].w * vk[q].w;
+          pi[j] += y[q][j].x * vk[q].x + y[q][j].y * vk[q].y + y[q][j].z * vk[q].z + y[q][j].w * vk[q].w;
+        }
+    }
+    double* part = a.part + (int64_t)(a.ka & 1) * SV_MAXR * SV_NA * 2;
+#pragma unroll
+    for (int j = 0; j < SV_RW; ++j) {
+      if (j < nr) {
+        const double sr = wave_sum_f64((double)pr[j]), si = wave_sum_f64((double)pi[j]);
+        if (lane == 0) {
+          const int rr = wave + NW * j;
+          part[((int64_t)rr * SV_NA + ch) * 2] = sr;
+          part[((int64_t)rr * SV_NA + ch) * 2 + 1] = si;
+        }
+      }
+    }
+    return;
+  }
+  if (a.kb < 0) return;
+  // job B: u of block kb, then float4 columns bb * SV_T + t, bb * SV_T + t
+  // + nB * SV_T, ... with four rows in flight
+  const int64_t bb = (int64_t)blockIdx.x - nA;
+  const int64_t b0 = a.kb * a.R;
+  const int nrow = (int)(b0 + a.R <= a.B ? a.R : a.B - b0);
+  const double* part = a.part + (int64_t)(a.kb & 1) * SV_MAXR * SV_NA * 2;
+  for (int rr = wave; rr < nrow; rr += NW) {
+    double sr = 0.0, si = 0.0;
+    for (int ch = lane; ch < SV_NA; ch += 64) {
+      sr += part[((int64_t)rr * SV_NA + ch) * 2];
+      si += part[((int64_t)rr * SV_NA + ch) * 2 + 1];
+    }
+    sr = wave_sum_f64(sr);
+    si = wave_sum_f64(si);
+    if (lane == 0) {
+      const float ur = (float)(sr - a.ov_c[0]), ui = (float)(si - a.ov_c[1]);
+      s_u[rr][0] = ur;
+      s_u[rr][1] = ui;
+      if (bb == 0) {
+        a.u[2 * (b0 + rr)] = ur;
+        a.u[2 * (b0 + rr) + 1] = ui;
+      }
+    }
+  }
+  __syncthreads();
+  for (int64_t c = bb * SV_T + threadIdx.x; c < a.P4; c += (int64_t)a.nB * SV_T) {
+    f32x4 s = {0.f, 0.f, 0.f, 0.f};
+    constexpr int UR = 4;
+    for (int r0 = 0; r0 < nrow; r0 += UR) {
+      f32x4 x[UR], y[UR];
+#pragma unroll
+      for (int q = 0; q < UR; ++q) {
+        const int64_t row = (b0 + r0 + q) * a.P4 + c;
+        x[q] = r0 + q < nrow ? ore4[row] : f32x4{0.f, 0.f, 0.f, 0.f};
+        y[q] = r0 + q < nrow ? oim4[row] : f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+      for (int q = 0; q < UR; ++q)
+        if (r0 + q < nrow) s += x[q] * s_u[r0 + q][0] + y[q] * s_u[r0 + q][1];
+    }
+    double2* acc = reinterpret_cast<double2*>(a.acc) + 2 * c;
+    if (a.kb == 0) {
+      acc[0] = make_double2((double)s.x, (double)s.y);
+      acc[1] = make_double2((double)s.z, (double)s.w);
+    } else {
+      const double2 p0 = acc[0], p1 = acc[1];
+      acc[0] = make_double2(p0.x + (double)s.x, p0.y + (double)s.y);
+      acc[1] = make_double2(p1.x + (double)s.z, p1.y + (double)s.w);
+    }
+  }
+}
+
+// out_k = scale * (acc_k - Re[conj(Obar_k) sum_b u_b]) + shift * v_k
+__global__ void sv_finish_kernel(const double* __restrict__ acc, const float* __restrict__ mre,
+                                 const float* __restrict__ mim, const double* __restrict__ usum, int64_t P,
+                                 double scale, double shift, const float* __restrict__ v, float* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  double r = scale * (acc[k] - ((double)mre[k] * usum[0] + (double)mim[k] * usum[1]));
+  if (v) r += shift * (double)v[k];
+  out[k] = (float)r;
+}
+
 // y = a*x + y (fp32 vectors, fp64 coefficient)
 __global__ void axpy_kernel(double alpha, const float* __restrict__ x, float* __restrict__ y, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -360,7 +520,64 @@ int fk_oh_times_u(const float* o_re, const float* o_im, const float* mean_re, co
   return FK_OK;
 }
 
-int64_t fk_sr_scratch_doubles(void) { return 258; }
+int64_t fk_sr_scratch_doubles(void) { return 260; }
+
+int64_t fk_sr_matvec_workspace_bytes(int64_t batch, int64_t n_params) {
+  (void)batch;
+  return (int64_t)(2 * SV_MAXR * SV_NA * 2) * 8 + ((n_params + 63) / 64) * 64 * 8;
+}
+
+int fk_sr_matvec(const float* o_re, const float* o_im, const float* mean_re, const float* mean_im, const float* v,
+                 int64_t batch, int64_t n_params, double scale, double shift, float* out, float* u,
+                 int32_t block_rows, void* workspace, int64_t workspace_bytes, double* scratch, void* stream) {
+  FK_REQUIRE(o_re && o_im && mean_re && mean_im && v && out && u && scratch, "fk_sr_matvec: null pointer");
+  FK_REQUIRE(batch > 0 && n_params > 0, "fk_sr_matvec: bad sizes");
+  FK_REQUIRE(block_rows >= 0 && block_rows <= SV_MAXR, "fk_sr_matvec: block_rows must be in [0, %d]", SV_MAXR);
+  if (block_rows == 0 || (n_params & 3)) {  // the two-pass products
+    const int rc = fk_o_times_v(o_re, o_im, mean_re, mean_im, v, batch, n_params, u, scratch, stream);
+    if (rc != FK_OK) return rc;
+    return fk_oh_times_u(o_re, o_im, mean_re, mean_im, u, batch, n_params, scale, shift, v, out, scratch, stream);
+  }
+  FK_REQUIRE(workspace && workspace_bytes >= fk_sr_matvec_workspace_bytes(batch, n_params),
+             "fk_sr_matvec: workspace of %lld bytes, need %lld", (long long)workspace_bytes,
+             (long long)fk_sr_matvec_workspace_bytes(batch, n_params));
+  FK_REQUIRE(((uintptr_t)workspace & 15) == 0, "fk_sr_matvec: workspace must be 16-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  // scratch: [0..255] dot partials, [256..257] Obar.v, [258..259] sum_b u_b
+  for (int part = 0; part < 2; ++part) {
+    hipLaunchKernelGGL(dot_f64_kernel, dim3(256), dim3(1024), 0, s, part ? mean_im : mean_re, v, n_params, scratch);
+    FK_CHECK_LAUNCH();
+    hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, s, scratch, 256, scratch + 256 + part);
+    FK_CHECK_LAUNCH();
+  }
+  SvArgs a{};
+  a.ore = o_re;
+  a.oim = o_im;
+  a.v = v;
+  a.ov_c = scratch + 256;
+  a.part = reinterpret_cast<double*>(workspace);
+  a.acc = a.part + 2 * SV_MAXR * SV_NA * 2;
+  a.u = u;
+  a.B = batch;
+  a.P4 = n_params / 4;
+  a.R = block_rows;
+  const int64_t nblk = (batch + a.R - 1) / a.R;
+  const int nB = div_up(a.P4, SV_T) < SV_NA ? div_up(a.P4, SV_T) : SV_NA;
+  a.nB = nB;
+  for (int64_t k = 0; k <= nblk; ++k) {
+    a.ka = k < nblk ? k : -1;
+    a.kb = k - 1;
+    const int grid = (a.ka >= 0 ? SV_NA : 0) + (a.kb >= 0 ? nB : 0);
+    hipLaunchKernelGGL(sv_rows_kernel, dim3(grid), dim3(SV_T), 0, s, a);
+    FK_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(sum_u_kernel, dim3(1), dim3(1024), 0, s, u, batch, scratch + 258);
+  FK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(sv_finish_kernel, dim3(div_up(n_params, 256)), dim3(256), 0, s, a.acc, mean_re, mean_im,
+                     scratch + 258, n_params, scale, shift, v, out);
+  FK_CHECK_LAUNCH();
+  return FK_OK;
+}
 
 int fk_dot_f64(const float* a, const float* b, int64_t n, double* scratch, double* out, void* stream) {
   FK_REQUIRE(a && b && scratch && out, "fk_dot_f64: null pointer");

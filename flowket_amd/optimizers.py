@@ -68,6 +68,8 @@ class _OFactor:
         self.scratch = torch.empty(int(_lib.load().fk_sr_scratch_doubles()), dtype=torch.float64, device=dev)
         self.u = torch.empty((self.B, 2), dtype=torch.float32, device=dev)
         self.stream = stream
+        self.matvec_ws = None
+        self.block_rows = 0
 
     def oh(self, u, scale, shift=0.0, v=None, out=None):
         """scale * Re[(O-Obar)^H u] + shift * v."""
@@ -82,7 +84,23 @@ class _OFactor:
         return self.oh(u, 1.0 / self.B)
 
     def s_matvec(self, v, diag_shift, out=None):
-        """S v = Re[(O-Obar)^H ((O-Obar) v)]/B + shift v (optimizer.py:78-81)."""
+        """S v = Re[(O-Obar)^H ((O-Obar) v)]/B + shift v (optimizer.py:78-81)
+        through fk_sr_matvec: two passes over O (block_rows = 0, the faster
+        form on MI355X) or one pass in row blocks of block_rows."""
+        if out is None:
+            out = torch.empty(self.P, dtype=torch.float32, device=self.o_re.device)
+        if self.block_rows and self.matvec_ws is None:
+            nbytes = int(_lib.load().fk_sr_matvec_workspace_bytes(self.B, self.P))
+            self.matvec_ws = torch.empty(nbytes // 8 + 2, dtype=torch.float64, device=self.o_re.device)
+        ws = self.matvec_ws if self.block_rows else None
+        _lib.call('fk_sr_matvec', _lib.ptr(self.o_re), _lib.ptr(self.o_im), _lib.ptr(self.mean_re),
+                  _lib.ptr(self.mean_im), _lib.ptr(v), self.B, self.P, 1.0 / self.B, float(diag_shift),
+                  _lib.ptr(out), _lib.ptr(self.u), int(self.block_rows), _lib.ptr(ws),
+                  0 if ws is None else ws.numel() * 8, _lib.ptr(self.scratch), _lib.stream_ptr(self.stream))
+        return out
+
+    def s_matvec_two_pass(self, v, diag_shift, out=None):
+        """The same product as the two separate calls fk_o_times_v, fk_oh_times_u."""
         _lib.call('fk_o_times_v', _lib.ptr(self.o_re), _lib.ptr(self.o_im), _lib.ptr(self.mean_re),
                   _lib.ptr(self.mean_im), _lib.ptr(v), self.B, self.P, _lib.ptr(self.u), _lib.ptr(self.scratch),
                   _lib.stream_ptr(self.stream))

@@ -406,6 +406,22 @@ int fk_oh_times_u(const float* o_re, const float* o_im, const float* mean_re,
                   int64_t n_params, double scale, double shift, const float* v,
                   float* out, double* scratch2, void* stream);
 int64_t fk_sr_scratch_doubles(void);
+/* out = scale * Re[(O - Obar)^H (O - Obar) v] + shift * v, the S v of
+ * optimizer.py:78-81 on a stored O.  block_rows = 0: fk_o_times_v then
+ * fk_oh_times_u (two passes over O, the faster form on MI355X).  block_rows
+ * in [1, 32]: one pass over O from HBM -- the rows go in blocks of
+ * block_rows; each launch forms the partial dots O v of one block and adds
+ * Re[O^H u] of the previous one, whose rows are still in the last-level
+ * cache.  u (batch, 2) receives (O - Obar) v as fk_o_times_v's.  workspace:
+ * fk_sr_matvec_workspace_bytes(batch, n_params) bytes, 16-byte aligned (may
+ * be NULL for block_rows = 0); scratch: fk_sr_scratch_doubles() doubles.
+ * n_params not a multiple of 4 always takes the two-pass form. */
+int64_t fk_sr_matvec_workspace_bytes(int64_t batch, int64_t n_params);
+int fk_sr_matvec(const float* o_re, const float* o_im, const float* mean_re,
+                 const float* mean_im, const float* v, int64_t batch,
+                 int64_t n_params, double scale, double shift, float* out,
+                 float* u, int32_t block_rows, void* workspace,
+                 int64_t workspace_bytes, double* scratch, void* stream);
 int fk_dot_f64(const float* a, const float* b, int64_t n, double* scratch,
                double* out, void* stream);
 int fk_axpy(double alpha, const float* x, float* y, int64_t n, void* stream);

@@ -19,7 +19,13 @@ pytestmark = pytest.mark.gpu
                                                      ((6, 6), 3, 48, 'relu', 'bf16', 40),
                                                      ((4, 5), 2, 24, 'elu', 'fp32', 30),
                                                      ((10, 10), 3, 32, 'relu', 'fp16', 40),
-                                                     ((5, 6), 2, 64, 'elu', 'fp16', 17)])
+                                                     ((5, 6), 2, 64, 'elu', 'fp16', 17),
+                                                     # rows of 11-16 pixels (two staging batches, > 64 KB
+                                                     # of LDS), 17 pixels (per-pixel vertical pass), depth 20
+                                                     ((4, 13), 3, 32, 'elu', 'fp16', 20),
+                                                     ((3, 16), 4, 32, 'relu', 'bf16', 17),
+                                                     ((3, 17), 3, 32, 'relu', 'bf16', 16),
+                                                     ((10, 10), 20, 32, 'relu', 'fp16', 32)])
 def test_fast_equals_plain_given_uniforms(rng, shape, depth, C, act, prec, B):
     """Same uniforms -> the same draws as AutoregressiveSampler (identical
     arithmetic), and the conditionals it drew from equal the full forward's."""

@@ -228,6 +228,57 @@ def test_sr_products_c_abi(rng, B, P):
     assert np.abs(out.double().cpu().numpy() - ref).max() < 1e-4 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize('B,P,rows', [(37, 1000, 0), (37, 1000, 1), (37, 1000, 5), (64, 2048, 32), (70, 1001, 8),
+                                      (3, 4, 16)])
+def test_sr_matvec_one_pass_c_abi(rng, B, P, rows):
+    """fk_sr_matvec (row blocks: partial dots of one block and Re[O^H u] of the
+    previous one per launch) against dense float64 algebra and the two-pass
+    products; ragged last blocks, one-row blocks, and P % 4 != 0 (two-pass
+    path)."""
+    from flowket_amd import _lib
+    gpu_banner(f'SR one-pass matvec B={B} P={P} rows={rows}')
+    O = rng.normal(size=(B, P)) + 1j * rng.normal(size=(B, P))
+    v = rng.normal(size=P)
+    ore = torch.as_tensor(O.real, dtype=torch.float32).contiguous().cuda()
+    oim = torch.as_tensor(O.imag, dtype=torch.float32).contiguous().cuda()
+    mre, mim = torch.empty(P, device='cuda'), torch.empty(P, device='cuda')
+    _lib.call('fk_o_mean', _lib.ptr(ore), _lib.ptr(oim), B, P, _lib.ptr(mre), _lib.ptr(mim), _lib.stream_ptr())
+    scratch = torch.empty(int(_lib.load().fk_sr_scratch_doubles()), dtype=torch.float64, device='cuda')
+    nbytes = int(_lib.load().fk_sr_matvec_workspace_bytes(B, P))
+    ws = torch.empty(nbytes // 8, dtype=torch.float64, device='cuda')
+    vd = torch.as_tensor(v, dtype=torch.float32).cuda()
+    u = torch.empty((B, 2), dtype=torch.float32, device='cuda')
+    outs = []
+    for _ in range(2):  # the second call reuses the workspace's stale contents
+        out = torch.empty(P, device='cuda')
+        _lib.call('fk_sr_matvec', _lib.ptr(ore), _lib.ptr(oim), _lib.ptr(mre), _lib.ptr(mim), _lib.ptr(vd), B, P,
+                  0.5, 0.25, _lib.ptr(out), _lib.ptr(u), rows, _lib.ptr(ws), nbytes, _lib.ptr(scratch),
+                  _lib.stream_ptr())
+        outs.append(out.double().cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    Oc = O.astype(np.complex64).astype(np.complex128)
+    Oc = Oc - Oc.mean(axis=0)
+    v32 = v.astype(np.float32).astype(np.float64)
+    ref_u = Oc @ v32
+    got_u = u[:, 0].double().cpu().numpy() + 1j * u[:, 1].double().cpu().numpy()
+    assert np.abs(got_u - ref_u).max() < 1e-5 * np.abs(ref_u).max()
+    ref = 0.5 * np.real(np.conj(Oc).T @ ref_u) + 0.25 * v32
+    assert np.abs(outs[0] - ref).max() < 1e-5 * np.abs(ref).max()
+    # the two-pass products on the same inputs
+    u2 = torch.empty((B, 2), dtype=torch.float32, device='cuda')
+    _lib.call('fk_o_times_v', _lib.ptr(ore), _lib.ptr(oim), _lib.ptr(mre), _lib.ptr(mim), _lib.ptr(vd), B, P,
+              _lib.ptr(u2), _lib.ptr(scratch), _lib.stream_ptr())
+    out2 = torch.empty(P, device='cuda')
+    _lib.call('fk_oh_times_u', _lib.ptr(ore), _lib.ptr(oim), _lib.ptr(mre), _lib.ptr(mim), _lib.ptr(u2), B, P,
+              0.5, 0.25, _lib.ptr(vd), _lib.ptr(out2), _lib.ptr(scratch), _lib.stream_ptr())
+    assert np.abs(outs[0] - out2.double().cpu().numpy()).max() < 1e-5 * np.abs(ref).max()
+    # arguments the entry point refuses
+    with pytest.raises(RuntimeError):
+        _lib.call('fk_sr_matvec', _lib.ptr(ore), _lib.ptr(oim), _lib.ptr(mre), _lib.ptr(mim), _lib.ptr(vd), B, P,
+                  0.5, 0.25, _lib.ptr(out), _lib.ptr(u), 33, _lib.ptr(ws), nbytes, _lib.ptr(scratch),
+                  _lib.stream_ptr())
+
+
 def test_mini_batch_loss_gradients_add_up_to_the_energy_gradient():
     """VariationalMonteCarlo with mini_batch_size = B/4: the loss gradients of
     the four mini-batches (optimization/loss.py:4-5) sum to the energy
