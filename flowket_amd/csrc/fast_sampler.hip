@@ -144,6 +144,11 @@ constexpr int fast_lb() {
 }
 inline int fast_vrow_lds(int W, int lb) { return (6 * W + 1) * 64 * lb + 3 * W * 16 * 4; }
 
+// Four consecutive biases of one lane group (16-byte aligned: every bias
+// offset of the aux buffer is a multiple of 4 floats, and channels 16 m + 4 g
+// .. + 3 stay inside the C or C/2 biases of the M-tiles that read them).
+__device__ __forceinline__ f32x4 ldq4(const float* p) { return *reinterpret_cast<const f32x4*>(p); }
+
 template <int PREC, int C, int ACT, bool WR>
 __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
   using P = Prec<PREC>;
@@ -164,7 +169,10 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
   const frag* wall = reinterpret_cast<const frag*>(a.w);
 
   uint8_t* base = a.cache + (int64_t)blockIdx.x * a.tile_bytes;
-  const int64_t BUF = (int64_t)HW * PB;
+  // every buffer holds HW pixels and, at index HW, a zero pixel that padding
+  // taps read (their MFMAs add exact zeros, as the fused forward's do), so
+  // the batched steps have no per-tap branches
+  const int64_t BUF = (int64_t)(HW + 1) * PB;
   auto CATp = [&](int l, int q) { return base + (int64_t)l * BUF + (int64_t)q * PB + lane * LB; };
   auto HAp = [&](int l, int q) { return base + (int64_t)(nb + l - 1) * BUF + (int64_t)q * PB + lane * LB; };
   auto VAp = [&](int l, int q) { return base + (int64_t)(2 * nb - 1 + l - 1) * BUF + (int64_t)q * PB + lane * LB; };
@@ -180,12 +188,14 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
   constexpr int VROW = VROW_MAX;
   extern __shared__ __align__(16) uint8_t s_dyn[];
 
+  for (int k = 0; k < 3 * nb - 2; ++k) LD{}.store(base + (int64_t)k * BUF + (int64_t)HW * PB + lane * LB);
+
   // row 0 of every block's concatenation: y = act(1x1(0) + b) (DownShift zeros)
   for (int l = 0; l <= F; ++l) {
     const float* by = a.aux + lay.aux_bias(l) + 2 * C + C / 2;
     LD d;
 #pragma unroll
-    for (int m = 0; m < MTH; ++m) d.set(MTH + m, act4<ACT>(load_bias4(by, 16 * m + 4 * g, C / 2)));
+    for (int m = 0; m < MTH; ++m) d.set(MTH + m, act4<ACT>(ldq4(by + 16 * m + 4 * g)));
     for (int j = 0; j < W; ++j) d.store_quads(CATp(l, j), MTH, MT);
   }
 
@@ -201,12 +211,13 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           const float* bias = a.aux + lay.aux_bias(F);
           f32x4 bh[MT], bc[MT], bxv[MTH];
 #pragma unroll
-          for (int m = 0; m < MT; ++m) bh[m] = load_bias4(bias + C, 16 * m + 4 * g, C);
+          for (int m = 0; m < MT; ++m) bh[m] = ldq4(bias + C + 16 * m + 4 * g);
 #pragma unroll
-          for (int m = 0; m < MT; ++m) bc[m] = load_bias4(bias + 3 * C, 16 * m + 4 * g, C);
+          for (int m = 0; m < MT; ++m) bc[m] = ldq4(bias + 3 * C + 16 * m + 4 * g);
 #pragma unroll
-          for (int m = 0; m < MTH; ++m) bxv[m] = load_bias4(bias + 2 * C, 16 * m + 4 * g, C / 2);
-          const f32x4 zb = load_bias4(a.aux + lay.aux_head_bias(nb), 4 * g, 4);
+          for (int m = 0; m < MTH; ++m) bxv[m] = ldq4(bias + 2 * C + 16 * m + 4 * g);
+          const f32x4 hb4 = ldq4(a.aux + lay.aux_head_bias(nb));  // the head's 4 biases (lane group 0)
+          const f32x4 zb = g == 0 ? hb4 : f32x4{0.f, 0.f, 0.f, 0.f};
           frag wh[3][MT][KS], wx[MTH][KS], wc[9][MT][KS], wz[KS];
 #pragma unroll
           for (int t = 0; t < 3; ++t)
@@ -228,12 +239,11 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           for (int ks = 0; ks < KS; ++ks) wz[ks] = wfrag(lay.head_base(nb) + ks);
           LD hd[3], cd[8], cur;
 #pragma unroll
-          for (int t = 0; t < 3; ++t)
-            if (j - 3 + t >= 0) hd[t].load(HAp(F, i * W + j - 3 + t));
+          for (int t = 0; t < 3; ++t) hd[t].load(HAp(F, j - 3 + t >= 0 ? i * W + j - 3 + t : HW));
 #pragma unroll
           for (int t = 0; t < 8; ++t) {
             const int r = i + t / 3 - 2, c = j + t % 3 - 2;
-            if (r >= 0 && c >= 0) cd[t].load(CATp(F, r * W + c));
+            cd[t].load(CATp(F, r >= 0 && c >= 0 ? r * W + c : HW));
           }
           cur.load_quads(CATp(F, q), MTH, MT);
           f32x4 x2[MTH];
@@ -243,7 +253,6 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
             for (int m = 0; m < MT; ++m) acc[m] = bh[m];
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
-              if (j - 3 + t < 0) continue;
               frag b[KS];
               hd[t].template frags<KS>(b);
 #pragma unroll
@@ -274,8 +283,6 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           for (int m = 0; m < MT; ++m) acc[m] = bc[m];
 #pragma unroll
           for (int t = 0; t < 9; ++t) {
-            const int r = i + t / 3 - 2, c = j + t % 3 - 2;
-            if (r < 0 || c < 0) continue;
             frag b[KS];
             if (t == 8) cur.template frags<KS>(b);
             else cd[t < 8 ? t : 0].template frags<KS>(b);
@@ -300,7 +307,7 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         if (j >= 1) {
           f32x4 acc[MT];
 #pragma unroll
-          for (int m = 0; m < MT; ++m) acc[m] = load_bias4(bias + C, 16 * m + 4 * g, C);
+          for (int m = 0; m < MT; ++m) acc[m] = ldq4(bias + C + 16 * m + 4 * g);
           for (int t = 0; t < 3; ++t) {
             const int c = j - 1 + t - 2;
             if (c < 0) continue;
@@ -321,14 +328,14 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           d_to_b<PREC, MT, KS>(ax, b);
 #pragma unroll
           for (int m = 0; m < MTH; ++m) {
-            f32x4 x = load_bias4(bx, 16 * m + 4 * g, C / 2);
+            f32x4 x = ldq4(bx + 16 * m + 4 * g);
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) x = P::mfma(wfrag(lay.off_x(F) + m * KS + ks), b[ks], x);
             x2[m] = act4<ACT>(x);
           }
         } else {
 #pragma unroll
-          for (int m = 0; m < MTH; ++m) x2[m] = act4<ACT>(load_bias4(bx, 16 * m + 4 * g, C / 2));
+          for (int m = 0; m < MTH; ++m) x2[m] = act4<ACT>(ldq4(bx + 16 * m + 4 * g));
         }
         LD cur;
         cur.load_quads(CATp(F, q), MTH, MT);
@@ -337,7 +344,7 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         cur.store_quads(CATp(F, q), 0, MTH);
         f32x4 acc[MT];
 #pragma unroll
-        for (int m = 0; m < MT; ++m) acc[m] = load_bias4(bias + 3 * C, 16 * m + 4 * g, C);
+        for (int m = 0; m < MT; ++m) acc[m] = ldq4(bias + 3 * C + 16 * m + 4 * g);
         for (int t = 0; t < 9; ++t) {
           const int r = i + t / 3 - 2, c = j + t % 3 - 2;
           if (r < 0 || c < 0) continue;
@@ -393,20 +400,29 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         // are requested before step l computes, so their latency hides behind
         // step l's dependent MFMA chain; weights (L2) are requested at the
         // start of their own step.
+        // this site's pixel offsets inside a buffer (32-bit, per lane): the
+        // loads then take a uniform buffer base plus one offset register
+        uint32_t o_hd[2], o_cd[8];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) o_hd[t] = (uint32_t)((j + t - 2 >= 0 ? i * W + j + t - 2 : HW) * PB + lane * LB);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int r = i + t / 3 - 2, c = j + t % 3 - 2;
+          o_cd[t] = (uint32_t)((r >= 0 && c >= 0 ? r * W + c : HW) * PB + lane * LB);
+        }
+        const uint32_t o_cur = (uint32_t)(q * PB + lane * LB);
         auto issue = [&](HLoads<PREC, C, BATCH>& L, int l) {
+          const uint8_t* cb = base + (int64_t)l * BUF;
           if constexpr (BATCH) {
             if (l > 0) {
+              const uint8_t* hb = base + (int64_t)(nb + l - 1) * BUF;
 #pragma unroll
-              for (int t = 0; t < 2; ++t)
-                if (j + t - 2 >= 0) L.hd[t].load(HAp(l, i * W + j + t - 2));
+              for (int t = 0; t < 2; ++t) L.hd[t].load(hb + o_hd[t]);
             }
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-              const int r = i + t / 3 - 2, c = j + t % 3 - 2;
-              if (r >= 0 && c >= 0) L.cd[t].load(CATp(l, r * W + c));
-            }
+            for (int t = 0; t < 8; ++t) L.cd[t].load(cb + o_cd[t]);
           }
-          L.cur2.load_quads(CATp(l, q), MTH, MT);
+          L.cur2.load_quads(cb + o_cur, MTH, MT);
         };
         // One block step: its own loads (biases, weights, the first block's
         // spins) are requested first, then the next step's activations
@@ -431,11 +447,11 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           const float* bl = a.aux + lay.aux_bias(l);
           f32x4 hacc[MT], cacc[MT], xb[MTH];
 #pragma unroll
-          for (int m = 0; m < MT; ++m) hacc[m] = load_bias4(bl + C, 16 * m + 4 * g, C);
+          for (int m = 0; m < MT; ++m) hacc[m] = ldq4(bl + C + 16 * m + 4 * g);
 #pragma unroll
-          for (int m = 0; m < MT; ++m) cacc[m] = load_bias4(bl + 3 * C, 16 * m + 4 * g, C);
+          for (int m = 0; m < MT; ++m) cacc[m] = ldq4(bl + 3 * C + 16 * m + 4 * g);
 #pragma unroll
-          for (int m = 0; m < MTH; ++m) xb[m] = load_bias4(bl + 2 * C, 16 * m + 4 * g, C / 2);
+          for (int m = 0; m < MTH; ++m) xb[m] = ldq4(bl + 2 * C + 16 * m + 4 * g);
           frag wh[BATCH ? 3 : 1][MT][KS], wc[BT][MT][KS], wx[MTH][KS];
 #pragma unroll
           for (int m = 0; m < MTH; ++m)
@@ -468,8 +484,9 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           flush();
           if (l == 0) {
             const float* kh = a.aux + lay.aux_kh0(nb);
+#pragma unroll
             for (int t = 0; t < 3; ++t) {
-              if (j + t - 2 < 0) continue;
+              if (!BATCH && j + t - 2 < 0) continue;  // (batched: sv = 0 adds zeros)
 #pragma unroll
               for (int m = 0; m < MT; ++m)
                 hacc[m] += *reinterpret_cast<const f32x4*>(kh + t * C + 16 * m + 4 * g) * sv[t];
@@ -478,7 +495,7 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
 #pragma unroll UNR
             for (int t = 0; t < 3; ++t) {
               const int c = j + t - 2;
-              if (c < 0) continue;
+              if (!BATCH && c < 0) continue;  // (batched: the zero pixel)
               frag b[KS];
               if (t == 2) {
                 d_to_b<PREC, MT, KS>(hcur, b);
@@ -517,7 +534,7 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
 #pragma unroll UNR
           for (int t = 0; t < 9; ++t) {
             const int r = i + t / 3 - 2, c = j + t % 3 - 2;
-            if (r < 0 || c < 0) continue;
+            if (!BATCH && (r < 0 || c < 0)) continue;  // (batched: the zero pixel)
             frag b[KS];
             if (t == 8) {
               L.cur2.template frags<KS>(b);
@@ -626,10 +643,10 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         auto vfetch = [&](VPre& V, int l) {
           const float* bl = a.aux + lay.aux_bias(l);
 #pragma unroll
-          for (int m = 0; m < MT; ++m) V.bv0[m] = load_bias4(bl, 16 * m + 4 * g, C);
+          for (int m = 0; m < MT; ++m) V.bv0[m] = ldq4(bl + 16 * m + 4 * g);
 #pragma unroll
           for (int m = 0; m < MTH; ++m) {
-            V.by0[m] = load_bias4(bl + 2 * C + C / 2, 16 * m + 4 * g, C / 2);
+            V.by0[m] = ldq4(bl + 2 * C + C / 2 + 16 * m + 4 * g);
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) V.wy[m][ks] = wfrag(lay.off_y(l) + m * KS + ks);
           }
@@ -733,10 +750,10 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
             for (int m = 0; m < MT; ++m) kv0[t][m] = *reinterpret_cast<const f32x4*>(kv + t * C + 16 * m + 4 * g);
           const float* bl = a.aux + lay.aux_bias(0);
 #pragma unroll
-          for (int m = 0; m < MT; ++m) VB.bv0[m] = load_bias4(bl, 16 * m + 4 * g, C);
+          for (int m = 0; m < MT; ++m) VB.bv0[m] = ldq4(bl + 16 * m + 4 * g);
 #pragma unroll
           for (int m = 0; m < MTH; ++m) {
-            VB.by0[m] = load_bias4(bl + 2 * C + C / 2, 16 * m + 4 * g, C / 2);
+            VB.by0[m] = ldq4(bl + 2 * C + C / 2 + 16 * m + 4 * g);
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) VB.wy[m][ks] = wfrag(lay.off_y(0) + m * KS + ks);
           }
@@ -755,10 +772,10 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
       frag wv[BT][MT][KS], wy[MTH][KS];
       f32x4 bv0[MT], by0[MTH];
 #pragma unroll
-      for (int m = 0; m < MT; ++m) bv0[m] = load_bias4(bl, 16 * m + 4 * g, C);
+      for (int m = 0; m < MT; ++m) bv0[m] = ldq4(bl + 16 * m + 4 * g);
 #pragma unroll
       for (int m = 0; m < MTH; ++m) {
-        by0[m] = load_bias4(bl + 2 * C + C / 2, 16 * m + 4 * g, C / 2);
+        by0[m] = ldq4(bl + 2 * C + C / 2 + 16 * m + 4 * g);
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) wy[m][ks] = wfrag(lay.off_y(l) + m * KS + ks);
       }
@@ -889,7 +906,7 @@ bool fast_sampler_supported(int channels, int precision) { return fast_channels(
 int64_t fast_tile_bytes(int HW, int nb, int channels, int precision) {
   const int esz = precision == FK_PREC_FP32 ? 4 : 2;
   const int64_t pb = 16LL * channels * esz;
-  return (int64_t)(3 * nb - 2) * HW * pb + (int64_t)HW * 16 * 4;
+  return (int64_t)(3 * nb - 2) * (HW + 1) * pb + (int64_t)HW * 16 * 4;
 }
 
 template <int PREC, int C, bool WR>
