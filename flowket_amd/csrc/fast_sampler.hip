@@ -144,6 +144,16 @@ constexpr int fast_lb() {
 }
 inline int fast_vrow_lds(int W, int lb) { return (6 * W + 1) * 64 * lb + 3 * W * 16 * 4; }
 
+// Whether a block step's operands are all requested ahead of its MFMAs (the
+// tap fragments are small: bf16/fp16 C=32), and the tap-loop unroll that goes
+// with it.
+template <int PREC, int C>
+struct FastBatch {
+  static constexpr bool value =
+      (C / 16) * (C / Prec<PREC>::KW) * (int)(sizeof(typename Prec<PREC>::frag) / 4) <= 8;
+  static constexpr int unr = value ? 9 : 1;
+};
+
 // Four consecutive biases of one lane group (16-byte aligned: every bias
 // offset of the aux buffer is a multiple of 4 floats, and channels 16 m + 4 g
 // .. + 3 stay inside the C or C/2 biases of the M-tiles that read them).
@@ -169,10 +179,11 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
   const frag* wall = reinterpret_cast<const frag*>(a.w);
 
   uint8_t* base = a.cache + (int64_t)blockIdx.x * a.tile_bytes;
-  // every buffer holds HW pixels and, at index HW, a zero pixel that padding
-  // taps read (their MFMAs add exact zeros, as the fused forward's do), so
-  // the batched steps have no per-tap branches
-  const int64_t BUF = (int64_t)(HW + 1) * PB;
+  // every buffer holds HW pixels, at index HW a zero pixel that padding taps
+  // read (their MFMAs add exact zeros, as the fused forward's do) and at HW+1
+  // a pixel that takes the stores a step has nothing for, so the batched
+  // steps have no branches around their memory operations
+  const int64_t BUF = (int64_t)(HW + 2) * PB;
   auto CATp = [&](int l, int q) { return base + (int64_t)l * BUF + (int64_t)q * PB + lane * LB; };
   auto HAp = [&](int l, int q) { return base + (int64_t)(nb + l - 1) * BUF + (int64_t)q * PB + lane * LB; };
   auto VAp = [&](int l, int q) { return base + (int64_t)(2 * nb - 1 + l - 1) * BUF + (int64_t)q * PB + lane * LB; };
@@ -181,9 +192,8 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
   auto wfrag = [&](int64_t fragidx) { return wall[fragidx * 64 + lane]; };
   // batch every load of a block step ahead of its MFMAs when a tap's
   // fragments are small (bf16/fp16 C=32); wider nets load tap by tap
-  constexpr bool BATCH = MT * KS * (int)(sizeof(frag) / 4) <= 8;
+  constexpr bool BATCH = FastBatch<PREC, C>::value;
   constexpr int BT = BATCH ? 9 : 1;
-  constexpr int UNR = BATCH ? 9 : 1;  // tap loops unrolled only with batched loads
   static_assert(!WR || BATCH, "the whole-row vertical pass needs the batched loads");
   constexpr int VROW = VROW_MAX;
   extern __shared__ __align__(16) uint8_t s_dyn[];
@@ -434,15 +444,16 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         // operations complete in issue order, so a store issued before a load
         // would make the wait for the load wait for the store as well.
         LD pcx, phn;
-        uint8_t *ppc = nullptr, *pph = nullptr;
+        uint8_t* const trash = base + (int64_t)(HW + 1) * PB + lane * LB;
+        uint8_t *ppc = trash, *pph = trash;
         auto flush = [&]() {
-          if (ppc) {
-            pcx.store_quads(ppc, 0, MTH);
-            phn.store(pph);
-            ppc = nullptr;
-          }
+          pcx.store_quads(ppc, 0, MTH);
+          phn.store(pph);
+          ppc = pph = trash;
         };
-        auto step = [&](HLoads<PREC, C, BATCH>& L, int l, HLoads<PREC, C, BATCH>& Ln, int ln) {
+        // FIRST: block 0, whose horizontal input is the spins
+        auto step = [&](auto first, HLoads<PREC, C, BATCH>& L, int l, HLoads<PREC, C, BATCH>& Ln, int ln) {
+          constexpr bool FIRST = decltype(first)::value;
           const int kind = block_kind(l, nb);
           const float* bl = a.aux + lay.aux_bias(l);
           f32x4 hacc[MT], cacc[MT], xb[MTH];
@@ -458,12 +469,20 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) wx[m][ks] = wfrag(lay.off_x(l) + m * KS + ks);
           float sv[3] = {0.0f, 0.0f, 0.0f};
-          if (l == 0) {
+          f32x4 kh0[FIRST ? 3 : 1][MT];
+          if constexpr (FIRST) {
+#pragma unroll
             for (int t = 0; t < 2; ++t) {
               const int c = j + t - 2;
-              if (c >= 0) sv[t] = spins[(i * W + c) * 16 + n];
+              const float v = spins[(i * W + (c >= 0 ? c : 0)) * 16 + n];
+              sv[t] = c >= 0 ? v : 0.0f;
             }
             sv[2] = s;
+            const float* kh = a.aux + lay.aux_kh0(nb);
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+#pragma unroll
+              for (int m = 0; m < MT; ++m) kh0[t][m] = ldq4(kh + t * C + 16 * m + 4 * g);
           } else if constexpr (BATCH) {
 #pragma unroll
             for (int t = 0; t < 3; ++t)
@@ -480,19 +499,22 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks) wc[t][m][ks] = wfrag(lay.off_c(l) + (t * MT + m) * KS + ks);
           }
-          if (ln >= 0) issue(Ln, ln);
+          // scheduling fences: the compiler would otherwise sink loads next
+          // to their first use, and a load issued after the prefetch makes
+          // its wait cover the prefetch too
+          __builtin_amdgcn_sched_barrier(0);
+          if constexpr (BATCH) issue(Ln, ln);
           flush();
-          if (l == 0) {
-            const float* kh = a.aux + lay.aux_kh0(nb);
+          __builtin_amdgcn_sched_barrier(0);
+          if constexpr (FIRST) {
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
               if (!BATCH && j + t - 2 < 0) continue;  // (batched: sv = 0 adds zeros)
 #pragma unroll
-              for (int m = 0; m < MT; ++m)
-                hacc[m] += *reinterpret_cast<const f32x4*>(kh + t * C + 16 * m + 4 * g) * sv[t];
+              for (int m = 0; m < MT; ++m) hacc[m] += kh0[t][m] * sv[t];
             }
           } else {
-#pragma unroll UNR
+#pragma unroll FastBatch<PREC, C>::unr
             for (int t = 0; t < 3; ++t) {
               const int c = j + t - 2;
               if (!BATCH && c < 0) continue;  // (batched: the zero pixel)
@@ -531,7 +553,7 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           }
           pcx = L.cur2;
           ppc = CATp(l, q);
-#pragma unroll UNR
+#pragma unroll FastBatch<PREC, C>::unr
           for (int t = 0; t < 9; ++t) {
             const int r = i + t / 3 - 2, c = j + t % 3 - 2;
             if (!BATCH && (r < 0 || c < 0)) continue;  // (batched: the zero pixel)
@@ -573,15 +595,23 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         };
         HLoads<PREC, C, BATCH> LA, LB;
         if constexpr (BATCH) {
-          if (F > 0) issue(LA, 0);
-          for (int l = 0; l < F; l += 2) {
-            step(LA, l, LB, l + 1 < F ? l + 1 : -1);
-            if (l + 1 < F) step(LB, l + 1, LA, l + 2 < F ? l + 2 : -1);
+          // every step requests a next step's activations (the last one its
+          // own again), so the memory operations per step are fixed
+          if (F > 0) {
+            issue(LA, 0);
+            step(std::true_type{}, LA, 0, LB, 1 < F ? 1 : 0);
           }
+          int l = 1;
+          for (; l + 1 < F; l += 2) {
+            step(std::false_type{}, LB, l, LA, l + 1);
+            step(std::false_type{}, LA, l + 1, LB, l + 2 < F ? l + 2 : l + 1);
+          }
+          if (l < F) step(std::false_type{}, LB, l, LA, l);
         } else {
           for (int l = 0; l < F; ++l) {
             issue(LA, l);
-            step(LA, l, LB, -1);
+            if (l == 0) step(std::true_type{}, LA, l, LB, -1);
+            else step(std::false_type{}, LA, l, LB, -1);
           }
         }
         flush();
@@ -825,7 +855,7 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
               acc[m] += *reinterpret_cast<const f32x4*>(kv + t * C + 16 * m + 4 * g) * sv[t];
           }
         } else {
-#pragma unroll UNR
+#pragma unroll FastBatch<PREC, C>::unr
           for (int t = 0; t < 9; ++t) {
             const int r = i + t / 3 - 2, c = j + t % 3 - 1;
             if (r < 0 || c < 0 || c >= W) continue;
@@ -906,7 +936,7 @@ bool fast_sampler_supported(int channels, int precision) { return fast_channels(
 int64_t fast_tile_bytes(int HW, int nb, int channels, int precision) {
   const int esz = precision == FK_PREC_FP32 ? 4 : 2;
   const int64_t pb = 16LL * channels * esz;
-  return (int64_t)(3 * nb - 2) * (HW + 1) * pb + (int64_t)HW * 16 * 4;
+  return (int64_t)(3 * nb - 2) * (HW + 2) * pb + (int64_t)HW * 16 * 4;
 }
 
 template <int PREC, int C, bool WR>
