@@ -559,9 +559,13 @@ __device__ __forceinline__ void stage_in(float* dst, int SC, const float* src, i
 
 // out[p][co_off + co] = bias[co] + sum_{tap, ci} K[tap][ci][co] in[src(tap, p)][ci],
 // the kernel given transposed (KT[tap][co][ci], transpose_kernels_kernel)
-__device__ void mconv_fwd(float* out, int cs_out, int co_off, const float* in, int cs_in, int cin, int cout,
-                          const float* KT, const float* bias, const int16_t* tab, int ntap, int HW,
-                          bool accumulate = false) {
+// The same contraction with the store left to post(p, co_off + co, r), r =
+// bias + the sums of channels co..co+3 of pixel p: the elementwise work that
+// follows a convolution (activation, residual) runs in its epilogue.
+template <class Post>
+__device__ __forceinline__ void mconv_fwd_post(int co_off, const float* in, int cs_in, int cin, int cout,
+                                               const float* KT, const float* bias, const int16_t* tab, int ntap,
+                                               int HW, Post post) {
   const int lane = threadIdx.x & 63, l16 = lane & 15, kk = lane >> 4;
   const f32x4 z = {0.f, 0.f, 0.f, 0.f};
   wave_gemm4(
@@ -580,10 +584,18 @@ __device__ void mconv_fwd(float* out, int cs_out, int co_off, const float* in, i
         const int co = 16 * mt + 4 * kk;
         f32x4 r = c;
         if (bias) r += *reinterpret_cast<const f32x4*>(bias + co);
-        f32x4* dst = reinterpret_cast<f32x4*>(out + (int64_t)p * cs_out + co_off + co);
-        if (accumulate) r += *dst;
-        *dst = r;
+        post(p, co_off + co, r);
       });
+}
+
+__device__ void mconv_fwd(float* out, int cs_out, int co_off, const float* in, int cs_in, int cin, int cout,
+                          const float* KT, const float* bias, const int16_t* tab, int ntap, int HW,
+                          bool accumulate = false) {
+  mconv_fwd_post(co_off, in, cs_in, cin, cout, KT, bias, tab, ntap, HW, [&](int p, int co, f32x4 r) {
+    f32x4* dst = reinterpret_cast<f32x4*>(out + (int64_t)p * cs_out + co);
+    if (accumulate) r += *dst;
+    *dst = r;
+  });
 }
 
 // din[q][ci] = sum_{tap, co} K[tap][ci][co] dout[inv(tap, q)][co_off + co]
@@ -718,7 +730,7 @@ __device__ void build_tap_tables(int16_t* ftab, int16_t* itab, int H, int W) {
 // scratch tensors of the primal and the tangent.
 // STG: the kxk and 1xk convolutions read their inputs from the LDS staging
 // tiles S0, S1 (row stride SC); otherwise from L2.
-template <int ACT, bool STG>
+template <int ACT, bool STG, bool TAN>
 __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan, const float* kt, const float* tkt,
                                              float* ws, float* wt, const int16_t* ftab, int H, int W, int C, int nb,
                                              float* t0, float* t1, float* u0, float* u1, float* S0, float* S1,
@@ -736,7 +748,31 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
     float* CAT = ws_tensor(ws, HW, C, blk, T_CAT);
     float* VO = ws_tensor(ws, HW, C, blk, T_VO);
     float* HO = ws_tensor(ws, HW, C, blk, T_HO);
-    if (first) {
+    // C-channel tensors in 16-byte units: element (p, co..co+3) is quad (p C + co) / 4
+    auto q4 = [&](int p, int co) { return (p * C + co) >> 2; };
+    const float* VOr = kb ? ws_tensor(ws, HW, C, blk - 2, T_VO) : nullptr;  // the residuals
+    const float* HOr = kb ? ws_tensor(ws, HW, C, blk - 2, T_HO) : nullptr;
+    if (!TAN && !first) {
+      // primal only: the activations and residuals run in the convolutions'
+      // epilogues (the same operations on the same values as the passes of
+      // the tangent form below)
+      const float* Vb = STG ? S0 : Vin;
+      const float* Hb = STG ? S1 : Hin;
+      const int vs = STG ? SC : C;
+      if constexpr (STG) {
+        stage_in(S0, SC, Vin, C, 0, C, HW);
+        stage_in(S1, SC, Hin, C, 0, C, HW);
+        __syncthreads();
+      }
+      mconv_fwd_post(0, Vb, vs, C, C, kt + o.kv, prm + o.bv, ftab + GT_V * HW, 9, HW, [&](int p, int co, f32x4 v) {
+        const int i = q4(p, co);
+        const f32x4 va = bact4<ACT>(v);
+        S4(VA, i) = va;
+        if (!fin) S4(VO, i) = kb ? bact4<ACT>(L4(VOr, i) + v) : va;
+      });
+      mconv_fwd_post(0, Hb, vs, C, C, kt + o.kh, prm + o.bh, ftab + GT_H * HW, 3, HW,
+                     [&](int p, int co, f32x4 h) { S4(XA, q4(p, co)) = bact4<ACT>(h); });
+    } else if (first) {
       conv_fwd(t0, C, Vin, 1, 1, C, prm + o.kv, prm + o.bv, 0, 9, H, W);
       conv_fwd(t1, C, Hin, 1, 1, C, prm + o.kh, prm + o.bh, 1, 3, H, W);
       if (tan) {  // the spins carry no tangent
@@ -765,14 +801,14 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
       }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < HW * C / 4; i += blockDim.x) {
+    for (int i = threadIdx.x; i < ((TAN || first) ? HW * C / 4 : 0); i += blockDim.x) {
       const f32x4 v = L4(t0, i);
       const f32x4 va = bact4<ACT>(v), xa = bact4<ACT>(L4(t1, i));
       S4(VA, i) = va;
       S4(XA, i) = xa;
       f32x4 vo = {0.f, 0.f, 0.f, 0.f};
       if (!fin) {
-        vo = kb ? bact4<ACT>(L4(ws_tensor(ws, HW, C, blk - 2, T_VO), i) + v) : va;
+        vo = kb ? bact4<ACT>(L4(VOr, i) + v) : va;
         S4(VO, i) = vo;
       }
       if (tan) {
@@ -785,10 +821,16 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
         }
       }
     }
-    __syncthreads();
-    mconv_fwd(CAT, C, 0, XA, C, C, C / 2, kt + o.kx, prm + o.bx, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW);
-    mconv_fwd(CAT, C, C / 2, VA, C, C, C / 2, kt + o.ky, prm + o.by, ftab + GT_Y * HW, 1, HW);
-    if (tan) {
+    if (TAN || first) __syncthreads();
+    if constexpr (!TAN) {
+      auto cat_post = [&](int p, int co, f32x4 c) { S4(CAT, q4(p, co)) = bact4<ACT>(c); };
+      mconv_fwd_post(0, XA, C, C, C / 2, kt + o.kx, prm + o.bx, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW, cat_post);
+      mconv_fwd_post(C / 2, VA, C, C, C / 2, kt + o.ky, prm + o.by, ftab + GT_Y * HW, 1, HW, cat_post);
+    } else {
+      mconv_fwd(CAT, C, 0, XA, C, C, C / 2, kt + o.kx, prm + o.bx, ftab + (fin ? GT_XF : GT_X) * HW, 1, HW);
+      mconv_fwd(CAT, C, C / 2, VA, C, C, C / 2, kt + o.ky, prm + o.by, ftab + GT_Y * HW, 1, HW);
+    }
+    if (TAN) {
       float* CATd = ws_tensor(wt, HW, C, blk, T_CAT);
       const float* XAd = ws_tensor(wt, HW, C, blk, T_XA);
       const float* VAd = ws_tensor(wt, HW, C, blk, T_VA);
@@ -799,20 +841,30 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
       mconv_fwd(CATd, C, C / 2, VA, C, C, C / 2, tkt + o.ky, nullptr, ftab + GT_Y * HW, 1, HW, true);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < HW * C / 4; i += blockDim.x) {
-      const f32x4 c = bact4<ACT>(L4(CAT, i));
-      S4(CAT, i) = c;
-      if (tan) S4(ws_tensor(wt, HW, C, blk, T_CAT), i) *= bdact4<ACT>(c);
+    if constexpr (TAN) {
+      for (int i = threadIdx.x; i < HW * C / 4; i += blockDim.x) {
+        const f32x4 c = bact4<ACT>(L4(CAT, i));
+        S4(CAT, i) = c;
+        S4(ws_tensor(wt, HW, C, blk, T_CAT), i) *= bdact4<ACT>(c);
+      }
+      __syncthreads();
     }
-    __syncthreads();
     const float* Cb = STG ? S0 : CAT;
     const int cs = STG ? SC : C;
     if constexpr (STG) {
       stage_in(S0, SC, CAT, C, 0, C, HW);
       __syncthreads();
     }
+    if constexpr (!TAN) {
+      mconv_fwd_post(0, Cb, cs, C, C, kt + o.kc, prm + o.bc, ftab + GT_C * HW, 9, HW, [&](int p, int co, f32x4 h) {
+        const int i = q4(p, co);
+        S4(HO, i) = kb ? bact4<ACT>(L4(HOr, i) + h) : bact4<ACT>(h);
+      });
+      __syncthreads();
+      continue;
+    }
     mconv_fwd(t0, C, 0, Cb, cs, C, C, kt + o.kc, prm + o.bc, ftab + GT_C * HW, 9, HW);
-    if (tan) {
+    {
       mconv_fwd(u0, C, 0, ws_tensor(wt, HW, C, blk, T_CAT), C, C, C, kt + o.kc, tan + o.bc, ftab + GT_C * HW, 9,
                 HW);
       __syncthreads();
@@ -821,9 +873,9 @@ __device__ __forceinline__ void mfma_forward(const float* prm, const float* tan,
     __syncthreads();
     for (int i = threadIdx.x; i < HW * C / 4; i += blockDim.x) {
       const f32x4 h = L4(t0, i);
-      const f32x4 ho = kb ? bact4<ACT>(L4(ws_tensor(ws, HW, C, blk - 2, T_HO), i) + h) : bact4<ACT>(h);
+      const f32x4 ho = kb ? bact4<ACT>(L4(HOr, i) + h) : bact4<ACT>(h);
       S4(HO, i) = ho;
-      if (tan) {
+      {
         const f32x4 din = kb ? L4(ws_tensor(wt, HW, C, blk - 2, T_HO), i) + L4(u0, i) : L4(u0, i);
         S4(ws_tensor(wt, HW, C, blk, T_HO), i) = bdact4<ACT>(ho) * din;
       }
@@ -1000,7 +1052,8 @@ __global__ __launch_bounds__(256, MODE == GRAD_VJP ? 3 : 4) void convar_grad_mfm
     for (int q = threadIdx.x; q < HW; q += blockDim.x) sp[q] = (float)a.spins[b * HW + q];
     if (threadIdx.x == 0) s_lp[0] = s_lp[1] = 0.0f;
     __syncthreads();
-    mfma_forward<ACT, STG>(prm, nullptr, a.kt, nullptr, ws, nullptr, ftab, H, W, C, nb, t0, t1, nullptr, nullptr,
+    mfma_forward<ACT, STG, false>(prm, nullptr, a.kt, nullptr, ws, nullptr, ftab, H, W, C, nb, t0, t1, nullptr,
+                                  nullptr,
                            S0, S1, a.stage);
     const int64_t ho = head_offset(nb, C);
     const float* HF = ws_tensor(ws, HW, C, F, T_HO);
@@ -1066,7 +1119,7 @@ __global__ __launch_bounds__(256, 4) void convar_jvp_mfma_kernel(BwdArgs a) {
     for (int q = threadIdx.x; q < HW; q += blockDim.x) sp[q] = (float)a.spins[b * HW + q];
     if (threadIdx.x < 4) s_acc[threadIdx.x] = 0.0f;
     __syncthreads();
-    mfma_forward<ACT, STG>(prm, tan, a.kt, a.tkt, ws, wt, ftab, H, W, C, nb, t0, t1, u0, u1, S0, S1, a.stage);
+    mfma_forward<ACT, STG, true>(prm, tan, a.kt, a.tkt, ws, wt, ftab, H, W, C, nb, t0, t1, u0, u1, S0, S1, a.stage);
     const int64_t ho = head_offset(nb, C);
     const float* HF = ws_tensor(ws, HW, C, F, T_HO);
     const float* HFd = ws_tensor(wt, HW, C, F, T_HO);
