@@ -74,6 +74,9 @@ def parse():
     ap.add_argument('--profile-grad', type=int, default=0, metavar='N',
                     help='only run the per-sample gradient kernel twice, then one batched VJP and one JVP, over '
                          'N fast samples of the headline network (a short program for rocprofv3) and exit')
+    ap.add_argument('--profile-sampler', type=int, default=0, metavar='N',
+                    help='only draw N fast samples of the headline network twice (FP16; a short program for '
+                         'rocprofv3) and exit')
     ap.add_argument('--sr-large', action='store_true',
                     help='only run one SR step of the largest configuration (20x20, depth 24, 64 channels, '
                          '16384 fast samples, implicit S products, 50 CG iterations; minutes) and exit')
@@ -129,6 +132,19 @@ def profile_grad(n):
     m.jvp(spins, torch.randn(m.n_trainable, device=spins.device))
     torch.cuda.synchronize()
     print(json.dumps({'profile_grad': {'samples': n, 'params': m.n_params}}))
+    return 0
+
+
+def profile_sampler(n):
+    import torch
+    import flowket_amd as fk
+    m = fk.ConvNetAutoregressive2D((H, W), depth=DEPTH, num_of_channels=CHANNELS, activation=ACT,
+                                   weights_normalization=False, precision='fp16', seed=1234)
+    fs = fk.FastAutoregressiveSampler(m, n, seed=7)
+    for _ in range(2):
+        next(fs)
+    torch.cuda.synchronize()
+    print(json.dumps({'profile_sampler': {'samples': n}}))
     return 0
 
 
@@ -671,6 +687,8 @@ def main():
                                args.row_first)
     if args.profile_grad:
         return profile_grad(args.profile_grad)
+    if args.profile_sampler:
+        return profile_sampler(args.profile_sampler)
     if args.sr_large:
         print(json.dumps({'sr_large': bench_sr_large()}))
         return 0
