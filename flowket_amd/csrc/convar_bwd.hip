@@ -1018,7 +1018,7 @@ enum { GRAD_O = 0, GRAD_VJP = 1 };
 // pass with the combined head cotangent) into its own partial gradient row.
 // STG: staging tiles in LDS (a.stage is their row stride).
 template <int ACT, int MODE, bool STG>
-__global__ __launch_bounds__(256, MODE == GRAD_VJP ? 3 : 4) void convar_grad_mfma_kernel(BwdArgs a) {
+__global__ __launch_bounds__(256, 4) void convar_grad_mfma_kernel(BwdArgs a) {
   extern __shared__ int16_t tabs[];  // GT_N forward rows, then GT_N inverse rows, of HW entries
   __shared__ float s_lp[2];
   const int H = a.H, W = a.W, HW = a.HW, C = a.C, nb = a.nb;
@@ -1183,7 +1183,7 @@ int64_t implicit_grid(int64_t batch, int per_cu) {
   const int64_t g = (int64_t)per_cu * device_cu_count();
   return batch < g ? (batch > 0 ? batch : 1) : g;
 }
-constexpr int VJP_PER_CU = 3, JVP_PER_CU = 4;
+constexpr int VJP_PER_CU = 4, JVP_PER_CU = 4;
 
 size_t grad_lds_bytes(int HW) {
   return (size_t)2 * GT_N * HW * sizeof(int16_t) + 4 + (size_t)4 * HW * sizeof(float);
