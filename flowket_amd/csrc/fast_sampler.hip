@@ -107,12 +107,16 @@ struct LaneData<FK_PREC_FP32, C> {
 // Per-lane activation loads of one horizontal-stack block step of the fast
 // sampler (from the per-tile cache in HBM): the two earlier horizontal-stack
 // pixels, the eight earlier concatenation pixels, and the y half of the
-// current one (the batched form; otherwise only the current pixel).
-template <int PREC, int C, bool BATCH>
+// current one (the batched form; otherwise only the current pixel).  With
+// row partials (RP) the six concatenation pixels of the two rows above are
+// replaced by the partial sum of their taps (FP32, bias included), computed
+// for the whole row after the row above it.
+template <int PREC, int C, bool BATCH, bool RP = false>
 struct HLoads {
   LaneData<PREC, C> hd[BATCH ? 2 : 1];
-  LaneData<PREC, C> cd[BATCH ? 8 : 1];
+  LaneData<PREC, C> cd[RP ? 2 : (BATCH ? 8 : 1)];
   LaneData<PREC, C> cur2;
+  f32x4 r[RP ? C / 16 : 1];
 };
 
 // Per-lane activation loads of one pixel of the vertical-stack pass: the
@@ -209,6 +213,93 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
     for (int j = 0; j < W; ++j) d.store_quads(CATp(l, j), MTH, MT);
   }
 
+  // Whole-row kernel (WR) staging: LDS rows of W pixels (the vertical pass
+  // and the row partials below), then one zero pixel and the spins of three
+  // rows.  Direct global-to-LDS loads move 16 bytes per lane,
+  // lane-consecutive, which is the pixels' lane layout; every lane reads back
+  // only its own bytes.
+  uint8_t* const s_vrow = s_dyn;
+  uint8_t* const vzero = s_dyn + 6 * W * PB + lane * LB;  // the zero pixel
+  auto vrow = [&](int which, int c) { return s_vrow + ((which * W + c) * 64 + lane) * LB; };
+  auto vdma = [&](const uint8_t* src, int which, int c) {
+    static_assert(!WR || LB == 16, "direct LDS loads move 16 bytes per lane");
+    __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)(src),
+                                     (void __attribute__((address_space(3)))*)(s_vrow + (which * W + c) * PB), 16, 0,
+                                     0);
+  };
+
+  // Row partials (WR).  R(l, j) of row i is the bias plus taps 0..5 of block
+  // l's 3x3 concatenation conv at site (i, j) -- the taps that read rows i-2
+  // and i-1 -- in FP32, for every block l = 0..F and column j.  The step of
+  // site (i, j) continues the same accumulator with taps 6..8, so every sum
+  // is the one the fused forward forms, in the same order.  Row i's partials
+  // of block l are made in the vertical pass of row i-1, block l: the block's
+  // two rows (VROW_MAX pixels each, in registers; columns from W on and rows
+  // above the lattice read the cache's zero pixel) and weights are requested
+  // with the pass's own operands, so they share its one wait per block.
+  // Columns from W on fill spare slots.
+  uint8_t* const rbuf = reinterpret_cast<uint8_t*>(spins + HW * 16);
+  constexpr int RB = MT * 16;  // bytes per lane of one partial
+  struct RPre {
+    frag w[6][MT][KS];
+    f32x4 b[MT];
+    LD x[2][VROW_MAX];
+  };
+  // block l's operands for the partials of row i
+  auto rfetch = [&](RPre& R, int l, int i) {
+    const float* bl = a.aux + lay.aux_bias(l) + 3 * C;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) R.b[m] = ldq4(bl + 16 * m + 4 * g);
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) R.w[t][m][ks] = wfrag(lay.off_c(l) + (t * MT + m) * KS + ks);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int row = i - 2 + r;
+#pragma unroll
+      for (int c = 0; c < VROW_MAX; ++c) R.x[r][c].load(CATp(l, row >= 0 && c < W ? row * W + c : HW));
+    }
+  };
+  auto rpixels = [&](const RPre& R, int l) {
+    uint8_t* const rl = rbuf + (int64_t)l * VROW_MAX * 64 * RB + lane * RB;
+#pragma unroll
+    for (int j = 0; j < VROW_MAX; ++j) {
+      f32x4 acc[MT];
+#pragma unroll
+      for (int m = 0; m < MT; ++m) acc[m] = R.b[m];
+#pragma unroll
+      for (int t = 0; t < 6; ++t) {
+        const int c = j + t % 3 - 2;
+        frag b[KS];
+        if (c >= 0) {
+          R.x[t / 3][c >= 0 ? c : 0].template frags<KS>(b);
+        } else {
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) b[ks] = frag{};
+        }
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) acc[m] = P::mfma(R.w[t][m][ks], b[ks], acc[m]);
+      }
+#pragma unroll
+      for (int m = 0; m < MT; ++m) *reinterpret_cast<f32x4*>(rl + j * 64 * RB + 16 * m) = acc[m];
+    }
+  };
+  if constexpr (WR) {
+    // the zero pixel of the LDS rows, and row 0's partials (bias and taps
+    // that read the zero rows above the lattice)
+    LD{}.store(vzero);
+    for (int l = 0; l <= F; ++l) {
+      RPre R;
+      rfetch(R, l, 0);
+      rpixels(R, l);
+    }
+  }
+
   for (int i = 0; i < H; ++i) {
     for (int j = 0; j < W; ++j) {
       const int q = i * W + j;
@@ -219,16 +310,23 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           // Every load of the conditional (weights, biases, the earlier
           // pixels it reads) is requested first and waited for once.
           const float* bias = a.aux + lay.aux_bias(F);
+          // (whole-row kernel: the concatenation conv starts from the row
+          // partial, so only its taps 6..8 and two earlier pixels remain)
+          constexpr int CT0 = WR ? 6 : 0;
           f32x4 bh[MT], bc[MT], bxv[MTH];
 #pragma unroll
           for (int m = 0; m < MT; ++m) bh[m] = ldq4(bias + C + 16 * m + 4 * g);
 #pragma unroll
-          for (int m = 0; m < MT; ++m) bc[m] = ldq4(bias + 3 * C + 16 * m + 4 * g);
+          for (int m = 0; m < MT; ++m) {
+            if constexpr (WR)
+              bc[m] = *reinterpret_cast<const f32x4*>(rbuf + ((int64_t)F * VROW_MAX + j) * 64 * RB + lane * RB + 16 * m);
+            else bc[m] = ldq4(bias + 3 * C + 16 * m + 4 * g);
+          }
 #pragma unroll
           for (int m = 0; m < MTH; ++m) bxv[m] = ldq4(bias + 2 * C + 16 * m + 4 * g);
           const f32x4 hb4 = ldq4(a.aux + lay.aux_head_bias(nb));  // the head's 4 biases (lane group 0)
           const f32x4 zb = g == 0 ? hb4 : f32x4{0.f, 0.f, 0.f, 0.f};
-          frag wh[3][MT][KS], wx[MTH][KS], wc[9][MT][KS], wz[KS];
+          frag wh[3][MT][KS], wx[MTH][KS], wc[9 - CT0][MT][KS], wz[KS];
 #pragma unroll
           for (int t = 0; t < 3; ++t)
 #pragma unroll
@@ -240,20 +338,20 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) wx[m][ks] = wfrag(lay.off_x(F) + m * KS + ks);
 #pragma unroll
-          for (int t = 0; t < 9; ++t)
+          for (int t = CT0; t < 9; ++t)
 #pragma unroll
             for (int m = 0; m < MT; ++m)
 #pragma unroll
-              for (int ks = 0; ks < KS; ++ks) wc[t][m][ks] = wfrag(lay.off_c(F) + (t * MT + m) * KS + ks);
+              for (int ks = 0; ks < KS; ++ks) wc[t - CT0][m][ks] = wfrag(lay.off_c(F) + (t * MT + m) * KS + ks);
 #pragma unroll
           for (int ks = 0; ks < KS; ++ks) wz[ks] = wfrag(lay.head_base(nb) + ks);
-          LD hd[3], cd[8], cur;
+          LD hd[3], cd[8 - CT0], cur;
 #pragma unroll
           for (int t = 0; t < 3; ++t) hd[t].load(HAp(F, j - 3 + t >= 0 ? i * W + j - 3 + t : HW));
 #pragma unroll
-          for (int t = 0; t < 8; ++t) {
+          for (int t = CT0; t < 8; ++t) {
             const int r = i + t / 3 - 2, c = j + t % 3 - 2;
-            cd[t].load(CATp(F, r >= 0 && c >= 0 ? r * W + c : HW));
+            cd[t - CT0].load(CATp(F, r >= 0 && c >= 0 ? r * W + c : HW));
           }
           cur.load_quads(CATp(F, q), MTH, MT);
           f32x4 x2[MTH];
@@ -292,14 +390,14 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
 #pragma unroll
           for (int m = 0; m < MT; ++m) acc[m] = bc[m];
 #pragma unroll
-          for (int t = 0; t < 9; ++t) {
+          for (int t = CT0; t < 9; ++t) {
             frag b[KS];
             if (t == 8) cur.template frags<KS>(b);
-            else cd[t < 8 ? t : 0].template frags<KS>(b);
+            else cd[t < 8 ? t - CT0 : 0].template frags<KS>(b);
 #pragma unroll
             for (int m = 0; m < MT; ++m)
 #pragma unroll
-              for (int ks = 0; ks < KS; ++ks) acc[m] = P::mfma(wc[t][m][ks], b[ks], acc[m]);
+              for (int ks = 0; ks < KS; ++ks) acc[m] = P::mfma(wc[t - CT0][m][ks], b[ks], acc[m]);
           }
           f32x4 ah[MT];
 #pragma unroll
@@ -421,7 +519,11 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           o_cd[t] = (uint32_t)((r >= 0 && c >= 0 ? r * W + c : HW) * PB + lane * LB);
         }
         const uint32_t o_cur = (uint32_t)(q * PB + lane * LB);
-        auto issue = [&](HLoads<PREC, C, BATCH>& L, int l) {
+        // whole-row kernel: the concatenation conv of a step starts from the
+        // row partial and reads two earlier pixels (taps 6, 7) and the current
+        constexpr int CT0 = WR ? 6 : 0;
+        const uint32_t o_r = (uint32_t)(j * 64 * RB + lane * RB);
+        auto issue = [&](HLoads<PREC, C, BATCH, WR>& L, int l) {
           const uint8_t* cb = base + (int64_t)l * BUF;
           if constexpr (BATCH) {
             if (l > 0) {
@@ -430,7 +532,12 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
               for (int t = 0; t < 2; ++t) L.hd[t].load(hb + o_hd[t]);
             }
 #pragma unroll
-            for (int t = 0; t < 8; ++t) L.cd[t].load(cb + o_cd[t]);
+            for (int t = CT0; t < 8; ++t) L.cd[t - CT0].load(cb + o_cd[t]);
+            if constexpr (WR) {
+              const uint8_t* rb = rbuf + (int64_t)l * VROW_MAX * 64 * RB;
+#pragma unroll
+              for (int m = 0; m < MT; ++m) L.r[m] = *reinterpret_cast<const f32x4*>(rb + o_r + 16 * m);
+            }
           }
           L.cur2.load_quads(cb + o_cur, MTH, MT);
         };
@@ -452,18 +559,20 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           ppc = pph = trash;
         };
         // FIRST: block 0, whose horizontal input is the spins
-        auto step = [&](auto first, HLoads<PREC, C, BATCH>& L, int l, HLoads<PREC, C, BATCH>& Ln, int ln) {
+        auto step = [&](auto first, HLoads<PREC, C, BATCH, WR>& L, int l, HLoads<PREC, C, BATCH, WR>& Ln, int ln) {
           constexpr bool FIRST = decltype(first)::value;
           const int kind = block_kind(l, nb);
           const float* bl = a.aux + lay.aux_bias(l);
           f32x4 hacc[MT], cacc[MT], xb[MTH];
 #pragma unroll
           for (int m = 0; m < MT; ++m) hacc[m] = ldq4(bl + C + 16 * m + 4 * g);
+          if constexpr (!WR) {
 #pragma unroll
-          for (int m = 0; m < MT; ++m) cacc[m] = ldq4(bl + 3 * C + 16 * m + 4 * g);
+            for (int m = 0; m < MT; ++m) cacc[m] = ldq4(bl + 3 * C + 16 * m + 4 * g);
+          }
 #pragma unroll
           for (int m = 0; m < MTH; ++m) xb[m] = ldq4(bl + 2 * C + 16 * m + 4 * g);
-          frag wh[BATCH ? 3 : 1][MT][KS], wc[BT][MT][KS], wx[MTH][KS];
+          frag wh[BATCH ? 3 : 1][MT][KS], wc[BATCH ? 9 - CT0 : 1][MT][KS], wx[MTH][KS];
 #pragma unroll
           for (int m = 0; m < MTH; ++m)
 #pragma unroll
@@ -493,11 +602,11 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           }
           if constexpr (BATCH) {
 #pragma unroll
-            for (int t = 0; t < 9; ++t)
+            for (int t = CT0; t < 9; ++t)
 #pragma unroll
               for (int m = 0; m < MT; ++m)
 #pragma unroll
-                for (int ks = 0; ks < KS; ++ks) wc[t][m][ks] = wfrag(lay.off_c(l) + (t * MT + m) * KS + ks);
+                for (int ks = 0; ks < KS; ++ks) wc[t - CT0][m][ks] = wfrag(lay.off_c(l) + (t * MT + m) * KS + ks);
           }
           // scheduling fences: the compiler would otherwise sink loads next
           // to their first use, and a load issued after the prefetch makes
@@ -553,15 +662,19 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           }
           pcx = L.cur2;
           ppc = CATp(l, q);
+          if constexpr (WR) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m) cacc[m] = L.r[m];
+          }
 #pragma unroll FastBatch<PREC, C>::unr
-          for (int t = 0; t < 9; ++t) {
+          for (int t = CT0; t < 9; ++t) {
             const int r = i + t / 3 - 2, c = j + t % 3 - 2;
             if (!BATCH && (r < 0 || c < 0)) continue;  // (batched: the zero pixel)
             frag b[KS];
             if (t == 8) {
               L.cur2.template frags<KS>(b);
             } else if constexpr (BATCH) {
-              L.cd[t].template frags<KS>(b);
+              L.cd[t - CT0].template frags<KS>(b);
             } else {
               LD cd;
               cd.load(CATp(l, r * W + c));
@@ -572,7 +685,7 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
 #pragma unroll
               for (int ks = 0; ks < KS; ++ks) {
                 frag wk;
-                if constexpr (BATCH) wk = wc[t][m][ks];
+                if constexpr (BATCH) wk = wc[t - CT0][m][ks];
                 else wk = wfrag(lay.off_c(l) + (t * MT + m) * KS + ks);
                 cacc[m] = P::mfma(wk, b[ks], cacc[m]);
               }
@@ -593,7 +706,7 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
 #pragma unroll
           for (int m = 0; m < MT; ++m) hcur[m] = hn.get(m);
         };
-        HLoads<PREC, C, BATCH> LA, LB;
+        HLoads<PREC, C, BATCH, WR> LA, LB;
         if constexpr (BATCH) {
           // every step requests a next step's activations (the last one its
           // own again), so the memory operations per step are fixed
@@ -634,18 +747,7 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         // Every lane reads back only the bytes it wrote, except the spins
         // (a barrier after they are staged).
         int cur = 0;
-        uint8_t* s_vrow = s_dyn;
-        uint8_t* vzero = s_dyn + 6 * W * PB + lane * LB;  // the zero pixel
         float* s_vspin = reinterpret_cast<float*>(s_dyn + 6 * W * PB + PB);
-        auto vrow = [&](int which, int c) { return s_vrow + ((which * W + c) * 64 + lane) * LB; };
-        // pixel src (16 bytes per lane) -> LDS row `which`, column c
-        auto vdma = [&](const uint8_t* src, int which, int c) {
-          static_assert(LB == 16, "direct LDS loads move 16 bytes per lane");
-          __builtin_amdgcn_global_load_lds(
-              (const void __attribute__((address_space(1)))*)(src),
-              (void __attribute__((address_space(3)))*)(s_vrow + (which * W + c) * PB), 16, 0, 0);
-        };
-        LD{}.store(vzero);
         for (int idx = lane; idx < 3 * W * 16; idx += 64) {
           const int r = idx / (W * 16), c = (idx / 16) % W, nn = idx % 16;
           const int rr = i + r - 2;
@@ -764,10 +866,15 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         // block ln's operands into Vn, then run the pixel loop
         f32x4 kv0[9][MT];
         VPre V;
+        // (with row i+1's partials of the same block)
+        const bool rnext = i + 1 < H;
         auto vblock = [&](int l) {
           vfetch(V, l);
+          RPre RR;
+          if (rnext) rfetch(RR, l, i + 1);
           __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
           if (l + 1 <= F) vrows(l + 1);
+          if (rnext) rpixels(RR, l);
           vpixels(std::false_type{}, V, kv0, l);
         };
         VPre& VB = V;
@@ -787,8 +894,11 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) VB.wy[m][ks] = wfrag(lay.off_y(0) + m * KS + ks);
           }
+          RPre RR;
+          if (rnext) rfetch(RR, 0, i + 1);
           __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
           if (F >= 1) vrows(1);
+          if (rnext) rpixels(RR, 0);
           vpixels(std::true_type{}, VB, kv0, 0);
         }
         for (int l = 1; l <= F; ++l) vblock(l);
@@ -933,10 +1043,22 @@ static int fast_channels(int channels, int precision) {
 }
 bool fast_sampler_supported(int channels, int precision) { return fast_channels(channels, precision) > 0; }
 
-int64_t fast_tile_bytes(int HW, int nb, int channels, int precision) {
+// whether a net takes the whole-row kernel (16-bit 32-channel nets, rows of
+// at most VROW_MAX pixels; see launch_fast)
+static bool fast_whole_row(int W, int channels, int precision) {
+  return precision != FK_PREC_FP32 && channels == 32 && W <= VROW_MAX;
+}
+
+// Per-tile cache: (3 nb - 2) activation buffers of HW + 2 pixels, the spins
+// and, for the whole-row kernel, one row of concatenation partial sums per
+// block (FP32, MT quads per lane, VROW_MAX columns).
+int64_t fast_tile_bytes(int H, int W, int nb, int channels, int precision) {
   const int esz = precision == FK_PREC_FP32 ? 4 : 2;
   const int64_t pb = 16LL * channels * esz;
-  return (int64_t)(3 * nb - 2) * (HW + 2) * pb + (int64_t)HW * 16 * 4;
+  const int64_t HW = (int64_t)H * W;
+  int64_t b = (int64_t)(3 * nb - 2) * (HW + 2) * pb + HW * 16 * 4;
+  if (fast_whole_row(W, channels, precision)) b += (int64_t)nb * VROW_MAX * 64 * (channels / 16) * 16;
+  return b;
 }
 
 template <int PREC, int C, bool WR>
@@ -959,7 +1081,7 @@ static int launch_fast_wr(const SampArgs& a, int act, int64_t tiles, hipStream_t
 template <int PREC, int C>
 static int launch_fast(const SampArgs& a, int act, int64_t tiles, hipStream_t s) {
   if constexpr (PREC != FK_PREC_FP32 && C == 32) {
-    if (a.W <= VROW_MAX) return launch_fast_wr<PREC, C, true>(a, act, tiles, s);
+    if (fast_whole_row(a.W, C, PREC)) return launch_fast_wr<PREC, C, true>(a, act, tiles, s);
   }
   return launch_fast_wr<PREC, C, false>(a, act, tiles, s);
 }
@@ -974,7 +1096,7 @@ int64_t fk_fast_sampler_workspace_bytes(int32_t H, int32_t W, int32_t depth, int
                                         int64_t batch) {
   if (H < 1 || W < 1 || depth < 1 || batch < 0 || !fast_sampler_supported(channels, precision)) return -1;
   const int64_t tiles = (batch + 15) / 16;
-  return tiles * fast_tile_bytes(H * W, num_causal_blocks(depth), fast_channels(channels, precision), precision);
+  return tiles * fast_tile_bytes(H, W, num_causal_blocks(depth), fast_channels(channels, precision), precision);
 }
 
 int fk_fast_sample(const void* wpacked, const float* aux, int32_t precision, int32_t H, int32_t W, int32_t depth,
@@ -1002,7 +1124,7 @@ int fk_fast_sample(const void* wpacked, const float* aux, int32_t precision, int
   a.cond = cond_logp;
   a.cache = reinterpret_cast<uint8_t*>(workspace);
   const int CP = fast_channels(channels, precision);
-  a.tile_bytes = fast_tile_bytes(H * W, a.nb, CP, precision);
+  a.tile_bytes = fast_tile_bytes(H, W, a.nb, CP, precision);
   const int64_t tiles = (batch + 15) / 16;
   FK_REQUIRE(tiles <= 0x7fffffff, "fk_fast_sample: batch too large");
   hipStream_t s = (hipStream_t)stream;
