@@ -263,30 +263,43 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
       for (int c = 0; c < VROW_MAX; ++c) R.x[r][c].load(CATp(l, row >= 0 && c < W ? row * W + c : HW));
     }
   };
+  // Pixels in groups of RG: the MFMAs of a group's pixels alternate, so the
+  // accumulator chains overlap in the MFMA pipeline.
+  constexpr int RG = 2;
+  static_assert(VROW_MAX % RG == 0, "whole pixel groups");
   auto rpixels = [&](const RPre& R, int l) {
     uint8_t* const rl = rbuf + (int64_t)l * VROW_MAX * 64 * RB + lane * RB;
 #pragma unroll
-    for (int j = 0; j < VROW_MAX; ++j) {
-      f32x4 acc[MT];
+    for (int j0 = 0; j0 < VROW_MAX; j0 += RG) {
+      f32x4 acc[RG][MT];
 #pragma unroll
-      for (int m = 0; m < MT; ++m) acc[m] = R.b[m];
+      for (int u = 0; u < RG; ++u)
+#pragma unroll
+        for (int m = 0; m < MT; ++m) acc[u][m] = R.b[m];
 #pragma unroll
       for (int t = 0; t < 6; ++t) {
-        const int c = j + t % 3 - 2;
-        frag b[KS];
-        if (c >= 0) {
-          R.x[t / 3][c >= 0 ? c : 0].template frags<KS>(b);
-        } else {
+        frag b[RG][KS];
 #pragma unroll
-          for (int ks = 0; ks < KS; ++ks) b[ks] = frag{};
+        for (int u = 0; u < RG; ++u) {
+          const int c = j0 + u + t % 3 - 2;
+          if (c >= 0) {
+            R.x[t / 3][c >= 0 ? c : 0].template frags<KS>(b[u]);
+          } else {
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) b[u][ks] = frag{};
+          }
         }
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
-          for (int ks = 0; ks < KS; ++ks) acc[m] = P::mfma(R.w[t][m][ks], b[ks], acc[m]);
+          for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int u = 0; u < RG; ++u) acc[u][m] = P::mfma(R.w[t][m][ks], b[u][ks], acc[u][m]);
       }
 #pragma unroll
-      for (int m = 0; m < MT; ++m) *reinterpret_cast<f32x4*>(rl + j * 64 * RB + 16 * m) = acc[m];
+      for (int u = 0; u < RG; ++u)
+#pragma unroll
+        for (int m = 0; m < MT; ++m) *reinterpret_cast<f32x4*>(rl + (j0 + u) * 64 * RB + 16 * m) = acc[u][m];
     }
   };
   if constexpr (WR) {
@@ -794,69 +807,86 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         auto vpixels = [&](auto first, const VPre& V, const f32x4 (&kv0)[9][MT], int l) {
           constexpr bool FIRST = decltype(first)::value;
           const int kind = block_kind(l, nb);
-          for (int j = 0; j < W; ++j) {
-            const int q = i * W + j;
-            f32x4 acc[MT];
+          // pixels in pairs (the second of an odd row's last pair reads only
+          // the zero pixel and stores nothing): the two pixels' MFMAs
+          // alternate, so their accumulator chains overlap
+          for (int j0 = 0; j0 < W; j0 += 2) {
+            f32x4 acc[2][MT];
 #pragma unroll
-            for (int m = 0; m < MT; ++m) acc[m] = V.bv0[m];
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+              for (int m = 0; m < MT; ++m) acc[u][m] = V.bv0[m];
             if constexpr (FIRST) {
 #pragma unroll
-              for (int t = 0; t < 9; ++t) {
-                const int c = j + t % 3 - 1;
-                const float sv = (c >= 0 && c < W) ? s_vspin[((t / 3) * W + c) * 16 + n] : 0.0f;
+              for (int u = 0; u < 2; ++u)
 #pragma unroll
-                for (int m = 0; m < MT; ++m) acc[m] += kv0[t][m] * sv;
-              }
+                for (int t = 0; t < 9; ++t) {
+                  const int c = j0 + u + t % 3 - 1;
+                  const float sv = (c >= 0 && c < W) ? s_vspin[((t / 3) * W + c) * 16 + n] : 0.0f;
+#pragma unroll
+                  for (int m = 0; m < MT; ++m) acc[u][m] += kv0[t][m] * sv;
+                }
             } else {
               // every tap's fragment first (padding taps read the zero pixel:
               // their MFMAs add exact zeros, as the fused forward's do), then
               // the MFMAs
-              LD x[9];
+              LD x[2][9];
+#pragma unroll
+              for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int t = 0; t < 9; ++t) {
+                  const int r = i + t / 3 - 2, c = j0 + u + t % 3 - 1;
+                  const bool pad = r < 0 || c < 0 || c >= W;
+                  x[u][t].load(pad ? vzero : vrow(t / 3 == 2 ? 2 + cur : vbase(l) + t / 3, c));
+                }
 #pragma unroll
               for (int t = 0; t < 9; ++t) {
-                const int r = i + t / 3 - 2, c = j + t % 3 - 1;
-                const bool pad = r < 0 || c < 0 || c >= W;
-                x[t].load(pad ? vzero : vrow(t / 3 == 2 ? 2 + cur : vbase(l) + t / 3, c));
-              }
+                frag b[2][KS];
 #pragma unroll
-              for (int t = 0; t < 9; ++t) {
-                frag b[KS];
-                x[t].template frags<KS>(b);
+                for (int u = 0; u < 2; ++u) x[u][t].template frags<KS>(b[u]);
 #pragma unroll
                 for (int m = 0; m < MT; ++m)
 #pragma unroll
-                  for (int ks = 0; ks < KS; ++ks) acc[m] = P::mfma(V.wv[t][m][ks], b[ks], acc[m]);
+                  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) acc[u][m] = P::mfma(V.wv[t][m][ks], b[u][ks], acc[u][m]);
               }
             }
-            f32x4 av[MT];
 #pragma unroll
-            for (int m = 0; m < MT; ++m) av[m] = act4<ACT>(acc[m]);
-            if (l < F) {
-              LD vn;
-              if (kind == BK_B) {
-                LD vr;
-                vr.load(vrow(2 + (cur ^ 1), j));
+            for (int u = 0; u < 2; ++u) {
+              const int j = j0 + u;
+              if (j >= W) break;
+              const int q = i * W + j;
+              f32x4 av[MT];
 #pragma unroll
-                for (int m = 0; m < MT; ++m) vn.set(m, act4<ACT>(vr.get(m) + acc[m]));
-              } else {
+              for (int m = 0; m < MT; ++m) av[m] = act4<ACT>(acc[u][m]);
+              if (l < F) {
+                LD vn;
+                if (kind == BK_B) {
+                  LD vr;
+                  vr.load(vrow(2 + (cur ^ 1), j));
 #pragma unroll
-                for (int m = 0; m < MT; ++m) vn.set(m, av[m]);
+                  for (int m = 0; m < MT; ++m) vn.set(m, act4<ACT>(vr.get(m) + acc[u][m]));
+                } else {
+#pragma unroll
+                  for (int m = 0; m < MT; ++m) vn.set(m, av[m]);
+                }
+                vn.store(VAp(l + 1, q));
+                vn.store(vrow(2 + (cur ^ 1), j));
               }
-              vn.store(VAp(l + 1, q));
-              vn.store(vrow(2 + (cur ^ 1), j));
-            }
-            if (i + 1 < H) {
-              frag b[KS];
-              d_to_b<PREC, MT, KS>(av, b);
-              LD yd;
+              if (i + 1 < H) {
+                frag b[KS];
+                d_to_b<PREC, MT, KS>(av, b);
+                LD yd;
 #pragma unroll
-              for (int m = 0; m < MTH; ++m) {
-                f32x4 y = V.by0[m];
+                for (int m = 0; m < MTH; ++m) {
+                  f32x4 y = V.by0[m];
 #pragma unroll
-                for (int ks = 0; ks < KS; ++ks) y = P::mfma(V.wy[m][ks], b[ks], y);
-                yd.set(MTH + m, act4<ACT>(y));
+                  for (int ks = 0; ks < KS; ++ks) y = P::mfma(V.wy[m][ks], b[ks], y);
+                  yd.set(MTH + m, act4<ACT>(y));
+                }
+                yd.store_quads(CATp(l, q + W), MTH, MT);
               }
-              yd.store_quads(CATp(l, q + W), MTH, MT);
             }
           }
           cur ^= 1;
