@@ -48,6 +48,24 @@ struct LaneData {
     for (int m = 0; m < MT; ++m)
       if (m >= m0 && m < m1) q[m] = *reinterpret_cast<const uint2*>(p + 8 * m);
   }
+  // the same loads through a buffer resource: per-lane offset voff, uniform
+  // offset soff (bytes)
+  __device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
+#pragma unroll
+    for (int k = 0; k < MT / 2; ++k) {
+      const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * k, soff, 0);
+      q[2 * k] = make_uint2(v.x, v.y);
+      q[2 * k + 1] = make_uint2(v.z, v.w);
+    }
+  }
+  __device__ __forceinline__ void bload_quads(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, int m0, int m1) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+      if (m >= m0 && m < m1) {
+        const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff + 8 * m, soff, 0);
+        q[m] = make_uint2(v.x, v.y);
+      }
+  }
   __device__ __forceinline__ void store_quads(uint8_t* p, int m0, int m1) const {
 #pragma unroll
     for (int m = 0; m < MT; ++m)
@@ -93,6 +111,18 @@ struct LaneData<FK_PREC_FP32, C> {
 #pragma unroll
     for (int m = 0; m < MT; ++m)
       if (m >= m0 && m < m1) *reinterpret_cast<f32x4*>(p + 16 * m) = q[m];
+  }
+  __device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * m, soff, 0);
+      q[m] = __builtin_bit_cast(f32x4, v);
+    }
+  }
+  __device__ __forceinline__ void bload_quads(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, int m0, int m1) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+      if (m >= m0 && m < m1) q[m] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * m, soff, 0));
   }
   __device__ __forceinline__ void store(uint8_t* p) const { store_quads(p, 0, MT); }
   __device__ __forceinline__ void set(int m, f32x4 v) { q[m] = v; }
@@ -193,7 +223,30 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
   auto VAp = [&](int l, int q) { return base + (int64_t)(2 * nb - 1 + l - 1) * BUF + (int64_t)q * PB + lane * LB; };
   float* spins = reinterpret_cast<float*>(base + (int64_t)(3 * nb - 2) * BUF);  // [HW][16]
 
-  auto wfrag = [&](int64_t fragidx) { return wall[fragidx * 64 + lane]; };
+  // Buffer resources over the weights, the biases and this tile's cache: a
+  // load takes a per-lane 32-bit offset (mostly loop-invariant) and a uniform
+  // offset in a scalar register, so no 64-bit address arithmetic per load.
+  constexpr int BUF_FLAGS = 0x00020000;  // raw buffer, 32-bit elements (gfx9)
+  const __amdgpu_buffer_rsrc_t rs_w = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.w), (short)0, 0x7fffffff, BUF_FLAGS);
+  const __amdgpu_buffer_rsrc_t rs_a =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(a.aux), (short)0, 0x7fffffff, BUF_FLAGS);
+  auto wfrag = [&](int64_t fragidx) {
+    const int soff = (int)(fragidx * 64 * (int64_t)sizeof(frag));
+    const uint32_t voff = lane * (uint32_t)sizeof(frag);
+    if constexpr (sizeof(frag) == 16) {
+      return __builtin_bit_cast(frag, __builtin_amdgcn_raw_buffer_load_b128(rs_w, voff, soff, 0));
+    } else if constexpr (sizeof(frag) == 8) {
+      return __builtin_bit_cast(frag, __builtin_amdgcn_raw_buffer_load_b64(rs_w, voff, soff, 0));
+    } else {
+      return __builtin_bit_cast(frag, __builtin_amdgcn_raw_buffer_load_b32(rs_w, voff, soff, 0));
+    }
+  };
+  const __amdgpu_buffer_rsrc_t rs_t =
+      __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)(a.tile_bytes < 0x7fffffff ? a.tile_bytes : 0x7fffffff), BUF_FLAGS);
+  // four biases of this lane's group: aux[fidx + 4 g .. + 3] (fidx uniform)
+  auto ldaux4 = [&](int fidx) {
+    return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs_a, 16u * g, fidx * 4, 0));
+  };
   // batch every load of a block step ahead of its MFMAs when a tap's
   // fragments are small (bf16/fp16 C=32); wider nets load tap by tap
   constexpr bool BATCH = FastBatch<PREC, C>::value;
@@ -536,23 +589,26 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         // row partial and reads two earlier pixels (taps 6, 7) and the current
         constexpr int CT0 = WR ? 6 : 0;
         const uint32_t o_r = (uint32_t)(j * 64 * RB + lane * RB);
+        // (tile-relative byte offsets: buffers at l * BUF, partials at rbuf)
+        const int rbuf_off = (int)(rbuf - base);
         auto issue = [&](HLoads<PREC, C, BATCH, WR>& L, int l) {
-          const uint8_t* cb = base + (int64_t)l * BUF;
+          const int cs = (int)(l * BUF);
           if constexpr (BATCH) {
             if (l > 0) {
-              const uint8_t* hb = base + (int64_t)(nb + l - 1) * BUF;
+              const int hs = (int)((nb + l - 1) * BUF);
 #pragma unroll
-              for (int t = 0; t < 2; ++t) L.hd[t].load(hb + o_hd[t]);
+              for (int t = 0; t < 2; ++t) L.hd[t].bload(rs_t, o_hd[t], hs);
             }
 #pragma unroll
-            for (int t = CT0; t < 8; ++t) L.cd[t - CT0].load(cb + o_cd[t]);
+            for (int t = CT0; t < 8; ++t) L.cd[t - CT0].bload(rs_t, o_cd[t], cs);
             if constexpr (WR) {
-              const uint8_t* rb = rbuf + (int64_t)l * VROW_MAX * 64 * RB;
+              const int rs = rbuf_off + l * VROW_MAX * 64 * RB;
 #pragma unroll
-              for (int m = 0; m < MT; ++m) L.r[m] = *reinterpret_cast<const f32x4*>(rb + o_r + 16 * m);
+              for (int m = 0; m < MT; ++m)
+                L.r[m] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs_t, o_r + 16 * m, rs, 0));
             }
           }
-          L.cur2.load_quads(cb + o_cur, MTH, MT);
+          L.cur2.bload_quads(rs_t, o_cur, cs, MTH, MT);
         };
         // One block step: its own loads (biases, weights, the first block's
         // spins) are requested first, then the next step's activations
@@ -577,14 +633,15 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
           const int kind = block_kind(l, nb);
           const float* bl = a.aux + lay.aux_bias(l);
           f32x4 hacc[MT], cacc[MT], xb[MTH];
+          const int bo = lay.aux_bias(l);
 #pragma unroll
-          for (int m = 0; m < MT; ++m) hacc[m] = ldq4(bl + C + 16 * m + 4 * g);
+          for (int m = 0; m < MT; ++m) hacc[m] = ldaux4(bo + C + 16 * m);
           if constexpr (!WR) {
 #pragma unroll
-            for (int m = 0; m < MT; ++m) cacc[m] = ldq4(bl + 3 * C + 16 * m + 4 * g);
+            for (int m = 0; m < MT; ++m) cacc[m] = ldaux4(bo + 3 * C + 16 * m);
           }
 #pragma unroll
-          for (int m = 0; m < MTH; ++m) xb[m] = ldq4(bl + 2 * C + 16 * m + 4 * g);
+          for (int m = 0; m < MTH; ++m) xb[m] = ldaux4(bo + 2 * C + 16 * m);
           frag wh[BATCH ? 3 : 1][MT][KS], wc[BATCH ? 9 - CT0 : 1][MT][KS], wx[MTH][KS];
 #pragma unroll
           for (int m = 0; m < MTH; ++m)
