@@ -112,18 +112,6 @@ struct LaneData<FK_PREC_FP32, C> {
     for (int m = 0; m < MT; ++m)
       if (m >= m0 && m < m1) *reinterpret_cast<f32x4*>(p + 16 * m) = q[m];
   }
-  __device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
-#pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * m, soff, 0);
-      q[m] = __builtin_bit_cast(f32x4, v);
-    }
-  }
-  __device__ __forceinline__ void bload_quads(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, int m0, int m1) {
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-      if (m >= m0 && m < m1) q[m] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * m, soff, 0));
-  }
   __device__ __forceinline__ void store(uint8_t* p) const { store_quads(p, 0, MT); }
   __device__ __forceinline__ void set(int m, f32x4 v) { q[m] = v; }
   __device__ __forceinline__ f32x4 get(int m) const { return q[m]; }
@@ -230,22 +218,24 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
   const __amdgpu_buffer_rsrc_t rs_w = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.w), (short)0, 0x7fffffff, BUF_FLAGS);
   const __amdgpu_buffer_rsrc_t rs_a =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(a.aux), (short)0, 0x7fffffff, BUF_FLAGS);
+  // (16-bit nets; the FP32 kernels, whose fragments are single floats, keep
+  // plain global loads, which measured faster for them)
+  constexpr bool BUFLD = Prec<PREC>::H16;
   auto wfrag = [&](int64_t fragidx) {
-    const int soff = (int)(fragidx * 64 * (int64_t)sizeof(frag));
-    const uint32_t voff = lane * (uint32_t)sizeof(frag);
-    if constexpr (sizeof(frag) == 16) {
-      return __builtin_bit_cast(frag, __builtin_amdgcn_raw_buffer_load_b128(rs_w, voff, soff, 0));
-    } else if constexpr (sizeof(frag) == 8) {
-      return __builtin_bit_cast(frag, __builtin_amdgcn_raw_buffer_load_b64(rs_w, voff, soff, 0));
+    if constexpr (BUFLD) {
+      static_assert(sizeof(frag) == 16, "16-byte fragments");
+      const int soff = (int)(fragidx * 64 * (int64_t)sizeof(frag));
+      return __builtin_bit_cast(frag, __builtin_amdgcn_raw_buffer_load_b128(rs_w, lane * 16u, soff, 0));
     } else {
-      return __builtin_bit_cast(frag, __builtin_amdgcn_raw_buffer_load_b32(rs_w, voff, soff, 0));
+      return wall[fragidx * 64 + lane];
     }
   };
   const __amdgpu_buffer_rsrc_t rs_t =
       __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)(a.tile_bytes < 0x7fffffff ? a.tile_bytes : 0x7fffffff), BUF_FLAGS);
   // four biases of this lane's group: aux[fidx + 4 g .. + 3] (fidx uniform)
   auto ldaux4 = [&](int fidx) {
-    return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs_a, 16u * g, fidx * 4, 0));
+    if constexpr (BUFLD) return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs_a, 16u * g, fidx * 4, 0));
+    else return ldq4(a.aux + fidx + 4 * g);
   };
   // batch every load of a block step ahead of its MFMAs when a tap's
   // fragments are small (bf16/fp16 C=32); wider nets load tap by tap
@@ -608,7 +598,8 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
                 L.r[m] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs_t, o_r + 16 * m, rs, 0));
             }
           }
-          L.cur2.bload_quads(rs_t, o_cur, cs, MTH, MT);
+          if constexpr (BUFLD) L.cur2.bload_quads(rs_t, o_cur, cs, MTH, MT);
+          else L.cur2.load_quads(base + cs + o_cur, MTH, MT);
         };
         // One block step: its own loads (biases, weights, the first block's
         // spins) are requested first, then the next step's activations
