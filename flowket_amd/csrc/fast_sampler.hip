@@ -303,7 +303,11 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
     for (int r = 0; r < 2; ++r) {
       const int row = i - 2 + r;
 #pragma unroll
-      for (int c = 0; c < VROW_MAX; ++c) R.x[r][c].load(CATp(l, row >= 0 && c < W ? row * W + c : HW));
+      for (int c = 0; c < VROW_MAX; ++c) {
+        const int q = row >= 0 && c < W ? row * W + c : HW;
+        if constexpr (BUFLD) R.x[r][c].bload(rs_t, lane * LB, (int)(l * BUF) + q * PB);
+        else R.x[r][c].load(CATp(l, q));
+      }
     }
   };
   // Pixels in groups of RG: the MFMAs of a group's pixels alternate, so the
