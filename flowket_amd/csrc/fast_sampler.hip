@@ -76,6 +76,18 @@ struct LaneData {
     for (int k = 0; k < MT / 2; ++k)
       *reinterpret_cast<uint4*>(p + 16 * k) = make_uint4(q[2 * k].x, q[2 * k].y, q[2 * k + 1].x, q[2 * k + 1].y);
   }
+  __device__ __forceinline__ void bstore_quads(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff, int m0,
+                                               int m1) const {
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+      if (m >= m0 && m < m1) __builtin_amdgcn_raw_buffer_store_b64(u32x2{q[m].x, q[m].y}, r, voff + 8 * m, soff, 0);
+  }
+  __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) const {
+#pragma unroll
+    for (int k = 0; k < MT / 2; ++k)
+      __builtin_amdgcn_raw_buffer_store_b128(u32x4{q[2 * k].x, q[2 * k].y, q[2 * k + 1].x, q[2 * k + 1].y}, r,
+                                             voff + 16 * k, soff, 0);
+  }
   __device__ __forceinline__ void set(int m, f32x4 v) {
     typename Prec<PREC>::quad h = to_quad<PREC>(v);
     q[m] = *reinterpret_cast<uint2*>(&h);
@@ -617,10 +629,20 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
         LD pcx, phn;
         uint8_t* const trash = base + (int64_t)(HW + 1) * PB + lane * LB;
         uint8_t *ppc = trash, *pph = trash;
+        // (16-bit: the pending stores as uniform offsets from this pixel's
+        // offset o_cur; the trash pixel of buffer 0 is HW + 1 - q pixels on)
+        const int strash = (HW + 1 - q) * PB;
+        int spc = strash, sph = strash;
         auto flush = [&]() {
-          pcx.store_quads(ppc, 0, MTH);
-          phn.store(pph);
-          ppc = pph = trash;
+          if constexpr (BUFLD) {
+            pcx.bstore_quads(rs_t, o_cur, spc, 0, MTH);
+            phn.bstore(rs_t, o_cur, sph);
+            spc = sph = strash;
+          } else {
+            pcx.store_quads(ppc, 0, MTH);
+            phn.store(pph);
+            ppc = pph = trash;
+          }
         };
         // FIRST: block 0, whose horizontal input is the spins
         auto step = [&](auto first, HLoads<PREC, C, BATCH, WR>& L, int l, HLoads<PREC, C, BATCH, WR>& Ln, int ln) {
@@ -726,7 +748,8 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
             L.cur2.set(m, act4<ACT>(x));
           }
           pcx = L.cur2;
-          ppc = CATp(l, q);
+          if constexpr (BUFLD) spc = (int)(l * BUF);
+          else ppc = CATp(l, q);
           if constexpr (WR) {
 #pragma unroll
             for (int m = 0; m < MT; ++m) cacc[m] = L.r[m];
@@ -766,7 +789,8 @@ __global__ __launch_bounds__(64) void fast_sample_kernel(SampArgs a) {
             hn.set(m, v);
           }
           phn = hn;
-          pph = HAp(l + 1, q);
+          if constexpr (BUFLD) sph = (int)((nb + l) * BUF);
+          else pph = HAp(l + 1, q);
           // the value the next block sees is the stored (rounded) one
 #pragma unroll
           for (int m = 0; m < MT; ++m) hcur[m] = hn.get(m);
